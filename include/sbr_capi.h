@@ -1,0 +1,165 @@
+/* C-ABI of the MI355X (gfx950) shoot-bounce-reflect library, libsbr_amd.so.
+ *
+ * This is the thin layer under the C++ RTSolver / Preprocessor / FieldCompute call surface
+ * (include/sbrsolver/).  Plain pointers and sizes only; no C++ exceptions cross it.
+ *
+ * Conventions
+ *   - Status: every entry point returns int, 0 = OK, negative = error class (SBR_E_*); the
+ *     message of the last error on a context is returned by sbr_last_error().
+ *   - Ownership: the caller owns every host buffer it passes; the context owns every device
+ *     buffer and frees it in sbr_destroy().  Host buffers are read during the call only.
+ *   - Threading: one context per host thread; a context is not thread-safe.
+ *
+ * Reference interfaces each entry point replaces (paths relative to SBRSolver_rev2/):
+ *   sbr_scene_from_stl     Preprocessor::StlToGeometry + BVH<Triangle>::ConstructBVH
+ *                          (Preprocessor.cpp:69-114, BVH/BVH.h:51-55, SBRSolver_rev2.cpp:67-68)
+ *   sbr_scene_from_soup    same, from the facet arrays stl_reader delivers (StlReader.h:637-700)
+ *   sbr_scene_upload       the BVH<Triangle>& / std::vector<Triangle*>& pair that
+ *                          RTSolver::Init (RTSolver.cpp:13-22) and ExecuteRayTracing take
+ *   sbr_materials          the MaterialProperties[] of RTSolver::Init (RTSolver.cpp:13-22)
+ *   sbr_trace              RTSolver::ExecuteRayTracing (RTSolver.cpp:52-126) through
+ *                          FieldCompute::FieldAtReceiver (FieldCompute.cpp:11-25)
+ *   sbr_result_fields      RTSolver::_efield (RTSolver.h:49) as SaveFieldAsCsv writes it
+ *                          (RTSolver.cpp:692-717)
+ *   sbr_result_paths       RTSolver::_rayPaths after ImagePathCorrection + RemoveDuplicatePath
+ *                          (RTSolver.h:45, RTSolver.cpp:116-121) as SavePathsAsVtk walks it
+ *                          (RTSolver.cpp:532-538)
+ *   sbr_generate_rays      GenerateRaysOnIcosahedron (Icosahedron.cpp:80-110)
+ *   sbr_closest_hit        BVH<Triangle>::RayIntersects (BVH/BVH.h:276-328)
+ *   sbr_submit / sbr_poll  (no reference counterpart: coverage-query serving loop)
+ */
+#ifndef SBR_CAPI_H
+#define SBR_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SBR_OK 0
+#define SBR_E_ARG -1       /* invalid argument */
+#define SBR_E_STATE -2     /* call out of order (no scene, no result, ...) */
+#define SBR_E_HIP -3       /* HIP runtime error */
+#define SBR_E_NOMEM -4     /* device memory exhausted or over the budget */
+#define SBR_E_IO -5        /* file could not be read / parsed */
+#define SBR_E_LIMIT -6     /* a fixed limit was exceeded (tree depth, hash table, stack) */
+#define SBR_E_INTERNAL -7  /* internal consistency check failed */
+
+typedef struct sbr_ctx sbr_ctx;
+
+/* MaterialProperties (common/DataStructures.h:20-28), same field order and layout. */
+typedef struct {
+  double frequency;
+  double transmissionLoss;
+  double reflectionLoss;
+  double relPermittivityRe;
+  double relPermittivityIm;
+  double relConductivity;
+} sbr_material;
+
+/* Flattened scene as sbr_scene_upload takes it.  Triangles are addressed by triangleId
+ * (0..n_tri-1); the BVH lists nodes in the reference's creation order with their triangle lists.
+ *   tri_v        [n_tri][9]  v1, v2, v3 (FP64)
+ *   tri_n        [n_tri][3]  normal stored in the triangle (the STL file normal)
+ *   tri_cs       [n_tri][9]  surface frame coordSys, row-major (Triangle.cpp:37-49)
+ *   tri_cop      [n_tri]     coplanar id
+ *   tri_mat      [n_tri]     material id
+ *   mesh_tri_id  [n_tri]     triangleId of the triangle at position k of the caller's mesh
+ *                            vector (the reference indexes that vector by triangleId, Q1)
+ *   node_box     [n_node][6] min xyz, max xyz
+ *   node_child   [n_node][2] children (-1 for a leaf)
+ *   node_first, node_count   leaf triangle list as a slice of leaf_tri
+ *   leaf_tri     [n_leaf_tri] triangle ids */
+typedef struct {
+  int32_t n_tri;
+  const double* tri_v;
+  const double* tri_n;
+  const double* tri_cs;
+  const int32_t* tri_cop;
+  const int32_t* tri_mat;
+  const int32_t* mesh_tri_id;
+  int32_t n_node;
+  const double* node_box;
+  const int32_t* node_child;
+  const int32_t* node_first;
+  const int32_t* node_count;
+  int32_t n_leaf_tri;
+  const int32_t* leaf_tri;
+} sbr_scene_desc;
+
+/* One transmitter and the trace options (RTSolver::ExecuteRayTracing arguments + extensions). */
+typedef struct {
+  double tx[3];
+  double txCoordSys[9];  /* row-major Mat3 */
+  int32_t maxR;          /* maxReflectionCount */
+  int32_t maxT;          /* maxTransmissionCount */
+  int32_t maxTotal;      /* extension: cap on reflections + transmissions; <= 0: none (reference) */
+  double frequency;
+  double pt;
+  int32_t useFresnel;    /* useFresnelCoeff */
+  int32_t tess;          /* txTesslation: icosahedron subdivision level */
+  int32_t antennaType;   /* 0 isotropic, 1 Hertzian (reference default), 2 half-wave dipole */
+  int32_t receptionModel;/* 0 reference sphere (RTSolver.cpp:838-841); 1 alpha*L/sqrt(3) */
+  int32_t launchStride;  /* trace launches i with i % stride == offset (1 = all) */
+  int32_t launchOffset;
+  int32_t q1MeshIndexing;/* 1 (reference): surface data of mesh[hitSurfaceID]; 0: by id */
+  int32_t exactTraversal;/* 1: visit every box the line touches (reference order); 0: clipped */
+} sbr_trace_params;
+
+/* Counters and stage times of the last trace. */
+typedef struct {
+  int64_t launches;
+  int64_t bounces;          /* RayLaunch calls = interaction-tree nodes */
+  int64_t capture_records;  /* captured (launch, node, receiver) with no capturing ancestor */
+  int64_t corrected;        /* records that survived image correction */
+  int64_t winners;          /* paths kept after duplicate removal */
+  int64_t max_depth;
+  double ms_raygen, ms_trace, ms_capture, ms_field, ms_total;
+} sbr_stats;
+
+/* device >= 0: HIP device ordinal.  device == -1: host-only context (scene preparation and
+ * scene queries work; device entry points return SBR_E_STATE). */
+int sbr_create(int device, sbr_ctx** out);
+int sbr_destroy(sbr_ctx* ctx);
+const char* sbr_last_error(const sbr_ctx* ctx);
+int sbr_version(void);
+
+int sbr_scene_from_stl(sbr_ctx* ctx, const char* path);
+int sbr_scene_from_soup(sbr_ctx* ctx, const float* coords9, const float* normals3, const int32_t* material_ids,
+                        int32_t n_facets);
+int sbr_scene_upload(sbr_ctx* ctx, const sbr_scene_desc* desc);
+/* Scene as stored: per triangleId coplanar id, material id, vertices, normal; mesh order. */
+int sbr_scene_info(sbr_ctx* ctx, int32_t* n_tri, int32_t* n_node, int32_t* n_coplanar);
+int sbr_scene_triangles(sbr_ctx* ctx, int32_t* coplanar, int32_t* material, double* verts9, double* norm3,
+                        int32_t* mesh_tri_id);
+/* BVH in DFS order as the device holds it: node_box [6], node_a/node_b (children, or
+ * first/count for a leaf marked by node_leaf=1), leaf triangle ids in DFS order. */
+int sbr_scene_bvh_dfs(sbr_ctx* ctx, double* node_box, int32_t* node_a, int32_t* node_b, int32_t* node_leaf,
+                      int32_t* tri_dfs_id);
+
+int sbr_materials(sbr_ctx* ctx, const sbr_material* mats, int32_t count);
+
+int sbr_generate_rays(sbr_ctx* ctx, int32_t tess, double* dirs_out, int64_t capacity, int64_t* n_out);
+int sbr_closest_hit(sbr_ctx* ctx, const double* orig, const double* dirs, int32_t n, int32_t exact,
+                    int32_t* tri_out, double* point_out, double* dist_out);
+
+int sbr_trace(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx, int32_t n_rx);
+int sbr_result_stats(sbr_ctx* ctx, sbr_stats* out);
+/* Per-receiver complex field: out[n_rx][3][2] (Ex re, im, Ey re, im, Ez re, im). */
+int sbr_result_fields(sbr_ctx* ctx, double* out, int32_t n_rx);
+/* Kept paths, receiver ascending, then launch ascending, then the path order of a launch. */
+int sbr_result_path_counts(sbr_ctx* ctx, int64_t* n_paths, int64_t* n_segs);
+int sbr_result_paths(sbr_ctx* ctx, int32_t* path_rx, int32_t* path_launch, int32_t* path_nseg, int64_t* path_off,
+                     int32_t* seg_tri, int32_t* seg_cop, int32_t* seg_type, double* seg_src, double* seg_tgt);
+/* Field of every kept path, in the order of sbr_result_paths: out[n_paths][3][2]. */
+int sbr_result_path_fields(sbr_ctx* ctx, double* out, int64_t n_paths);
+/* Interaction trees of the last trace, one row per node, ordered by (launch, depth, branch). */
+int sbr_result_tree_count(sbr_ctx* ctx, int64_t* n_nodes);
+int sbr_result_trees(sbr_ctx* ctx, int32_t* launch, int32_t* depth, int64_t* code, int32_t* tri, int32_t* cop,
+                     int32_t* type, double* src, double* tgt, double* pl);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SBR_CAPI_H */

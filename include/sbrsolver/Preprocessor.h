@@ -1,0 +1,31 @@
+// Host scene preparation of the reference call surface (Preprocessor.h:18-76).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "Triangle.h"
+#include "VecMatDef.h"
+
+namespace Preprocessor {
+
+// STL (ASCII or binary) -> triangles with the file normals and coplanar ids
+// (Preprocessor.cpp:69-114).  Returns false when the file cannot be read.
+bool StlToGeometry(std::string fileName, std::vector<Triangle*>& output, bool saveEdges = false,
+                   bool saveFaces = false, std::string dataDir = "");
+
+// Facet arrays as a binary STL holds them (9 vertex floats + 3 normal floats per facet), with
+// optional per-facet material ids; same vertex dedup, normal indexing and coplanar grouping.
+bool SoupToGeometry(const float* coords9, const float* normals3, const int32_t* materialIds, int nFacets,
+                    std::vector<Triangle*>& output);
+
+// Raw STL parse with exact vertex dedup (the stl_reader contract, StlReader.h:448-700).
+bool ReadStl(const std::string& fileName, std::vector<float>& coords, std::vector<float>& normals,
+             std::vector<unsigned>& tris);
+
+bool ReadLocationFile(std::string fileName, VecVec3& output);                       // :37-67
+void GenerateRxPlane(double xMin, double yMin, double xMax, double yMax, double height, double resolution,
+                     VecVec3& output);                                                 // :254-268
+void GenerateRXLine(Vec3 start, Vec3 end, double resolution, VecVec3& output);         // :270-284
+
+}  // namespace Preprocessor

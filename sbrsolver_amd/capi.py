@@ -1,0 +1,248 @@
+"""ctypes binding of libsbr_amd.so (the C-ABI in include/sbr_capi.h).
+
+There is no CPU fallback: every call runs the HIP library, and loading fails loudly when the
+library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsbr_amd.so")
+_lib = None
+
+
+class SbrError(RuntimeError):
+    pass
+
+
+class TraceParams(C.Structure):
+    """sbr_trace_params: RTSolver::ExecuteRayTracing arguments plus extensions."""
+    _fields_ = [
+        ("tx", C.c_double * 3), ("txCoordSys", C.c_double * 9),
+        ("maxR", C.c_int32), ("maxT", C.c_int32), ("maxTotal", C.c_int32),
+        ("frequency", C.c_double), ("pt", C.c_double),
+        ("useFresnel", C.c_int32), ("tess", C.c_int32), ("antennaType", C.c_int32),
+        ("receptionModel", C.c_int32), ("launchStride", C.c_int32), ("launchOffset", C.c_int32),
+        ("q1MeshIndexing", C.c_int32), ("exactTraversal", C.c_int32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("launches", C.c_int64), ("bounces", C.c_int64), ("capture_records", C.c_int64),
+        ("corrected", C.c_int64), ("winners", C.c_int64), ("max_depth", C.c_int64),
+        ("ms_raygen", C.c_double), ("ms_trace", C.c_double), ("ms_capture", C.c_double),
+        ("ms_field", C.c_double), ("ms_total", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Material(C.Structure):
+    _fields_ = [("frequency", C.c_double), ("transmissionLoss", C.c_double), ("reflectionLoss", C.c_double),
+                ("relPermittivityRe", C.c_double), ("relPermittivityIm", C.c_double),
+                ("relConductivity", C.c_double)]
+
+
+# Every symbol the header declares, with its ctypes signature.
+_vp, _i32, _i64, _dp = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
+SIGNATURES = {
+    "sbr_version": ([], C.c_int),
+    "sbr_create": ([C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "sbr_destroy": ([_vp], C.c_int),
+    "sbr_last_error": ([_vp], C.c_char_p),
+    "sbr_scene_from_stl": ([_vp, C.c_char_p], C.c_int),
+    "sbr_scene_from_soup": ([_vp, _vp, _vp, _vp, _i32], C.c_int),
+    "sbr_scene_upload": ([_vp, _vp], C.c_int),
+    "sbr_scene_info": ([_vp, _vp, _vp, _vp], C.c_int),
+    "sbr_scene_triangles": ([_vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
+    "sbr_scene_bvh_dfs": ([_vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
+    "sbr_materials": ([_vp, _vp, _i32], C.c_int),
+    "sbr_generate_rays": ([_vp, _i32, _vp, _i64, _vp], C.c_int),
+    "sbr_closest_hit": ([_vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp], C.c_int),
+    "sbr_trace": ([_vp, _vp, _vp, _i32], C.c_int),
+    "sbr_result_stats": ([_vp, _vp], C.c_int),
+    "sbr_result_fields": ([_vp, _vp, _i32], C.c_int),
+    "sbr_result_path_counts": ([_vp, _vp, _vp], C.c_int),
+    "sbr_result_paths": ([_vp] + [_vp] * 9, C.c_int),
+    "sbr_result_path_fields": ([_vp, _vp, _i64], C.c_int),
+    "sbr_result_tree_count": ([_vp, _vp], C.c_int),
+    "sbr_result_trees": ([_vp] + [_vp] * 9, C.c_int),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SbrError(f"{LIB_PATH} is not built: run python sbrsolver_amd/build.py "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _a(x):
+    return x.ctypes.data if x is not None else None
+
+
+def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None, maxTotal=0, useFresnel=True,
+                antennaType=1, receptionModel=0, launchStride=1, launchOffset=0, q1MeshIndexing=1,
+                exactTraversal=0):
+    p = TraceParams()
+    for i in range(3):
+        p.tx[i] = float(tx[i])
+    cs = np.eye(3) if txCoordSys is None else np.asarray(txCoordSys, dtype=np.float64).reshape(3, 3)
+    for i in range(9):
+        p.txCoordSys[i] = float(cs.reshape(-1)[i])
+    p.maxR, p.maxT, p.maxTotal = int(maxR), int(maxT), int(maxTotal)
+    p.frequency, p.pt = float(frequency), float(pt)
+    p.useFresnel, p.tess, p.antennaType = int(bool(useFresnel)), int(tess), int(antennaType)
+    p.receptionModel, p.launchStride, p.launchOffset = int(receptionModel), int(launchStride), int(launchOffset)
+    p.q1MeshIndexing, p.exactTraversal = int(q1MeshIndexing), int(exactTraversal)
+    return p
+
+
+DEFAULT_MATERIALS = np.array([[1.8e9, 221.13, 0.05, 1.0, 0.0, 1e6]], dtype=np.float64)
+
+
+class Result:
+    def __init__(self, ctx, nrx):
+        self.ctx, self.nrx = ctx, nrx
+        st = Stats()
+        ctx._chk(lib().sbr_result_stats(ctx.h, C.byref(st)))
+        self.stats = st.as_dict()
+        self.bounces = st.bounces
+        f = np.zeros((nrx, 3, 2))
+        ctx._chk(lib().sbr_result_fields(ctx.h, _a(f), nrx))
+        self.fields = f[..., 0] + 1j * f[..., 1]
+
+    def paths(self):
+        L, h = lib(), self.ctx.h
+        npth, nseg = C.c_int64(), C.c_int64()
+        self.ctx._chk(L.sbr_result_path_counts(h, C.byref(npth), C.byref(nseg)))
+        n, s = npth.value, nseg.value
+        out = dict(prx=np.zeros(n, np.int32), plaunch=np.zeros(n, np.int32), pnseg=np.zeros(n, np.int32),
+                   poff=np.zeros(n, np.int64), stri=np.zeros(s, np.int32), scop=np.zeros(s, np.int32),
+                   stype=np.zeros(s, np.int32), ssrc=np.zeros((s, 3)), stgt=np.zeros((s, 3)))
+        if n:
+            self.ctx._chk(L.sbr_result_paths(h, *[_a(out[k]) for k in ("prx", "plaunch", "pnseg", "poff", "stri",
+                                                                         "scop", "stype", "ssrc", "stgt")]))
+        return out
+
+    def path_fields(self):
+        n = self.stats["winners"]
+        f = np.zeros((n, 3, 2))
+        self.ctx._chk(lib().sbr_result_path_fields(self.ctx.h, _a(f), n))
+        return f[..., 0] + 1j * f[..., 1]
+
+    def trees(self):
+        L, h = lib(), self.ctx.h
+        nn = C.c_int64()
+        self.ctx._chk(L.sbr_result_tree_count(h, C.byref(nn)))
+        n = nn.value
+        out = dict(launch=np.zeros(n, np.int32), depth=np.zeros(n, np.int32), code=np.zeros(n, np.int64),
+                   tri=np.zeros(n, np.int32), cop=np.zeros(n, np.int32), type=np.zeros(n, np.int32),
+                   src=np.zeros((n, 3)), tgt=np.zeros((n, 3)), pl=np.zeros(n))
+        if n:
+            self.ctx._chk(L.sbr_result_trees(h, *[_a(out[k]) for k in ("launch", "depth", "code", "tri", "cop", "type",
+                                                                         "src", "tgt", "pl")]))
+        return out
+
+
+class Context:
+    """One sbr_ctx on one HIP device (not thread-safe, as the C-ABI)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib().sbr_create(int(device), C.byref(h))
+        if rc != 0:
+            raise SbrError(f"sbr_create failed ({rc})")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sbr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc != 0:
+            msg = lib().sbr_last_error(self.h)
+            raise SbrError(f"sbr error {rc}: {msg.decode() if msg else ''}")
+
+    # ---- scene
+    def scene_from_stl(self, path):
+        self._chk(lib().sbr_scene_from_stl(self.h, path.encode()))
+
+    def scene_from_soup(self, coords9, normals3, materials=None):
+        c = np.ascontiguousarray(coords9, dtype=np.float32).reshape(-1, 9)
+        n = np.ascontiguousarray(normals3, dtype=np.float32).reshape(-1, 3)
+        m = None if materials is None else np.ascontiguousarray(materials, dtype=np.int32)
+        self._chk(lib().sbr_scene_from_soup(self.h, _a(c), _a(n), _a(m), c.shape[0]))
+
+    def scene_info(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(lib().sbr_scene_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return dict(n_tri=a.value, n_node=b.value, n_coplanar=c.value)
+
+    def scene_triangles(self):
+        n = self.scene_info()["n_tri"]
+        cop = np.zeros(n, np.int32); mat = np.zeros(n, np.int32); mesh = np.zeros(n, np.int32)
+        v = np.zeros((n, 9)); nm = np.zeros((n, 3))
+        self._chk(lib().sbr_scene_triangles(self.h, _a(cop), _a(mat), _a(v), _a(nm), _a(mesh)))
+        return dict(coplanar=cop, material=mat, verts=v, normals=nm, mesh_tri_id=mesh)
+
+    def scene_bvh_dfs(self):
+        info = self.scene_info()
+        nn, nt = info["n_node"], info["n_tri"]
+        box = np.zeros((nn, 6)); a = np.zeros(nn, np.int32); b = np.zeros(nn, np.int32)
+        leaf = np.zeros(nn, np.int32); tri = np.zeros(nt, np.int32)
+        self._chk(lib().sbr_scene_bvh_dfs(self.h, _a(box), _a(a), _a(b), _a(leaf), _a(tri)))
+        return dict(box=box, a=a, b=b, leaf=leaf, tri_dfs_id=tri)
+
+    def materials(self, mats=None):
+        m = DEFAULT_MATERIALS if mats is None else np.ascontiguousarray(mats, dtype=np.float64).reshape(-1, 6)
+        self._chk(lib().sbr_materials(self.h, _a(m), m.shape[0]))
+
+    # ---- kernels
+    def generate_rays(self, tess):
+        n = 10 * 4 ** int(tess) + 2
+        out = np.zeros((n, 3))
+        got = C.c_int64()
+        self._chk(lib().sbr_generate_rays(self.h, int(tess), _a(out), n, C.byref(got)))
+        assert got.value == n
+        return out
+
+    def closest_hit(self, org, dirs, exact=False):
+        o = np.ascontiguousarray(org, np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+        if o.shape[0] == 1 and d.shape[0] > 1:
+            o = np.ascontiguousarray(np.repeat(o, d.shape[0], axis=0))
+        n = d.shape[0]
+        tri = np.zeros(n, np.int32); pt = np.zeros((n, 3)); dist = np.zeros(n)
+        self._chk(lib().sbr_closest_hit(self.h, _a(o), _a(d), n, int(bool(exact)), _a(tri), _a(pt), _a(dist)))
+        return tri, pt, dist
+
+    def trace(self, params, rx):
+        r = np.ascontiguousarray(rx, dtype=np.float64).reshape(-1, 3)
+        self._chk(lib().sbr_trace(self.h, C.byref(params), _a(r), r.shape[0]))
+        return Result(self, r.shape[0])

@@ -1,0 +1,368 @@
+// C-ABI entry points (include/sbr_capi.h) and the scene upload into the device layout.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "engine.h"
+#include "sbr_capi.h"
+#include "sbrsolver/BVH.h"
+#include "sbrsolver/Preprocessor.h"
+#include "sbrsolver/SceneDesc.h"
+
+struct sbr_ctx {
+  sbr::Engine eng;
+};
+
+namespace sbr {
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) return fail(SBR_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+static int put(DBuf<T>& b, const std::vector<T>& h, hipStream_t s) {
+  b.ensure(std::max<size_t>(h.size(), 1));
+  if (!b.p) return -1;
+  if (!h.empty() && hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess)
+    return -1;
+  return 0;
+}
+
+// Flattens the BVH into depth-first order (children[0] subtree first), so that the DFS position of a
+// triangle is the reference's visiting order, and builds the surface tables addressed by
+// hitSurfaceID (through the mesh vector, and by id).
+int Engine::upload(const sbr_scene_desc* d) {
+  if (!d || d->n_tri <= 0 || d->n_node <= 0 || !d->tri_v || !d->tri_n || !d->tri_cs || !d->tri_cop || !d->tri_mat ||
+      !d->mesh_tri_id || !d->node_box || !d->node_child || !d->node_first || !d->node_count || !d->leaf_tri)
+    return fail(SBR_E_ARG, "scene: missing arrays");
+  const int n = d->n_tri, nn = d->n_node;
+  for (int i = 0; i < n; i++)
+    if (d->mesh_tri_id[i] < 0 || d->mesh_tri_id[i] >= n) return fail(SBR_E_ARG, "scene: mesh_tri_id out of range");
+  std::vector<DNode> nodesOut;
+  std::vector<DTri> trisOut;
+  std::vector<double> triN;
+  nodesOut.reserve(nn);
+  // Iterative pre-order walk; `slot` remembers where a parent expects its second child index.
+  struct Item { int node; int parentSlot; int depth; };
+  std::vector<Item> st{{0, -1, 1}};
+  int height = 0;
+  int visited = 0;
+  while (!st.empty()) {
+    Item it = st.back();
+    st.pop_back();
+    if (it.node < 0 || it.node >= nn || ++visited > nn) return fail(SBR_E_ARG, "scene: malformed BVH");
+    height = std::max(height, it.depth);
+    const int me = (int)nodesOut.size();
+    if (it.parentSlot >= 0) nodesOut[it.parentSlot].b = me;
+    DNode dn{};
+    for (int a = 0; a < 3; a++) { dn.bmin[a] = d->node_box[6 * it.node + a]; dn.bmax[a] = d->node_box[6 * it.node + 3 + a]; }
+    const int c0 = d->node_child[2 * it.node], c1 = d->node_child[2 * it.node + 1];
+    if (c0 == -1) {
+      const int first = d->node_first[it.node], cnt = d->node_count[it.node];
+      if (cnt <= 0 || first < 0 || first + cnt > d->n_leaf_tri) return fail(SBR_E_ARG, "scene: bad leaf");
+      dn.a = (int32_t)trisOut.size();
+      dn.count = cnt;
+      dn.b = -1;
+      for (int k = 0; k < cnt; k++) {
+        int id = d->leaf_tri[first + k];
+        if (id < 0 || id >= n) return fail(SBR_E_ARG, "scene: leaf triangle id out of range");
+        const double* v = d->tri_v + 9 * (size_t)id;
+        DTri t{};
+        for (int a = 0; a < 3; a++) {
+          t.v1[a] = v[a];
+          t.e1[a] = v[3 + a] - v[a];
+          t.e2[a] = v[6 + a] - v[a];
+        }
+        t.id = id;
+        t.cop = d->tri_cop[id];
+        t.mat = d->tri_mat[id];
+        trisOut.push_back(t);
+        for (int a = 0; a < 3; a++) triN.push_back(d->tri_n[3 * (size_t)id + a]);
+      }
+      nodesOut.push_back(dn);
+    } else {
+      dn.a = me + 1;
+      dn.count = 0;
+      nodesOut.push_back(dn);
+      st.push_back({c1, me, it.depth + 1});
+      st.push_back({c0, -1, it.depth + 1});
+    }
+  }
+  if (height > kMaxStack) return fail(SBR_E_LIMIT, "BVH taller than the traversal stack");
+  auto surf_of = [&](int id) {
+    DSurf s{};
+    const double* v = d->tri_v + 9 * (size_t)id;
+    for (int a = 0; a < 3; a++) {
+      s.v1[a] = v[a];
+      s.e1[a] = v[3 + a] - v[a];
+      s.e2[a] = v[6 + a] - v[a];
+      s.n[a] = d->tri_n[3 * (size_t)id + a];
+    }
+    for (int k = 0; k < 9; k++) s.cs[k] = d->tri_cs[9 * (size_t)id + k];
+    return s;
+  };
+  std::vector<DSurf> sq(n), si(n);
+  for (int t = 0; t < n; t++) {
+    sq[t] = surf_of(d->mesh_tri_id[t]);
+    si[t] = surf_of(t);
+  }
+  if (device >= 0) {
+    HIPCHK(hipSetDevice(device));
+    if (put(dNodes, nodesOut, stream) || put(dTris, trisOut, stream) || put(dTriN, triN, stream) ||
+        put(dSurfQ1, sq, stream) || put(dSurfId, si, stream))
+      return fail(SBR_E_NOMEM, "scene upload");
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  hNodes.swap(nodesOut);
+  hTris.swap(trisOut);
+  hTriN.swap(triN);
+  hSurfQ1.swap(sq);
+  hSurfId.swap(si);
+  hCop.assign(d->tri_cop, d->tri_cop + n);
+  hMat.assign(d->tri_mat, d->tri_mat + n);
+  hMeshTriId.assign(d->mesh_tri_id, d->mesh_tri_id + n);
+  hVerts.assign(d->tri_v, d->tri_v + 9 * (size_t)n);
+  hNorms.assign(d->tri_n, d->tri_n + 3 * (size_t)n);
+  ntri = n;
+  nnode = nn;
+  treeHeight = height;
+  ncop = 0;
+  for (int t = 0; t < n; t++) ncop = std::max(ncop, hCop[t] + 1);
+  haveScene = true;
+  haveResult = false;
+  return SBR_OK;
+}
+
+}  // namespace sbr
+
+namespace {
+
+int from_triangles(sbr_ctx* ctx, std::vector<Triangle*>& mesh) {
+  BVH<Triangle> bvh;
+  bvh.ConstructBVH(mesh);   // sorts mesh in place, as SBRSolver_rev2.cpp:67-68 does
+  SceneArrays sa;
+  int rc;
+  if (!sa.build(mesh, bvh)) rc = ctx->eng.fail(SBR_E_INTERNAL, "scene flattening failed");
+  else rc = ctx->eng.upload(&sa.desc);
+  for (Triangle* t : mesh) delete t;
+  mesh.clear();
+  return rc;
+}
+
+template <typename F>
+int guarded(sbr_ctx* ctx, F&& f) {
+  if (!ctx) return SBR_E_ARG;
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return ctx->eng.fail(SBR_E_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return ctx->eng.fail(SBR_E_INTERNAL, e.what());
+  } catch (...) {
+    return ctx->eng.fail(SBR_E_INTERNAL, "unknown exception");
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbr_version(void) { return 1; }
+
+int sbr_create(int device, sbr_ctx** out) {
+  if (!out) return SBR_E_ARG;
+  *out = nullptr;
+  sbr_ctx* c = new (std::nothrow) sbr_ctx();
+  if (!c) return SBR_E_NOMEM;
+  int rc = c->eng.init(device);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return SBR_OK;
+}
+
+int sbr_destroy(sbr_ctx* ctx) {
+  if (!ctx) return SBR_E_ARG;
+  if (ctx->eng.device >= 0) (void)hipSetDevice(ctx->eng.device);
+  delete ctx;
+  return SBR_OK;
+}
+
+const char* sbr_last_error(const sbr_ctx* ctx) { return ctx ? ctx->eng.err.c_str() : "null context"; }
+
+int sbr_scene_from_stl(sbr_ctx* ctx, const char* path) {
+  return guarded(ctx, [&] {
+    if (!path) return ctx->eng.fail(SBR_E_ARG, "null path");
+    std::vector<Triangle*> mesh;
+    if (!Preprocessor::StlToGeometry(path, mesh)) {
+      for (Triangle* t : mesh) delete t;
+      return ctx->eng.fail(SBR_E_IO, std::string("cannot read STL file ") + path);
+    }
+    return from_triangles(ctx, mesh);
+  });
+}
+
+int sbr_scene_from_soup(sbr_ctx* ctx, const float* coords9, const float* normals3, const int32_t* mats, int32_t n) {
+  return guarded(ctx, [&] {
+    if (!coords9 || !normals3 || n <= 0) return ctx->eng.fail(SBR_E_ARG, "soup: empty input");
+    std::vector<Triangle*> mesh;
+    if (!Preprocessor::SoupToGeometry(coords9, normals3, mats, n, mesh)) {
+      for (Triangle* t : mesh) delete t;
+      return ctx->eng.fail(SBR_E_ARG, "soup: no valid triangle");
+    }
+    return from_triangles(ctx, mesh);
+  });
+}
+
+int sbr_scene_upload(sbr_ctx* ctx, const sbr_scene_desc* desc) {
+  return guarded(ctx, [&] { return ctx->eng.upload(desc); });
+}
+
+int sbr_scene_info(sbr_ctx* ctx, int32_t* n_tri, int32_t* n_node, int32_t* n_coplanar) {
+  return guarded(ctx, [&] {
+    if (!ctx->eng.haveScene) return ctx->eng.fail(SBR_E_STATE, "no scene");
+    if (n_tri) *n_tri = ctx->eng.ntri;
+    if (n_node) *n_node = ctx->eng.nnode;
+    if (n_coplanar) *n_coplanar = ctx->eng.ncop;
+    return SBR_OK;
+  });
+}
+
+int sbr_scene_triangles(sbr_ctx* ctx, int32_t* cop, int32_t* mat, double* verts9, double* norm3, int32_t* mesh) {
+  return guarded(ctx, [&] {
+    sbr::Engine& e = ctx->eng;
+    if (!e.haveScene) return e.fail(SBR_E_STATE, "no scene");
+    const size_t n = e.ntri;
+    if (cop) memcpy(cop, e.hCop.data(), n * sizeof(int32_t));
+    if (mat) memcpy(mat, e.hMat.data(), n * sizeof(int32_t));
+    if (verts9) memcpy(verts9, e.hVerts.data(), 9 * n * sizeof(double));
+    if (norm3) memcpy(norm3, e.hNorms.data(), 3 * n * sizeof(double));
+    if (mesh) memcpy(mesh, e.hMeshTriId.data(), n * sizeof(int32_t));
+    return SBR_OK;
+  });
+}
+
+int sbr_scene_bvh_dfs(sbr_ctx* ctx, double* box, int32_t* a, int32_t* b, int32_t* leaf, int32_t* triId) {
+  return guarded(ctx, [&] {
+    sbr::Engine& e = ctx->eng;
+    if (!e.haveScene) return e.fail(SBR_E_STATE, "no scene");
+    for (size_t k = 0; k < e.hNodes.size(); k++) {
+      const sbr::DNode& d = e.hNodes[k];
+      for (int q = 0; q < 3; q++) { box[6 * k + q] = d.bmin[q]; box[6 * k + 3 + q] = d.bmax[q]; }
+      a[k] = d.a;
+      b[k] = d.count > 0 ? d.count : d.b;
+      leaf[k] = d.count > 0 ? 1 : 0;
+    }
+    for (size_t k = 0; k < e.hTris.size(); k++) triId[k] = e.hTris[k].id;
+    return SBR_OK;
+  });
+}
+
+int sbr_materials(sbr_ctx* ctx, const sbr_material* mats, int32_t count) {
+  return guarded(ctx, [&] { return ctx->eng.set_materials(mats, count); });
+}
+
+int sbr_generate_rays(sbr_ctx* ctx, int32_t tess, double* out, int64_t cap, int64_t* n_out) {
+  return guarded(ctx, [&] {
+    sbr::Engine& e = ctx->eng;
+    if (hipSetDevice(e.device) != hipSuccess) return e.fail(SBR_E_HIP, "hipSetDevice");
+    int64_t n = 0;
+    int rc = e.generate_rays(tess, &n);
+    if (rc) return rc;
+    if (n_out) *n_out = n;
+    if (out && cap > 0) {
+      int64_t k = std::min(cap, n);
+      if (hipMemcpy(out, e.dRays.p, 3 * sizeof(double) * k, hipMemcpyDeviceToHost) != hipSuccess)
+        return e.fail(SBR_E_HIP, "copy rays");
+    }
+    return SBR_OK;
+  });
+}
+
+int sbr_closest_hit(sbr_ctx* ctx, const double* o, const double* d, int32_t n, int32_t exact, int32_t* tri, double* pt,
+                    double* dist) {
+  return guarded(ctx, [&] {
+    if (n < 0 || (n > 0 && (!o || !d || !tri || !pt || !dist))) return ctx->eng.fail(SBR_E_ARG, "closest_hit: bad args");
+    return ctx->eng.closest_hit(o, d, n, exact, tri, pt, dist);
+  });
+}
+
+int sbr_trace(sbr_ctx* ctx, const sbr_trace_params* p, const double* rx, int32_t n_rx) {
+  return guarded(ctx, [&] { return ctx->eng.trace(p, rx, n_rx); });
+}
+
+int sbr_result_stats(sbr_ctx* ctx, sbr_stats* out) {
+  return guarded(ctx, [&] {
+    if (!ctx->eng.haveResult) return ctx->eng.fail(SBR_E_STATE, "no trace result");
+    if (!out) return ctx->eng.fail(SBR_E_ARG, "null output");
+    *out = ctx->eng.stats;
+    return SBR_OK;
+  });
+}
+
+int sbr_result_fields(sbr_ctx* ctx, double* out, int32_t n_rx) {
+  return guarded(ctx, [&] {
+    sbr::Engine& e = ctx->eng;
+    if (!e.haveResult) return e.fail(SBR_E_STATE, "no trace result");
+    if (n_rx != e.nrx || (!out && n_rx > 0)) return e.fail(SBR_E_ARG, "fields: receiver count mismatch");
+    if (n_rx > 0) memcpy(out, e.hField.data(), 6 * sizeof(double) * n_rx);
+    return SBR_OK;
+  });
+}
+
+int sbr_result_path_counts(sbr_ctx* ctx, int64_t* n_paths, int64_t* n_segs) {
+  return guarded(ctx, [&] {
+    int64_t a = 0, b = 0;
+    int rc = ctx->eng.paths(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &a, &b, true);
+    if (n_paths) *n_paths = a;
+    if (n_segs) *n_segs = b;
+    return rc;
+  });
+}
+
+int sbr_result_paths(sbr_ctx* ctx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
+                     int32_t* scop, int32_t* stype, double* ssrc, double* stgt) {
+  return guarded(ctx, [&] {
+    if (!prx || !plaunch || !pnseg || !poff || !stri || !scop || !stype || !ssrc || !stgt)
+      return ctx->eng.fail(SBR_E_ARG, "paths: null output");
+    int64_t a = 0, b = 0;
+    return ctx->eng.paths(prx, plaunch, pnseg, poff, stri, scop, stype, ssrc, stgt, &a, &b, false);
+  });
+}
+
+int sbr_result_path_fields(sbr_ctx* ctx, double* out, int64_t n_paths) {
+  return guarded(ctx, [&] {
+    sbr::Engine& e = ctx->eng;
+    if (!e.haveResult) return e.fail(SBR_E_STATE, "no trace result");
+    if (n_paths != e.winners || (!out && n_paths > 0)) return e.fail(SBR_E_ARG, "path fields: count mismatch");
+    if (n_paths > 0 && hipMemcpy(out, e.dWinField.p, 6 * sizeof(double) * n_paths, hipMemcpyDeviceToHost) != hipSuccess)
+      return e.fail(SBR_E_HIP, "copy path fields");
+    return SBR_OK;
+  });
+}
+
+int sbr_result_tree_count(sbr_ctx* ctx, int64_t* n) {
+  return guarded(ctx, [&] {
+    if (!ctx->eng.haveResult) return ctx->eng.fail(SBR_E_STATE, "no trace result");
+    if (n) *n = ctx->eng.nodes;
+    return SBR_OK;
+  });
+}
+
+int sbr_result_trees(sbr_ctx* ctx, int32_t* launch, int32_t* depth, int64_t* code, int32_t* tri, int32_t* cop,
+                     int32_t* type, double* src, double* tgt, double* pl) {
+  return guarded(ctx, [&] {
+    if (!launch || !depth || !code || !tri || !cop || !type || !src || !tgt || !pl)
+      return ctx->eng.fail(SBR_E_ARG, "trees: null output");
+    return ctx->eng.trees(launch, depth, code, tri, cop, type, src, tgt, pl);
+  });
+}
+
+}  // extern "C"

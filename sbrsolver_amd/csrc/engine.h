@@ -1,0 +1,162 @@
+// Internal interface of the gfx950 engine behind the C-ABI (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "sbr_capi.h"
+#include "sbrsolver/sbr_math.h"
+
+namespace sbr {
+
+constexpr int kMaxDepth = 24;       // reflections + transmissions per path (tree depth)
+constexpr int kMaxStack = 64;       // BVH traversal stack (tree height bound, checked on upload)
+
+// ---- scene, device layout ---------------------------------------------------------------------
+// BVH node in depth-first order (children[0] subtree first), 64 B.
+struct alignas(64) DNode {
+  double bmin[3];
+  double bmax[3];
+  int32_t a;      // inner: first child (DFS index); leaf: first triangle (DFS index)
+  int32_t b;      // inner: second child; leaf: unused
+  int32_t count;  // leaf: triangle count (>0); inner: 0
+  int32_t pad;
+};
+// Triangle as the traversal reads it, in leaf (DFS) order, 96 B.
+struct alignas(32) DTri {
+  double v1[3];
+  double e1[3];   // v2 - v1
+  double e2[3];   // v3 - v1
+  int32_t id;     // triangleId
+  int32_t cop;    // coplanarId
+  int32_t mat;    // materialId
+  int32_t pad;
+  double pad2;
+};
+// Surface data addressed by a path's hitSurfaceID (image points, re-intersection, field frame).
+struct alignas(64) DSurf {
+  double v1[3];
+  double e1[3];
+  double e2[3];
+  double n[3];
+  double cs[9];   // coordSys, row-major
+  double pad[3];
+};
+
+// ---- interaction tree ---------------------------------------------------------------------------
+// One RayLaunch call (one bounce), 128 B.
+struct alignas(64) TNode {
+  double src[3];
+  double tgt[3];
+  double dir[3];     // direction the segment was traced with
+  double pl;         // pathLength as stored by the reference (cumulative; INF for a miss)
+  double plArg;      // totalPathLength that RayCapture hands to this node (sum of ancestors' pl)
+  double trueLen;    // geometric length from the transmitter to src
+  int32_t parent;    // node index of the parent, -1 for a root
+  int32_t launch;
+  int32_t tri;       // hitSurfaceID, -1 on a miss
+  int32_t cop;       // hitCoplanarId, -1 on a miss
+  int32_t mat;       // material of the hit surface
+  uint32_t code;     // branch code: bit per level, transmit = 0, reflect = 1, first level most significant
+  uint8_t depth;
+  uint8_t type;      // 0 from the transmitter, 1 reflected, 2 transmitted
+  uint8_t rc, tc;    // reflections / transmissions so far
+  int32_t pad;
+};
+static_assert(sizeof(TNode) == 128, "TNode layout");
+
+struct TraceConsts {
+  double tx[3];
+  double txcs[9];
+  double freq, pt, rcoef;
+  int32_t maxR, maxT, maxTotal;
+  int32_t useFresnel, antennaType, receptionModel, exact;
+};
+
+struct RxGrid {
+  double bmin[3], bmax[3];
+  double gx0, gy0, cell;
+  int32_t nx, ny;
+};
+
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;  // capacity in elements
+  void ensure(size_t want);
+  void release();
+  ~DBuf() { release(); }
+};
+
+struct Engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // scene (host copies for queries)
+  int ntri = 0, nnode = 0, ncop = 0, treeHeight = 0;
+  std::vector<DNode> hNodes;
+  std::vector<DTri> hTris;
+  std::vector<double> hTriN;                // normals in DFS order
+  std::vector<DSurf> hSurfQ1, hSurfId;      // by hitSurfaceID: mesh-vector mapping / identity
+  std::vector<int32_t> hCop, hMat, hMeshTriId;
+  std::vector<double> hVerts, hNorms;       // by triangleId
+  DBuf<DNode> dNodes;
+  DBuf<DTri> dTris;
+  DBuf<double> dTriN;
+  DBuf<DSurf> dSurfQ1, dSurfId;
+  std::vector<double> mats;                 // 6 per material
+  DBuf<double> dMats;
+  bool haveScene = false;
+
+  // per-trace buffers
+  DBuf<double> dRays;            // 3 per launch
+  DBuf<int32_t> dFaces[2], dAdj[2];
+  DBuf<int32_t> dFlag, dScan;
+  DBuf<TNode> dTree;
+  DBuf<int32_t> dQueue;          // (parent << 1) | reflect
+  DBuf<int32_t> dChild;          // child mask per node of a level
+  DBuf<int32_t> dChildScan;
+  DBuf<uint8_t> dScratch;        // library temp storage (scan / sort)
+  DBuf<double> dRx;
+  DBuf<int32_t> dCellStart, dCellRx;
+  DBuf<uint8_t> dLos;
+  DBuf<unsigned long long> dTabA, dTabB, dTabOrder;
+  DBuf<int32_t> dTabRx;
+  DBuf<unsigned long long> dCounters;
+  DBuf<unsigned long long> dWinKey, dWinKey2;
+  DBuf<int32_t> dWinRx, dWinRx2;
+  DBuf<double> dWinField;
+  DBuf<int32_t> dRxStart;
+  DBuf<double> dField;
+
+  // last result
+  bool haveResult = false;
+  sbr_trace_params lastParams{};
+  sbr_stats stats{};
+  int nrx = 0;
+  std::vector<int64_t> levelBase;  // node index of the first node of each depth
+  int64_t nodes = 0;
+  int64_t winners = 0;
+  std::vector<double> hField;
+  std::vector<double> hRx;
+  RxGrid grid{};
+
+  ~Engine();
+  int fail(int code, const std::string& msg) { err = msg; return code; }
+  int init(int dev);
+  int upload(const sbr_scene_desc* d);
+  int set_materials(const sbr_material* m, int n);
+  int generate_rays(int tess, int64_t* nOut);   // fills dRays
+  int trace(const sbr_trace_params* p, const double* rx, int m);
+  int closest_hit(const double* o, const double* d, int n, int exact, int32_t* tri, double* pt, double* dist);
+  int paths(int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri, int32_t* scop,
+            int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs, bool countOnly);
+  int trees(int32_t* launch, int32_t* depth, int64_t* code, int32_t* tri, int32_t* cop, int32_t* type, double* src,
+            double* tgt, double* pl);
+};
+
+}  // namespace sbr

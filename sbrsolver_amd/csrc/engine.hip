@@ -1,0 +1,1153 @@
+// gfx950 engine of the shoot-bounce-reflect trace-and-field step.
+//
+// Pipeline of one sbr_trace call (RTSolver::ExecuteRayTracing, RTSolver.cpp:52-126):
+//   1. k_ico_flag / k_ico_split / k_ico_dirs: icosahedral launch directions, built level by level
+//      in the vertex order of Icosahedron.cpp:3-110.
+//   2. k_trace_level<EXACT> + k_emit_level: the interaction tree of every launch, one kernel per
+//      tree depth (RayLaunch, RTSolver.cpp:128-257).  Each level's queue is compacted with a stable
+//      scan, so within a depth the node index grows with (launch, branch code); the triple
+//      (launch, depth, index-in-level) is then an order key equal to the reference's
+//      (launch, BFS position) order.
+//   3. k_los: direct-path line of sight per receiver (DirectPathCorrection, RTSolver.cpp:375-394).
+//   4. k_capture: every tree node against the receivers binned in a uniform xy grid
+//      (HitReceptionSphere, RTSolver.cpp:805-863); a capture counts when no ancestor captures
+//      (RayCapture + Paths, RTSolver.cpp:259-297, Ray.cpp:136-182); image correction in FP64
+//      (RTSolver.cpp:299-433); duplicate removal keeps the minimum order key per (receiver,
+//      coplanar-id sequence) in a lock-free hash table (RemoveDuplicatePath, RTSolver.cpp:435-459).
+//   5. k_collect + two stable radix sorts + k_field + k_rx_sum: the field of every kept path
+//      (FieldForPath, FieldCompute.cpp:71-151) summed per receiver in (launch, BFS) order
+//      (FieldAtReceiver, FieldCompute.cpp:11-25).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "device_common.h"
+#include "engine.h"
+
+namespace sbr {
+
+#define HIPCHK(x)                                                                     \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) return fail(SBR_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+void DBuf<T>::ensure(size_t want) {
+  if (want <= n && p) return;
+  release();
+  size_t cap = want + want / 4 + 64;
+  if (hipMalloc((void**)&p, cap * sizeof(T)) != hipSuccess) {
+    p = nullptr;
+    n = 0;
+    (void)hipGetLastError();
+    return;
+  }
+  n = cap;
+}
+template <typename T>
+void DBuf<T>::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  n = 0;
+}
+
+template struct DBuf<double>;
+template struct DBuf<int32_t>;
+template struct DBuf<uint8_t>;
+template struct DBuf<unsigned long long>;
+template struct DBuf<int64_t>;
+template struct DBuf<DNode>;
+template struct DBuf<DTri>;
+template struct DBuf<DSurf>;
+template struct DBuf<TNode>;
+
+#define ENSURE(buf, cnt)                                                                         \
+  do {                                                                                           \
+    (buf).ensure((size_t)(cnt));                                                                 \
+    if (!(buf).p) return fail(SBR_E_NOMEM, std::string("device allocation failed: ") + #buf);  \
+  } while (0)
+
+// Grow a buffer keeping its first `keep` elements.
+template <typename T>
+static int grow_keep(DBuf<T>& b, size_t want, size_t keep, hipStream_t s) {
+  if (want <= b.n && b.p) return 0;
+  size_t cap = want + want / 2 + 1024;
+  T* np = nullptr;
+  if (hipMalloc((void**)&np, cap * sizeof(T)) != hipSuccess) { (void)hipGetLastError(); return -1; }
+  if (b.p && keep) {
+    if (hipMemcpyAsync(np, b.p, keep * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
+    if (hipStreamSynchronize(s) != hipSuccess) return -1;
+  }
+  b.release();
+  b.p = np;
+  b.n = cap;
+  return 0;
+}
+
+static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
+
+// =============================================================================================
+// 1. Icosahedral launch directions (Icosahedron.cpp:3-110)
+// =============================================================================================
+// Each edge of the closed mesh belongs to two faces.  Subdivide() appends a new vertex the first
+// time an edge is met in (face, edge slot) order, i.e. in the face with the smaller index.
+__global__ void k_ico_flag(const int32_t* __restrict__ adj, int32_t nf, int32_t* __restrict__ flag) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= 3 * (int64_t)nf) return;
+  int32_t f = (int32_t)(s / 3);
+  flag[s] = f < adj[2 * s] ? 1 : 0;
+}
+
+// Faces (v0, v1, v2) get midpoints m0 (v0v1), m1 (v1v2), m2 (v2v0) and the children
+// (v0,m0,m2), (v1,m1,m0), (v2,m2,m1), (m0,m1,m2) at 4f..4f+3 (Icosahedron.cpp:23-41).
+// adj[2*(3f+e)] = neighbour face across edge e, adj[2*(3f+e)+1] = that edge's slot in it.
+__global__ void k_ico_split(const int32_t* __restrict__ faces, const int32_t* __restrict__ adj,
+                            const int32_t* __restrict__ flag, const int32_t* __restrict__ pos, int32_t nf,
+                            int64_t nv, double* __restrict__ verts, int32_t* __restrict__ faces2,
+                            int32_t* __restrict__ adj2, int writeChildren) {
+  int32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int32_t v[3] = {faces[3 * f], faces[3 * f + 1], faces[3 * f + 2]};
+  int32_t m[3];
+#pragma unroll
+  for (int e = 0; e < 3; e++) {
+    int64_t s = 3 * (int64_t)f + e;
+    if (flag[s]) {
+      m[e] = (int32_t)(nv + pos[s]);
+      int a = v[e], b = v[(e + 1) % 3];
+      V3 p = vnormalized(vadd(ld3(verts + 3 * (int64_t)a), ld3(verts + 3 * (int64_t)b)));
+      double* o = verts + 3 * (int64_t)m[e];
+      o[0] = p.x; o[1] = p.y; o[2] = p.z;
+    } else {
+      int64_t t = 3 * (int64_t)adj[2 * s] + adj[2 * s + 1];
+      m[e] = (int32_t)(nv + pos[t]);
+    }
+  }
+  if (!writeChildren) return;
+  int32_t c[4][3] = {{v[0], m[0], m[2]}, {v[1], m[1], m[0]}, {v[2], m[2], m[1]}, {m[0], m[1], m[2]}};
+  for (int k = 0; k < 4; k++)
+    for (int j = 0; j < 3; j++) faces2[3 * (4 * (int64_t)f + k) + j] = c[k][j];
+  // Half edges: child x, slot 0 lies on parent edge x and touches v_x; slot 2 lies on parent
+  // edge (x+2)%3 and touches v_x.  Across parent edge E the neighbour g holds the matching half:
+  // its child at the slot of the shared vertex.
+  auto half = [&](int E, int w, int32_t* outF, int32_t* outS) {
+    int64_t s = 3 * (int64_t)f + E;
+    int32_t g = adj[2 * s], es = adj[2 * s + 1];
+    if (faces[3 * g + es] == w) { *outF = 4 * g + es; *outS = 0; }
+    else { *outF = 4 * g + (es + 1) % 3; *outS = 2; }
+  };
+  for (int x = 0; x < 3; x++) {
+    int64_t base = 2 * 3 * (4 * (int64_t)f + x);
+    int32_t nf0, ns0, nf2, ns2;
+    half(x, v[x], &nf0, &ns0);
+    half((x + 2) % 3, v[x], &nf2, &ns2);
+    adj2[base + 0] = nf0; adj2[base + 1] = ns0;
+    adj2[base + 2] = 4 * f + 3; adj2[base + 3] = (x + 2) % 3;   // interior edge, centre child
+    adj2[base + 4] = nf2; adj2[base + 5] = ns2;
+  }
+  int64_t base = 2 * 3 * (4 * (int64_t)f + 3);
+  adj2[base + 0] = 4 * f + 1; adj2[base + 1] = 1;
+  adj2[base + 2] = 4 * f + 2; adj2[base + 3] = 1;
+  adj2[base + 4] = 4 * f + 0; adj2[base + 5] = 1;
+}
+
+// GenerateRaysOnIcosahedron: directions are the normalized vertices, in vertex order.
+__global__ void k_ico_dirs(const double* __restrict__ verts, int64_t nv, double* __restrict__ dirs) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  V3 d = vnormalized(ld3(verts + 3 * i));
+  dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+}
+
+// =============================================================================================
+// 2. Interaction tree, one depth per launch (RayLaunch, RTSolver.cpp:128-257)
+// =============================================================================================
+struct LevelArgs {
+  int32_t depth;
+  int32_t n;              // items in this level
+  int64_t base;           // node index of item 0
+  int32_t launchStride, launchOffset;
+};
+
+template <bool EXACT>
+__global__ void __launch_bounds__(256) k_trace_level(LevelArgs L, TraceConsts K, const DNode* __restrict__ nodes,
+                                                     const DTri* __restrict__ tris, const double* __restrict__ triN,
+                                                     const double* __restrict__ rays, const int32_t* __restrict__ queue,
+                                                     TNode* __restrict__ tree, int32_t* __restrict__ childMask) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L.n) return;
+  TNode nd;
+  V3 src, dir;
+  double totalPL;
+  if (L.depth == 0) {
+    int32_t launch = L.launchOffset + i * L.launchStride;
+    src = v3(K.tx[0], K.tx[1], K.tx[2]);
+    dir = ld3(rays + 3 * (int64_t)launch);
+    nd.parent = -1;
+    nd.launch = launch;
+    nd.code = 0;
+    nd.type = 0;
+    nd.rc = nd.tc = 0;
+    nd.plArg = 0.0;
+    nd.trueLen = 0.0;
+    totalPL = 0.0;
+  } else {
+    int32_t q = queue[i];
+    int32_t pi = q >> 1;
+    bool refl = q & 1;
+    const TNode& P = tree[pi];
+    src = ld3(P.tgt);
+    V3 pd = ld3(P.dir);
+    if (refl) {
+      // Reflect about the hit triangle's stored normal, then normalize (RTSolver.cpp:186-203).
+      V3 n = ld3(triN + 3 * (int64_t)P.pad);
+      dir = vnormalized(vreflect(pd, n));
+    } else {
+      dir = pd;
+    }
+    nd.parent = pi;
+    nd.launch = P.launch;
+    nd.code = (P.code << 1) | (refl ? 1u : 0u);
+    nd.type = refl ? 1 : 2;
+    nd.rc = P.rc + (refl ? 1 : 0);
+    nd.tc = P.tc + (refl ? 0 : 1);
+    nd.plArg = P.plArg + P.pl;
+    nd.trueLen = P.trueLen + vnorm(vsub(ld3(P.tgt), ld3(P.src)));
+    totalPL = P.pl;
+  }
+  nd.depth = (uint8_t)L.depth;
+  nd.src[0] = src.x; nd.src[1] = src.y; nd.src[2] = src.z;
+  nd.dir[0] = dir.x; nd.dir[1] = dir.y; nd.dir[2] = dir.z;
+  Hit h;
+  int mask = 0;
+  if (closest_hit<EXACT>(nodes, tris, src, dir, h)) {
+    const DTri& t = tris[h.idx];
+    nd.tgt[0] = h.p.x; nd.tgt[1] = h.p.y; nd.tgt[2] = h.p.z;
+    nd.pl = totalPL + h.dist;
+    nd.tri = t.id;
+    nd.cop = t.cop;
+    nd.mat = t.mat;
+    nd.pad = h.idx;   // DFS index of the hit triangle (normal lookup for the reflected child)
+    bool depthOk = K.maxTotal <= 0 || (nd.rc + nd.tc) < K.maxTotal;
+    if (nd.tc < K.maxT && depthOk) mask |= 1;   // transmitted child (BFS order: first)
+    if (nd.rc < K.maxR && depthOk) mask |= 2;   // reflected child
+  } else {
+    // A miss leaves a 100 m stub below the root, a 1e10 m segment at the root (RTSolver.cpp:224-256).
+    double len = L.depth == 0 ? 1E10 : 1E2;
+    V3 t = vadd(src, vmul(dir, len));
+    nd.tgt[0] = t.x; nd.tgt[1] = t.y; nd.tgt[2] = t.z;
+    nd.pl = INFINITY;
+    nd.tri = -1;
+    nd.cop = -1;
+    nd.mat = -1;
+    nd.pad = -1;
+  }
+  tree[L.base + i] = nd;
+  childMask[i] = mask;
+}
+
+__global__ void k_child_count(const int32_t* __restrict__ mask, int32_t n, int32_t* __restrict__ cnt) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int m = mask[i];
+  cnt[i] = (m & 1) + ((m >> 1) & 1);
+}
+
+__global__ void k_emit_level(const int32_t* __restrict__ mask, const int32_t* __restrict__ off, int32_t n,
+                             int64_t base, int32_t* __restrict__ next) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int m = mask[i];
+  int32_t o = off[i];
+  int32_t node = (int32_t)(base + i);
+  if (m & 1) next[o++] = node << 1;         // transmit
+  if (m & 2) next[o++] = (node << 1) | 1;   // reflect
+}
+
+// =============================================================================================
+// 3. Direct-path line of sight (DirectPathCorrection, RTSolver.cpp:375-394)
+// =============================================================================================
+template <bool EXACT>
+__global__ void k_los(TraceConsts K, const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
+                      const double* __restrict__ rx, int32_t m, uint8_t* __restrict__ los) {
+  int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  V3 tx = v3(K.tx[0], K.tx[1], K.tx[2]);
+  V3 d = vsub(ld3(rx + 3 * (int64_t)r), tx);
+  Hit h;
+  bool hh = closest_hit<EXACT>(nodes, tris, tx, d, h);
+  los[r] = (!hh || vnorm(d) < h.dist) ? 1 : 0;
+}
+
+// =============================================================================================
+// 4. Capture, image correction, duplicate removal
+// =============================================================================================
+struct CaptureArgs {
+  int64_t nodes;
+  int64_t levelBase[kMaxDepth + 2];
+  int32_t nrx;
+  RxGrid g;
+  unsigned long long tabMask;
+};
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebull;
+  x ^= x >> 31;
+  return x;
+}
+
+// Counters: [0] capture records, [1] corrected, [2] table slots used, [3] flags (1 overflow,
+// 2 hash collision).
+__global__ void __launch_bounds__(256) k_capture(CaptureArgs A, TraceConsts K, const TNode* __restrict__ tree,
+                                                 const DSurf* __restrict__ surf, const double* __restrict__ rx,
+                                                 const int32_t* __restrict__ cellStart,
+                                                 const int32_t* __restrict__ cellRx, const uint8_t* __restrict__ los,
+                                                 unsigned long long* __restrict__ tabA,
+                                                 unsigned long long* __restrict__ tabB,
+                                                 unsigned long long* __restrict__ tabOrder,
+                                                 int32_t* __restrict__ tabRx, unsigned long long* __restrict__ counters) {
+  __shared__ unsigned long long sRec, sCor, sUsed;
+  if (threadIdx.x == 0) { sRec = 0; sCor = 0; sUsed = 0; }
+  __syncthreads();
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long myRec = 0, myCor = 0, myUsed = 0;
+  if (n < A.nodes) {
+    const TNode nd = tree[n];
+    const int depth = nd.depth;
+    const V3 S = ld3(nd.src), T = ld3(nd.tgt);
+    const V3 D = vsub(T, S);
+    // Conservative search radius: a capture at C has its closest point P within
+    // (base + |C - ref|) * rc / (1 - rc) of C, with base/ref per reception model.
+    const double rc = K.rcoef;
+    double far2 = 0.0;
+    for (int a = 0; a < 3; a++) {
+      double sa = vget(K.receptionModel == 0 ? v3(0, 0, 0) : S, a);
+      double e = fmax(fabs(A.g.bmin[a] - sa), fabs(A.g.bmax[a] - sa));
+      far2 += e * e;
+    }
+    double base = K.receptionModel == 0 ? nd.plArg : nd.trueLen;
+    double R = (base + sqrt(far2)) * rc / (1.0 - rc);
+    R = R * (1.0 + 1e-6) + 1e-6;
+    bool all = !(R < 1e300) || rc >= 1.0;
+    // Clip the segment to the receivers' box grown by R.
+    double u0 = 0.0, u1 = 1.0;
+    bool any = true;
+    if (!all) {
+      for (int a = 0; a < 3 && any; a++) {
+        double lo = A.g.bmin[a] - R, hi = A.g.bmax[a] + R, s = vget(S, a), d = vget(D, a);
+        if (fabs(d) < 1e-300) {
+          if (s < lo || s > hi) any = false;
+        } else {
+          double ua = (lo - s) / d, ub = (hi - s) / d;
+          u0 = fmax(u0, fmin(ua, ub) - 1e-12);
+          u1 = fmin(u1, fmax(ua, ub) + 1e-12);
+          if (u0 > u1) any = false;
+        }
+      }
+    }
+    if (any) {
+      // Per-node path data: image sources and surfaces (GetImageSources, RTSolver.cpp:396-433)
+      // and the hash of the coplanar-id sequence of the ancestors (the dedup key without rx).
+      int32_t chain[kMaxDepth + 1];
+      {
+        int64_t c = n;
+        for (int k = depth; k >= 0; k--) { chain[k] = (int32_t)c; c = tree[c].parent; }
+      }
+      V3 img[kMaxDepth + 1];
+      int32_t isurf[kMaxDepth + 1];
+      unsigned long long seqA = 0x243f6a8885a308d3ull ^ (unsigned long long)depth;
+      unsigned long long seqB = 0x13198a2e03707344ull + (unsigned long long)depth;
+      bool chainOk = true;
+      {
+        V3 cur = v3(K.tx[0], K.tx[1], K.tx[2]);
+        img[0] = cur;
+        isurf[0] = -1;
+        for (int k = 0; k < depth; k++) {
+          const TNode& a = tree[chain[k]];
+          const TNode& b = tree[chain[k + 1]];
+          seqA = mix64(seqA + (unsigned long long)(uint32_t)a.cop + 0x9e3779b97f4a7c15ull);
+          seqB = mix64(seqB ^ ((unsigned long long)(uint32_t)a.cop * 0xd6e8feb86659fd93ull + (unsigned long long)k));
+          if (a.tri < 0) { chainOk = false; break; }
+          if (b.type == 1) cur = image_point(surf[a.tri], cur);
+          else if (b.type != 2) { chainOk = false; break; }
+          img[k + 1] = cur;
+          isurf[k + 1] = a.tri;
+        }
+      }
+      const unsigned long long orderKey = ((unsigned long long)nd.launch << 38) |
+                                          ((unsigned long long)depth << 33) |
+                                          (unsigned long long)(n - A.levelBase[depth]);
+      // Candidate receivers: grid rows crossed by the clipped segment's xy band.
+      V3 P0 = vadd(S, vmul(D, u0)), P1 = vadd(S, vmul(D, u1));
+      double ylo = fmin(P0.y, P1.y) - R, yhi = fmax(P0.y, P1.y) + R;
+      int j0 = 0, j1 = A.g.ny - 1;
+      if (!all) {
+        j0 = (int)fmax(0.0, floor((ylo - A.g.gy0) / A.g.cell));
+        j1 = (int)fmin((double)(A.g.ny - 1), floor((yhi - A.g.gy0) / A.g.cell));
+      }
+      for (int j = j0; j <= j1; j++) {
+        int i0 = 0, i1 = A.g.nx - 1;
+        if (!all) {
+          double xa, xb;
+          double by0 = A.g.gy0 + j * A.g.cell - R, by1 = A.g.gy0 + (j + 1) * A.g.cell + R;
+          double dy = P1.y - P0.y;
+          if (fabs(dy) < 1e-300) {
+            xa = fmin(P0.x, P1.x);
+            xb = fmax(P0.x, P1.x);
+          } else {
+            double va = (by0 - P0.y) / dy, vb = (by1 - P0.y) / dy;
+            double w0 = fmax(0.0, fmin(va, vb) - 1e-12), w1 = fmin(1.0, fmax(va, vb) + 1e-12);
+            if (w0 > w1) continue;
+            double xa0 = P0.x + (P1.x - P0.x) * w0, xa1 = P0.x + (P1.x - P0.x) * w1;
+            xa = fmin(xa0, xa1);
+            xb = fmax(xa0, xa1);
+          }
+          i0 = (int)fmax(0.0, floor((xa - R - A.g.gx0) / A.g.cell));
+          i1 = (int)fmin((double)(A.g.nx - 1), floor((xb + R - A.g.gx0) / A.g.cell));
+        }
+        for (int ci = i0; ci <= i1; ci++) {
+          int cell = j * A.g.nx + ci;
+          for (int q = cellStart[cell]; q < cellStart[cell + 1]; q++) {
+            int32_t r = cellRx[q];
+            V3 C = ld3(rx + 3 * (int64_t)r);
+            if (!sphere_hit(S, T, C, nd.plArg, nd.trueLen, K.receptionModel, rc)) continue;
+            bool blocked = false;
+            for (int k = depth - 1; k >= 0 && !blocked; k--) {
+              const TNode& a = tree[chain[k]];
+              blocked = sphere_hit(ld3(a.src), ld3(a.tgt), C, a.plArg, a.trueLen, K.receptionModel, rc);
+            }
+            if (blocked) continue;
+            myRec++;
+            // Image correction (MultiPathCorrection / DirectPathCorrection).
+            bool ok;
+            if (depth == 0) {
+              ok = los[r] != 0;
+            } else if (!chainOk) {
+              ok = false;
+            } else {
+              ok = true;
+              V3 target = C;
+              for (int k = depth; k >= 1 && ok; k--) {
+                const DSurf& s = surf[isurf[k]];
+                V3 p;
+                double dist;
+                ok = tri_intersect(ld3(s.v1), ld3(s.e1), ld3(s.e2), img[k], vsub(target, img[k]), &p, &dist);
+                target = p;
+              }
+            }
+            if (!ok) continue;
+            myCor++;
+            unsigned long long hA = mix64(seqA ^ ((unsigned long long)(uint32_t)r * 0x9e3779b97f4a7c15ull));
+            unsigned long long hB = mix64(seqB + ((unsigned long long)(uint32_t)r << 7) + 0x632be59bd9b4e019ull);
+            if (hA == 0) hA = 1;
+            if (hB == 0) hB = 1;
+            unsigned long long slot = hA & A.tabMask;
+            bool placed = false;
+            for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
+              unsigned long long prev = atomicCAS(&tabA[slot], 0ull, hA);
+              if (prev == 0ull || prev == hA) {
+                if (prev == 0ull) { tabRx[slot] = r; myUsed++; }
+                unsigned long long pb = atomicCAS(&tabB[slot], 0ull, hB);
+                if (pb != 0ull && pb != hB) atomicOr(&counters[3], 2ull);
+                atomicMin(&tabOrder[slot], orderKey);
+                placed = true;
+                break;
+              }
+              slot = (slot + 1) & A.tabMask;
+            }
+            if (!placed) atomicOr(&counters[3], 1ull);
+          }
+        }
+      }
+    }
+  }
+  if (myRec) atomicAdd(&sRec, myRec);
+  if (myCor) atomicAdd(&sCor, myCor);
+  if (myUsed) atomicAdd(&sUsed, myUsed);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (sRec) atomicAdd(&counters[0], sRec);
+    if (sCor) atomicAdd(&counters[1], sCor);
+    if (sUsed) atomicAdd(&counters[2], sUsed);
+  }
+}
+
+__global__ void k_collect(const unsigned long long* __restrict__ tabA, const unsigned long long* __restrict__ tabOrder,
+                          const int32_t* __restrict__ tabRx, unsigned long long cap,
+                          unsigned long long* __restrict__ winKey, int32_t* __restrict__ winRx,
+                          unsigned long long* __restrict__ counter) {
+  unsigned long long s = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= cap) return;
+  if (tabA[s] == 0ull) return;
+  unsigned long long w = atomicAdd(counter, 1ull);
+  winKey[w] = tabOrder[s];
+  winRx[w] = tabRx[s];
+}
+
+// =============================================================================================
+// 5. Field of every kept path (FieldForPath, FieldCompute.cpp:71-151) and per-receiver sums
+// =============================================================================================
+struct FieldArgs {
+  int64_t winners;
+  int64_t levelBase[kMaxDepth + 2];
+};
+
+// Corrected vertices of the path of node n to receiver C: pts[0] = tx, pts[1..depth] the
+// re-intersections, pts[depth+1] = C.  Returns false if the correction fails.
+__device__ bool corrected_points(const TraceConsts& K, const TNode* __restrict__ tree, const DSurf* __restrict__ surf,
+                                 const int32_t* chain, int depth, V3 C, V3* pts) {
+  V3 img[kMaxDepth + 1];
+  int32_t isurf[kMaxDepth + 1];
+  V3 cur = v3(K.tx[0], K.tx[1], K.tx[2]);
+  img[0] = cur;
+  for (int k = 0; k < depth; k++) {
+    const TNode& a = tree[chain[k]];
+    const TNode& b = tree[chain[k + 1]];
+    if (a.tri < 0) return false;
+    if (b.type == 1) cur = image_point(surf[a.tri], cur);
+    else if (b.type != 2) return false;
+    img[k + 1] = cur;
+    isurf[k + 1] = a.tri;
+  }
+  pts[0] = img[0];
+  pts[depth + 1] = C;
+  V3 target = C;
+  for (int k = depth; k >= 1; k--) {
+    const DSurf& s = surf[isurf[k]];
+    V3 p;
+    double dist;
+    if (!tri_intersect(ld3(s.v1), ld3(s.e1), ld3(s.e2), img[k], vsub(target, img[k]), &p, &dist)) return false;
+    pts[k] = p;
+    target = p;
+  }
+  return true;
+}
+
+__device__ __forceinline__ int64_t node_of(const int64_t* levelBase, unsigned long long key) {
+  int depth = (int)((key >> 33) & 31ull);
+  return levelBase[depth] + (int64_t)(key & ((1ull << 33) - 1));
+}
+
+// Spherical angles of a local vector (CartesianToSpherical, VecMatDef.h:132-155).
+__device__ __forceinline__ void cart_to_sph(V3 v, double* r, double* th, double* ph) {
+  *r = vnorm(v);
+  if (v.x == 0.0 && v.y == 0.0 && v.z == 0.0) *th = SBR_PI / 2;
+  else *th = atan2(sqrt(v.x * v.x + v.y * v.y), v.z);
+  double p = atan2(v.y, v.x);
+  if (p < 0) p += 2 * SBR_PI;
+  *ph = p;
+}
+
+__global__ void __launch_bounds__(256) k_field(FieldArgs F, TraceConsts K, const TNode* __restrict__ tree,
+                                               const DSurf* __restrict__ surf, const double* __restrict__ mats,
+                                               const double* __restrict__ rx, const unsigned long long* __restrict__ winKey,
+                                               const int32_t* __restrict__ winRx, double* __restrict__ winField) {
+  int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= F.winners) return;
+  unsigned long long key = winKey[w];
+  int64_t n = node_of(F.levelBase, key);
+  int depth = tree[n].depth;
+  int32_t chain[kMaxDepth + 1];
+  {
+    int64_t c = n;
+    for (int k = depth; k >= 0; k--) { chain[k] = (int32_t)c; c = tree[c].parent; }
+  }
+  V3 C = ld3(rx + 3 * (int64_t)winRx[w]);
+  V3 pts[kMaxDepth + 2];
+  if (depth == 0) {
+    pts[0] = v3(K.tx[0], K.tx[1], K.tx[2]);
+    pts[1] = C;
+  } else {
+    (void)corrected_points(K, tree, surf, chain, depth, C, pts);
+  }
+  const double lamda = SBR_C0 / K.freq;
+  const double kw = (2 * SBR_PI) / lamda;
+  const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  const double* next = K.txcs;
+  const double* cur = I3;
+  V3C tot = {mkc(0, 0), mkc(0, 0), mkc(0, 0)};
+  double L = 0.0;
+  for (int k = 0; k <= depth; k++) {
+    const TNode& sg = tree[chain[k]];
+    V3 vg = vsub(pts[k + 1], pts[k]);
+    V3 vl = mat_vec(next, vg);
+    double r, th, ph;
+    cart_to_sph(vl, &r, &th, &ph);
+    L += r;
+    cur = next;
+    int hitSurf = k == depth ? -1 : sg.tri;
+    next = hitSurf == -1 ? I3 : surf[hitSurf].cs;
+    V3C inc;
+    if (k == 0) {
+      inc.x = mkc(0, 0);
+      inc.y = mkc(0, 0);
+      inc.z = mkc(0, 0);
+      if (K.antennaType == 0) inc.x = mkc(0, sqrt(60 * K.pt));
+      else if (K.antennaType == 1) inc.y = mkc(0, sqrt(60 * K.pt * 1.5) * sin(th));
+      else if (K.antennaType == 2) inc.y = mkc(0, sqrt(60 * K.pt) * (cos(SBR_PI * cos(th) / 2.0) / sin(th)));
+    } else {
+      inc = matT_vec_c(cur, tot);
+      double st = sin(th), sp = sin(ph), ct = cos(th), cp = cos(ph);
+      const double Tm[9] = {st * cp, ct * cp, -1 * sp, st * sp, ct * sp, cp, ct, -1 * st, 0.0};
+      inc = matT_vec_c(Tm, inc);
+    }
+    if (sg.type == 1 || sg.type == 2) {
+      double ang = angle_between_rad(vg, v3(cur[6], cur[7], cur[8]));
+      int mat = tree[chain[k - 1]].mat;
+      cx fTM, fTE;
+      fresnel(sg.type == 1, ang, mats[6 * mat + 3], mats[6 * mat + 5], K.freq, K.useFresnel != 0, &fTM, &fTE);
+      tot.x = inc.x;
+      tot.y = cmul(inc.y, fTM);
+      tot.z = cmul(inc.z, fTE);
+    } else {
+      tot = inc;
+    }
+    double st = sin(th), sp = sin(ph), ct = cos(th), cp = cos(ph);
+    const double Tm[9] = {st * cp, ct * cp, -1 * sp, st * sp, ct * sp, cp, ct, -1 * st, 0.0};
+    tot = mat_vec_c(Tm, tot);
+    tot = matT_vec_c(cur, tot);
+  }
+  cx e = cexp_(mkc(-0.0 * kw * L, -1.0 * kw * L));
+  cx f[3] = {cdiv(cmul(tot.x, e), mkc(L, 0)), cdiv(cmul(tot.y, e), mkc(L, 0)), cdiv(cmul(tot.z, e), mkc(L, 0))};
+  double* o = winField + 6 * w;
+  for (int c = 0; c < 3; c++) { o[2 * c] = f[c].r; o[2 * c + 1] = f[c].i; }
+}
+
+// Per receiver: sum of its kept paths' fields in (launch, BFS) order.
+__global__ void k_rx_sum(const int32_t* __restrict__ winRx, int64_t W, const double* __restrict__ winField,
+                         int32_t m, double* __restrict__ field) {
+  int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  int64_t lo = 0, hi = W;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (winRx[mid] < r) lo = mid + 1;
+    else hi = mid;
+  }
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t w = lo; w < W && winRx[w] == r; w++)
+    for (int c = 0; c < 6; c++) acc[c] += winField[6 * w + c];
+  for (int c = 0; c < 6; c++) field[6 * (int64_t)r + c] = acc[c];
+}
+
+// Path export for the kept paths (one row per segment).
+__global__ void k_paths(FieldArgs F, TraceConsts K, const TNode* __restrict__ tree, const DSurf* __restrict__ surf,
+                        const double* __restrict__ rx, const unsigned long long* __restrict__ winKey,
+                        const int32_t* __restrict__ winRx, const int64_t* __restrict__ off, int32_t* __restrict__ stri,
+                        int32_t* __restrict__ scop, int32_t* __restrict__ stype, double* __restrict__ ssrc,
+                        double* __restrict__ stgt) {
+  int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= F.winners) return;
+  int64_t n = node_of(F.levelBase, winKey[w]);
+  int depth = tree[n].depth;
+  int32_t chain[kMaxDepth + 1];
+  int64_t c = n;
+  for (int k = depth; k >= 0; k--) { chain[k] = (int32_t)c; c = tree[c].parent; }
+  V3 C = ld3(rx + 3 * (int64_t)winRx[w]);
+  V3 pts[kMaxDepth + 2];
+  if (depth == 0) {
+    pts[0] = v3(K.tx[0], K.tx[1], K.tx[2]);
+    pts[1] = C;
+  } else {
+    (void)corrected_points(K, tree, surf, chain, depth, C, pts);
+  }
+  int64_t o = off[w];
+  for (int k = 0; k <= depth; k++) {
+    const TNode& sg = tree[chain[k]];
+    stri[o + k] = k == depth ? -1 : sg.tri;
+    scop[o + k] = k == depth ? -1 : sg.cop;
+    stype[o + k] = sg.type;
+    ssrc[3 * (o + k)] = pts[k].x; ssrc[3 * (o + k) + 1] = pts[k].y; ssrc[3 * (o + k) + 2] = pts[k].z;
+    stgt[3 * (o + k)] = pts[k + 1].x; stgt[3 * (o + k) + 1] = pts[k + 1].y; stgt[3 * (o + k) + 2] = pts[k + 1].z;
+  }
+}
+
+template <bool EXACT>
+__global__ void k_closest_batch(const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
+                                const double* __restrict__ org, const double* __restrict__ dirs, int32_t n,
+                                int32_t* __restrict__ triOut, double* __restrict__ ptOut, double* __restrict__ distOut) {
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Hit h;
+  bool hh = closest_hit<EXACT>(nodes, tris, ld3(org + 3 * (int64_t)i), ld3(dirs + 3 * (int64_t)i), h);
+  triOut[i] = hh ? tris[h.idx].id : -1;
+  ptOut[3 * i] = h.p.x; ptOut[3 * i + 1] = h.p.y; ptOut[3 * i + 2] = h.p.z;
+  distOut[i] = h.dist;
+}
+
+// =============================================================================================
+// Host orchestration
+// =============================================================================================
+Engine::~Engine() {
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Engine::init(int dev) {
+  if (dev == -1) {   // host-only context: scene preparation and queries, no device work
+    device = -1;
+    return SBR_OK;
+  }
+  int count = 0;
+  HIPCHK(hipGetDeviceCount(&count));
+  if (dev < 0 || dev >= count) return fail(SBR_E_ARG, "no such HIP device");
+  device = dev;
+  HIPCHK(hipSetDevice(dev));
+  HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  return SBR_OK;
+}
+
+int Engine::set_materials(const sbr_material* m, int n) {
+  if (!m || n <= 0) return fail(SBR_E_ARG, "materials: empty list");
+  mats.assign((const double*)m, (const double*)m + 6 * (size_t)n);
+  if (device < 0) return SBR_OK;
+  HIPCHK(hipSetDevice(device));
+  ENSURE(dMats, mats.size());
+  HIPCHK(hipMemcpyAsync(dMats.p, mats.data(), mats.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SBR_OK;
+}
+
+// Base icosahedron (Icosahedron.cpp:50-67) and its edge adjacency.
+static void ico_base(std::vector<double>& v, std::vector<int32_t>& f, std::vector<int32_t>& adj) {
+  const double X = .525731112119133606, Z = .850650808352039932, N = 0.0;
+  const double b[12][3] = {{-X, N, Z}, {X, N, Z}, {-X, N, -Z}, {X, N, -Z}, {N, Z, X}, {N, Z, -X},
+                           {N, -Z, X}, {N, -Z, -X}, {Z, X, N}, {-Z, X, N}, {Z, -X, N}, {-Z, -X, N}};
+  const int t[20][3] = {{0, 4, 1}, {0, 9, 4}, {9, 5, 4}, {4, 5, 8}, {4, 8, 1}, {8, 10, 1}, {8, 3, 10},
+                        {5, 3, 8}, {5, 2, 3}, {2, 7, 3}, {7, 10, 3}, {7, 6, 10}, {7, 11, 6}, {11, 0, 6},
+                        {0, 1, 6}, {6, 1, 10}, {9, 0, 11}, {9, 11, 2}, {9, 2, 5}, {7, 2, 11}};
+  v.assign(&b[0][0], &b[0][0] + 36);
+  f.assign(&t[0][0], &t[0][0] + 60);
+  adj.assign(120, -1);
+  for (int a = 0; a < 20; a++)
+    for (int e = 0; e < 3; e++) {
+      int p = t[a][e], q = t[a][(e + 1) % 3];
+      for (int g = 0; g < 20; g++) {
+        if (g == a) continue;
+        for (int e2 = 0; e2 < 3; e2++) {
+          int p2 = t[g][e2], q2 = t[g][(e2 + 1) % 3];
+          if ((p2 == p && q2 == q) || (p2 == q && q2 == p)) { adj[2 * (3 * a + e)] = g; adj[2 * (3 * a + e) + 1] = e2; }
+        }
+      }
+    }
+}
+
+int Engine::generate_rays(int tess, int64_t* nOut) {
+  if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
+  if (tess < 0 || tess > 12) return fail(SBR_E_ARG, "tessellation must be in [0, 12]");
+  const int64_t nvFinal = 10 * (int64_t(1) << (2 * tess)) + 2;
+  std::vector<double> v0;
+  std::vector<int32_t> f0, a0;
+  ico_base(v0, f0, a0);
+  DBuf<double> verts;
+  verts.ensure(3 * (size_t)nvFinal);
+  if (!verts.p) return fail(SBR_E_NOMEM, "icosahedron vertices");
+  const int64_t nfMax = 20 * (int64_t(1) << (2 * std::max(0, tess - 1)));
+  ENSURE(dFaces[0], 3 * nfMax);
+  ENSURE(dAdj[0], 6 * nfMax);
+  if (tess > 1) {
+    ENSURE(dFaces[1], 3 * nfMax);
+    ENSURE(dAdj[1], 6 * nfMax);
+  }
+  ENSURE(dFlag, 3 * nfMax);
+  ENSURE(dScan, 3 * nfMax);
+  HIPCHK(hipMemcpyAsync(verts.p, v0.data(), v0.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dFaces[0].p, f0.data(), f0.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dAdj[0].p, a0.data(), a0.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+  int cur = 0;
+  int64_t nv = 12, nf = 20;
+  for (int l = 0; l < tess; l++) {
+    const int64_t ns = 3 * nf;
+    k_ico_flag<<<blocks_for(ns, 256), 256, 0, stream>>>(dAdj[cur].p, (int32_t)nf, dFlag.p);
+    size_t tmpBytes = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, dFlag.p, dScan.p, (int)ns, stream));
+    ENSURE(dScratch, tmpBytes);
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(dScratch.p, tmpBytes, dFlag.p, dScan.p, (int)ns, stream));
+    const bool last = l == tess - 1;
+    k_ico_split<<<blocks_for(nf, 256), 256, 0, stream>>>(dFaces[cur].p, dAdj[cur].p, dFlag.p, dScan.p,
+                                                         (int32_t)nf, nv, verts.p, last ? nullptr : dFaces[1 - cur].p,
+                                                         last ? nullptr : dAdj[1 - cur].p, last ? 0 : 1);
+    HIPCHK(hipGetLastError());
+    nv = nv + 3 * nf / 2;   // each of the 3F/2 edges adds one vertex
+    nf *= 4;
+    cur = 1 - cur;
+  }
+  if (nv != nvFinal) return fail(SBR_E_INTERNAL, "icosahedron vertex count mismatch");
+  ENSURE(dRays, 3 * nv);
+  k_ico_dirs<<<blocks_for(nv, 256), 256, 0, stream>>>(verts.p, nv, dRays.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(stream));
+  *nOut = nv;
+  return SBR_OK;
+}
+
+int Engine::closest_hit(const double* o, const double* d, int n, int exact, int32_t* tri, double* pt, double* dist) {
+  if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
+  if (!haveScene) return fail(SBR_E_STATE, "no scene uploaded");
+  if (n <= 0) return SBR_OK;
+  HIPCHK(hipSetDevice(device));
+  DBuf<double> dO, dD, dP, dDist;
+  DBuf<int32_t> dT;
+  dO.ensure(3 * (size_t)n); dD.ensure(3 * (size_t)n); dP.ensure(3 * (size_t)n); dDist.ensure(n); dT.ensure(n);
+  if (!dO.p || !dD.p || !dP.p || !dDist.p || !dT.p) return fail(SBR_E_NOMEM, "closest_hit buffers");
+  HIPCHK(hipMemcpyAsync(dO.p, o, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dD.p, d, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
+  if (exact)
+    k_closest_batch<true><<<blocks_for(n, 128), 128, 0, stream>>>(dNodes.p, dTris.p, dO.p, dD.p, n, dT.p, dP.p, dDist.p);
+  else
+    k_closest_batch<false><<<blocks_for(n, 128), 128, 0, stream>>>(dNodes.p, dTris.p, dO.p, dD.p, n, dT.p, dP.p, dDist.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(tri, dT.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(pt, dP.p, 3 * sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(dist, dDist.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SBR_OK;
+}
+
+static double ms_since(std::chrono::steady_clock::time_point t) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+}
+
+int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
+  if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
+  if (!haveScene) return fail(SBR_E_STATE, "no scene uploaded");
+  if (mats.empty()) return fail(SBR_E_STATE, "no materials set");
+  if (!p || m < 0 || (m > 0 && !rx)) return fail(SBR_E_ARG, "trace: bad arguments");
+  if (p->maxR < 0 || p->maxT < 0) return fail(SBR_E_ARG, "maxR / maxT must be >= 0");
+  int maxDepth = p->maxR + p->maxT;
+  if (p->maxTotal > 0) maxDepth = std::min(maxDepth, p->maxTotal);
+  if (maxDepth > kMaxDepth) return fail(SBR_E_LIMIT, "tree depth above the device limit");
+  for (int32_t t = 0; t < ntri; t++)
+    if (hMat[t] < 0 || hMat[t] * 6 >= (int)mats.size()) return fail(SBR_E_ARG, "triangle material id without a material");
+  const int stride = p->launchStride > 0 ? p->launchStride : 1;
+  if (p->launchOffset < 0 || p->launchOffset >= stride) return fail(SBR_E_ARG, "launchOffset must be in [0, stride)");
+  if (p->antennaType < 0 || p->antennaType > 2) return fail(SBR_E_ARG, "antennaType must be 0, 1 or 2");
+  HIPCHK(hipSetDevice(device));
+  haveResult = false;
+  lastParams = *p;
+  nrx = m;
+  hRx.assign(rx, rx + 3 * (size_t)m);
+  stats = sbr_stats{};
+
+  TraceConsts K{};
+  memcpy(K.tx, p->tx, sizeof K.tx);
+  memcpy(K.txcs, p->txCoordSys, sizeof K.txcs);
+  K.freq = p->frequency;
+  K.pt = p->pt;
+  K.maxR = p->maxR;
+  K.maxT = p->maxT;
+  K.maxTotal = p->maxTotal;
+  K.useFresnel = p->useFresnel;
+  K.antennaType = p->antennaType;
+  K.receptionModel = p->receptionModel;
+  K.exact = p->exactTraversal;
+  if (p->receptionModel == 0) {
+    // RTSolver.cpp:839, 841 with txTesslation = tess + 1 (RTSolver.cpp:70)
+    double beta = 1.0 / (3.0 * (p->tess + 1)) * std::acos(-1.0 / std::sqrt(5.0));
+    K.rcoef = std::tan(beta);
+  } else if (p->receptionModel == 1) {
+    double alpha = (69.0 * 3.141592653589793 / 180.0) / std::ldexp(1.0, p->tess);
+    K.rcoef = alpha / std::sqrt(3.0);
+  } else {
+    return fail(SBR_E_ARG, "receptionModel must be 0 or 1");
+  }
+
+  hipEvent_t ev[6];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 6; i++) (void)hipEventDestroy(e[i]); } } guard{ev};
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(hipEventRecord(ev[0], stream));
+
+  // ---- 1. launch directions
+  int64_t nl = 0;
+  {
+    int rcode = generate_rays(p->tess, &nl);
+    if (rcode) return rcode;
+  }
+  HIPCHK(hipEventRecord(ev[1], stream));
+  if (nl >= (int64_t(1) << 26)) return fail(SBR_E_LIMIT, "too many launches for the order key");
+  int64_t nsel = nl > p->launchOffset ? (nl - p->launchOffset + stride - 1) / stride : 0;
+  stats.launches = nsel;
+
+  // ---- 2. interaction trees, depth by depth
+  levelBase.assign(kMaxDepth + 2, 0);
+  int64_t total = 0;
+  int64_t n = nsel;
+  if (grow_keep(dTree, std::max<int64_t>(n * 2, 1), 0, stream)) return fail(SBR_E_NOMEM, "tree buffer");
+  ENSURE(dQueue, std::max<int64_t>(n, 1));
+  int depth = 0;
+  int maxDepthSeen = 0;
+  while (n > 0) {
+    if (n >= (int64_t(1) << 31) / 2) return fail(SBR_E_LIMIT, "tree level above 2^30 nodes");
+    levelBase[depth] = total;
+    if (total + n >= (int64_t(1) << 30)) return fail(SBR_E_LIMIT, "more than 2^30 tree nodes");
+    if (grow_keep(dTree, (size_t)(total + n), (size_t)total, stream)) return fail(SBR_E_NOMEM, "tree buffer");
+    ENSURE(dChild, n);
+    ENSURE(dChildScan, n + 1);
+    LevelArgs LA{depth, (int32_t)n, total, stride, p->launchOffset};
+    if (p->exactTraversal)
+      k_trace_level<true><<<blocks_for(n, 256), 256, 0, stream>>>(LA, K, dNodes.p, dTris.p, dTriN.p, dRays.p, dQueue.p,
+                                                                   dTree.p, dChild.p);
+    else
+      k_trace_level<false><<<blocks_for(n, 256), 256, 0, stream>>>(LA, K, dNodes.p, dTris.p, dTriN.p, dRays.p,
+                                                                    dQueue.p, dTree.p, dChild.p);
+    HIPCHK(hipGetLastError());
+    total += n;
+    maxDepthSeen = depth;
+    if (depth >= maxDepth) break;
+    // child counts -> stable exclusive scan -> next queue
+    DBuf<int32_t>& cnt = dChildScan;
+    ENSURE(dFlag, n + 1);
+    k_child_count<<<blocks_for(n, 256), 256, 0, stream>>>(dChild.p, (int32_t)n, dFlag.p);
+    size_t tmpBytes = 0;
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, dFlag.p, cnt.p, (int)n, stream));
+    ENSURE(dScratch, tmpBytes);
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(dScratch.p, tmpBytes, dFlag.p, cnt.p, (int)n, stream));
+    int32_t lastOff = 0, lastCnt = 0;
+    HIPCHK(hipMemcpyAsync(&lastOff, cnt.p + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(&lastCnt, dFlag.p + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    int64_t nn = (int64_t)lastOff + lastCnt;
+    if (nn == 0) break;
+    DBuf<int32_t> nextQ;
+    nextQ.ensure(nn);
+    if (!nextQ.p) return fail(SBR_E_NOMEM, "queue");
+    k_emit_level<<<blocks_for(n, 256), 256, 0, stream>>>(dChild.p, cnt.p, (int32_t)n, levelBase[depth], nextQ.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+    std::swap(dQueue.p, nextQ.p);
+    std::swap(dQueue.n, nextQ.n);
+    n = nn;
+    depth++;
+  }
+  levelBase[maxDepthSeen + 1] = total;
+  nodes = total;
+  stats.bounces = total;
+  stats.max_depth = maxDepthSeen;
+  HIPCHK(hipEventRecord(ev[2], stream));
+
+  // ---- 3/4. receivers: grid, line of sight, capture + correction + dedup
+  winners = 0;
+  if (m > 0) {
+    // Uniform xy grid with about one receiver per cell.
+    RxGrid g{};
+    for (int a = 0; a < 3; a++) { g.bmin[a] = INFINITY; g.bmax[a] = -INFINITY; }
+    for (int r = 0; r < m; r++)
+      for (int a = 0; a < 3; a++) {
+        g.bmin[a] = std::min(g.bmin[a], rx[3 * r + a]);
+        g.bmax[a] = std::max(g.bmax[a], rx[3 * r + a]);
+      }
+    double ex = g.bmax[0] - g.bmin[0], ey = g.bmax[1] - g.bmin[1];
+    double cell = std::sqrt(std::max(ex * ey, 1e-12) / std::max(1, m));
+    cell = std::max(cell, std::max(ex, ey) / 4096.0);
+    if (!(cell > 0)) cell = 1.0;
+    g.cell = cell;
+    g.gx0 = g.bmin[0];
+    g.gy0 = g.bmin[1];
+    g.nx = std::max(1, (int)std::floor(ex / cell) + 1);
+    g.ny = std::max(1, (int)std::floor(ey / cell) + 1);
+    grid = g;
+    std::vector<int32_t> cellCount((size_t)g.nx * g.ny + 1, 0), cellOf(m), cellRx(m);
+    for (int r = 0; r < m; r++) {
+      int ix = std::min(g.nx - 1, std::max(0, (int)std::floor((rx[3 * r] - g.gx0) / cell)));
+      int iy = std::min(g.ny - 1, std::max(0, (int)std::floor((rx[3 * r + 1] - g.gy0) / cell)));
+      cellOf[r] = iy * g.nx + ix;
+      cellCount[cellOf[r] + 1]++;
+    }
+    for (size_t c = 1; c < cellCount.size(); c++) cellCount[c] += cellCount[c - 1];
+    std::vector<int32_t> fillPos(cellCount.begin(), cellCount.end() - 1);
+    for (int r = 0; r < m; r++) cellRx[fillPos[cellOf[r]]++] = r;
+    ENSURE(dRx, 3 * (size_t)m);
+    ENSURE(dCellStart, cellCount.size());
+    ENSURE(dCellRx, m);
+    ENSURE(dLos, m);
+    HIPCHK(hipMemcpyAsync(dRx.p, rx, 3 * sizeof(double) * m, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dCellStart.p, cellCount.data(), sizeof(int32_t) * cellCount.size(), hipMemcpyHostToDevice,
+                          stream));
+    HIPCHK(hipMemcpyAsync(dCellRx.p, cellRx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, stream));
+    if (p->exactTraversal)
+      k_los<true><<<blocks_for(m, 128), 128, 0, stream>>>(K, dNodes.p, dTris.p, dRx.p, m, dLos.p);
+    else
+      k_los<false><<<blocks_for(m, 128), 128, 0, stream>>>(K, dNodes.p, dTris.p, dRx.p, m, dLos.p);
+    HIPCHK(hipGetLastError());
+
+    const DSurf* surf = p->q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
+    size_t cap = 1;
+    while (cap < (size_t)std::max<int64_t>(1 << 16, 8 * (int64_t)m * 16)) cap <<= 1;
+    ENSURE(dCounters, 8);
+    for (int attempt = 0; attempt < 6; attempt++) {
+      ENSURE(dTabA, cap);
+      ENSURE(dTabB, cap);
+      ENSURE(dTabOrder, cap);
+      ENSURE(dTabRx, cap);
+      HIPCHK(hipMemsetAsync(dTabA.p, 0, cap * sizeof(unsigned long long), stream));
+      HIPCHK(hipMemsetAsync(dTabB.p, 0, cap * sizeof(unsigned long long), stream));
+      HIPCHK(hipMemsetAsync(dTabOrder.p, 0xFF, cap * sizeof(unsigned long long), stream));
+      HIPCHK(hipMemsetAsync(dCounters.p, 0, 8 * sizeof(unsigned long long), stream));
+      CaptureArgs CA{};
+      CA.nodes = total;
+      for (int k = 0; k < kMaxDepth + 2; k++) CA.levelBase[k] = levelBase[k];
+      CA.nrx = m;
+      CA.g = g;
+      CA.tabMask = cap - 1;
+      k_capture<<<blocks_for(total, 256), 256, 0, stream>>>(CA, K, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p,
+                                                             dLos.p, dTabA.p, dTabB.p, dTabOrder.p, dTabRx.p,
+                                                             dCounters.p);
+      HIPCHK(hipGetLastError());
+      unsigned long long ctr[4];
+      HIPCHK(hipMemcpyAsync(ctr, dCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      if (ctr[3] & 2ull) return fail(SBR_E_INTERNAL, "dedup hash collision detected");
+      if ((ctr[3] & 1ull) || ctr[2] * 10 > cap * 7) {
+        cap <<= 2;
+        continue;
+      }
+      stats.capture_records = (int64_t)ctr[0];
+      stats.corrected = (int64_t)ctr[1];
+      winners = (int64_t)ctr[2];
+      break;
+    }
+    if (winners == 0 && stats.corrected > 0) return fail(SBR_E_LIMIT, "dedup table could not be sized");
+    HIPCHK(hipEventRecord(ev[3], stream));
+
+    // ---- 5. kept paths: collect, sort by (receiver, order key), fields, per-receiver sums
+    ENSURE(dWinKey, std::max<int64_t>(winners, 1));
+    ENSURE(dWinKey2, std::max<int64_t>(winners, 1));
+    ENSURE(dWinRx, std::max<int64_t>(winners, 1));
+    ENSURE(dWinRx2, std::max<int64_t>(winners, 1));
+    ENSURE(dWinField, 6 * std::max<int64_t>(winners, 1));
+    ENSURE(dField, 6 * (size_t)m);
+    HIPCHK(hipMemsetAsync(dCounters.p + 4, 0, sizeof(unsigned long long), stream));
+    k_collect<<<blocks_for((int64_t)cap, 256), 256, 0, stream>>>(dTabA.p, dTabOrder.p, dTabRx.p, cap, dWinKey.p,
+                                                                  dWinRx.p, dCounters.p + 4);
+    HIPCHK(hipGetLastError());
+    if (winners > 0) {
+      size_t tb1 = 0, tb2 = 0;
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, dWinKey.p, dWinKey2.p, dWinRx.p, dWinRx2.p,
+                                                (int)winners, 0, 64, stream));
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dWinRx2.p, dWinRx.p, dWinKey2.p, dWinKey.p,
+                                                (int)winners, 0, 32, stream));
+      ENSURE(dScratch, std::max(tb1, tb2));
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb1, dWinKey.p, dWinKey2.p, dWinRx.p, dWinRx2.p,
+                                                (int)winners, 0, 64, stream));
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb2, dWinRx2.p, dWinRx.p, dWinKey2.p, dWinKey.p,
+                                                (int)winners, 0, 32, stream));
+      FieldArgs FA{};
+      FA.winners = winners;
+      for (int k = 0; k < kMaxDepth + 2; k++) FA.levelBase[k] = levelBase[k];
+      k_field<<<blocks_for(winners, 256), 256, 0, stream>>>(FA, K, dTree.p, surf, dMats.p, dRx.p, dWinKey.p,
+                                                             dWinRx.p, dWinField.p);
+      HIPCHK(hipGetLastError());
+    }
+    k_rx_sum<<<blocks_for(m, 256), 256, 0, stream>>>(dWinRx.p, winners, dWinField.p, m, dField.p);
+    HIPCHK(hipGetLastError());
+  } else {
+    HIPCHK(hipEventRecord(ev[3], stream));
+  }
+  HIPCHK(hipEventRecord(ev[4], stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  stats.winners = winners;
+  float a = 0;
+  HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
+  stats.ms_raygen = a;
+  HIPCHK(hipEventElapsedTime(&a, ev[1], ev[2]));
+  stats.ms_trace = a;
+  HIPCHK(hipEventElapsedTime(&a, ev[2], ev[3]));
+  stats.ms_capture = a;
+  HIPCHK(hipEventElapsedTime(&a, ev[3], ev[4]));
+  stats.ms_field = a;
+  stats.ms_total = ms_since(t0);
+  hField.assign(6 * (size_t)m, 0.0);
+  if (m > 0)
+    HIPCHK(hipMemcpyAsync(hField.data(), dField.p, 6 * sizeof(double) * m, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  haveResult = true;
+  return SBR_OK;
+}
+
+int Engine::paths(int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri, int32_t* scop,
+                  int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs, bool countOnly) {
+  if (!haveResult) return fail(SBR_E_STATE, "no trace result");
+  HIPCHK(hipSetDevice(device));
+  std::vector<unsigned long long> keys(winners);
+  std::vector<int32_t> rxs(winners);
+  if (winners > 0) {
+    HIPCHK(hipMemcpyAsync(keys.data(), dWinKey.p, sizeof(unsigned long long) * winners, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(rxs.data(), dWinRx.p, sizeof(int32_t) * winners, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  std::vector<int64_t> off(winners + 1, 0);
+  for (int64_t w = 0; w < winners; w++) off[w + 1] = off[w] + (int64_t)((keys[w] >> 33) & 31ull) + 1;
+  *nPaths = winners;
+  *nSegs = off[winners];
+  if (countOnly) return SBR_OK;
+  for (int64_t w = 0; w < winners; w++) {
+    prx[w] = rxs[w];
+    plaunch[w] = (int32_t)(keys[w] >> 38);
+    pnseg[w] = (int32_t)(off[w + 1] - off[w]);
+    poff[w] = off[w];
+  }
+  if (winners == 0) return SBR_OK;
+  const int64_t ns = off[winners];
+  DBuf<int64_t> dOff;
+  DBuf<int32_t> dI;
+  DBuf<double> dS;
+  dOff.ensure(winners);
+  dI.ensure(3 * ns);
+  dS.ensure(6 * ns);
+  if (!dOff.p || !dI.p || !dS.p) return fail(SBR_E_NOMEM, "path export buffers");
+  HIPCHK(hipMemcpyAsync(dOff.p, off.data(), sizeof(int64_t) * winners, hipMemcpyHostToDevice, stream));
+  FieldArgs FA{};
+  FA.winners = winners;
+  for (int k = 0; k < kMaxDepth + 2; k++) FA.levelBase[k] = levelBase[k];
+  TraceConsts K{};
+  memcpy(K.tx, lastParams.tx, sizeof K.tx);
+  const DSurf* surf = lastParams.q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
+  k_paths<<<blocks_for(winners, 128), 128, 0, stream>>>(FA, K, dTree.p, surf, dRx.p, dWinKey.p, dWinRx.p, dOff.p, dI.p,
+                                                         dI.p + ns, dI.p + 2 * ns, dS.p, dS.p + 3 * ns);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(stri, dI.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(scop, dI.p + ns, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(stype, dI.p + 2 * ns, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(ssrc, dS.p, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(stgt, dS.p + 3 * ns, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SBR_OK;
+}
+
+int Engine::trees(int32_t* launch, int32_t* depth, int64_t* code, int32_t* tri, int32_t* cop, int32_t* type,
+                  double* src, double* tgt, double* pl) {
+  if (!haveResult) return fail(SBR_E_STATE, "no trace result");
+  HIPCHK(hipSetDevice(device));
+  std::vector<TNode> h(nodes);
+  if (nodes > 0) {
+    HIPCHK(hipMemcpyAsync(h.data(), dTree.p, sizeof(TNode) * nodes, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  std::vector<int64_t> order(nodes);
+  for (int64_t i = 0; i < nodes; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    if (h[a].launch != h[b].launch) return h[a].launch < h[b].launch;
+    if (h[a].depth != h[b].depth) return h[a].depth < h[b].depth;
+    return h[a].code < h[b].code;
+  });
+  for (int64_t k = 0; k < nodes; k++) {
+    const TNode& t = h[order[k]];
+    launch[k] = t.launch;
+    depth[k] = t.depth;
+    code[k] = t.code;
+    tri[k] = t.tri;
+    cop[k] = t.cop;
+    type[k] = t.type;
+    for (int a = 0; a < 3; a++) { src[3 * k + a] = t.src[a]; tgt[3 * k + a] = t.tgt[a]; }
+    pl[k] = t.pl;
+  }
+  return SBR_OK;
+}
+
+}  // namespace sbr

@@ -1,0 +1,122 @@
+"""Seeded synthetic scenes, receiver sets and trace settings for the benchmark configurations.
+
+Scenes are triangle soups in the layout a binary STL holds (9 vertex floats and 3 normal floats per
+facet, plus a material id), so they enter the library exactly as a file read by
+Preprocessor::StlToGeometry would.  Randomness comes from SplitMix64 with an explicit uniform
+mapping, so every platform generates the same scene for a seed.
+"""
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+SEED_BASE = 20261203
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.s = seed & MASK64
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & MASK64
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, a=0.0, b=1.0):
+        return a + (b - a) * ((self.next() >> 11) * (1.0 / (1 << 53)))
+
+    def randint(self, lo, hi):
+        """Integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1)) if hi > lo else lo
+
+
+CONCRETE = [0.0, -1.0, -1.0, 5.31, 0.0, 0.0326]   # [frequency, transLoss, reflLoss, er, er_im, sigma]
+
+
+class Soup:
+    def __init__(self):
+        self.v, self.n, self.m = [], [], []
+
+    def tri(self, a, b, c, n, mat=0):
+        self.v.append(list(a) + list(b) + list(c))
+        self.n.append(list(n))
+        self.m.append(mat)
+
+    def quad(self, a, b, c, d, n, mat=0):
+        """Quad a-b-c-d as triangles (a, b, c) and (a, c, d)."""
+        self.tri(a, b, c, n, mat)
+        self.tri(a, c, d, n, mat)
+
+    def box(self, x0, y0, z0, x1, y1, z1, inward=False, mat=0):
+        s = -1.0 if inward else 1.0
+        self.quad((x0, y0, z0), (x0, y1, z0), (x1, y1, z0), (x1, y0, z0), (0, 0, -s), mat)   # bottom
+        self.quad((x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1), (0, 0, s), mat)    # top
+        self.quad((x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1), (0, -s, 0), mat)   # y = y0
+        self.quad((x0, y1, z0), (x0, y1, z1), (x1, y1, z1), (x1, y1, z0), (0, s, 0), mat)    # y = y1
+        self.quad((x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0), (-s, 0, 0), mat)   # x = x0
+        self.quad((x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1), (s, 0, 0), mat)    # x = x1
+
+    def arrays(self):
+        return (np.asarray(self.v, np.float32).reshape(-1, 9), np.asarray(self.n, np.float32).reshape(-1, 3),
+                np.asarray(self.m, np.int32))
+
+
+def config1():
+    """Closed box room 10x8x3 m (12 triangles, inward normals), concrete, 2.4 GHz, half-wave
+    dipole Tx at (2, 2, 1.5), icosahedron level 5, 4 reflections, 10x10 receivers at z = 1.2."""
+    s = Soup()
+    s.box(0.0, 0.0, 0.0, 10.0, 8.0, 3.0, inward=True)
+    rx = np.array([[0.5 + i, 0.4 + 0.8 * j, 1.2] for i in range(10) for j in range(10)])
+    mats = np.array([[2.4e9] + CONCRETE[1:]])
+    trace = dict(tx=(2.0, 2.0, 1.5), tess=5, maxR=4, maxT=0, frequency=2.4e9, pt=1.0, antennaType=2)
+    return dict(name="box_room_12tri_L5_R4", soup=s.arrays(), rx=rx, materials=mats, trace=trace)
+
+
+def urban_buildings(seed, count=40, half=250.0, fmin=10.0, fmax=40.0, hmin=10.0, hmax=60.0, gap=2.0):
+    rng = SplitMix64(seed)
+    boxes = []
+    tries = 0
+    while len(boxes) < count and tries < 100000:
+        tries += 1
+        w, d = rng.uniform(fmin, fmax), rng.uniform(fmin, fmax)
+        h = rng.uniform(hmin, hmax)
+        cx, cy = rng.uniform(-half + w / 2 + 5, half - w / 2 - 5), rng.uniform(-half + d / 2 + 5, half - d / 2 - 5)
+        b = (cx - w / 2, cy - d / 2, cx + w / 2, cy + d / 2, h)
+        if any(b[0] < o[2] + gap and o[0] < b[2] + gap and b[1] < o[3] + gap and o[1] < b[3] + gap for o in boxes):
+            continue
+        boxes.append(b)
+    return boxes, rng
+
+
+def config2(tess=9, rx_n=100):
+    """Synthetic urban canyon at 28 GHz: 500x500 m ground (2 triangles) plus 40 seeded box buildings
+    (footprints U[10,40] m, heights U[10,60] m; 482 triangles), concrete; 1 Tx at 25 m in a street
+    cell; icosahedron level 9; 6 reflections plus one transmission; 100x100 receivers at z = 1.5."""
+    boxes, rng = urban_buildings(SEED_BASE + 2)
+    s = Soup()
+    h = 250.0
+    s.quad((-h, -h, 0.0), (h, -h, 0.0), (h, h, 0.0), (-h, h, 0.0), (0, 0, 1))
+    for (x0, y0, x1, y1, bh) in boxes:
+        s.box(x0, y0, 0.0, x1, y1, bh)
+    # Tx in a street cell: the first seeded point at least 3 m from every footprint.
+    while True:
+        tx = (rng.uniform(-200, 200), rng.uniform(-200, 200), 25.0)
+        if all(not (x0 - 3 < tx[0] < x1 + 3 and y0 - 3 < tx[1] < y1 + 3) for (x0, y0, x1, y1, _) in boxes):
+            break
+    pitch = 500.0 / rx_n
+    g = -250.0 + pitch / 2 + pitch * np.arange(rx_n)
+    rx = np.array([[x, y, 1.5] for x in g for y in g])
+    mats = np.array([[28e9] + CONCRETE[1:]])
+    trace = dict(tx=tx, tess=tess, maxR=6, maxT=1, frequency=28e9, pt=1.0, antennaType=2)
+    return dict(name=f"urban_canyon_482tri_L{tess}_R6T1", soup=s.arrays(), rx=rx, materials=mats, trace=trace)
+
+
+def ground_fixture():
+    """The reference's committed main() scenario (SBRSolver_rev2.cpp:18-144): ground.stl, 3000
+    receivers on x = 0, z = 3, y = 0..2999, Tx (0, 0, 5), txCoordSys rows (0,1,0), (0,0,1), (1,0,0),
+    icosahedron level 2, 1 reflection, 1.8 GHz, metal (er = 1, sigma = 1e6)."""
+    rx = np.array([[0.0, float(i), 3.0] for i in range(3000)])
+    mats = np.array([[1.8e9, 221.13, 0.05, 1.0, 0.0, 1e6]])
+    trace = dict(tx=(0.0, 0.0, 5.0), tess=2, maxR=1, maxT=0, frequency=1.8e9, pt=1.0,
+                 txCoordSys=[[0, 1, 0], [0, 0, 1], [1, 0, 0]], antennaType=1)
+    return dict(name="ground_fixture", rx=rx, materials=mats, trace=trace)

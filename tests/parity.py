@@ -1,0 +1,64 @@
+"""Comparison helpers shared by the parity tests (HIP library against the FP64 oracle)."""
+import numpy as np
+
+FIELD_RTOL = 1e-4   # per-receiver complex field, relative to |E| of the oracle (0.01 dB class)
+
+
+def path_table(p):
+    """List of (rx, launch, tri sequence, coplanar sequence, type sequence) per kept path."""
+    out = []
+    for k in range(len(p["prx"])):
+        o, n = int(p["poff"][k]), int(p["pnseg"][k])
+        out.append((int(p["prx"][k]), int(p["plaunch"][k]), tuple(p["stri"][o:o + n].tolist()),
+                    tuple(p["scop"][o:o + n].tolist()), tuple(p["stype"][o:o + n].tolist())))
+    return out
+
+
+def oracle_paths(r):
+    return dict(prx=r.prx, plaunch=r.plaunch, pnseg=r.pnseg, poff=r.poff, stri=r.stri, scop=r.scop, stype=r.stype,
+                ssrc=r.ssrc, stgt=r.stgt)
+
+
+def compare(orc, gpu_res, label=""):
+    """Asserts identical kept-path sets (as ordered triangle-id sequences, per receiver, in the
+    reference's order), bit-identical corrected vertices, equal bounce counts and fields within
+    FIELD_RTOL.  Returns the worst relative field error."""
+    gp = gpu_res.paths()
+    op = oracle_paths(orc)
+    assert gpu_res.bounces == orc.bounces, f"{label}: bounces {gpu_res.bounces} != {orc.bounces}"
+    tg, to = path_table(gp), path_table(op)
+    if tg != to:
+        sg, so = set(tg), set(to)
+        missing = sorted(so - sg)[:5]
+        extra = sorted(sg - so)[:5]
+        raise AssertionError(f"{label}: path sets differ: {len(tg)} gpu vs {len(to)} oracle; "
+                             f"missing {missing} extra {extra}")
+    assert np.array_equal(gp["ssrc"], op["ssrc"]) and np.array_equal(gp["stgt"], op["stgt"]), \
+        f"{label}: corrected vertices differ (max {np.abs(gp['ssrc'] - op['ssrc']).max()})"
+    pf_g, pf_o = gpu_res.path_fields(), orc.path_fields
+    worst_path = _worst(pf_g, pf_o, f"{label} (per path)", to)
+    worst = _worst(gpu_res.fields, orc.fields, f"{label} (per receiver)")
+    return max(worst, worst_path)
+
+
+def _worst(eg, eo, label, table=None):
+    # NaN where the reference produces NaN (the half-wave pattern on its axis is 0/0): the HIP
+    # path must produce NaN on exactly the same entries.
+    ng, no = ~np.isfinite(eg).all(axis=1), ~np.isfinite(eo).all(axis=1)
+    assert np.array_equal(ng, no), f"{label}: non-finite entries differ: gpu {np.flatnonzero(ng)[:5]} " \
+                                    f"oracle {np.flatnonzero(no)[:5]}"
+    eg, eo = eg[~no], eo[~no]
+    mag = np.linalg.norm(eo, axis=1)
+    err = np.linalg.norm(eg - eo, axis=1)
+    zero = mag == 0
+    assert np.all(err[zero] == 0), f"{label}: nonzero field where the oracle has none"
+    if not np.any(~zero):
+        return 0.0
+    rel = err[~zero] / mag[~zero]
+    worst = float(rel.max())
+    if worst > FIELD_RTOL:
+        i = int(np.flatnonzero(~zero)[np.argmax(rel)])
+        extra = f" path {table[np.flatnonzero(~no)[i]]}" if table is not None else ""
+        raise AssertionError(f"{label}: field rel err {worst} > {FIELD_RTOL} at {i}: gpu {eg[i]} oracle {eo[i]}"
+                             + extra)
+    return worst
