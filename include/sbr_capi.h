@@ -105,6 +105,7 @@ typedef struct {
   int32_t launchOffset;
   int32_t q1MeshIndexing;/* 1 (reference): surface data of mesh[hitSurfaceID]; 0: by id */
   int32_t exactTraversal;/* 1: visit every box the line touches (reference order); 0: clipped */
+  int32_t countTraversal;/* 1: count BVH node visits and triangle tests (sbr_stats) */
 } sbr_trace_params;
 
 /* Counters and stage times of the last trace. */
@@ -115,7 +116,12 @@ typedef struct {
   int64_t corrected;        /* records that survived image correction */
   int64_t winners;          /* paths kept after duplicate removal */
   int64_t max_depth;
-  double ms_raygen, ms_trace, ms_capture, ms_field, ms_total;
+  double ms_raygen, ms_trace, ms_capture, ms_field, ms_total;  /* stage times (HIP events) */
+  double ms_k_trace;        /* summed duration of the closest-hit kernel launches (one per depth) */
+  int64_t n_k_trace;        /* number of those launches */
+  double ms_k_capture;      /* duration of the capture kernel */
+  int64_t node_visits;      /* BVH nodes entered (countTraversal = 1) */
+  int64_t tri_tests;        /* ray-triangle tests (countTraversal = 1) */
 } sbr_stats;
 
 /* device >= 0: HIP device ordinal.  device == -1: host-only context (scene preparation and

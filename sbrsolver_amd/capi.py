@@ -25,7 +25,7 @@ class TraceParams(C.Structure):
         ("frequency", C.c_double), ("pt", C.c_double),
         ("useFresnel", C.c_int32), ("tess", C.c_int32), ("antennaType", C.c_int32),
         ("receptionModel", C.c_int32), ("launchStride", C.c_int32), ("launchOffset", C.c_int32),
-        ("q1MeshIndexing", C.c_int32), ("exactTraversal", C.c_int32),
+        ("q1MeshIndexing", C.c_int32), ("exactTraversal", C.c_int32), ("countTraversal", C.c_int32),
     ]
 
 
@@ -35,6 +35,8 @@ class Stats(C.Structure):
         ("corrected", C.c_int64), ("winners", C.c_int64), ("max_depth", C.c_int64),
         ("ms_raygen", C.c_double), ("ms_trace", C.c_double), ("ms_capture", C.c_double),
         ("ms_field", C.c_double), ("ms_total", C.c_double),
+        ("ms_k_trace", C.c_double), ("n_k_trace", C.c_int64), ("ms_k_capture", C.c_double),
+        ("node_visits", C.c_int64), ("tri_tests", C.c_int64),
     ]
 
     def as_dict(self):
@@ -95,7 +97,7 @@ def _a(x):
 
 def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None, maxTotal=0, useFresnel=True,
                 antennaType=1, receptionModel=0, launchStride=1, launchOffset=0, q1MeshIndexing=1,
-                exactTraversal=0):
+                exactTraversal=0, countTraversal=0):
     p = TraceParams()
     for i in range(3):
         p.tx[i] = float(tx[i])
@@ -107,6 +109,7 @@ def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None
     p.useFresnel, p.tess, p.antennaType = int(bool(useFresnel)), int(tess), int(antennaType)
     p.receptionModel, p.launchStride, p.launchOffset = int(receptionModel), int(launchStride), int(launchOffset)
     p.q1MeshIndexing, p.exactTraversal = int(q1MeshIndexing), int(exactTraversal)
+    p.countTraversal = int(countTraversal)
     return p
 
 

@@ -31,6 +31,8 @@ struct Hit {
   double dist;
   double t;
   V3 p;
+  int nodes;     // BVH nodes entered
+  int tests;     // ray-triangle tests
 };
 
 __device__ __forceinline__ bool box_of(const DNode& n, V3 o, V3 d, double* lo, double* hi) {
@@ -51,6 +53,8 @@ __device__ bool closest_hit(const DNode* __restrict__ nodes, const DTri* __restr
   h.dist = INFINITY;
   h.t = INFINITY;
   h.p = v3(0, 0, 0);
+  h.nodes = 0;
+  h.tests = 0;
   double lo, hi;
   if (!box_of(nodes[0], o, d, &lo, &hi) || !keep_box<EXACT>(lo, hi, h)) return false;
   int stackN[kMaxStack];
@@ -60,7 +64,9 @@ __device__ bool closest_hit(const DNode* __restrict__ nodes, const DTri* __restr
   while (true) {
     const DNode nd = nodes[cur];
     bool descended = false;
+    h.nodes++;
     if (nd.count > 0) {
+      h.tests += nd.count;
       for (int k = nd.a; k < nd.a + nd.count; k++) {
         const DTri& tr = tris[k];
         V3 p;

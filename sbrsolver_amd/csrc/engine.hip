@@ -176,9 +176,15 @@ template <bool EXACT>
 __global__ void __launch_bounds__(256) k_trace_level(LevelArgs L, TraceConsts K, const DNode* __restrict__ nodes,
                                                      const DTri* __restrict__ tris, const double* __restrict__ triN,
                                                      const double* __restrict__ rays, const int32_t* __restrict__ queue,
-                                                     TNode* __restrict__ tree, int32_t* __restrict__ childMask) {
+                                                     TNode* __restrict__ tree, int32_t* __restrict__ childMask,
+                                                     unsigned long long* __restrict__ travCount) {
+  __shared__ unsigned long long sNodes, sTests;
+  if (travCount) {
+    if (threadIdx.x == 0) { sNodes = 0; sTests = 0; }
+    __syncthreads();
+  }
   int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= L.n) return;
+  if (i < L.n) {
   TNode nd;
   V3 src, dir;
   double totalPL;
@@ -247,6 +253,18 @@ __global__ void __launch_bounds__(256) k_trace_level(LevelArgs L, TraceConsts K,
   }
   tree[L.base + i] = nd;
   childMask[i] = mask;
+  if (travCount) {
+    atomicAdd(&sNodes, (unsigned long long)h.nodes);
+    atomicAdd(&sTests, (unsigned long long)h.tests);
+  }
+  }
+  if (travCount) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      atomicAdd(&travCount[0], sNodes);
+      atomicAdd(&travCount[1], sTests);
+    }
+  }
 }
 
 __global__ void k_child_count(const int32_t* __restrict__ mask, int32_t n, int32_t* __restrict__ cnt) {
@@ -876,6 +894,11 @@ int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
 
   // ---- 2. interaction trees, depth by depth
   levelBase.assign(kMaxDepth + 2, 0);
+  ENSURE(dCounters, 8);
+  HIPCHK(hipMemsetAsync(dCounters.p + 6, 0, 2 * sizeof(unsigned long long), stream));
+  hipEvent_t kev[2];
+  for (auto& e : kev) HIPCHK(hipEventCreate(&e));
+  struct KevGuard { hipEvent_t* e; ~KevGuard() { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); } } kguard{kev};
   int64_t total = 0;
   int64_t n = nsel;
   if (grow_keep(dTree, std::max<int64_t>(n * 2, 1), 0, stream)) return fail(SBR_E_NOMEM, "tree buffer");
@@ -890,13 +913,23 @@ int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
     ENSURE(dChild, n);
     ENSURE(dChildScan, n + 1);
     LevelArgs LA{depth, (int32_t)n, total, stride, p->launchOffset};
+    unsigned long long* tc = p->countTraversal ? dCounters.p + 6 : nullptr;
+    HIPCHK(hipEventRecord(kev[0], stream));
     if (p->exactTraversal)
       k_trace_level<true><<<blocks_for(n, 256), 256, 0, stream>>>(LA, K, dNodes.p, dTris.p, dTriN.p, dRays.p, dQueue.p,
-                                                                   dTree.p, dChild.p);
+                                                                   dTree.p, dChild.p, tc);
     else
       k_trace_level<false><<<blocks_for(n, 256), 256, 0, stream>>>(LA, K, dNodes.p, dTris.p, dTriN.p, dRays.p,
-                                                                    dQueue.p, dTree.p, dChild.p);
+                                                                    dQueue.p, dTree.p, dChild.p, tc);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(kev[1], stream));
+    HIPCHK(hipEventSynchronize(kev[1]));
+    {
+      float kms = 0;
+      HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
+      stats.ms_k_trace += kms;
+      stats.n_k_trace++;
+    }
     total += n;
     maxDepthSeen = depth;
     if (depth >= maxDepth) break;
@@ -927,6 +960,13 @@ int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
   }
   levelBase[maxDepthSeen + 1] = total;
   nodes = total;
+  if (p->countTraversal) {
+    unsigned long long tcnt[2];
+    HIPCHK(hipMemcpyAsync(tcnt, dCounters.p + 6, sizeof tcnt, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    stats.node_visits = (int64_t)tcnt[0];
+    stats.tri_tests = (int64_t)tcnt[1];
+  }
   stats.bounces = total;
   stats.max_depth = maxDepthSeen;
   HIPCHK(hipEventRecord(ev[2], stream));
@@ -979,7 +1019,6 @@ int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
     const DSurf* surf = p->q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
     size_t cap = 1;
     while (cap < (size_t)std::max<int64_t>(1 << 16, 8 * (int64_t)m * 16)) cap <<= 1;
-    ENSURE(dCounters, 8);
     for (int attempt = 0; attempt < 6; attempt++) {
       ENSURE(dTabA, cap);
       ENSURE(dTabB, cap);
@@ -988,20 +1027,27 @@ int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
       HIPCHK(hipMemsetAsync(dTabA.p, 0, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dTabB.p, 0, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dTabOrder.p, 0xFF, cap * sizeof(unsigned long long), stream));
-      HIPCHK(hipMemsetAsync(dCounters.p, 0, 8 * sizeof(unsigned long long), stream));
+      HIPCHK(hipMemsetAsync(dCounters.p, 0, 6 * sizeof(unsigned long long), stream));
       CaptureArgs CA{};
       CA.nodes = total;
       for (int k = 0; k < kMaxDepth + 2; k++) CA.levelBase[k] = levelBase[k];
       CA.nrx = m;
       CA.g = g;
       CA.tabMask = cap - 1;
+      HIPCHK(hipEventRecord(kev[0], stream));
       k_capture<<<blocks_for(total, 256), 256, 0, stream>>>(CA, K, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p,
                                                              dLos.p, dTabA.p, dTabB.p, dTabOrder.p, dTabRx.p,
                                                              dCounters.p);
       HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(kev[1], stream));
       unsigned long long ctr[4];
       HIPCHK(hipMemcpyAsync(ctr, dCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
+      {
+        float kms = 0;
+        HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
+        stats.ms_k_capture = kms;
+      }
       if (ctr[3] & 2ull) return fail(SBR_E_INTERNAL, "dedup hash collision detected");
       if ((ctr[3] & 1ull) || ctr[2] * 10 > cap * 7) {
         cap <<= 2;

@@ -2,6 +2,10 @@
 import numpy as np
 
 FIELD_RTOL = 1e-4   # per-receiver complex field, relative to |E| of the oracle (0.01 dB class)
+# Absolute floor, relative to the strongest entry: a field that is zero in exact arithmetic (a
+# transmission through the reference's infinite wall keeps only the radial component, about
+# 1e-17 of the incident field) is rounding noise on both sides and has no meaningful relative error.
+FIELD_ATOL_REL = 1e-12
 
 
 def path_table(p):
@@ -50,15 +54,14 @@ def _worst(eg, eo, label, table=None):
     eg, eo = eg[~no], eo[~no]
     mag = np.linalg.norm(eo, axis=1)
     err = np.linalg.norm(eg - eo, axis=1)
-    zero = mag == 0
-    assert np.all(err[zero] == 0), f"{label}: nonzero field where the oracle has none"
-    if not np.any(~zero):
+    if mag.size == 0:
         return 0.0
-    rel = err[~zero] / mag[~zero]
-    worst = float(rel.max())
-    if worst > FIELD_RTOL:
-        i = int(np.flatnonzero(~zero)[np.argmax(rel)])
+    atol = FIELD_ATOL_REL * mag.max()
+    bad = err > FIELD_RTOL * mag + atol
+    if np.any(bad):
+        i = int(np.flatnonzero(bad)[np.argmax((err / np.maximum(mag, 1e-300))[bad])])
         extra = f" path {table[np.flatnonzero(~no)[i]]}" if table is not None else ""
-        raise AssertionError(f"{label}: field rel err {worst} > {FIELD_RTOL} at {i}: gpu {eg[i]} oracle {eo[i]}"
-                             + extra)
-    return worst
+        raise AssertionError(f"{label}: field err {err[i]} > {FIELD_RTOL} * {mag[i]} + {atol} at {i}: "
+                             f"gpu {eg[i]} oracle {eo[i]}" + extra)
+    sig = mag > atol / FIELD_ATOL_REL * 1e-9     # entries above -180 dB of the strongest
+    return float((err[sig] / mag[sig]).max()) if np.any(sig) else 0.0

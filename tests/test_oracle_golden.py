@@ -1,0 +1,224 @@
+"""Pins the FP64 oracle: reference golden fixtures, closed forms, brute force and invariants.
+
+Fixtures are the outputs of the reference's committed main() run (data/output/*, copied under
+tests/golden/output/): ground.stl, 3000 receivers, Tx (0,0,5), tess 2, R = 1, 1.8 GHz, metal.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sbrsolver_amd import scenes
+from tests import fixtures_io
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+OUT = os.path.join(GOLDEN, "output")
+
+
+def stl(name):
+    return os.path.join(GOLDEN, "stl", name)
+
+
+@pytest.fixture(scope="module")
+def ground_run():
+    cfg = scenes.ground_fixture()
+    sc = O.Scene.from_stl(stl("ground.stl"))
+    r = O.trace(sc, O.make_params(**cfg["trace"]), cfg["rx"], cfg["materials"])
+    return cfg, sc, r
+
+
+def test_faces_fixture():
+    """Faces.vtk: RemoveDoubles vertex order, triangle indices, coplanar and material ids."""
+    f = fixtures_io.read_faces_vtk(os.path.join(OUT, "Faces.vtk"))
+    sc = O.Scene.from_stl(stl("ground.stl"))
+    coords, tris = sc.geometry()
+    assert np.array_equal(coords, f["points"].astype(np.float32))
+    assert np.array_equal(tris, f["tris"])
+    t = sc.triangles()
+    assert np.array_equal(t["coplanar"], f["cop"]) and np.array_equal(t["material"], f["mat"])
+
+
+def test_edges_fixture():
+    edges, conn = fixtures_io.read_edges_vtk(os.path.join(OUT, "Edges.vtk"))
+    sc = O.Scene.from_stl(stl("ground.stl"))
+    pairs, counts = sc.edges()
+    assert np.array_equal(pairs, edges) and np.array_equal(counts, conn)
+
+
+def test_icosahedron_fixture():
+    """Icosahedron.vtk: level-3 launch directions in the reference's vertex order (6 digits)."""
+    org, dirs = fixtures_io.read_icosahedron_vtk(os.path.join(OUT, "Icosahedron.vtk"))
+    mine = O.icosahedron(3)
+    assert mine.shape == dirs.shape == (642, 3)
+    assert np.abs(mine - dirs).max() < 1e-5
+
+
+@pytest.mark.parametrize("tess", range(0, 7))
+def test_icosahedron_counts_and_unit_norm(tess):
+    d = O.icosahedron(tess)
+    assert d.shape[0] == 10 * 4 ** tess + 2
+    assert np.abs(np.linalg.norm(d, axis=1) - 1).max() < 4e-16
+    assert len(np.unique(np.round(d, 12), axis=0)) == d.shape[0]
+
+
+def test_electric_field_fixture(ground_run):
+    cfg, _, r = ground_run
+    xyz, E = fixtures_io.read_field_csv(os.path.join(OUT, "ElectricField.csv"))
+    assert np.array_equal(xyz, cfg["rx"])
+    rel = np.linalg.norm(r.fields - E, axis=1) / np.linalg.norm(E, axis=1)
+    assert rel.max() < 1e-5
+
+
+def test_raypath_fixture(ground_run):
+    """RayPath.vtk: kept paths in (launch, receiver) order, corrected vertices to 6 digits."""
+    _, _, r = ground_run
+    vtk = fixtures_io.read_raypath_vtk(os.path.join(OUT, "RayPath.vtk"))
+    order = np.lexsort((r.prx, r.plaunch))
+    assert len(order) == len(vtk["launch"]) == 3133
+    assert np.array_equal(r.plaunch[order], vtk["launch"]) and np.array_equal(r.prx[order], vtk["rx"])
+    for k, p in enumerate(order[:400]):
+        o, n = r.poff[p], r.pnseg[p]
+        verts = np.vstack([r.ssrc[o:o + 1], r.stgt[o:o + n]])
+        ref = vtk["points"][vtk["lines"][k]]
+        assert np.allclose(verts, ref, rtol=1e-5, atol=1e-4), (k, verts, ref)
+    assert sorted(set(np.unique(r.pnseg))) == [1, 2]
+    assert int(np.sum(r.pnseg == 2)) == 133
+
+
+def test_rxlocations_fixture(ground_run):
+    """RxLocations.vtk: |E|, 20 log10(|E| 1e6) and the radian phase sum (degree conversion discarded)."""
+    _, _, r = ground_run
+    v = fixtures_io.read_rxloc_vtk(os.path.join(OUT, "RxLocations.vtk"))
+    eabs = np.sqrt((np.abs(r.fields) ** 2).sum(axis=1))
+    assert np.allclose(eabs, v["eabs"], rtol=2e-5)
+    assert np.allclose(20 * np.log10(eabs * 1e6), v["edb"], atol=2e-4)
+    phase = np.angle(r.fields).sum(axis=1)
+    assert np.allclose(phase, v["phase"], atol=5e-5)
+
+
+def test_direct_field_closed_form(ground_run):
+    """Receiver 0 sees the direct path only: E = j sqrt(90 Pt) sin(theta) e^{-jkr}/r, rotated by
+    txCoordSys (theta = 90 deg in the dipole frame, r = 2 m)."""
+    _, _, r = ground_run
+    k = 2 * np.pi * 1.8e9 / 2.99792458e8
+    ex = -1j * np.sqrt(90.0) * np.exp(-1j * k * 2.0) / 2.0
+    assert abs(r.fields[0, 0] - ex) < 1e-12 * abs(ex)
+    assert abs(r.fields[0, 1]) < 1e-12 and abs(r.fields[0, 2]) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["triangle.stl", "two_planes.stl", "corner.stl", "cube.stl", "tetrahedron.stl",
+                                  "geometry_test.stl", "geometry_test2.stl", "office.stl", "two_tetrahedron.stl"])
+def test_bvh_closest_hit_equals_brute_force(name):
+    sc = O.Scene.from_stl(stl(name))
+    v = sc.triangles()["verts"].reshape(-1, 3)
+    lo, hi = v.min(0), v.max(0)
+    rng = np.random.default_rng(11)
+    for _ in range(1500):
+        o = lo + (hi - lo + 1e-3) * (rng.random(3) * 1.4 - 0.2)
+        d = rng.normal(size=3)
+        t1, p1, d1 = sc.closest_hit(o, d)
+        t2, p2, d2 = sc.closest_hit(o, d, brute=True)
+        assert (t1 < 0) == (t2 < 0)
+        if t1 >= 0:
+            assert d1 == d2          # same closest distance (ties may pick a different triangle)
+            if t1 != t2:
+                assert np.array_equal(p1, p2) or abs(d1 - d2) == 0
+
+
+def test_fresnel_normal_incidence_closed_form():
+    """At normal incidence Gamma_TM = Gamma_TE = (n_t - 1)/(n_t + 1) in the code's sign convention."""
+    for er, sig, f in [(5.31, 0.0326, 2.4e9), (2.5, 0.03, 1.8e9), (1.0, 1e6, 1.8e9), (6.0, 0.0, 28e9)]:
+        c = O.fresnel(0.0, er, sig, f)
+        eps0 = 1.0 / (4 * np.pi * 1e-7 * 2.99792458e8 ** 2)
+        nt = np.sqrt(er - 1j * sig / (eps0 * 2 * np.pi * f))
+        g = (nt - 1) / (nt + 1)
+        assert abs(c["rTM"] - g) < 1e-12 * max(1, abs(g)) and abs(c["rTE"] - g) < 1e-12 * max(1, abs(g))
+        assert c["tTM"] == 0 and c["tTE"] == 0      # infinite wall: no transmission (Q8)
+
+
+def test_fresnel_brewster_and_grazing():
+    er = 5.31
+    thb = np.arctan(np.sqrt(er))
+    assert abs(O.fresnel(thb, er, 0.0, 2.4e9)["rTM"]) < 1e-12
+    # grazing incidence, in the code's sign convention (FieldCompute.cpp:255, 283): TE -> +1, TM -> -1
+    g = O.fresnel(np.pi / 2 - 1e-9, er, 0.0, 2.4e9)
+    assert abs(g["rTE"] - 1) < 1e-6 and abs(g["rTM"] + 1) < 1e-6
+
+
+def test_fresnel_zero_width_slab():
+    """useFresnelCoeff = false: zero-width slab, R = 0 and T = 1 (Q8)."""
+    c = O.fresnel(0.3, 5.31, 0.0326, 2.4e9, width=0.0, inf_wall=False)
+    assert abs(c["rTM"]) < 1e-15 and abs(c["rTE"]) < 1e-15
+    assert abs(c["tTM"] - 1) < 1e-12 and abs(c["tTE"] - 1) < 1e-12
+
+
+def test_closed_room_tree_has_exactly_R_plus_one_nodes_per_launch():
+    cfg = scenes.config1()
+    sc = O.Scene.from_soup(*cfg["soup"])
+    tr = dict(cfg["trace"], tess=3)
+    t = O.trees(sc, O.make_params(**tr))
+    counts = np.bincount(t["launch"])
+    assert np.all(counts == tr["maxR"] + 1)
+    assert np.all(t["tri"] >= 0)    # closed room: every segment hits
+
+
+def test_tree_node_count_bound_and_branch_order():
+    """Nodes per launch <= sum over (r <= R, t <= T) of C(r+t, r); BFS order = (depth, code)."""
+    from math import comb
+    cfg = scenes.config2(tess=2)
+    sc = O.Scene.from_soup(*cfg["soup"])
+    t = O.trees(sc, O.make_params(**cfg["trace"]))
+    R, T = cfg["trace"]["maxR"], cfg["trace"]["maxT"]
+    bound = sum(comb(r + q, r) for r in range(R + 1) for q in range(T + 1))
+    assert np.bincount(t["launch"]).max() <= bound
+    for l in np.unique(t["launch"])[:50]:
+        sel = np.flatnonzero(t["launch"] == l)
+        key = list(zip(t["depth"][sel], t["code"][sel]))
+        assert key == sorted(key)
+        assert np.sum(t["type"][sel] == 2) <= len(sel)
+
+
+def test_image_point_reflection_paths_are_specular():
+    """Corrected single-reflection ground paths obey the mirror law: the bounce point lies on
+    z = 0 on the line from the image source (0, 0, -5) to the receiver."""
+    cfg = scenes.ground_fixture()
+    sc = O.Scene.from_stl(stl("ground.stl"))
+    r = O.trace(sc, O.make_params(**cfg["trace"]), cfg["rx"], cfg["materials"])
+    for p in np.flatnonzero(r.pnseg == 2):
+        o = r.poff[p]
+        bounce = r.stgt[o]
+        rxp = cfg["rx"][r.prx[p]]
+        t = 5.0 / (5.0 + rxp[2])
+        assert abs(bounce[2]) < 1e-12
+        assert abs(bounce[1] - t * rxp[1]) < 1e-9
+
+
+def test_dedup_survivor_is_first_by_launch():
+    """Per receiver, a coplanar-id sequence is kept once, from the smallest launch reaching it."""
+    cfg = scenes.config1()
+    sc = O.Scene.from_soup(*cfg["soup"])
+    tr = dict(cfg["trace"], tess=2)
+    r = O.trace(sc, O.make_params(**tr), cfg["rx"], cfg["materials"])
+    seen = {}
+    for p in range(r.np):
+        o, n = r.poff[p], r.pnseg[p]
+        key = (int(r.prx[p]), tuple(r.scop[o:o + n]))
+        assert key not in seen
+        seen[key] = int(r.plaunch[p])
+        assert r.scop[o + n - 1] == -1 and r.stri[o + n - 1] == -1
+
+
+def test_planted_mistake_is_caught():
+    """A wrong reflection material (planted in a copy of the inputs) changes the field: the field
+    comparison used by the parity tests is sensitive to it."""
+    from tests.parity import _worst
+    cfg = scenes.config1()
+    sc = O.Scene.from_soup(*cfg["soup"])
+    tr = dict(cfg["trace"], tess=2)
+    a = O.trace(sc, O.make_params(**tr), cfg["rx"], cfg["materials"])
+    bad = cfg["materials"].copy()
+    bad[0, 3] = 5.2
+    b = O.trace(sc, O.make_params(**tr), cfg["rx"], bad)
+    with pytest.raises(AssertionError):
+        _worst(b.fields, a.fields, "planted")
