@@ -26,7 +26,6 @@
  *                          (RTSolver.cpp:532-538)
  *   sbr_generate_rays      GenerateRaysOnIcosahedron (Icosahedron.cpp:80-110)
  *   sbr_closest_hit        BVH<Triangle>::RayIntersects (BVH/BVH.h:276-328)
- *   sbr_submit / sbr_poll  (no reference counterpart: coverage-query serving loop)
  */
 #ifndef SBR_CAPI_H
 #define SBR_CAPI_H
@@ -122,7 +121,17 @@ typedef struct {
   double ms_k_capture;      /* duration of the capture kernel */
   int64_t node_visits;      /* BVH nodes entered (countTraversal = 1) */
   int64_t tri_tests;        /* ray-triangle tests (countTraversal = 1) */
+  int32_t trace_lds;        /* 1 if the traversal staged the scene in LDS */
+  int32_t trace_blocks_per_cu;
 } sbr_stats;
+
+/* Launch configuration of the traversal kernel (no reference counterpart).  lds_scene: -1 stage
+ * the scene in LDS when it fits, 0 never; blocks_per_cu: persistent blocks per compute unit, 0 =
+ * the occupancy maximum.  Results do not depend on it. */
+typedef struct {
+  int32_t lds_scene;
+  int32_t blocks_per_cu;
+} sbr_tuning;
 
 /* device >= 0: HIP device ordinal.  device == -1: host-only context (scene preparation and
  * scene queries work; device entry points return SBR_E_STATE). */
@@ -145,19 +154,29 @@ int sbr_scene_bvh_dfs(sbr_ctx* ctx, double* node_box, int32_t* node_a, int32_t* 
                       int32_t* tri_dfs_id);
 
 int sbr_materials(sbr_ctx* ctx, const sbr_material* mats, int32_t count);
+int sbr_set_tuning(sbr_ctx* ctx, const sbr_tuning* tuning);
+int sbr_get_tuning(sbr_ctx* ctx, sbr_tuning* tuning);
 
 int sbr_generate_rays(sbr_ctx* ctx, int32_t tess, double* dirs_out, int64_t capacity, int64_t* n_out);
 int sbr_closest_hit(sbr_ctx* ctx, const double* orig, const double* dirs, int32_t n, int32_t exact,
                     int32_t* tri_out, double* point_out, double* dist_out);
 
 int sbr_trace(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx, int32_t n_rx);
+/* Several transmitters over the same scene and receivers in one pass (no reference counterpart:
+ * the reference runs one transmitter per ExecuteRayTracing call).  params[n_tx]; every
+ * transmitter's result equals what sbr_trace gives for it alone. */
+int sbr_trace_batch(sbr_ctx* ctx, const sbr_trace_params* params, int32_t n_tx, const double* rx, int32_t n_rx);
 int sbr_result_stats(sbr_ctx* ctx, sbr_stats* out);
-/* Per-receiver complex field: out[n_rx][3][2] (Ex re, im, Ey re, im, Ez re, im). */
-int sbr_result_fields(sbr_ctx* ctx, double* out, int32_t n_rx);
-/* Kept paths, receiver ascending, then launch ascending, then the path order of a launch. */
+/* Complex field per (transmitter, receiver): out[n_tx][n_rx][3][2] (Ex re, im, Ey re, im, Ez re,
+ * im); n = n_tx * n_rx. */
+int sbr_result_fields(sbr_ctx* ctx, double* out, int32_t n);
+/* Kept paths, (transmitter, receiver) ascending, then launch ascending, then the path order of a
+ * launch (breadth first, transmitted child before reflected child). */
 int sbr_result_path_counts(sbr_ctx* ctx, int64_t* n_paths, int64_t* n_segs);
 int sbr_result_paths(sbr_ctx* ctx, int32_t* path_rx, int32_t* path_launch, int32_t* path_nseg, int64_t* path_off,
                      int32_t* seg_tri, int32_t* seg_cop, int32_t* seg_type, double* seg_src, double* seg_tgt);
+/* Transmitter index of every kept path (batched traces), in the order of sbr_result_paths. */
+int sbr_result_path_tx(sbr_ctx* ctx, int32_t* out, int64_t n_paths);
 /* Field of every kept path, in the order of sbr_result_paths: out[n_paths][3][2]. */
 int sbr_result_path_fields(sbr_ctx* ctx, double* out, int64_t n_paths);
 /* Interaction trees of the last trace, one row per node, ordered by (launch, depth, branch). */

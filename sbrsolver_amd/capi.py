@@ -37,10 +37,15 @@ class Stats(C.Structure):
         ("ms_field", C.c_double), ("ms_total", C.c_double),
         ("ms_k_trace", C.c_double), ("n_k_trace", C.c_int64), ("ms_k_capture", C.c_double),
         ("node_visits", C.c_int64), ("tri_tests", C.c_int64),
+        ("trace_lds", C.c_int32), ("trace_blocks_per_cu", C.c_int32),
     ]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(C.Structure):
+    _fields_ = [("lds_scene", C.c_int32), ("blocks_per_cu", C.c_int32)]
 
 
 class Material(C.Structure):
@@ -63,9 +68,13 @@ SIGNATURES = {
     "sbr_scene_triangles": ([_vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "sbr_scene_bvh_dfs": ([_vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "sbr_materials": ([_vp, _vp, _i32], C.c_int),
+    "sbr_set_tuning": ([_vp, _vp], C.c_int),
+    "sbr_get_tuning": ([_vp, _vp], C.c_int),
     "sbr_generate_rays": ([_vp, _i32, _vp, _i64, _vp], C.c_int),
     "sbr_closest_hit": ([_vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp], C.c_int),
     "sbr_trace": ([_vp, _vp, _vp, _i32], C.c_int),
+    "sbr_trace_batch": ([_vp, _vp, _i32, _vp, _i32], C.c_int),
+    "sbr_result_path_tx": ([_vp, _vp, _i64], C.c_int),
     "sbr_result_stats": ([_vp, _vp], C.c_int),
     "sbr_result_fields": ([_vp, _vp, _i32], C.c_int),
     "sbr_result_path_counts": ([_vp, _vp, _vp], C.c_int),
@@ -117,15 +126,22 @@ DEFAULT_MATERIALS = np.array([[1.8e9, 221.13, 0.05, 1.0, 0.0, 1e6]], dtype=np.fl
 
 
 class Result:
-    def __init__(self, ctx, nrx):
-        self.ctx, self.nrx = ctx, nrx
+    def __init__(self, ctx, nrx, ntx=1):
+        self.ctx, self.nrx, self.ntx = ctx, nrx, ntx
         st = Stats()
         ctx._chk(lib().sbr_result_stats(ctx.h, C.byref(st)))
         self.stats = st.as_dict()
         self.bounces = st.bounces
-        f = np.zeros((nrx, 3, 2))
-        ctx._chk(lib().sbr_result_fields(ctx.h, _a(f), nrx))
-        self.fields = f[..., 0] + 1j * f[..., 1]
+        f = np.zeros((ntx * nrx, 3, 2))
+        ctx._chk(lib().sbr_result_fields(ctx.h, _a(f), ntx * nrx))
+        f = f[..., 0] + 1j * f[..., 1]
+        self.fields = f if ntx == 1 else f.reshape(ntx, nrx, 3)
+
+    def path_tx(self):
+        n = self.stats["winners"]
+        out = np.zeros(n, np.int32)
+        self.ctx._chk(lib().sbr_result_path_tx(self.ctx.h, _a(out), n))
+        return out
 
     def paths(self):
         L, h = lib(), self.ctx.h
@@ -226,6 +242,15 @@ class Context:
         m = DEFAULT_MATERIALS if mats is None else np.ascontiguousarray(mats, dtype=np.float64).reshape(-1, 6)
         self._chk(lib().sbr_materials(self.h, _a(m), m.shape[0]))
 
+    def set_tuning(self, lds_scene=-1, blocks_per_cu=0):
+        t = Tuning(int(lds_scene), int(blocks_per_cu))
+        self._chk(lib().sbr_set_tuning(self.h, C.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        self._chk(lib().sbr_get_tuning(self.h, C.byref(t)))
+        return dict(lds_scene=t.lds_scene, blocks_per_cu=t.blocks_per_cu)
+
     # ---- kernels
     def generate_rays(self, tess):
         n = 10 * 4 ** int(tess) + 2
@@ -249,3 +274,9 @@ class Context:
         r = np.ascontiguousarray(rx, dtype=np.float64).reshape(-1, 3)
         self._chk(lib().sbr_trace(self.h, C.byref(params), _a(r), r.shape[0]))
         return Result(self, r.shape[0])
+
+    def trace_batch(self, params_list, rx):
+        r = np.ascontiguousarray(rx, dtype=np.float64).reshape(-1, 3)
+        arr = (TraceParams * len(params_list))(*params_list)
+        self._chk(lib().sbr_trace_batch(self.h, arr, len(params_list), _a(r), r.shape[0]))
+        return Result(self, r.shape[0], len(params_list))

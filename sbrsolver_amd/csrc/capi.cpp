@@ -131,6 +131,7 @@ int Engine::upload(const sbr_scene_desc* d) {
   ntri = n;
   nnode = nn;
   treeHeight = height;
+  sceneBytes = hNodes.size() * sizeof(DNode) + hTris.size() * sizeof(DTri);
   ncop = 0;
   for (int t = 0; t < n; t++) ncop = std::max(ncop, hCop[t] + 1);
   haveScene = true;
@@ -269,17 +270,38 @@ int sbr_materials(sbr_ctx* ctx, const sbr_material* mats, int32_t count) {
   return guarded(ctx, [&] { return ctx->eng.set_materials(mats, count); });
 }
 
+int sbr_set_tuning(sbr_ctx* ctx, const sbr_tuning* t) {
+  return guarded(ctx, [&] {
+    if (!t || t->lds_scene < -1 || t->lds_scene > 1 || t->blocks_per_cu < 0 || t->blocks_per_cu > 64)
+      return ctx->eng.fail(SBR_E_ARG, "tuning out of range");
+    ctx->eng.tune.ldsScene = t->lds_scene;
+    ctx->eng.tune.blocksPerCU = t->blocks_per_cu;
+    return SBR_OK;
+  });
+}
+
+int sbr_get_tuning(sbr_ctx* ctx, sbr_tuning* t) {
+  return guarded(ctx, [&] {
+    if (!t) return ctx->eng.fail(SBR_E_ARG, "null tuning");
+    t->lds_scene = ctx->eng.tune.ldsScene;
+    t->blocks_per_cu = ctx->eng.tune.blocksPerCU;
+    return SBR_OK;
+  });
+}
+
 int sbr_generate_rays(sbr_ctx* ctx, int32_t tess, double* out, int64_t cap, int64_t* n_out) {
   return guarded(ctx, [&] {
     sbr::Engine& e = ctx->eng;
+    if (e.device < 0) return e.fail(SBR_E_STATE, "host-only context has no device");
     if (hipSetDevice(e.device) != hipSuccess) return e.fail(SBR_E_HIP, "hipSetDevice");
     int64_t n = 0;
-    int rc = e.generate_rays(tess, &n);
+    const double* rays = nullptr;
+    int rc = e.rays_for(tess, &rays, &n);
     if (rc) return rc;
     if (n_out) *n_out = n;
     if (out && cap > 0) {
       int64_t k = std::min(cap, n);
-      if (hipMemcpy(out, e.dRays.p, 3 * sizeof(double) * k, hipMemcpyDeviceToHost) != hipSuccess)
+      if (hipMemcpy(out, rays, 3 * sizeof(double) * k, hipMemcpyDeviceToHost) != hipSuccess)
         return e.fail(SBR_E_HIP, "copy rays");
     }
     return SBR_OK;
@@ -295,7 +317,11 @@ int sbr_closest_hit(sbr_ctx* ctx, const double* o, const double* d, int32_t n, i
 }
 
 int sbr_trace(sbr_ctx* ctx, const sbr_trace_params* p, const double* rx, int32_t n_rx) {
-  return guarded(ctx, [&] { return ctx->eng.trace(p, rx, n_rx); });
+  return guarded(ctx, [&] { return ctx->eng.trace(p, 1, rx, n_rx); });
+}
+
+int sbr_trace_batch(sbr_ctx* ctx, const sbr_trace_params* p, int32_t n_tx, const double* rx, int32_t n_rx) {
+  return guarded(ctx, [&] { return ctx->eng.trace(p, n_tx, rx, n_rx); });
 }
 
 int sbr_result_stats(sbr_ctx* ctx, sbr_stats* out) {
@@ -307,12 +333,13 @@ int sbr_result_stats(sbr_ctx* ctx, sbr_stats* out) {
   });
 }
 
-int sbr_result_fields(sbr_ctx* ctx, double* out, int32_t n_rx) {
+int sbr_result_fields(sbr_ctx* ctx, double* out, int32_t n) {
   return guarded(ctx, [&] {
     sbr::Engine& e = ctx->eng;
     if (!e.haveResult) return e.fail(SBR_E_STATE, "no trace result");
-    if (n_rx != e.nrx || (!out && n_rx > 0)) return e.fail(SBR_E_ARG, "fields: receiver count mismatch");
-    if (n_rx > 0) memcpy(out, e.hField.data(), 6 * sizeof(double) * n_rx);
+    if ((int64_t)n != (int64_t)e.nrx * e.ntx || (!out && n > 0))
+      return e.fail(SBR_E_ARG, "fields: count must be transmitters x receivers");
+    if (n > 0) memcpy(out, e.hField.data(), 6 * sizeof(double) * (size_t)n);
     return SBR_OK;
   });
 }
@@ -320,7 +347,8 @@ int sbr_result_fields(sbr_ctx* ctx, double* out, int32_t n_rx) {
 int sbr_result_path_counts(sbr_ctx* ctx, int64_t* n_paths, int64_t* n_segs) {
   return guarded(ctx, [&] {
     int64_t a = 0, b = 0;
-    int rc = ctx->eng.paths(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &a, &b, true);
+    int rc = ctx->eng.paths(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &a,
+                            &b, true);
     if (n_paths) *n_paths = a;
     if (n_segs) *n_segs = b;
     return rc;
@@ -333,7 +361,21 @@ int sbr_result_paths(sbr_ctx* ctx, int32_t* prx, int32_t* plaunch, int32_t* pnse
     if (!prx || !plaunch || !pnseg || !poff || !stri || !scop || !stype || !ssrc || !stgt)
       return ctx->eng.fail(SBR_E_ARG, "paths: null output");
     int64_t a = 0, b = 0;
-    return ctx->eng.paths(prx, plaunch, pnseg, poff, stri, scop, stype, ssrc, stgt, &a, &b, false);
+    return ctx->eng.paths(nullptr, prx, plaunch, pnseg, poff, stri, scop, stype, ssrc, stgt, &a, &b, false);
+  });
+}
+
+int sbr_result_path_tx(sbr_ctx* ctx, int32_t* out, int64_t n_paths) {
+  return guarded(ctx, [&] {
+    sbr::Engine& e = ctx->eng;
+    if (!e.haveResult) return e.fail(SBR_E_STATE, "no trace result");
+    if (n_paths != e.winners || (!out && n_paths > 0)) return e.fail(SBR_E_ARG, "path tx: count mismatch");
+    if (n_paths == 0) return SBR_OK;
+    std::vector<int32_t> txrx(n_paths);
+    if (hipMemcpy(txrx.data(), e.dWinRx.p, sizeof(int32_t) * n_paths, hipMemcpyDeviceToHost) != hipSuccess)
+      return e.fail(SBR_E_HIP, "copy path keys");
+    for (int64_t w = 0; w < n_paths; w++) out[w] = txrx[w] / e.nrx;
+    return SBR_OK;
   });
 }
 

@@ -25,6 +25,12 @@ __device__ __forceinline__ V3 ld3(const double* p) { return v3(p[0], p[1], p[2])
 // margin of 1e-7; children are visited nearest entry first so the cut bites early.  A box is
 // only entered when the reference's own slab test passes, so the reference's grazing misses are
 // reproduced too.
+//
+// The slab test runs with a per-ray reciprocal direction (one multiply instead of a divide per
+// slab).  The reciprocal product differs from the reference's quotient by at most ~2 ulp and has
+// the same inf / NaN pattern (both come only from zero direction components), so the pass / fail
+// decision can differ only when the entry and exit parameters agree to ~1e-15; inside a band of
+// 1e-13 relative the test is redone with the reference's divisions.
 // ---------------------------------------------------------------------------------------------
 struct Hit {
   int idx;       // DFS triangle index, INT_MAX if none
@@ -35,8 +41,44 @@ struct Hit {
   int tests;     // ray-triangle tests
 };
 
-__device__ __forceinline__ bool box_of(const DNode& n, V3 o, V3 d, double* lo, double* hi) {
-  return box_line(n.bmin, n.bmax, o, d, lo, hi);
+struct RayPre {
+  V3 o, d, inv;
+  bool safe;     // no subnormal direction component (reciprocal exact in inf / NaN pattern)
+};
+
+__device__ __forceinline__ RayPre ray_pre(V3 o, V3 d) {
+  RayPre r;
+  r.o = o;
+  r.d = d;
+  r.inv = v3(1.0 / d.x, 1.0 / d.y, 1.0 / d.z);
+  const double tiny = 1e-290;
+  r.safe = !((d.x != 0.0 && fabs(d.x) < tiny) || (d.y != 0.0 && fabs(d.y) < tiny) || (d.z != 0.0 && fabs(d.z) < tiny));
+  return r;
+}
+
+__device__ __forceinline__ double fin_abs(double x) { return isfinite(x) ? fabs(x) : 0.0; }
+
+__device__ __forceinline__ bool box_test(const double* bmin, const double* bmax, const RayPre& r, double* lo,
+                                         double* hi) {
+  if (r.safe) {
+    double t_max = INFINITY, t_min = -INFINITY;
+    const double oo[3] = {r.o.x, r.o.y, r.o.z}, iv[3] = {r.inv.x, r.inv.y, r.inv.z};
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      double t1 = (bmin[i] - oo[i]) * iv[i];
+      double t2 = (bmax[i] - oo[i]) * iv[i];
+      t_max = smin(t_max, smax(t1, t2));
+      t_min = smax(t_min, smin(t1, t2));
+    }
+    double m = 1e-13 * (fin_abs(t_min) + fin_abs(t_max));
+    if (t_min > t_max + m) return false;
+    if (t_min < t_max - m) {
+      *lo = t_min;
+      *hi = t_max;
+      return true;
+    }
+  }
+  return box_line(bmin, bmax, r.o, r.d, lo, hi);
 }
 
 template <bool EXACT>
@@ -47,23 +89,24 @@ __device__ __forceinline__ bool keep_box(double lo, double hi, const Hit& h) {
   return true;
 }
 
+// Traversal stack: entry e of the calling thread is at stkN[e * stride] / stkT[e * stride]
+// (stride = threads per block, so a wave's entries are contiguous).
 template <bool EXACT>
-__device__ bool closest_hit(const DNode* __restrict__ nodes, const DTri* __restrict__ tris, V3 o, V3 d, Hit& h) {
+__device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, const DTri* __restrict__ tris, V3 o, V3 d,
+                                            Hit& h, int* stkN, float* stkT, int stride) {
   h.idx = INT_MAX;
   h.dist = INFINITY;
   h.t = INFINITY;
   h.p = v3(0, 0, 0);
   h.nodes = 0;
   h.tests = 0;
+  const RayPre r = ray_pre(o, d);
   double lo, hi;
-  if (!box_of(nodes[0], o, d, &lo, &hi) || !keep_box<EXACT>(lo, hi, h)) return false;
-  int stackN[kMaxStack];
-  double stackT[kMaxStack];
+  if (!box_test(nodes[0].bmin, nodes[0].bmax, r, &lo, &hi) || !keep_box<EXACT>(lo, hi, h)) return false;
   int sp = 0;
   int cur = 0;
   while (true) {
     const DNode nd = nodes[cur];
-    bool descended = false;
     h.nodes++;
     if (nd.count > 0) {
       h.tests += nd.count;
@@ -82,31 +125,28 @@ __device__ bool closest_hit(const DNode* __restrict__ nodes, const DTri* __restr
       }
     } else {
       double la, ha, lb, hb;
-      bool ina = box_of(nodes[nd.a], o, d, &la, &ha) && keep_box<EXACT>(la, ha, h);
-      bool inb = box_of(nodes[nd.b], o, d, &lb, &hb) && keep_box<EXACT>(lb, hb, h);
+      bool ina = box_test(nodes[nd.a].bmin, nodes[nd.a].bmax, r, &la, &ha) && keep_box<EXACT>(la, ha, h);
+      bool inb = box_test(nodes[nd.b].bmin, nodes[nd.b].bmax, r, &lb, &hb) && keep_box<EXACT>(lb, hb, h);
       if (ina && inb) {
         int nearN = nd.a, farN = nd.b;
         double farT = lb;
         if (!EXACT && lb < la) { nearN = nd.b; farN = nd.a; farT = la; }
-        stackN[sp] = farN;
-        stackT[sp] = farT;
+        stkN[sp * stride] = farN;
+        stkT[sp * stride] = __double2float_rd(farT);
         sp++;
         cur = nearN;
-        descended = true;
-      } else if (ina) {
-        cur = nd.a;
-        descended = true;
-      } else if (inb) {
-        cur = nd.b;
-        descended = true;
+        continue;
+      }
+      if (ina || inb) {
+        cur = ina ? nd.a : nd.b;
+        continue;
       }
     }
-    if (descended) continue;
     bool popped = false;
     while (sp > 0) {
       --sp;
-      if (!EXACT && h.idx != INT_MAX && stackT[sp] > h.t * (1.0 + 1e-7) + 1e-7) continue;
-      cur = stackN[sp];
+      if (!EXACT && h.idx != INT_MAX && (double)stkT[sp * stride] > h.t * (1.0 + 1e-7) + 1e-7) continue;
+      cur = stkN[sp * stride];
       popped = true;
       break;
     }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -14,6 +15,8 @@ namespace sbr {
 
 constexpr int kMaxDepth = 24;       // reflections + transmissions per path (tree depth)
 constexpr int kMaxStack = 64;       // BVH traversal stack (tree height bound, checked on upload)
+constexpr int kBlock = 256;         // threads per block of the traversal kernels
+constexpr size_t kSceneLdsMax = 96 * 1024;   // scenes up to this size are staged in LDS
 
 // ---- scene, device layout ---------------------------------------------------------------------
 // BVH node in depth-first order (children[0] subtree first), 64 B.
@@ -46,8 +49,23 @@ struct alignas(64) DSurf {
   double pad[3];
 };
 
+// ---- transmitters --------------------------------------------------------------------------------
+struct TxDesc {
+  double tx[3];
+  double txcs[9];
+  double freq, pt, rcoef;
+  const double* rays;        // launch directions of this transmitter's icosahedron level
+  int64_t launchStart;       // global index of its first root item
+  int64_t nLaunch;           // root items (selected launches)
+  int32_t launchStride, launchOffset;
+  int32_t maxR, maxT, maxTotal;
+  int32_t useFresnel, antennaType, receptionModel;
+  int32_t pad[2];
+};
+
 // ---- interaction tree ---------------------------------------------------------------------------
-// One RayLaunch call (one bounce), 128 B.
+// One RayLaunch call (one bounce), 128 B.  The hit triangle's material is tris[hitDfs].mat; the
+// launch index within the transmitter follows from gl (see TxDesc).
 struct alignas(64) TNode {
   double src[3];
   double tgt[3];
@@ -56,24 +74,23 @@ struct alignas(64) TNode {
   double plArg;      // totalPathLength that RayCapture hands to this node (sum of ancestors' pl)
   double trueLen;    // geometric length from the transmitter to src
   int32_t parent;    // node index of the parent, -1 for a root
-  int32_t launch;
+  int32_t gl;        // global root index of the tree (transmitters' root items concatenated)
   int32_t tri;       // hitSurfaceID, -1 on a miss
   int32_t cop;       // hitCoplanarId, -1 on a miss
-  int32_t mat;       // material of the hit surface
+  int32_t hitDfs;    // DFS index of the hit triangle, -1 on a miss
   uint32_t code;     // branch code: bit per level, transmit = 0, reflect = 1, first level most significant
   uint8_t depth;
   uint8_t type;      // 0 from the transmitter, 1 reflected, 2 transmitted
   uint8_t rc, tc;    // reflections / transmissions so far
-  int32_t pad;
+  int16_t txi;       // transmitter index in the batch
+  int16_t pad;
 };
 static_assert(sizeof(TNode) == 128, "TNode layout");
 
-struct TraceConsts {
-  double tx[3];
-  double txcs[9];
-  double freq, pt, rcoef;
-  int32_t maxR, maxT, maxTotal;
-  int32_t useFresnel, antennaType, receptionModel, exact;
+// Per-depth work bookkeeping on the device.
+struct LevelRec {
+  unsigned long long base;   // node index of the level's first node
+  unsigned long long count;  // items (nodes) of the level
 };
 
 struct RxGrid {
@@ -91,13 +108,22 @@ struct DBuf {
   ~DBuf() { release(); }
 };
 
+// Launch configuration of the traversal kernel (sbr_set_tuning; the autotuner fills it).
+struct Tuning {
+  int ldsScene = -1;      // -1: stage the scene in LDS when it fits, 0: never
+  int blocksPerCU = 0;    // persistent blocks per CU; 0: occupancy maximum
+};
+
 struct Engine {
   int device = 0;
+  Tuning tune;
   hipStream_t stream = nullptr;
   std::string err;
+  int numCU = 256;
 
   // scene (host copies for queries)
   int ntri = 0, nnode = 0, ncop = 0, treeHeight = 0;
+  size_t sceneBytes = 0;
   std::vector<DNode> hNodes;
   std::vector<DTri> hTris;
   std::vector<double> hTriN;                // normals in DFS order
@@ -112,14 +138,17 @@ struct Engine {
   DBuf<double> dMats;
   bool haveScene = false;
 
-  // per-trace buffers
-  DBuf<double> dRays;            // 3 per launch
+  // launch directions per icosahedron level (generated once, kept)
+  std::map<int, DBuf<double>*> rayCache;
   DBuf<int32_t> dFaces[2], dAdj[2];
   DBuf<int32_t> dFlag, dScan;
+  DBuf<double> dVerts;
+
+  // per-trace buffers
+  DBuf<TxDesc> dTx;
   DBuf<TNode> dTree;
-  DBuf<int32_t> dQueue;          // (parent << 1) | reflect
-  DBuf<int32_t> dChild;          // child mask per node of a level
-  DBuf<int32_t> dChildScan;
+  DBuf<int32_t> dQueue[2];       // (parent << 1) | reflect
+  DBuf<LevelRec> dLevels;
   DBuf<uint8_t> dScratch;        // library temp storage (scan / sort)
   DBuf<double> dRx;
   DBuf<int32_t> dCellStart, dCellRx;
@@ -128,21 +157,19 @@ struct Engine {
   DBuf<int32_t> dTabRx;
   DBuf<unsigned long long> dCounters;
   DBuf<unsigned long long> dWinKey, dWinKey2;
-  DBuf<int32_t> dWinRx, dWinRx2;
+  DBuf<int32_t> dWinRx, dWinRx2, dWinIdx, dWinIdx2, dWinNode;
   DBuf<double> dWinField;
-  DBuf<int32_t> dRxStart;
   DBuf<double> dField;
 
   // last result
   bool haveResult = false;
-  sbr_trace_params lastParams{};
+  std::vector<sbr_trace_params> lastParams;
   sbr_stats stats{};
-  int nrx = 0;
-  std::vector<int64_t> levelBase;  // node index of the first node of each depth
+  int nrx = 0, ntx = 0;
   int64_t nodes = 0;
   int64_t winners = 0;
+  size_t nodeCap = 0;
   std::vector<double> hField;
-  std::vector<double> hRx;
   RxGrid grid{};
 
   ~Engine();
@@ -150,13 +177,18 @@ struct Engine {
   int init(int dev);
   int upload(const sbr_scene_desc* d);
   int set_materials(const sbr_material* m, int n);
-  int generate_rays(int tess, int64_t* nOut);   // fills dRays
-  int trace(const sbr_trace_params* p, const double* rx, int m);
+  int rays_for(int tess, const double** out, int64_t* nOut);   // cached launch directions
+  int trace(const sbr_trace_params* p, int ntx, const double* rx, int m);
   int closest_hit(const double* o, const double* d, int n, int exact, int32_t* tri, double* pt, double* dist);
-  int paths(int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri, int32_t* scop,
-            int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs, bool countOnly);
+  int paths(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
+            int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
+            bool countOnly);
   int trees(int32_t* launch, int32_t* depth, int64_t* code, int32_t* tri, int32_t* cop, int32_t* type, double* src,
             double* tgt, double* pl);
+
+ private:
+  int generate_rays(int tess, DBuf<double>& out, int64_t* nOut);
+  int trace_stage(const std::vector<TxDesc>& txs, int64_t nRoot, int maxDepth, bool exact, bool count);
 };
 
 }  // namespace sbr

@@ -1,21 +1,25 @@
 // gfx950 engine of the shoot-bounce-reflect trace-and-field step.
 //
-// Pipeline of one sbr_trace call (RTSolver::ExecuteRayTracing, RTSolver.cpp:52-126):
-//   1. k_ico_flag / k_ico_split / k_ico_dirs: icosahedral launch directions, built level by level
-//      in the vertex order of Icosahedron.cpp:3-110.
-//   2. k_trace_level<EXACT> + k_emit_level: the interaction tree of every launch, one kernel per
-//      tree depth (RayLaunch, RTSolver.cpp:128-257).  Each level's queue is compacted with a stable
-//      scan, so within a depth the node index grows with (launch, branch code); the triple
-//      (launch, depth, index-in-level) is then an order key equal to the reference's
-//      (launch, BFS position) order.
-//   3. k_los: direct-path line of sight per receiver (DirectPathCorrection, RTSolver.cpp:375-394).
+// Pipeline of one trace (RTSolver::ExecuteRayTracing, RTSolver.cpp:52-126), for a batch of one or
+// more transmitters sharing the scene and the receivers:
+//   1. k_ico_flag / k_ico_split / k_ico_dirs: icosahedral launch directions per subdivision level,
+//      in the vertex order of Icosahedron.cpp:3-110 (generated once per level and kept).
+//   2. k_trace<EXACT, LDS>: the interaction trees of every launch (RayLaunch, RTSolver.cpp:128-257),
+//      one persistent launch per tree depth.  Each launch reads its item count from device memory and
+//      appends the next depth's items with wave-aggregated atomics, so the bounce loop has no host
+//      round trip.  Small scenes are staged in LDS; the traversal stack always lives in LDS.
+//   3. k_los: direct-path line of sight per (transmitter, receiver) (DirectPathCorrection,
+//      RTSolver.cpp:375-394).
 //   4. k_capture: every tree node against the receivers binned in a uniform xy grid
 //      (HitReceptionSphere, RTSolver.cpp:805-863); a capture counts when no ancestor captures
 //      (RayCapture + Paths, RTSolver.cpp:259-297, Ray.cpp:136-182); image correction in FP64
-//      (RTSolver.cpp:299-433); duplicate removal keeps the minimum order key per (receiver,
-//      coplanar-id sequence) in a lock-free hash table (RemoveDuplicatePath, RTSolver.cpp:435-459).
-//   5. k_collect + two stable radix sorts + k_field + k_rx_sum: the field of every kept path
-//      (FieldForPath, FieldCompute.cpp:71-151) summed per receiver in (launch, BFS) order
+//      (RTSolver.cpp:299-433); duplicate removal keeps the minimum order key per (transmitter,
+//      receiver, coplanar-id sequence) in a lock-free hash table (RemoveDuplicatePath,
+//      RTSolver.cpp:435-459).  The order key (root, depth, branch code) is the reference's
+//      (launch, BFS position) order.
+//   5. k_collect, a radix sort by order key, k_resolve (order key -> tree node), a stable radix sort
+//      by (transmitter, receiver), k_field and k_rx_sum: the field of every kept path
+//      (FieldForPath, FieldCompute.cpp:71-151) summed per receiver in the reference's order
 //      (FieldAtReceiver, FieldCompute.cpp:11-25).
 #include <hipcub/hipcub.hpp>
 
@@ -63,29 +67,14 @@ template struct DBuf<DNode>;
 template struct DBuf<DTri>;
 template struct DBuf<DSurf>;
 template struct DBuf<TNode>;
+template struct DBuf<TxDesc>;
+template struct DBuf<LevelRec>;
 
 #define ENSURE(buf, cnt)                                                                         \
   do {                                                                                           \
     (buf).ensure((size_t)(cnt));                                                                 \
     if (!(buf).p) return fail(SBR_E_NOMEM, std::string("device allocation failed: ") + #buf);  \
   } while (0)
-
-// Grow a buffer keeping its first `keep` elements.
-template <typename T>
-static int grow_keep(DBuf<T>& b, size_t want, size_t keep, hipStream_t s) {
-  if (want <= b.n && b.p) return 0;
-  size_t cap = want + want / 2 + 1024;
-  T* np = nullptr;
-  if (hipMalloc((void**)&np, cap * sizeof(T)) != hipSuccess) { (void)hipGetLastError(); return -1; }
-  if (b.p && keep) {
-    if (hipMemcpyAsync(np, b.p, keep * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess) return -1;
-    if (hipStreamSynchronize(s) != hipSuccess) return -1;
-  }
-  b.release();
-  b.p = np;
-  b.n = cap;
-  return 0;
-}
 
 static inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)std::max<int64_t>(1, (n + bs - 1) / bs); }
 
@@ -163,141 +152,213 @@ __global__ void k_ico_dirs(const double* __restrict__ verts, int64_t nv, double*
 }
 
 // =============================================================================================
-// 2. Interaction tree, one depth per launch (RayLaunch, RTSolver.cpp:128-257)
+// 2. Interaction trees, one persistent launch per depth (RayLaunch, RTSolver.cpp:128-257)
 // =============================================================================================
-struct LevelArgs {
+struct TraceArgs {
   int32_t depth;
-  int32_t n;              // items in this level
-  int64_t base;           // node index of item 0
-  int32_t launchStride, launchOffset;
+  int32_t nTx;
+  int32_t stackDepth;
+  int32_t nodeBytes, triBytes;   // scene staged in LDS (LDS variant)
+  unsigned long long nodeCap, queueCap;
 };
 
-template <bool EXACT>
-__global__ void __launch_bounds__(256) k_trace_level(LevelArgs L, TraceConsts K, const DNode* __restrict__ nodes,
-                                                     const DTri* __restrict__ tris, const double* __restrict__ triN,
-                                                     const double* __restrict__ rays, const int32_t* __restrict__ queue,
-                                                     TNode* __restrict__ tree, int32_t* __restrict__ childMask,
-                                                     unsigned long long* __restrict__ travCount) {
-  __shared__ unsigned long long sNodes, sTests;
-  if (travCount) {
-    if (threadIdx.x == 0) { sNodes = 0; sTests = 0; }
+// Flags: [0] bit 0 node overflow, bit 1 queue overflow; [6], [7] node visits / triangle tests.
+template <bool EXACT, bool LDS>
+__global__ void __launch_bounds__(kBlock) k_trace(TraceArgs A, const TxDesc* __restrict__ txs,
+                                                  const DNode* __restrict__ gNodes, const DTri* __restrict__ gTris,
+                                                  const double* __restrict__ triN, const int32_t* __restrict__ qIn,
+                                                  int32_t* __restrict__ qOut, TNode* __restrict__ tree,
+                                                  LevelRec* __restrict__ lv, unsigned long long* __restrict__ flags,
+                                                  int countTraversal) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const DNode* nodes = gNodes;
+  const DTri* tris = gTris;
+  size_t sceneB = 0;
+  if (LDS) {
+    const int4* s0 = reinterpret_cast<const int4*>(gNodes);
+    int4* d0 = reinterpret_cast<int4*>(smem);
+    for (int i = threadIdx.x; i < A.nodeBytes / 16; i += blockDim.x) d0[i] = s0[i];
+    const int4* s1 = reinterpret_cast<const int4*>(gTris);
+    int4* d1 = reinterpret_cast<int4*>(smem + A.nodeBytes);
+    for (int i = threadIdx.x; i < A.triBytes / 16; i += blockDim.x) d1[i] = s1[i];
+    sceneB = (size_t)A.nodeBytes + A.triBytes;
+    nodes = reinterpret_cast<const DNode*>(smem);
+    tris = reinterpret_cast<const DTri*>(smem + A.nodeBytes);
     __syncthreads();
   }
-  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < L.n) {
-  TNode nd;
-  V3 src, dir;
-  double totalPL;
-  if (L.depth == 0) {
-    int32_t launch = L.launchOffset + i * L.launchStride;
-    src = v3(K.tx[0], K.tx[1], K.tx[2]);
-    dir = ld3(rays + 3 * (int64_t)launch);
-    nd.parent = -1;
-    nd.launch = launch;
-    nd.code = 0;
-    nd.type = 0;
-    nd.rc = nd.tc = 0;
-    nd.plArg = 0.0;
-    nd.trueLen = 0.0;
-    totalPL = 0.0;
-  } else {
-    int32_t q = queue[i];
-    int32_t pi = q >> 1;
-    bool refl = q & 1;
-    const TNode& P = tree[pi];
-    src = ld3(P.tgt);
-    V3 pd = ld3(P.dir);
-    if (refl) {
-      // Reflect about the hit triangle's stored normal, then normalize (RTSolver.cpp:186-203).
-      V3 n = ld3(triN + 3 * (int64_t)P.pad);
-      dir = vnormalized(vreflect(pd, n));
-    } else {
-      dir = pd;
-    }
-    nd.parent = pi;
-    nd.launch = P.launch;
-    nd.code = (P.code << 1) | (refl ? 1u : 0u);
-    nd.type = refl ? 1 : 2;
-    nd.rc = P.rc + (refl ? 1 : 0);
-    nd.tc = P.tc + (refl ? 0 : 1);
-    nd.plArg = P.plArg + P.pl;
-    nd.trueLen = P.trueLen + vnorm(vsub(ld3(P.tgt), ld3(P.src)));
-    totalPL = P.pl;
-  }
-  nd.depth = (uint8_t)L.depth;
-  nd.src[0] = src.x; nd.src[1] = src.y; nd.src[2] = src.z;
-  nd.dir[0] = dir.x; nd.dir[1] = dir.y; nd.dir[2] = dir.z;
-  Hit h;
-  int mask = 0;
-  if (closest_hit<EXACT>(nodes, tris, src, dir, h)) {
-    const DTri& t = tris[h.idx];
-    nd.tgt[0] = h.p.x; nd.tgt[1] = h.p.y; nd.tgt[2] = h.p.z;
-    nd.pl = totalPL + h.dist;
-    nd.tri = t.id;
-    nd.cop = t.cop;
-    nd.mat = t.mat;
-    nd.pad = h.idx;   // DFS index of the hit triangle (normal lookup for the reflected child)
-    bool depthOk = K.maxTotal <= 0 || (nd.rc + nd.tc) < K.maxTotal;
-    if (nd.tc < K.maxT && depthOk) mask |= 1;   // transmitted child (BFS order: first)
-    if (nd.rc < K.maxR && depthOk) mask |= 2;   // reflected child
-  } else {
-    // A miss leaves a 100 m stub below the root, a 1e10 m segment at the root (RTSolver.cpp:224-256).
-    double len = L.depth == 0 ? 1E10 : 1E2;
-    V3 t = vadd(src, vmul(dir, len));
-    nd.tgt[0] = t.x; nd.tgt[1] = t.y; nd.tgt[2] = t.z;
-    nd.pl = INFINITY;
-    nd.tri = -1;
-    nd.cop = -1;
-    nd.mat = -1;
-    nd.pad = -1;
-  }
-  tree[L.base + i] = nd;
-  childMask[i] = mask;
-  if (travCount) {
-    atomicAdd(&sNodes, (unsigned long long)h.nodes);
-    atomicAdd(&sTests, (unsigned long long)h.tests);
-  }
-  }
-  if (travCount) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      atomicAdd(&travCount[0], sNodes);
-      atomicAdd(&travCount[1], sTests);
-    }
-  }
-}
+  int* stkN = reinterpret_cast<int*>(smem + sceneB) + threadIdx.x;
+  float* stkT = reinterpret_cast<float*>(smem + sceneB + (size_t)A.stackDepth * blockDim.x * 4) + threadIdx.x;
 
-__global__ void k_child_count(const int32_t* __restrict__ mask, int32_t n, int32_t* __restrict__ cnt) {
-  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int m = mask[i];
-  cnt[i] = (m & 1) + ((m >> 1) & 1);
-}
-
-__global__ void k_emit_level(const int32_t* __restrict__ mask, const int32_t* __restrict__ off, int32_t n,
-                             int64_t base, int32_t* __restrict__ next) {
-  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int m = mask[i];
-  int32_t o = off[i];
-  int32_t node = (int32_t)(base + i);
-  if (m & 1) next[o++] = node << 1;         // transmit
-  if (m & 2) next[o++] = (node << 1) | 1;   // reflect
+  const LevelRec L = lv[A.depth];
+  unsigned long long n = L.count;
+  if (n > A.queueCap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&flags[0], 2ull);
+    n = A.queueCap;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) lv[A.depth + 1].base = L.base + n;
+  unsigned long long cntNodes = 0, cntTests = 0;
+  const int lane = threadIdx.x & 63;
+  for (unsigned long long b0 = (unsigned long long)blockIdx.x * blockDim.x; b0 < n;
+       b0 += (unsigned long long)gridDim.x * blockDim.x) {
+    const unsigned long long i = b0 + threadIdx.x;
+    const unsigned long long nodeIdx = L.base + i;
+    int mask = 0;
+    if (i < n && nodeIdx >= A.nodeCap) atomicOr(&flags[0], 1ull);
+    if (i < n && nodeIdx < A.nodeCap) {
+      // The part of the record known before the traversal is stored first, so that only the ray,
+      // the running path length and the child permissions stay live across the traversal.
+      TNode* o = &tree[nodeIdx];
+      V3 src, dir;
+      double totalPL;
+      int allow;
+      {
+        TNode nd;
+        const TxDesc* T;
+        if (A.depth == 0) {
+          int lo = 0, hi = A.nTx - 1;   // transmitter owning root item i
+          while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if ((unsigned long long)txs[mid].launchStart <= i) lo = mid;
+            else hi = mid - 1;
+          }
+          T = &txs[lo];
+          const int64_t launch = T->launchOffset + ((int64_t)i - T->launchStart) * T->launchStride;
+          src = v3(T->tx[0], T->tx[1], T->tx[2]);
+          dir = ld3(T->rays + 3 * launch);
+          nd.parent = -1;
+          nd.gl = (int32_t)i;
+          nd.code = 0;
+          nd.type = 0;
+          nd.rc = nd.tc = 0;
+          nd.txi = (int16_t)lo;
+          nd.plArg = 0.0;
+          nd.trueLen = 0.0;
+          totalPL = 0.0;
+        } else {
+          const int32_t q = qIn[i];
+          const int32_t pi = q >> 1;
+          const bool refl = q & 1;
+          const TNode& P = tree[pi];
+          T = &txs[P.txi];
+          src = ld3(P.tgt);
+          const V3 pd = ld3(P.dir);
+          // Reflect about the hit triangle's stored normal, then normalize (RTSolver.cpp:186-203).
+          dir = refl ? vnormalized(vreflect(pd, ld3(triN + 3 * (int64_t)P.hitDfs))) : pd;
+          nd.parent = pi;
+          nd.gl = P.gl;
+          nd.code = (P.code << 1) | (refl ? 1u : 0u);
+          nd.type = refl ? 1 : 2;
+          nd.rc = P.rc + (refl ? 1 : 0);
+          nd.tc = P.tc + (refl ? 0 : 1);
+          nd.txi = P.txi;
+          nd.plArg = P.plArg + P.pl;
+          nd.trueLen = P.trueLen + vnorm(vsub(ld3(P.tgt), ld3(P.src)));
+          totalPL = P.pl;
+        }
+        const bool depthOk = T->maxTotal <= 0 || (nd.rc + nd.tc) < T->maxTotal;
+        allow = ((nd.tc < T->maxT && depthOk) ? 1 : 0) | ((nd.rc < T->maxR && depthOk) ? 2 : 0);
+        o->src[0] = src.x; o->src[1] = src.y; o->src[2] = src.z;
+        o->dir[0] = dir.x; o->dir[1] = dir.y; o->dir[2] = dir.z;
+        o->plArg = nd.plArg;
+        o->trueLen = nd.trueLen;
+        o->parent = nd.parent;
+        o->gl = nd.gl;
+        o->code = nd.code;
+        o->depth = (uint8_t)A.depth;
+        o->type = nd.type;
+        o->rc = nd.rc;
+        o->tc = nd.tc;
+        o->txi = nd.txi;
+        o->pad = 0;
+      }
+      Hit h;
+      if (closest_hit<EXACT>(nodes, tris, src, dir, h, stkN, stkT, blockDim.x)) {
+        const DTri& t = tris[h.idx];
+        o->tgt[0] = h.p.x; o->tgt[1] = h.p.y; o->tgt[2] = h.p.z;
+        o->pl = totalPL + h.dist;
+        o->tri = t.id;
+        o->cop = t.cop;
+        o->hitDfs = h.idx;
+        mask = allow;   // transmitted child (bit 0, BFS order: first), reflected child (bit 1)
+      } else {
+        // A miss leaves a 100 m stub below the root, a 1e10 m segment at the root (RTSolver.cpp:224-256).
+        const double len = A.depth == 0 ? 1E10 : 1E2;
+        const V3 t = vadd(src, vmul(dir, len));
+        o->tgt[0] = t.x; o->tgt[1] = t.y; o->tgt[2] = t.z;
+        o->pl = INFINITY;
+        o->tri = -1;
+        o->cop = -1;
+        o->hitDfs = -1;
+      }
+      cntNodes += h.nodes;
+      cntTests += h.tests;
+    }
+    // Wave-aggregated append of the children to the next depth's queue.
+    const int c = (mask & 1) + ((mask >> 1) & 1);
+    int incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    const int tot = __shfl(incl, 63, 64);
+    unsigned long long wbase = 0;
+    if (lane == 63 && tot > 0) wbase = atomicAdd(&lv[A.depth + 1].count, (unsigned long long)tot);
+    wbase = __shfl(wbase, 63, 64);
+    if (c > 0) {
+      unsigned long long pos = wbase + (unsigned long long)(incl - c);
+      if (pos + c > A.queueCap) {
+        atomicOr(&flags[0], 2ull);
+      } else {
+        const int32_t nodeI = (int32_t)nodeIdx;
+        if (mask & 1) qOut[pos++] = nodeI << 1;
+        if (mask & 2) qOut[pos] = (nodeI << 1) | 1;
+      }
+    }
+  }
+  if (countTraversal) {
+    atomicAdd(&flags[6], cntNodes);
+    atomicAdd(&flags[7], cntTests);
+  }
 }
 
 // =============================================================================================
 // 3. Direct-path line of sight (DirectPathCorrection, RTSolver.cpp:375-394)
 // =============================================================================================
 template <bool EXACT>
-__global__ void k_los(TraceConsts K, const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
-                      const double* __restrict__ rx, int32_t m, uint8_t* __restrict__ los) {
-  int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= m) return;
-  V3 tx = v3(K.tx[0], K.tx[1], K.tx[2]);
-  V3 d = vsub(ld3(rx + 3 * (int64_t)r), tx);
+__global__ void __launch_bounds__(kBlock) k_los(const TxDesc* __restrict__ txs, int32_t nTx,
+                                                const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
+                                                const double* __restrict__ rx, int32_t m, int32_t stackDepth,
+                                                uint8_t* __restrict__ los) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int* stkN = reinterpret_cast<int*>(smem) + threadIdx.x;
+  float* stkT = reinterpret_cast<float*>(smem + (size_t)stackDepth * blockDim.x * 4) + threadIdx.x;
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)nTx * m) return;
+  const TxDesc& T = txs[k / m];
+  V3 tx = v3(T.tx[0], T.tx[1], T.tx[2]);
+  V3 d = vsub(ld3(rx + 3 * (k % m)), tx);
   Hit h;
-  bool hh = closest_hit<EXACT>(nodes, tris, tx, d, h);
-  los[r] = (!hh || vnorm(d) < h.dist) ? 1 : 0;
+  bool hh = closest_hit<EXACT>(nodes, tris, tx, d, h, stkN, stkT, blockDim.x);
+  los[k] = (!hh || vnorm(d) < h.dist) ? 1 : 0;
+}
+
+template <bool EXACT>
+__global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
+                                                          const double* __restrict__ org, const double* __restrict__ dirs,
+                                                          int32_t n, int32_t stackDepth, int32_t* __restrict__ triOut,
+                                                          double* __restrict__ ptOut, double* __restrict__ distOut) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  int* stkN = reinterpret_cast<int*>(smem) + threadIdx.x;
+  float* stkT = reinterpret_cast<float*>(smem + (size_t)stackDepth * blockDim.x * 4) + threadIdx.x;
+  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Hit h;
+  bool hh = closest_hit<EXACT>(nodes, tris, ld3(org + 3 * (int64_t)i), ld3(dirs + 3 * (int64_t)i), h, stkN, stkT,
+                               blockDim.x);
+  triOut[i] = hh ? tris[h.idx].id : -1;
+  ptOut[3 * i] = h.p.x; ptOut[3 * i + 1] = h.p.y; ptOut[3 * i + 2] = h.p.z;
+  distOut[i] = h.dist;
 }
 
 // =============================================================================================
@@ -305,7 +366,6 @@ __global__ void k_los(TraceConsts K, const DNode* __restrict__ nodes, const DTri
 // =============================================================================================
 struct CaptureArgs {
   int64_t nodes;
-  int64_t levelBase[kMaxDepth + 2];
   int32_t nrx;
   RxGrid g;
   unsigned long long tabMask;
@@ -320,9 +380,15 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
   return x;
 }
 
+__device__ __forceinline__ unsigned long long order_key(const TNode& nd) {
+  return ((unsigned long long)(uint32_t)nd.gl << 32) | ((unsigned long long)nd.depth << 27) |
+         (unsigned long long)(nd.code & ((1u << 27) - 1));
+}
+
 // Counters: [0] capture records, [1] corrected, [2] table slots used, [3] flags (1 overflow,
 // 2 hash collision).
-__global__ void __launch_bounds__(256) k_capture(CaptureArgs A, TraceConsts K, const TNode* __restrict__ tree,
+__global__ void __launch_bounds__(256) k_capture(CaptureArgs A, const TxDesc* __restrict__ txs,
+                                                 const TNode* __restrict__ tree, const DTri* __restrict__ tris,
                                                  const DSurf* __restrict__ surf, const double* __restrict__ rx,
                                                  const int32_t* __restrict__ cellStart,
                                                  const int32_t* __restrict__ cellRx, const uint8_t* __restrict__ los,
@@ -337,22 +403,24 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, TraceConsts K, c
   unsigned long long myRec = 0, myCor = 0, myUsed = 0;
   if (n < A.nodes) {
     const TNode nd = tree[n];
+    const TxDesc& T = txs[nd.txi];
     const int depth = nd.depth;
-    const V3 S = ld3(nd.src), T = ld3(nd.tgt);
-    const V3 D = vsub(T, S);
+    const int model = T.receptionModel;
+    const double rc = T.rcoef;
+    const V3 S = ld3(nd.src), Tg = ld3(nd.tgt);
+    const V3 D = vsub(Tg, S);
     // Conservative search radius: a capture at C has its closest point P within
     // (base + |C - ref|) * rc / (1 - rc) of C, with base/ref per reception model.
-    const double rc = K.rcoef;
     double far2 = 0.0;
     for (int a = 0; a < 3; a++) {
-      double sa = vget(K.receptionModel == 0 ? v3(0, 0, 0) : S, a);
+      double sa = model == 0 ? 0.0 : vget(S, a);
       double e = fmax(fabs(A.g.bmin[a] - sa), fabs(A.g.bmax[a] - sa));
       far2 += e * e;
     }
-    double base = K.receptionModel == 0 ? nd.plArg : nd.trueLen;
+    const double base = model == 0 ? nd.plArg : nd.trueLen;
     double R = (base + sqrt(far2)) * rc / (1.0 - rc);
     R = R * (1.0 + 1e-6) + 1e-6;
-    bool all = !(R < 1e300) || rc >= 1.0;
+    const bool all = !(R < 1e300) || rc >= 1.0;
     // Clip the segment to the receivers' box grown by R.
     double u0 = 0.0, u1 = 1.0;
     bool any = true;
@@ -379,11 +447,11 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, TraceConsts K, c
       }
       V3 img[kMaxDepth + 1];
       int32_t isurf[kMaxDepth + 1];
-      unsigned long long seqA = 0x243f6a8885a308d3ull ^ (unsigned long long)depth;
-      unsigned long long seqB = 0x13198a2e03707344ull + (unsigned long long)depth;
+      unsigned long long seqA = 0x243f6a8885a308d3ull ^ ((unsigned long long)depth << 48) ^ (unsigned long long)nd.txi;
+      unsigned long long seqB = 0x13198a2e03707344ull + (unsigned long long)depth * 0x51ull + ((unsigned long long)nd.txi << 20);
       bool chainOk = true;
       {
-        V3 cur = v3(K.tx[0], K.tx[1], K.tx[2]);
+        V3 cur = v3(T.tx[0], T.tx[1], T.tx[2]);
         img[0] = cur;
         isurf[0] = -1;
         for (int k = 0; k < depth; k++) {
@@ -398,9 +466,8 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, TraceConsts K, c
           isurf[k + 1] = a.tri;
         }
       }
-      const unsigned long long orderKey = ((unsigned long long)nd.launch << 38) |
-                                          ((unsigned long long)depth << 33) |
-                                          (unsigned long long)(n - A.levelBase[depth]);
+      const unsigned long long orderKey = order_key(nd);
+      const int64_t losBase = (int64_t)nd.txi * A.nrx;
       // Candidate receivers: grid rows crossed by the clipped segment's xy band.
       V3 P0 = vadd(S, vmul(D, u0)), P1 = vadd(S, vmul(D, u1));
       double ylo = fmin(P0.y, P1.y) - R, yhi = fmax(P0.y, P1.y) + R;
@@ -430,22 +497,42 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, TraceConsts K, c
           i1 = (int)fmin((double)(A.g.nx - 1), floor((xb + R - A.g.gx0) / A.g.cell));
         }
         for (int ci = i0; ci <= i1; ci++) {
-          int cell = j * A.g.nx + ci;
+          const int cell = j * A.g.nx + ci;
           for (int q = cellStart[cell]; q < cellStart[cell + 1]; q++) {
-            int32_t r = cellRx[q];
-            V3 C = ld3(rx + 3 * (int64_t)r);
-            if (!sphere_hit(S, T, C, nd.plArg, nd.trueLen, K.receptionModel, rc)) continue;
+            const int32_t r = cellRx[q];
+            const V3 C = ld3(rx + 3 * (int64_t)r);
+            if (!sphere_hit(S, Tg, C, nd.plArg, nd.trueLen, model, rc)) continue;
             bool blocked = false;
             for (int k = depth - 1; k >= 0 && !blocked; k--) {
               const TNode& a = tree[chain[k]];
-              blocked = sphere_hit(ld3(a.src), ld3(a.tgt), C, a.plArg, a.trueLen, K.receptionModel, rc);
+              blocked = sphere_hit(ld3(a.src), ld3(a.tgt), C, a.plArg, a.trueLen, model, rc);
             }
             if (blocked) continue;
             myRec++;
+            unsigned long long hA = mix64(seqA ^ ((unsigned long long)(uint32_t)r * 0x9e3779b97f4a7c15ull));
+            unsigned long long hB = mix64(seqB + ((unsigned long long)(uint32_t)r << 7) + 0x632be59bd9b4e019ull);
+            if (hA == 0) hA = 1;
+            if (hB == 0) hB = 1;
+            // A record that cannot beat the key already stored for its group needs no correction:
+            // the stored key belongs to a corrected path and keys only decrease.
+            {
+              unsigned long long slot = hA & A.tabMask;
+              bool pruned = false;
+              for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
+                const unsigned long long a = __atomic_load_n(&tabA[slot], __ATOMIC_RELAXED);
+                if (a == 0ull) break;
+                if (a == hA) {
+                  pruned = __atomic_load_n(&tabOrder[slot], __ATOMIC_RELAXED) < orderKey;
+                  break;
+                }
+                slot = (slot + 1) & A.tabMask;
+              }
+              if (pruned) continue;
+            }
             // Image correction (MultiPathCorrection / DirectPathCorrection).
             bool ok;
             if (depth == 0) {
-              ok = los[r] != 0;
+              ok = los[losBase + r] != 0;
             } else if (!chainOk) {
               ok = false;
             } else {
@@ -461,16 +548,12 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, TraceConsts K, c
             }
             if (!ok) continue;
             myCor++;
-            unsigned long long hA = mix64(seqA ^ ((unsigned long long)(uint32_t)r * 0x9e3779b97f4a7c15ull));
-            unsigned long long hB = mix64(seqB + ((unsigned long long)(uint32_t)r << 7) + 0x632be59bd9b4e019ull);
-            if (hA == 0) hA = 1;
-            if (hB == 0) hB = 1;
             unsigned long long slot = hA & A.tabMask;
             bool placed = false;
             for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
               unsigned long long prev = atomicCAS(&tabA[slot], 0ull, hA);
               if (prev == 0ull || prev == hA) {
-                if (prev == 0ull) { tabRx[slot] = r; myUsed++; }
+                if (prev == 0ull) { tabRx[slot] = (int32_t)(losBase + r); myUsed++; }
                 unsigned long long pb = atomicCAS(&tabB[slot], 0ull, hB);
                 if (pb != 0ull && pb != hB) atomicOr(&counters[3], 2ull);
                 atomicMin(&tabOrder[slot], orderKey);
@@ -508,21 +591,36 @@ __global__ void k_collect(const unsigned long long* __restrict__ tabA, const uns
   winRx[w] = tabRx[s];
 }
 
+// Maps every kept path's order key (sorted) to its tree node.
+__global__ void k_resolve(const TNode* __restrict__ tree, int64_t nodes, const unsigned long long* __restrict__ keys,
+                          int64_t W, int32_t* __restrict__ winNode) {
+  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nodes) return;
+  const unsigned long long key = order_key(tree[n]);
+  int64_t lo = 0, hi = W;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  for (int64_t j = lo; j < W && keys[j] == key; j++) winNode[j] = (int32_t)n;
+}
+
+__global__ void k_iota(int32_t* __restrict__ a, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = (int32_t)i;
+}
+
 // =============================================================================================
 // 5. Field of every kept path (FieldForPath, FieldCompute.cpp:71-151) and per-receiver sums
 // =============================================================================================
-struct FieldArgs {
-  int64_t winners;
-  int64_t levelBase[kMaxDepth + 2];
-};
-
 // Corrected vertices of the path of node n to receiver C: pts[0] = tx, pts[1..depth] the
 // re-intersections, pts[depth+1] = C.  Returns false if the correction fails.
-__device__ bool corrected_points(const TraceConsts& K, const TNode* __restrict__ tree, const DSurf* __restrict__ surf,
+__device__ bool corrected_points(V3 tx, const TNode* __restrict__ tree, const DSurf* __restrict__ surf,
                                  const int32_t* chain, int depth, V3 C, V3* pts) {
   V3 img[kMaxDepth + 1];
   int32_t isurf[kMaxDepth + 1];
-  V3 cur = v3(K.tx[0], K.tx[1], K.tx[2]);
+  V3 cur = tx;
   img[0] = cur;
   for (int k = 0; k < depth; k++) {
     const TNode& a = tree[chain[k]];
@@ -547,11 +645,6 @@ __device__ bool corrected_points(const TraceConsts& K, const TNode* __restrict__
   return true;
 }
 
-__device__ __forceinline__ int64_t node_of(const int64_t* levelBase, unsigned long long key) {
-  int depth = (int)((key >> 33) & 31ull);
-  return levelBase[depth] + (int64_t)(key & ((1ull << 33) - 1));
-}
-
 // Spherical angles of a local vector (CartesianToSpherical, VecMatDef.h:132-155).
 __device__ __forceinline__ void cart_to_sph(V3 v, double* r, double* th, double* ph) {
   *r = vnorm(v);
@@ -562,27 +655,37 @@ __device__ __forceinline__ void cart_to_sph(V3 v, double* r, double* th, double*
   *ph = p;
 }
 
-__global__ void __launch_bounds__(256) k_field(FieldArgs F, TraceConsts K, const TNode* __restrict__ tree,
-                                               const DSurf* __restrict__ surf, const double* __restrict__ mats,
-                                               const double* __restrict__ rx, const unsigned long long* __restrict__ winKey,
-                                               const int32_t* __restrict__ winRx, double* __restrict__ winField) {
+struct FieldArgs {
+  int64_t winners;
+  int32_t nrx;
+};
+
+// winTxRx / winPerm: kept paths sorted by (transmitter * nrx + receiver), each pointing into the
+// order-key-sorted arrays (winNode).
+__global__ void __launch_bounds__(256) k_field(FieldArgs F, const TxDesc* __restrict__ txs, const TNode* __restrict__ tree,
+                                               const DTri* __restrict__ tris, const DSurf* __restrict__ surf,
+                                               const double* __restrict__ mats, const double* __restrict__ rx,
+                                               const int32_t* __restrict__ winTxRx, const int32_t* __restrict__ winPerm,
+                                               const int32_t* __restrict__ winNode, double* __restrict__ winField) {
   int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= F.winners) return;
-  unsigned long long key = winKey[w];
-  int64_t n = node_of(F.levelBase, key);
-  int depth = tree[n].depth;
+  const int64_t n = winNode[winPerm[w]];
+  const TNode& leaf = tree[n];
+  const int depth = leaf.depth;
+  const TxDesc& K = txs[leaf.txi];
   int32_t chain[kMaxDepth + 1];
   {
     int64_t c = n;
     for (int k = depth; k >= 0; k--) { chain[k] = (int32_t)c; c = tree[c].parent; }
   }
-  V3 C = ld3(rx + 3 * (int64_t)winRx[w]);
+  const V3 C = ld3(rx + 3 * (int64_t)(winTxRx[w] % F.nrx));
+  const V3 txp = v3(K.tx[0], K.tx[1], K.tx[2]);
   V3 pts[kMaxDepth + 2];
   if (depth == 0) {
-    pts[0] = v3(K.tx[0], K.tx[1], K.tx[2]);
+    pts[0] = txp;
     pts[1] = C;
   } else {
-    (void)corrected_points(K, tree, surf, chain, depth, C, pts);
+    (void)corrected_points(txp, tree, surf, chain, depth, C, pts);
   }
   const double lamda = SBR_C0 / K.freq;
   const double kw = (2 * SBR_PI) / lamda;
@@ -617,7 +720,7 @@ __global__ void __launch_bounds__(256) k_field(FieldArgs F, TraceConsts K, const
     }
     if (sg.type == 1 || sg.type == 2) {
       double ang = angle_between_rad(vg, v3(cur[6], cur[7], cur[8]));
-      int mat = tree[chain[k - 1]].mat;
+      int mat = tris[tree[chain[k - 1]].hitDfs].mat;
       cx fTM, fTE;
       fresnel(sg.type == 1, ang, mats[6 * mat + 3], mats[6 * mat + 5], K.freq, K.useFresnel != 0, &fTM, &fTE);
       tot.x = inc.x;
@@ -637,45 +740,48 @@ __global__ void __launch_bounds__(256) k_field(FieldArgs F, TraceConsts K, const
   for (int c = 0; c < 3; c++) { o[2 * c] = f[c].r; o[2 * c + 1] = f[c].i; }
 }
 
-// Per receiver: sum of its kept paths' fields in (launch, BFS) order.
-__global__ void k_rx_sum(const int32_t* __restrict__ winRx, int64_t W, const double* __restrict__ winField,
-                         int32_t m, double* __restrict__ field) {
-  int32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= m) return;
+// Per (transmitter, receiver): sum of its kept paths' fields in (launch, BFS) order.
+__global__ void k_rx_sum(const int32_t* __restrict__ winTxRx, int64_t W, const double* __restrict__ winField,
+                         int64_t nOut, double* __restrict__ field) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nOut) return;
   int64_t lo = 0, hi = W;
   while (lo < hi) {
     int64_t mid = (lo + hi) >> 1;
-    if (winRx[mid] < r) lo = mid + 1;
+    if (winTxRx[mid] < r) lo = mid + 1;
     else hi = mid;
   }
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int64_t w = lo; w < W && winRx[w] == r; w++)
+  for (int64_t w = lo; w < W && winTxRx[w] == r; w++)
     for (int c = 0; c < 6; c++) acc[c] += winField[6 * w + c];
-  for (int c = 0; c < 6; c++) field[6 * (int64_t)r + c] = acc[c];
+  for (int c = 0; c < 6; c++) field[6 * r + c] = acc[c];
 }
 
 // Path export for the kept paths (one row per segment).
-__global__ void k_paths(FieldArgs F, TraceConsts K, const TNode* __restrict__ tree, const DSurf* __restrict__ surf,
-                        const double* __restrict__ rx, const unsigned long long* __restrict__ winKey,
-                        const int32_t* __restrict__ winRx, const int64_t* __restrict__ off, int32_t* __restrict__ stri,
+__global__ void k_paths(int64_t W, int32_t nrx, const TxDesc* __restrict__ txs, const TNode* __restrict__ tree,
+                        const DSurf* __restrict__ surf, const double* __restrict__ rx,
+                        const int32_t* __restrict__ winTxRx, const int32_t* __restrict__ winPerm,
+                        const int32_t* __restrict__ winNode, const int64_t* __restrict__ off, int32_t* __restrict__ stri,
                         int32_t* __restrict__ scop, int32_t* __restrict__ stype, double* __restrict__ ssrc,
                         double* __restrict__ stgt) {
   int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= F.winners) return;
-  int64_t n = node_of(F.levelBase, winKey[w]);
-  int depth = tree[n].depth;
+  if (w >= W) return;
+  const int64_t n = winNode[winPerm[w]];
+  const int depth = tree[n].depth;
+  const TxDesc& K = txs[tree[n].txi];
   int32_t chain[kMaxDepth + 1];
   int64_t c = n;
   for (int k = depth; k >= 0; k--) { chain[k] = (int32_t)c; c = tree[c].parent; }
-  V3 C = ld3(rx + 3 * (int64_t)winRx[w]);
+  const V3 C = ld3(rx + 3 * (int64_t)(winTxRx[w] % nrx));
+  const V3 txp = v3(K.tx[0], K.tx[1], K.tx[2]);
   V3 pts[kMaxDepth + 2];
   if (depth == 0) {
-    pts[0] = v3(K.tx[0], K.tx[1], K.tx[2]);
+    pts[0] = txp;
     pts[1] = C;
   } else {
-    (void)corrected_points(K, tree, surf, chain, depth, C, pts);
+    (void)corrected_points(txp, tree, surf, chain, depth, C, pts);
   }
-  int64_t o = off[w];
+  const int64_t o = off[w];
   for (int k = 0; k <= depth; k++) {
     const TNode& sg = tree[chain[k]];
     stri[o + k] = k == depth ? -1 : sg.tri;
@@ -686,23 +792,11 @@ __global__ void k_paths(FieldArgs F, TraceConsts K, const TNode* __restrict__ tr
   }
 }
 
-template <bool EXACT>
-__global__ void k_closest_batch(const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
-                                const double* __restrict__ org, const double* __restrict__ dirs, int32_t n,
-                                int32_t* __restrict__ triOut, double* __restrict__ ptOut, double* __restrict__ distOut) {
-  int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  Hit h;
-  bool hh = closest_hit<EXACT>(nodes, tris, ld3(org + 3 * (int64_t)i), ld3(dirs + 3 * (int64_t)i), h);
-  triOut[i] = hh ? tris[h.idx].id : -1;
-  ptOut[3 * i] = h.p.x; ptOut[3 * i + 1] = h.p.y; ptOut[3 * i + 2] = h.p.z;
-  distOut[i] = h.dist;
-}
-
 // =============================================================================================
 // Host orchestration
 // =============================================================================================
 Engine::~Engine() {
+  for (auto& kv : rayCache) delete kv.second;
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -717,6 +811,9 @@ int Engine::init(int dev) {
   device = dev;
   HIPCHK(hipSetDevice(dev));
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  numCU = prop.multiProcessorCount;
   return SBR_OK;
 }
 
@@ -755,16 +852,14 @@ static void ico_base(std::vector<double>& v, std::vector<int32_t>& f, std::vecto
     }
 }
 
-int Engine::generate_rays(int tess, int64_t* nOut) {
+int Engine::generate_rays(int tess, DBuf<double>& out, int64_t* nOut) {
   if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
-  if (tess < 0 || tess > 12) return fail(SBR_E_ARG, "tessellation must be in [0, 12]");
+  if (tess < 0 || tess > 11) return fail(SBR_E_ARG, "tessellation must be in [0, 11]");
   const int64_t nvFinal = 10 * (int64_t(1) << (2 * tess)) + 2;
   std::vector<double> v0;
   std::vector<int32_t> f0, a0;
   ico_base(v0, f0, a0);
-  DBuf<double> verts;
-  verts.ensure(3 * (size_t)nvFinal);
-  if (!verts.p) return fail(SBR_E_NOMEM, "icosahedron vertices");
+  ENSURE(dVerts, 3 * (size_t)nvFinal);
   const int64_t nfMax = 20 * (int64_t(1) << (2 * std::max(0, tess - 1)));
   ENSURE(dFaces[0], 3 * nfMax);
   ENSURE(dAdj[0], 6 * nfMax);
@@ -774,7 +869,7 @@ int Engine::generate_rays(int tess, int64_t* nOut) {
   }
   ENSURE(dFlag, 3 * nfMax);
   ENSURE(dScan, 3 * nfMax);
-  HIPCHK(hipMemcpyAsync(verts.p, v0.data(), v0.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dVerts.p, v0.data(), v0.size() * sizeof(double), hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(dFaces[0].p, f0.data(), f0.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(dAdj[0].p, a0.data(), a0.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
   int cur = 0;
@@ -788,7 +883,7 @@ int Engine::generate_rays(int tess, int64_t* nOut) {
     HIPCHK(hipcub::DeviceScan::ExclusiveSum(dScratch.p, tmpBytes, dFlag.p, dScan.p, (int)ns, stream));
     const bool last = l == tess - 1;
     k_ico_split<<<blocks_for(nf, 256), 256, 0, stream>>>(dFaces[cur].p, dAdj[cur].p, dFlag.p, dScan.p,
-                                                         (int32_t)nf, nv, verts.p, last ? nullptr : dFaces[1 - cur].p,
+                                                         (int32_t)nf, nv, dVerts.p, last ? nullptr : dFaces[1 - cur].p,
                                                          last ? nullptr : dAdj[1 - cur].p, last ? 0 : 1);
     HIPCHK(hipGetLastError());
     nv = nv + 3 * nf / 2;   // each of the 3F/2 edges adds one vertex
@@ -796,13 +891,32 @@ int Engine::generate_rays(int tess, int64_t* nOut) {
     cur = 1 - cur;
   }
   if (nv != nvFinal) return fail(SBR_E_INTERNAL, "icosahedron vertex count mismatch");
-  ENSURE(dRays, 3 * nv);
-  k_ico_dirs<<<blocks_for(nv, 256), 256, 0, stream>>>(verts.p, nv, dRays.p);
+  ENSURE(out, 3 * nv);
+  k_ico_dirs<<<blocks_for(nv, 256), 256, 0, stream>>>(dVerts.p, nv, out.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(stream));
   *nOut = nv;
   return SBR_OK;
 }
+
+int Engine::rays_for(int tess, const double** out, int64_t* nOut) {
+  auto it = rayCache.find(tess);
+  if (it == rayCache.end()) {
+    DBuf<double>* b = new DBuf<double>();
+    int64_t n = 0;
+    int rc = generate_rays(tess, *b, &n);
+    if (rc) {
+      delete b;
+      return rc;
+    }
+    it = rayCache.emplace(tess, b).first;
+  }
+  *out = it->second->p;
+  *nOut = 10 * (int64_t(1) << (2 * tess)) + 2;
+  return SBR_OK;
+}
+
+static size_t stack_bytes(int height, int block) { return (size_t)height * block * 8; }
 
 int Engine::closest_hit(const double* o, const double* d, int n, int exact, int32_t* tri, double* pt, double* dist) {
   if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
@@ -815,10 +929,13 @@ int Engine::closest_hit(const double* o, const double* d, int n, int exact, int3
   if (!dO.p || !dD.p || !dP.p || !dDist.p || !dT.p) return fail(SBR_E_NOMEM, "closest_hit buffers");
   HIPCHK(hipMemcpyAsync(dO.p, o, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(dD.p, d, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
+  const size_t sm = stack_bytes(treeHeight, kBlock);
   if (exact)
-    k_closest_batch<true><<<blocks_for(n, 128), 128, 0, stream>>>(dNodes.p, dTris.p, dO.p, dD.p, n, dT.p, dP.p, dDist.p);
+    k_closest_batch<true><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dTris.p, dO.p, dD.p, n, treeHeight,
+                                                                         dT.p, dP.p, dDist.p);
   else
-    k_closest_batch<false><<<blocks_for(n, 128), 128, 0, stream>>>(dNodes.p, dTris.p, dO.p, dD.p, n, dT.p, dP.p, dDist.p);
+    k_closest_batch<false><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dTris.p, dO.p, dD.p, n, treeHeight,
+                                                                          dT.p, dP.p, dDist.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(tri, dT.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(pt, dP.p, 3 * sizeof(double) * n, hipMemcpyDeviceToHost, stream));
@@ -831,154 +948,175 @@ static double ms_since(std::chrono::steady_clock::time_point t) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
 
-int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
+// Bounce loop: one persistent k_trace launch per depth, no host synchronisation in between.  Returns
+// SBR_E_LIMIT (with nodeCap grown) when the node or queue buffers overflowed: run it again.
+int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t nRoot, int maxDepth, bool exact, bool count) {
+  bool lds = sceneBytes + stack_bytes(treeHeight, kBlock) <= kSceneLdsMax;
+  if (tune.ldsScene == 0) lds = false;
+  const size_t smem = (lds ? sceneBytes : 0) + stack_bytes(treeHeight, kBlock);
+  TraceArgs A{};
+  A.nTx = (int32_t)txs.size();
+  A.stackDepth = treeHeight;
+  A.nodeBytes = (int32_t)(hNodes.size() * sizeof(DNode));
+  A.triBytes = (int32_t)(hTris.size() * sizeof(DTri));
+  A.nodeCap = nodeCap;
+  A.queueCap = nodeCap;
+  auto kern = exact ? (lds ? k_trace<true, true> : k_trace<true, false>)
+                    : (lds ? k_trace<false, true> : k_trace<false, false>);
+  int perCU = 1;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, kBlock, smem));
+  perCU = std::max(1, perCU);
+  if (tune.blocksPerCU > 0) perCU = std::min(perCU, tune.blocksPerCU);
+  const unsigned grid = (unsigned)std::min<int64_t>((int64_t)numCU * perCU, blocks_for(std::max<int64_t>(nRoot, 1), kBlock));
+  stats.trace_lds = lds ? 1 : 0;
+  stats.trace_blocks_per_cu = perCU;
+  std::vector<LevelRec> lv0(kMaxDepth + 2, LevelRec{0, 0});
+  lv0[0].count = (unsigned long long)nRoot;
+  HIPCHK(hipMemcpyAsync(dLevels.p, lv0.data(), sizeof(LevelRec) * lv0.size(), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemsetAsync(dCounters.p, 0, 8 * sizeof(unsigned long long), stream));
+  hipEvent_t kev[2];
+  HIPCHK(hipEventCreate(&kev[0]));
+  HIPCHK(hipEventCreate(&kev[1]));
+  struct KevGuard { hipEvent_t* e; ~KevGuard() { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); } } kguard{kev};
+  HIPCHK(hipEventRecord(kev[0], stream));
+  for (int d = 0; d <= maxDepth; d++) {
+    A.depth = d;
+    const int32_t* qin = dQueue[d & 1].p;
+    int32_t* qout = dQueue[(d + 1) & 1].p;
+    kern<<<grid, kBlock, smem, stream>>>(A, dTx.p, dNodes.p, dTris.p, dTriN.p, qin, qout, dTree.p, dLevels.p,
+                                         dCounters.p, count ? 1 : 0);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(kev[1], stream));
+  std::vector<LevelRec> lv(kMaxDepth + 2);
+  unsigned long long ctr[8];
+  HIPCHK(hipMemcpyAsync(lv.data(), dLevels.p, sizeof(LevelRec) * lv.size(), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(ctr, dCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  float kms = 0;
+  HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
+  stats.ms_k_trace = kms;
+  stats.n_k_trace = maxDepth + 1;
+  unsigned long long total = 0;
+  int deepest = 0;
+  for (int d = 0; d <= maxDepth; d++) {
+    total += lv[d].count;
+    if (lv[d].count) deepest = d;
+  }
+  if ((ctr[0] & 3ull) || total > nodeCap) {
+    nodeCap = std::max<size_t>(nodeCap * 2, (size_t)(total + total / 4 + 1024));
+    return SBR_E_LIMIT;
+  }
+  nodes = (int64_t)total;
+  stats.bounces = nodes;
+  stats.max_depth = deepest;
+  stats.node_visits = count ? (int64_t)ctr[6] : 0;
+  stats.tri_tests = count ? (int64_t)ctr[7] : 0;
+  return SBR_OK;
+}
+
+int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
   if (!haveScene) return fail(SBR_E_STATE, "no scene uploaded");
   if (mats.empty()) return fail(SBR_E_STATE, "no materials set");
-  if (!p || m < 0 || (m > 0 && !rx)) return fail(SBR_E_ARG, "trace: bad arguments");
-  if (p->maxR < 0 || p->maxT < 0) return fail(SBR_E_ARG, "maxR / maxT must be >= 0");
-  int maxDepth = p->maxR + p->maxT;
-  if (p->maxTotal > 0) maxDepth = std::min(maxDepth, p->maxTotal);
-  if (maxDepth > kMaxDepth) return fail(SBR_E_LIMIT, "tree depth above the device limit");
+  if (!P || nt <= 0 || nt > 32767 || m < 0 || (m > 0 && !rx)) return fail(SBR_E_ARG, "trace: bad arguments");
+  if ((int64_t)nt * m >= (int64_t(1) << 31)) return fail(SBR_E_LIMIT, "transmitters x receivers above 2^31");
   for (int32_t t = 0; t < ntri; t++)
     if (hMat[t] < 0 || hMat[t] * 6 >= (int)mats.size()) return fail(SBR_E_ARG, "triangle material id without a material");
-  const int stride = p->launchStride > 0 ? p->launchStride : 1;
-  if (p->launchOffset < 0 || p->launchOffset >= stride) return fail(SBR_E_ARG, "launchOffset must be in [0, stride)");
-  if (p->antennaType < 0 || p->antennaType > 2) return fail(SBR_E_ARG, "antennaType must be 0, 1 or 2");
+  for (int t = 1; t < nt; t++)
+    if ((P[t].q1MeshIndexing != 0) != (P[0].q1MeshIndexing != 0))
+      return fail(SBR_E_ARG, "q1MeshIndexing must be the same for every transmitter of a batch");
   HIPCHK(hipSetDevice(device));
   haveResult = false;
-  lastParams = *p;
-  nrx = m;
-  hRx.assign(rx, rx + 3 * (size_t)m);
   stats = sbr_stats{};
-
-  TraceConsts K{};
-  memcpy(K.tx, p->tx, sizeof K.tx);
-  memcpy(K.txcs, p->txCoordSys, sizeof K.txcs);
-  K.freq = p->frequency;
-  K.pt = p->pt;
-  K.maxR = p->maxR;
-  K.maxT = p->maxT;
-  K.maxTotal = p->maxTotal;
-  K.useFresnel = p->useFresnel;
-  K.antennaType = p->antennaType;
-  K.receptionModel = p->receptionModel;
-  K.exact = p->exactTraversal;
-  if (p->receptionModel == 0) {
-    // RTSolver.cpp:839, 841 with txTesslation = tess + 1 (RTSolver.cpp:70)
-    double beta = 1.0 / (3.0 * (p->tess + 1)) * std::acos(-1.0 / std::sqrt(5.0));
-    K.rcoef = std::tan(beta);
-  } else if (p->receptionModel == 1) {
-    double alpha = (69.0 * 3.141592653589793 / 180.0) / std::ldexp(1.0, p->tess);
-    K.rcoef = alpha / std::sqrt(3.0);
-  } else {
-    return fail(SBR_E_ARG, "receptionModel must be 0 or 1");
-  }
-
-  hipEvent_t ev[6];
-  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-  struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 6; i++) (void)hipEventDestroy(e[i]); } } guard{ev};
   auto t0 = std::chrono::steady_clock::now();
+  hipEvent_t ev[5];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 5; i++) (void)hipEventDestroy(e[i]); } } guard{ev};
   HIPCHK(hipEventRecord(ev[0], stream));
 
-  // ---- 1. launch directions
-  int64_t nl = 0;
-  {
-    int rcode = generate_rays(p->tess, &nl);
-    if (rcode) return rcode;
-  }
-  HIPCHK(hipEventRecord(ev[1], stream));
-  if (nl >= (int64_t(1) << 26)) return fail(SBR_E_LIMIT, "too many launches for the order key");
-  int64_t nsel = nl > p->launchOffset ? (nl - p->launchOffset + stride - 1) / stride : 0;
-  stats.launches = nsel;
-
-  // ---- 2. interaction trees, depth by depth
-  levelBase.assign(kMaxDepth + 2, 0);
-  ENSURE(dCounters, 8);
-  HIPCHK(hipMemsetAsync(dCounters.p + 6, 0, 2 * sizeof(unsigned long long), stream));
-  hipEvent_t kev[2];
-  for (auto& e : kev) HIPCHK(hipEventCreate(&e));
-  struct KevGuard { hipEvent_t* e; ~KevGuard() { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); } } kguard{kev};
-  int64_t total = 0;
-  int64_t n = nsel;
-  if (grow_keep(dTree, std::max<int64_t>(n * 2, 1), 0, stream)) return fail(SBR_E_NOMEM, "tree buffer");
-  ENSURE(dQueue, std::max<int64_t>(n, 1));
-  int depth = 0;
-  int maxDepthSeen = 0;
-  while (n > 0) {
-    if (n >= (int64_t(1) << 31) / 2) return fail(SBR_E_LIMIT, "tree level above 2^30 nodes");
-    levelBase[depth] = total;
-    if (total + n >= (int64_t(1) << 30)) return fail(SBR_E_LIMIT, "more than 2^30 tree nodes");
-    if (grow_keep(dTree, (size_t)(total + n), (size_t)total, stream)) return fail(SBR_E_NOMEM, "tree buffer");
-    ENSURE(dChild, n);
-    ENSURE(dChildScan, n + 1);
-    LevelArgs LA{depth, (int32_t)n, total, stride, p->launchOffset};
-    unsigned long long* tc = p->countTraversal ? dCounters.p + 6 : nullptr;
-    HIPCHK(hipEventRecord(kev[0], stream));
-    if (p->exactTraversal)
-      k_trace_level<true><<<blocks_for(n, 256), 256, 0, stream>>>(LA, K, dNodes.p, dTris.p, dTriN.p, dRays.p, dQueue.p,
-                                                                   dTree.p, dChild.p, tc);
-    else
-      k_trace_level<false><<<blocks_for(n, 256), 256, 0, stream>>>(LA, K, dNodes.p, dTris.p, dTriN.p, dRays.p,
-                                                                    dQueue.p, dTree.p, dChild.p, tc);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(kev[1], stream));
-    HIPCHK(hipEventSynchronize(kev[1]));
-    {
-      float kms = 0;
-      HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
-      stats.ms_k_trace += kms;
-      stats.n_k_trace++;
+  // ---- transmitters and their launch directions
+  std::vector<TxDesc> txs(nt);
+  int maxDepth = 0;
+  bool exact = false, count = false;
+  int64_t nRoot = 0;
+  for (int t = 0; t < nt; t++) {
+    const sbr_trace_params& p = P[t];
+    if (p.maxR < 0 || p.maxT < 0) return fail(SBR_E_ARG, "maxR / maxT must be >= 0");
+    int md = p.maxR + p.maxT;
+    if (p.maxTotal > 0) md = std::min(md, p.maxTotal);
+    if (md > kMaxDepth) return fail(SBR_E_LIMIT, "tree depth above the device limit");
+    maxDepth = std::max(maxDepth, md);
+    const int stride = p.launchStride > 0 ? p.launchStride : 1;
+    if (p.launchOffset < 0 || p.launchOffset >= stride) return fail(SBR_E_ARG, "launchOffset must be in [0, stride)");
+    if (p.antennaType < 0 || p.antennaType > 2) return fail(SBR_E_ARG, "antennaType must be 0, 1 or 2");
+    if (p.receptionModel < 0 || p.receptionModel > 1) return fail(SBR_E_ARG, "receptionModel must be 0 or 1");
+    if (!(p.frequency > 0)) return fail(SBR_E_ARG, "frequency must be > 0");
+    exact = exact || p.exactTraversal;
+    count = count || p.countTraversal;
+    TxDesc& T = txs[t];
+    memset(&T, 0, sizeof T);
+    memcpy(T.tx, p.tx, sizeof T.tx);
+    memcpy(T.txcs, p.txCoordSys, sizeof T.txcs);
+    T.freq = p.frequency;
+    T.pt = p.pt;
+    if (p.receptionModel == 0) {
+      // RTSolver.cpp:839, 841 with txTesslation = tess + 1 (RTSolver.cpp:70)
+      double beta = 1.0 / (3.0 * (p.tess + 1)) * std::acos(-1.0 / std::sqrt(5.0));
+      T.rcoef = std::tan(beta);
+    } else {
+      double alpha = (69.0 * 3.141592653589793 / 180.0) / std::ldexp(1.0, p.tess);
+      T.rcoef = alpha / std::sqrt(3.0);
     }
-    total += n;
-    maxDepthSeen = depth;
-    if (depth >= maxDepth) break;
-    // child counts -> stable exclusive scan -> next queue
-    DBuf<int32_t>& cnt = dChildScan;
-    ENSURE(dFlag, n + 1);
-    k_child_count<<<blocks_for(n, 256), 256, 0, stream>>>(dChild.p, (int32_t)n, dFlag.p);
-    size_t tmpBytes = 0;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, dFlag.p, cnt.p, (int)n, stream));
-    ENSURE(dScratch, tmpBytes);
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(dScratch.p, tmpBytes, dFlag.p, cnt.p, (int)n, stream));
-    int32_t lastOff = 0, lastCnt = 0;
-    HIPCHK(hipMemcpyAsync(&lastOff, cnt.p + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(&lastCnt, dFlag.p + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    int64_t nn = (int64_t)lastOff + lastCnt;
-    if (nn == 0) break;
-    DBuf<int32_t> nextQ;
-    nextQ.ensure(nn);
-    if (!nextQ.p) return fail(SBR_E_NOMEM, "queue");
-    k_emit_level<<<blocks_for(n, 256), 256, 0, stream>>>(dChild.p, cnt.p, (int32_t)n, levelBase[depth], nextQ.p);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(stream));
-    std::swap(dQueue.p, nextQ.p);
-    std::swap(dQueue.n, nextQ.n);
-    n = nn;
-    depth++;
+    int64_t nl = 0;
+    int rc = rays_for(p.tess, &T.rays, &nl);
+    if (rc) return rc;
+    T.launchStride = stride;
+    T.launchOffset = p.launchOffset;
+    T.nLaunch = nl > p.launchOffset ? (nl - p.launchOffset + stride - 1) / stride : 0;
+    T.launchStart = nRoot;
+    nRoot += T.nLaunch;
+    T.maxR = p.maxR;
+    T.maxT = p.maxT;
+    T.maxTotal = p.maxTotal;
+    T.useFresnel = p.useFresnel;
+    T.antennaType = p.antennaType;
+    T.receptionModel = p.receptionModel;
   }
-  levelBase[maxDepthSeen + 1] = total;
-  nodes = total;
-  if (p->countTraversal) {
-    unsigned long long tcnt[2];
-    HIPCHK(hipMemcpyAsync(tcnt, dCounters.p + 6, sizeof tcnt, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    stats.node_visits = (int64_t)tcnt[0];
-    stats.tri_tests = (int64_t)tcnt[1];
+  if (nRoot >= (int64_t(1) << 31)) return fail(SBR_E_LIMIT, "more than 2^31 launches in one trace");
+  ENSURE(dTx, nt);
+  HIPCHK(hipMemcpyAsync(dTx.p, txs.data(), sizeof(TxDesc) * nt, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipEventRecord(ev[1], stream));
+  stats.launches = nRoot;
+
+  // ---- interaction trees
+  ENSURE(dLevels, kMaxDepth + 2);
+  ENSURE(dCounters, 8);
+  if (nodeCap < (size_t)nRoot * 4 + 1024) nodeCap = (size_t)nRoot * 4 + 1024;
+  for (int attempt = 0;; attempt++) {
+    if (nodeCap >= (size_t(1) << 30)) return fail(SBR_E_LIMIT, "more than 2^30 tree nodes");
+    ENSURE(dTree, nodeCap);
+    ENSURE(dQueue[0], nodeCap);
+    ENSURE(dQueue[1], nodeCap);
+    nodeCap = std::min(dTree.n, std::min(dQueue[0].n, dQueue[1].n));
+    int rc = trace_stage(txs, nRoot, maxDepth, exact, count);
+    if (rc == SBR_OK) break;
+    if (rc != SBR_E_LIMIT || attempt > 8) return rc == SBR_E_LIMIT ? fail(rc, "tree buffers could not be sized") : rc;
   }
-  stats.bounces = total;
-  stats.max_depth = maxDepthSeen;
   HIPCHK(hipEventRecord(ev[2], stream));
 
-  // ---- 3/4. receivers: grid, line of sight, capture + correction + dedup
+  // ---- receivers: grid, line of sight, capture + correction + dedup
   winners = 0;
+  nrx = m;
+  ntx = nt;
+  const int64_t nOut = (int64_t)nt * m;
   if (m > 0) {
     // Uniform xy grid with about one receiver per cell.
     RxGrid g{};
     for (int a = 0; a < 3; a++) { g.bmin[a] = INFINITY; g.bmax[a] = -INFINITY; }
     for (int r = 0; r < m; r++)
       for (int a = 0; a < 3; a++) {
+        if (!std::isfinite(rx[3 * r + a])) return fail(SBR_E_ARG, "receiver coordinates must be finite");
         g.bmin[a] = std::min(g.bmin[a], rx[3 * r + a]);
         g.bmax[a] = std::max(g.bmax[a], rx[3 * r + a]);
       }
@@ -1005,21 +1143,29 @@ int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
     ENSURE(dRx, 3 * (size_t)m);
     ENSURE(dCellStart, cellCount.size());
     ENSURE(dCellRx, m);
-    ENSURE(dLos, m);
+    ENSURE(dLos, nOut);
     HIPCHK(hipMemcpyAsync(dRx.p, rx, 3 * sizeof(double) * m, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(dCellStart.p, cellCount.data(), sizeof(int32_t) * cellCount.size(), hipMemcpyHostToDevice,
                           stream));
     HIPCHK(hipMemcpyAsync(dCellRx.p, cellRx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, stream));
-    if (p->exactTraversal)
-      k_los<true><<<blocks_for(m, 128), 128, 0, stream>>>(K, dNodes.p, dTris.p, dRx.p, m, dLos.p);
+    const size_t sm = stack_bytes(treeHeight, kBlock);
+    if (exact)
+      k_los<true><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dTris.p, dRx.p, m, treeHeight,
+                                                                     dLos.p);
     else
-      k_los<false><<<blocks_for(m, 128), 128, 0, stream>>>(K, dNodes.p, dTris.p, dRx.p, m, dLos.p);
+      k_los<false><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dTris.p, dRx.p, m, treeHeight,
+                                                                      dLos.p);
     HIPCHK(hipGetLastError());
 
-    const DSurf* surf = p->q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
+    const DSurf* surf = P[0].q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
     size_t cap = 1;
-    while (cap < (size_t)std::max<int64_t>(1 << 16, 8 * (int64_t)m * 16)) cap <<= 1;
-    for (int attempt = 0; attempt < 6; attempt++) {
+    while (cap < (size_t)std::max<int64_t>(1 << 16, 64 * nOut)) cap <<= 1;
+    hipEvent_t kev[2];
+    HIPCHK(hipEventCreate(&kev[0]));
+    HIPCHK(hipEventCreate(&kev[1]));
+    struct KevGuard { hipEvent_t* e; ~KevGuard() { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); } } kguard{kev};
+    bool sized = false;
+    for (int attempt = 0; attempt < 6 && !sized; attempt++) {
       ENSURE(dTabA, cap);
       ENSURE(dTabB, cap);
       ENSURE(dTabOrder, cap);
@@ -1029,25 +1175,22 @@ int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
       HIPCHK(hipMemsetAsync(dTabOrder.p, 0xFF, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dCounters.p, 0, 6 * sizeof(unsigned long long), stream));
       CaptureArgs CA{};
-      CA.nodes = total;
-      for (int k = 0; k < kMaxDepth + 2; k++) CA.levelBase[k] = levelBase[k];
+      CA.nodes = nodes;
       CA.nrx = m;
       CA.g = g;
       CA.tabMask = cap - 1;
       HIPCHK(hipEventRecord(kev[0], stream));
-      k_capture<<<blocks_for(total, 256), 256, 0, stream>>>(CA, K, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p,
-                                                             dLos.p, dTabA.p, dTabB.p, dTabOrder.p, dTabRx.p,
+      k_capture<<<blocks_for(nodes, 256), 256, 0, stream>>>(CA, dTx.p, dTree.p, dTris.p, surf, dRx.p, dCellStart.p,
+                                                             dCellRx.p, dLos.p, dTabA.p, dTabB.p, dTabOrder.p, dTabRx.p,
                                                              dCounters.p);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(kev[1], stream));
       unsigned long long ctr[4];
       HIPCHK(hipMemcpyAsync(ctr, dCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
-      {
-        float kms = 0;
-        HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
-        stats.ms_k_capture = kms;
-      }
+      float kms = 0;
+      HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
+      stats.ms_k_capture = kms;
       if (ctr[3] & 2ull) return fail(SBR_E_INTERNAL, "dedup hash collision detected");
       if ((ctr[3] & 1ull) || ctr[2] * 10 > cap * 7) {
         cap <<= 2;
@@ -1056,41 +1199,50 @@ int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
       stats.capture_records = (int64_t)ctr[0];
       stats.corrected = (int64_t)ctr[1];
       winners = (int64_t)ctr[2];
-      break;
+      sized = true;
     }
-    if (winners == 0 && stats.corrected > 0) return fail(SBR_E_LIMIT, "dedup table could not be sized");
+    if (!sized) return fail(SBR_E_LIMIT, "dedup table could not be sized");
     HIPCHK(hipEventRecord(ev[3], stream));
 
-    // ---- 5. kept paths: collect, sort by (receiver, order key), fields, per-receiver sums
-    ENSURE(dWinKey, std::max<int64_t>(winners, 1));
-    ENSURE(dWinKey2, std::max<int64_t>(winners, 1));
-    ENSURE(dWinRx, std::max<int64_t>(winners, 1));
-    ENSURE(dWinRx2, std::max<int64_t>(winners, 1));
-    ENSURE(dWinField, 6 * std::max<int64_t>(winners, 1));
-    ENSURE(dField, 6 * (size_t)m);
+    // ---- kept paths: collect, sort by order key, resolve nodes, sort by (tx, rx), fields, sums
+    const int64_t Wc = std::max<int64_t>(winners, 1);
+    ENSURE(dWinKey, Wc);
+    ENSURE(dWinKey2, Wc);
+    ENSURE(dWinRx, Wc);
+    ENSURE(dWinRx2, Wc);
+    ENSURE(dWinIdx, Wc);
+    ENSURE(dWinIdx2, Wc);
+    ENSURE(dWinNode, Wc);
+    ENSURE(dWinField, 6 * Wc);
+    ENSURE(dField, 6 * (size_t)nOut);
     HIPCHK(hipMemsetAsync(dCounters.p + 4, 0, sizeof(unsigned long long), stream));
     k_collect<<<blocks_for((int64_t)cap, 256), 256, 0, stream>>>(dTabA.p, dTabOrder.p, dTabRx.p, cap, dWinKey.p,
                                                                   dWinRx.p, dCounters.p + 4);
     HIPCHK(hipGetLastError());
     if (winners > 0) {
+      int txrxBits = 1;
+      while ((int64_t(1) << txrxBits) < nOut) txrxBits++;
       size_t tb1 = 0, tb2 = 0;
       HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, dWinKey.p, dWinKey2.p, dWinRx.p, dWinRx2.p,
                                                 (int)winners, 0, 64, stream));
-      HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dWinRx2.p, dWinRx.p, dWinKey2.p, dWinKey.p,
-                                                (int)winners, 0, 32, stream));
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dWinRx2.p, dWinRx.p, dWinIdx.p, dWinIdx2.p,
+                                                (int)winners, 0, txrxBits, stream));
       ENSURE(dScratch, std::max(tb1, tb2));
+      // by order key -> dWinKey2 (keys), dWinRx2 (tx * nrx + rx)
       HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb1, dWinKey.p, dWinKey2.p, dWinRx.p, dWinRx2.p,
                                                 (int)winners, 0, 64, stream));
-      HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb2, dWinRx2.p, dWinRx.p, dWinKey2.p, dWinKey.p,
-                                                (int)winners, 0, 32, stream));
-      FieldArgs FA{};
-      FA.winners = winners;
-      for (int k = 0; k < kMaxDepth + 2; k++) FA.levelBase[k] = levelBase[k];
-      k_field<<<blocks_for(winners, 256), 256, 0, stream>>>(FA, K, dTree.p, surf, dMats.p, dRx.p, dWinKey.p,
-                                                             dWinRx.p, dWinField.p);
+      k_resolve<<<blocks_for(nodes, 256), 256, 0, stream>>>(dTree.p, nodes, dWinKey2.p, winners, dWinNode.p);
+      k_iota<<<blocks_for(winners, 256), 256, 0, stream>>>(dWinIdx.p, winners);
+      HIPCHK(hipGetLastError());
+      // stable by tx * nrx + rx -> dWinRx (keys), dWinIdx2 (index into the key-sorted arrays)
+      HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb2, dWinRx2.p, dWinRx.p, dWinIdx.p, dWinIdx2.p,
+                                                (int)winners, 0, txrxBits, stream));
+      FieldArgs FA{winners, m};
+      k_field<<<blocks_for(winners, 256), 256, 0, stream>>>(FA, dTx.p, dTree.p, dTris.p, surf, dMats.p, dRx.p, dWinRx.p,
+                                                             dWinIdx2.p, dWinNode.p, dWinField.p);
       HIPCHK(hipGetLastError());
     }
-    k_rx_sum<<<blocks_for(m, 256), 256, 0, stream>>>(dWinRx.p, winners, dWinField.p, m, dField.p);
+    k_rx_sum<<<blocks_for(nOut, 256), 256, 0, stream>>>(dWinRx.p, winners, dWinField.p, nOut, dField.p);
     HIPCHK(hipGetLastError());
   } else {
     HIPCHK(hipEventRecord(ev[3], stream));
@@ -1107,34 +1259,43 @@ int Engine::trace(const sbr_trace_params* p, const double* rx, int m) {
   stats.ms_capture = a;
   HIPCHK(hipEventElapsedTime(&a, ev[3], ev[4]));
   stats.ms_field = a;
-  stats.ms_total = ms_since(t0);
-  hField.assign(6 * (size_t)m, 0.0);
-  if (m > 0)
-    HIPCHK(hipMemcpyAsync(hField.data(), dField.p, 6 * sizeof(double) * m, hipMemcpyDeviceToHost, stream));
+  hField.assign(6 * (size_t)nOut, 0.0);
+  if (nOut > 0)
+    HIPCHK(hipMemcpyAsync(hField.data(), dField.p, 6 * sizeof(double) * nOut, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
+  stats.ms_total = ms_since(t0);
+  lastParams.assign(P, P + nt);
   haveResult = true;
   return SBR_OK;
 }
 
-int Engine::paths(int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri, int32_t* scop,
-                  int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs, bool countOnly) {
+int Engine::paths(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
+                  int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
+                  bool countOnly) {
   if (!haveResult) return fail(SBR_E_STATE, "no trace result");
   HIPCHK(hipSetDevice(device));
-  std::vector<unsigned long long> keys(winners);
-  std::vector<int32_t> rxs(winners);
+  std::vector<int32_t> txrx(winners), perm(winners);
+  std::vector<unsigned long long> key(winners);
   if (winners > 0) {
-    HIPCHK(hipMemcpyAsync(keys.data(), dWinKey.p, sizeof(unsigned long long) * winners, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(rxs.data(), dWinRx.p, sizeof(int32_t) * winners, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(txrx.data(), dWinRx.p, sizeof(int32_t) * winners, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(perm.data(), dWinIdx2.p, sizeof(int32_t) * winners, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(key.data(), dWinKey2.p, sizeof(unsigned long long) * winners, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
   std::vector<int64_t> off(winners + 1, 0);
-  for (int64_t w = 0; w < winners; w++) off[w + 1] = off[w] + (int64_t)((keys[w] >> 33) & 31ull) + 1;
+  for (int64_t w = 0; w < winners; w++) off[w + 1] = off[w] + (int64_t)((key[perm[w]] >> 27) & 31ull) + 1;
   *nPaths = winners;
   *nSegs = off[winners];
   if (countOnly) return SBR_OK;
+  std::vector<TxDesc> txs(ntx);
+  HIPCHK(hipMemcpyAsync(txs.data(), dTx.p, sizeof(TxDesc) * ntx, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
   for (int64_t w = 0; w < winners; w++) {
-    prx[w] = rxs[w];
-    plaunch[w] = (int32_t)(keys[w] >> 38);
+    const int t = txrx[w] / nrx;
+    const int64_t gl = (int64_t)(key[perm[w]] >> 32);
+    if (ptx) ptx[w] = t;
+    prx[w] = txrx[w] % nrx;
+    plaunch[w] = (int32_t)(txs[t].launchOffset + (gl - txs[t].launchStart) * txs[t].launchStride);
     pnseg[w] = (int32_t)(off[w + 1] - off[w]);
     poff[w] = off[w];
   }
@@ -1148,14 +1309,10 @@ int Engine::paths(int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff,
   dS.ensure(6 * ns);
   if (!dOff.p || !dI.p || !dS.p) return fail(SBR_E_NOMEM, "path export buffers");
   HIPCHK(hipMemcpyAsync(dOff.p, off.data(), sizeof(int64_t) * winners, hipMemcpyHostToDevice, stream));
-  FieldArgs FA{};
-  FA.winners = winners;
-  for (int k = 0; k < kMaxDepth + 2; k++) FA.levelBase[k] = levelBase[k];
-  TraceConsts K{};
-  memcpy(K.tx, lastParams.tx, sizeof K.tx);
-  const DSurf* surf = lastParams.q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
-  k_paths<<<blocks_for(winners, 128), 128, 0, stream>>>(FA, K, dTree.p, surf, dRx.p, dWinKey.p, dWinRx.p, dOff.p, dI.p,
-                                                         dI.p + ns, dI.p + 2 * ns, dS.p, dS.p + 3 * ns);
+  const DSurf* surf = lastParams[0].q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
+  k_paths<<<blocks_for(winners, 128), 128, 0, stream>>>(winners, nrx, dTx.p, dTree.p, surf, dRx.p, dWinRx.p,
+                                                         dWinIdx2.p, dWinNode.p, dOff.p, dI.p, dI.p + ns, dI.p + 2 * ns,
+                                                         dS.p, dS.p + 3 * ns);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(stri, dI.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(scop, dI.p + ns, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, stream));
@@ -1171,20 +1328,23 @@ int Engine::trees(int32_t* launch, int32_t* depth, int64_t* code, int32_t* tri, 
   if (!haveResult) return fail(SBR_E_STATE, "no trace result");
   HIPCHK(hipSetDevice(device));
   std::vector<TNode> h(nodes);
+  std::vector<TxDesc> txs(ntx);
   if (nodes > 0) {
     HIPCHK(hipMemcpyAsync(h.data(), dTree.p, sizeof(TNode) * nodes, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(txs.data(), dTx.p, sizeof(TxDesc) * ntx, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
   std::vector<int64_t> order(nodes);
   for (int64_t i = 0; i < nodes; i++) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
-    if (h[a].launch != h[b].launch) return h[a].launch < h[b].launch;
+    if (h[a].gl != h[b].gl) return h[a].gl < h[b].gl;
     if (h[a].depth != h[b].depth) return h[a].depth < h[b].depth;
     return h[a].code < h[b].code;
   });
   for (int64_t k = 0; k < nodes; k++) {
     const TNode& t = h[order[k]];
-    launch[k] = t.launch;
+    const TxDesc& T = txs[t.txi];
+    launch[k] = (int32_t)(T.launchOffset + ((int64_t)t.gl - T.launchStart) * T.launchStride);
     depth[k] = t.depth;
     code[k] = t.code;
     tri[k] = t.tri;

@@ -156,3 +156,30 @@ def test_stl_scenes_match_oracle(gpu_ctx, name, tx, tess, maxR, maxT):
     o = O.trace(sc, O.make_params(**kw), rx, mats, nthreads=8)
     _trees_equal(g, O.trees(sc, O.make_params(**kw)), name)
     compare(o, g, name)
+
+
+def test_batch_equals_individual_traces(gpu_ctx):
+    """Transmitters traced together give, each, exactly what a trace of that transmitter alone gives."""
+    say("batch vs individual")
+    cfg = scenes.config2(tess=3)
+    gpu_ctx.scene_from_soup(*cfg["soup"])
+    gpu_ctx.materials(cfg["materials"])
+    base = dict(cfg["trace"])
+    variants = [dict(base, tess=3), dict(base, tess=4, maxR=4, receptionModel=1),
+                dict(base, tess=2, tx=(10.0, -60.0, 12.0), maxT=0), dict(base, tess=3, launchStride=3, launchOffset=1)]
+    params = [make_params(**v) for v in variants]
+    batch = gpu_ctx.trace_batch(params, cfg["rx"])
+    bp = batch.paths()
+    btx = batch.path_tx()
+    bpf = batch.path_fields()
+    total = 0
+    for t, p in enumerate(params):
+        single = gpu_ctx.trace(p, cfg["rx"])
+        total += single.bounces
+        sp = single.paths()
+        sel = np.flatnonzero(btx == t)
+        assert np.array_equal(bp["prx"][sel], sp["prx"]) and np.array_equal(bp["plaunch"][sel], sp["plaunch"])
+        assert np.array_equal(bp["pnseg"][sel], sp["pnseg"])
+        assert np.array_equal(batch.fields[t], single.fields)
+        assert np.array_equal(bpf[sel], single.path_fields())
+    assert batch.bounces == total
