@@ -123,6 +123,7 @@ typedef struct {
   int64_t tri_tests;        /* ray-triangle tests (countTraversal = 1) */
   int32_t trace_lds;        /* 1 if the traversal staged the scene in LDS */
   int32_t trace_blocks_per_cu;
+  int64_t heavy_nodes;      /* tree nodes captured wave-cooperatively (large candidate regions) */
 } sbr_stats;
 
 /* Launch configuration of the traversal kernel (no reference counterpart).  lds_scene: -1 stage

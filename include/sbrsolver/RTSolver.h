@@ -1,0 +1,86 @@
+// RTSolver: the reference's tracing entry point (RTSolver.h:18-83) on the gfx950 library.
+//
+// Same call surface: Init(materials, count, bvh) keeps a non-owning BVH pointer and copies the
+// materials; ExecuteRayTracing(...) takes the same arguments, keeps non-owning pointers to the
+// receivers and the mesh, and returns N_rays * N_rx like the reference (RTSolver.cpp:52-126); the
+// Save* writers produce the reference's file formats (RTSolver.cpp:510-766).  The trace itself runs
+// through the C-ABI (sbr_capi.h); the scene is uploaded once per (mesh, BVH) pair and stays
+// resident on the device.  Extensions: result getters, trace options, batch tracing.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../sbr_capi.h"
+#include "BVH.h"
+#include "Triangle.h"
+#include "VecMatDef.h"
+
+// One segment of a kept path (Ray.h:15-27, the fields the outputs and the field computation use).
+struct Ray {
+  Vec3 sourcePoint;
+  Vec3 targetPoint;
+  int penetrationMaterialId = -1;  // >= 0: the segment leaves its start surface by transmission
+  int reflectionMaterialId = -1;   // >= 0: ... by reflection
+  int hitSurfaceID = -1;
+  int hitCoplanarId = -1;
+};
+
+// Kept paths of one (launch, receiver) pair (Ray.h:59-67).
+class Paths {
+ public:
+  std::vector<std::vector<Ray>> rayPaths;
+};
+
+struct RTSolverOptions {
+  int antennaType = 1;        // FieldCompute.cpp:117 hard-codes 1 (Hertzian dipole)
+  int receptionModel = 0;     // 0: reference sphere (RTSolver.cpp:838-841); 1: alpha*L/sqrt(3)
+  int maxTotal = 0;           // cap on reflections + transmissions, 0 = none (reference)
+  int q1MeshIndexing = 1;     // 1: surface data of mesh[hitSurfaceID] (reference); 0: by id
+  int device = 0;
+};
+
+class RTSolver {
+ public:
+  RTSolver();
+  virtual ~RTSolver();
+  int Init(MaterialProperties materialProperties[], int materialPropertiesCount, BVH<Triangle>& triangles);
+  void Cleanup();
+  int ExecuteRayTracing(Vec3 sourcePoint, const Mat3& txCoordSys, int maxReflectionCount, int maxTransmissionCount,
+                        double frequency, double pt, bool useFresnelCoeff, VecVec3& receivers,
+                        std::vector<Triangle*>& triangleMesh, int txTesslation);
+  bool SavePathsAsVtk(std::string fname);
+  bool SaveReceiversAsVtk(std::string fname);
+  bool SaveFieldAsCsv(std::string fname);
+  bool SaveIcosahedronAsVtk(std::string fname, Vec3 rayOrg, int tessellation);
+  void CmdLineDebug();
+
+  // ---- extensions (no reference counterpart)
+  RTSolverOptions options;
+  const std::vector<Vec3c>& Fields() const { return _efield; }
+  // Kept paths per receiver: (launch id, segments), launch ascending then path order.
+  const std::vector<std::vector<std::pair<int, std::vector<Ray>>>>& ReceiverPaths() const { return _kept; }
+  const sbr_stats& Stats() const { return _stats; }
+  const char* LastError() const;
+
+ protected:
+  MaterialProperties* _materialProperties = nullptr;
+  int _materialCount = 0;
+  BVH<Triangle>* _bvh = nullptr;
+  VecVec3* _receivers = nullptr;
+  std::vector<Vec3> _shootRayList;
+  std::vector<Vec3c> _efield;
+  std::vector<std::vector<std::pair<int, std::vector<Ray>>>> _kept;
+  sbr_stats _stats{};
+  bool _isSceneInitialized = false;
+  int _txTesslation = 0;
+  int _pathsCount = 0;
+  int _receiverCount = 0;
+
+ private:
+  sbr_ctx* _ctx = nullptr;
+  const void* _uploadedMesh = nullptr;
+  const void* _uploadedBvh = nullptr;
+  size_t _uploadedSize = 0;
+  int EnsureContext();
+};

@@ -14,7 +14,9 @@ SOURCES = [
     os.path.join(HERE, "csrc", "engine.hip"),
     os.path.join(HERE, "csrc", "capi.cpp"),
     os.path.join(HERE, "csrc", "host", "preprocessor.cpp"),
+    os.path.join(HERE, "csrc", "host", "rtsolver.cpp"),
 ]
+DEMO = os.path.join(HERE, "sbr_demo")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -37,6 +39,11 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    # The reference's main() against the library (links libsbr_amd.so from its own directory).
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(HERE, "csrc", "host", "sbr_demo.cpp"), "-o", DEMO + ".tmp", "-L", HERE,
+                           "-lsbr_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(DEMO + ".tmp", DEMO)
     return LIB
 
 

@@ -155,6 +155,7 @@ struct Engine {
   DBuf<uint8_t> dLos;
   DBuf<unsigned long long> dTabA, dTabB, dTabOrder;
   DBuf<int32_t> dTabRx;
+  DBuf<int32_t> dHeavy;          // nodes left to the wave-per-node capture
   DBuf<unsigned long long> dCounters;
   DBuf<unsigned long long> dWinKey, dWinKey2;
   DBuf<int32_t> dWinRx, dWinRx2, dWinIdx, dWinIdx2, dWinNode;

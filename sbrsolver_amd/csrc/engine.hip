@@ -385,198 +385,271 @@ __device__ __forceinline__ unsigned long long order_key(const TNode& nd) {
          (unsigned long long)(nd.code & ((1u << 27) - 1));
 }
 
-// Counters: [0] capture records, [1] corrected, [2] table slots used, [3] flags (1 overflow,
-// 2 hash collision).
-__global__ void __launch_bounds__(256) k_capture(CaptureArgs A, const TxDesc* __restrict__ txs,
-                                                 const TNode* __restrict__ tree, const DTri* __restrict__ tris,
-                                                 const DSurf* __restrict__ surf, const double* __restrict__ rx,
-                                                 const int32_t* __restrict__ cellStart,
-                                                 const int32_t* __restrict__ cellRx, const uint8_t* __restrict__ los,
-                                                 unsigned long long* __restrict__ tabA,
-                                                 unsigned long long* __restrict__ tabB,
-                                                 unsigned long long* __restrict__ tabOrder,
-                                                 int32_t* __restrict__ tabRx, unsigned long long* __restrict__ counters) {
-  __shared__ unsigned long long sRec, sCor, sUsed;
-  if (threadIdx.x == 0) { sRec = 0; sCor = 0; sUsed = 0; }
-  __syncthreads();
-  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Per-node state of the capture: the segment, the conservative search region and the path data
+// shared by every receiver tested against the node.
+struct NodeCtx {
+  V3 S, Tg, D, P0, P1;
+  double plArg, trueLen, rc, R;
+  int model, depth;
+  bool all, chainOk;
+  unsigned long long seqA, seqB, orderKey;
+  int64_t losBase;
+  int j0, j1;
+  int32_t chain[kMaxDepth + 1];
+  int32_t isurf[kMaxDepth + 1];
+  V3 img[kMaxDepth + 1];
+};
+
+struct CaptureBufs {
+  const TxDesc* txs;
+  const TNode* tree;
+  const DSurf* surf;
+  const double* rx;
+  const int32_t* cellStart;
+  const int32_t* cellRx;
+  const uint8_t* los;
+  unsigned long long* tabA;
+  unsigned long long* tabB;
+  unsigned long long* tabOrder;
+  int32_t* tabRx;
+  unsigned long long* counters;   // [0] records [1] corrected [2] slots used [3] flags [5] heavy nodes
+  int32_t* heavy;
+};
+
+// Returns false when no receiver can be captured by the node.
+__device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n, NodeCtx& c) {
+  const TNode nd = B.tree[n];
+  const TxDesc& T = B.txs[nd.txi];
+  c.depth = nd.depth;
+  c.model = T.receptionModel;
+  c.rc = T.rcoef;
+  c.plArg = nd.plArg;
+  c.trueLen = nd.trueLen;
+  c.S = ld3(nd.src);
+  c.Tg = ld3(nd.tgt);
+  c.D = vsub(c.Tg, c.S);
+  // Conservative search radius: a capture at C has its closest point P within
+  // (base + |C - ref|) * rc / (1 - rc) of C, with base/ref per reception model.
+  double far2 = 0.0;
+  for (int a = 0; a < 3; a++) {
+    double sa = c.model == 0 ? 0.0 : vget(c.S, a);
+    double e = fmax(fabs(A.g.bmin[a] - sa), fabs(A.g.bmax[a] - sa));
+    far2 += e * e;
+  }
+  const double base = c.model == 0 ? c.plArg : c.trueLen;
+  double R = (base + sqrt(far2)) * c.rc / (1.0 - c.rc);
+  c.R = R * (1.0 + 1e-6) + 1e-6;
+  c.all = !(c.R < 1e300) || c.rc >= 1.0;
+  // Clip the segment to the receivers' box grown by R.
+  double u0 = 0.0, u1 = 1.0;
+  if (!c.all) {
+    for (int a = 0; a < 3; a++) {
+      double lo = A.g.bmin[a] - c.R, hi = A.g.bmax[a] + c.R, s = vget(c.S, a), d = vget(c.D, a);
+      if (fabs(d) < 1e-300) {
+        if (s < lo || s > hi) return false;
+      } else {
+        double ua = (lo - s) / d, ub = (hi - s) / d;
+        u0 = fmax(u0, fmin(ua, ub) - 1e-12);
+        u1 = fmin(u1, fmax(ua, ub) + 1e-12);
+        if (u0 > u1) return false;
+      }
+    }
+  }
+  c.P0 = vadd(c.S, vmul(c.D, u0));
+  c.P1 = vadd(c.S, vmul(c.D, u1));
+  c.j0 = 0;
+  c.j1 = A.g.ny - 1;
+  if (!c.all) {
+    double ylo = fmin(c.P0.y, c.P1.y) - c.R, yhi = fmax(c.P0.y, c.P1.y) + c.R;
+    c.j0 = (int)fmax(0.0, floor((ylo - A.g.gy0) / A.g.cell));
+    c.j1 = (int)fmin((double)(A.g.ny - 1), floor((yhi - A.g.gy0) / A.g.cell));
+  }
+  // Path data: image sources and surfaces (GetImageSources, RTSolver.cpp:396-433) and the hash of
+  // the coplanar-id sequence of the ancestors (the dedup key without the receiver).
+  {
+    int64_t q = n;
+    for (int k = c.depth; k >= 0; k--) { c.chain[k] = (int32_t)q; q = B.tree[q].parent; }
+  }
+  c.seqA = 0x243f6a8885a308d3ull ^ ((unsigned long long)c.depth << 48) ^ (unsigned long long)nd.txi;
+  c.seqB = 0x13198a2e03707344ull + (unsigned long long)c.depth * 0x51ull + ((unsigned long long)nd.txi << 20);
+  c.chainOk = true;
+  V3 cur = v3(T.tx[0], T.tx[1], T.tx[2]);
+  c.img[0] = cur;
+  c.isurf[0] = -1;
+  for (int k = 0; k < c.depth; k++) {
+    const TNode& a = B.tree[c.chain[k]];
+    const TNode& b = B.tree[c.chain[k + 1]];
+    c.seqA = mix64(c.seqA + (unsigned long long)(uint32_t)a.cop + 0x9e3779b97f4a7c15ull);
+    c.seqB = mix64(c.seqB ^ ((unsigned long long)(uint32_t)a.cop * 0xd6e8feb86659fd93ull + (unsigned long long)k));
+    if (a.tri < 0) { c.chainOk = false; break; }
+    if (b.type == 1) cur = image_point(B.surf[a.tri], cur);
+    else if (b.type != 2) { c.chainOk = false; break; }
+    c.img[k + 1] = cur;
+    c.isurf[k + 1] = a.tri;
+  }
+  c.orderKey = order_key(nd);
+  c.losBase = (int64_t)nd.txi * A.nrx;
+  return true;
+}
+
+// Grid cells [i0, i1] of row j that can hold a receiver within R of the clipped segment (xy).
+__device__ __forceinline__ bool row_range(const CaptureArgs& A, const NodeCtx& c, int j, int* i0, int* i1) {
+  if (c.all) {
+    *i0 = 0;
+    *i1 = A.g.nx - 1;
+    return true;
+  }
+  double xa, xb;
+  const double by0 = A.g.gy0 + j * A.g.cell - c.R, by1 = A.g.gy0 + (j + 1) * A.g.cell + c.R;
+  const double dy = c.P1.y - c.P0.y;
+  if (fabs(dy) < 1e-300) {
+    xa = fmin(c.P0.x, c.P1.x);
+    xb = fmax(c.P0.x, c.P1.x);
+  } else {
+    double va = (by0 - c.P0.y) / dy, vb = (by1 - c.P0.y) / dy;
+    double w0 = fmax(0.0, fmin(va, vb) - 1e-12), w1 = fmin(1.0, fmax(va, vb) + 1e-12);
+    if (w0 > w1) return false;
+    double xa0 = c.P0.x + (c.P1.x - c.P0.x) * w0, xa1 = c.P0.x + (c.P1.x - c.P0.x) * w1;
+    xa = fmin(xa0, xa1);
+    xb = fmax(xa0, xa1);
+  }
+  *i0 = (int)fmax(0.0, floor((xa - c.R - A.g.gx0) / A.g.cell));
+  *i1 = (int)fmin((double)(A.g.nx - 1), floor((xb + c.R - A.g.gx0) / A.g.cell));
+  return *i0 <= *i1;
+}
+
+// One receiver against one node: reception sphere, ancestor rule, image correction, dedup.
+__device__ void capture_one(const CaptureArgs& A, const CaptureBufs& B, const NodeCtx& c, int32_t r,
+                            unsigned long long& myRec, unsigned long long& myCor, unsigned long long& myUsed) {
+  const V3 C = ld3(B.rx + 3 * (int64_t)r);
+  if (!sphere_hit(c.S, c.Tg, C, c.plArg, c.trueLen, c.model, c.rc)) return;
+  for (int k = c.depth - 1; k >= 0; k--) {
+    const TNode& a = B.tree[c.chain[k]];
+    if (sphere_hit(ld3(a.src), ld3(a.tgt), C, a.plArg, a.trueLen, c.model, c.rc)) return;
+  }
+  myRec++;
+  unsigned long long hA = mix64(c.seqA ^ ((unsigned long long)(uint32_t)r * 0x9e3779b97f4a7c15ull));
+  unsigned long long hB = mix64(c.seqB + ((unsigned long long)(uint32_t)r << 7) + 0x632be59bd9b4e019ull);
+  if (hA == 0) hA = 1;
+  if (hB == 0) hB = 1;
+  // A record that cannot beat the key already stored for its group needs no correction: the stored
+  // key belongs to a corrected path and keys only decrease.  A stale read is only less selective.
+  {
+    unsigned long long slot = hA & A.tabMask;
+    for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
+      const unsigned long long a = B.tabA[slot];
+      if (a == 0ull) break;
+      if (a == hA) {
+        if (B.tabOrder[slot] < c.orderKey) return;
+        break;
+      }
+      slot = (slot + 1) & A.tabMask;
+    }
+  }
+  // Image correction (MultiPathCorrection / DirectPathCorrection).
+  bool ok;
+  if (c.depth == 0) {
+    ok = B.los[c.losBase + r] != 0;
+  } else if (!c.chainOk) {
+    ok = false;
+  } else {
+    ok = true;
+    V3 target = C;
+    for (int k = c.depth; k >= 1 && ok; k--) {
+      const DSurf& s = B.surf[c.isurf[k]];
+      V3 p;
+      double dist;
+      ok = tri_intersect(ld3(s.v1), ld3(s.e1), ld3(s.e2), c.img[k], vsub(target, c.img[k]), &p, &dist);
+      target = p;
+    }
+  }
+  if (!ok) return;
+  myCor++;
+  unsigned long long slot = hA & A.tabMask;
+  for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
+    unsigned long long prev = atomicCAS(&B.tabA[slot], 0ull, hA);
+    if (prev == 0ull || prev == hA) {
+      if (prev == 0ull) { B.tabRx[slot] = (int32_t)(c.losBase + r); myUsed++; }
+      unsigned long long pb = atomicCAS(&B.tabB[slot], 0ull, hB);
+      if (pb != 0ull && pb != hB) atomicOr(&B.counters[3], 2ull);
+      atomicMin(&B.tabOrder[slot], c.orderKey);
+      return;
+    }
+    slot = (slot + 1) & A.tabMask;
+  }
+  atomicOr(&B.counters[3], 1ull);
+}
+
+__device__ __forceinline__ void flush_counters(const CaptureBufs& B, unsigned long long myRec, unsigned long long myCor,
+                                               unsigned long long myUsed) {
+  // wave-level reduction, then one atomic per wave and counter
+  for (int off = 32; off > 0; off >>= 1) {
+    myRec += __shfl_xor(myRec, off, 64);
+    myCor += __shfl_xor(myCor, off, 64);
+    myUsed += __shfl_xor(myUsed, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (myRec) atomicAdd(&B.counters[0], myRec);
+    if (myCor) atomicAdd(&B.counters[1], myCor);
+    if (myUsed) atomicAdd(&B.counters[2], myUsed);
+  }
+}
+
+// Thread per node.  Nodes whose candidate region spans more than kHeavyCells grid cells are left to
+// k_capture_heavy, which spreads one node over a wave.
+constexpr int kHeavyCells = 48;
+
+__global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long myRec = 0, myCor = 0, myUsed = 0;
   if (n < A.nodes) {
-    const TNode nd = tree[n];
-    const TxDesc& T = txs[nd.txi];
-    const int depth = nd.depth;
-    const int model = T.receptionModel;
-    const double rc = T.rcoef;
-    const V3 S = ld3(nd.src), Tg = ld3(nd.tgt);
-    const V3 D = vsub(Tg, S);
-    // Conservative search radius: a capture at C has its closest point P within
-    // (base + |C - ref|) * rc / (1 - rc) of C, with base/ref per reception model.
-    double far2 = 0.0;
-    for (int a = 0; a < 3; a++) {
-      double sa = model == 0 ? 0.0 : vget(S, a);
-      double e = fmax(fabs(A.g.bmin[a] - sa), fabs(A.g.bmax[a] - sa));
-      far2 += e * e;
-    }
-    const double base = model == 0 ? nd.plArg : nd.trueLen;
-    double R = (base + sqrt(far2)) * rc / (1.0 - rc);
-    R = R * (1.0 + 1e-6) + 1e-6;
-    const bool all = !(R < 1e300) || rc >= 1.0;
-    // Clip the segment to the receivers' box grown by R.
-    double u0 = 0.0, u1 = 1.0;
-    bool any = true;
-    if (!all) {
-      for (int a = 0; a < 3 && any; a++) {
-        double lo = A.g.bmin[a] - R, hi = A.g.bmax[a] + R, s = vget(S, a), d = vget(D, a);
-        if (fabs(d) < 1e-300) {
-          if (s < lo || s > hi) any = false;
-        } else {
-          double ua = (lo - s) / d, ub = (hi - s) / d;
-          u0 = fmax(u0, fmin(ua, ub) - 1e-12);
-          u1 = fmin(u1, fmax(ua, ub) + 1e-12);
-          if (u0 > u1) any = false;
-        }
+    NodeCtx c;
+    if (node_setup(A, B, n, c)) {
+      int cells = 0;
+      for (int j = c.j0; j <= c.j1 && cells <= kHeavyCells; j++) {
+        int i0, i1;
+        if (row_range(A, c, j, &i0, &i1)) cells += i1 - i0 + 1;
       }
-    }
-    if (any) {
-      // Per-node path data: image sources and surfaces (GetImageSources, RTSolver.cpp:396-433)
-      // and the hash of the coplanar-id sequence of the ancestors (the dedup key without rx).
-      int32_t chain[kMaxDepth + 1];
-      {
-        int64_t c = n;
-        for (int k = depth; k >= 0; k--) { chain[k] = (int32_t)c; c = tree[c].parent; }
-      }
-      V3 img[kMaxDepth + 1];
-      int32_t isurf[kMaxDepth + 1];
-      unsigned long long seqA = 0x243f6a8885a308d3ull ^ ((unsigned long long)depth << 48) ^ (unsigned long long)nd.txi;
-      unsigned long long seqB = 0x13198a2e03707344ull + (unsigned long long)depth * 0x51ull + ((unsigned long long)nd.txi << 20);
-      bool chainOk = true;
-      {
-        V3 cur = v3(T.tx[0], T.tx[1], T.tx[2]);
-        img[0] = cur;
-        isurf[0] = -1;
-        for (int k = 0; k < depth; k++) {
-          const TNode& a = tree[chain[k]];
-          const TNode& b = tree[chain[k + 1]];
-          seqA = mix64(seqA + (unsigned long long)(uint32_t)a.cop + 0x9e3779b97f4a7c15ull);
-          seqB = mix64(seqB ^ ((unsigned long long)(uint32_t)a.cop * 0xd6e8feb86659fd93ull + (unsigned long long)k));
-          if (a.tri < 0) { chainOk = false; break; }
-          if (b.type == 1) cur = image_point(surf[a.tri], cur);
-          else if (b.type != 2) { chainOk = false; break; }
-          img[k + 1] = cur;
-          isurf[k + 1] = a.tri;
-        }
-      }
-      const unsigned long long orderKey = order_key(nd);
-      const int64_t losBase = (int64_t)nd.txi * A.nrx;
-      // Candidate receivers: grid rows crossed by the clipped segment's xy band.
-      V3 P0 = vadd(S, vmul(D, u0)), P1 = vadd(S, vmul(D, u1));
-      double ylo = fmin(P0.y, P1.y) - R, yhi = fmax(P0.y, P1.y) + R;
-      int j0 = 0, j1 = A.g.ny - 1;
-      if (!all) {
-        j0 = (int)fmax(0.0, floor((ylo - A.g.gy0) / A.g.cell));
-        j1 = (int)fmin((double)(A.g.ny - 1), floor((yhi - A.g.gy0) / A.g.cell));
-      }
-      for (int j = j0; j <= j1; j++) {
-        int i0 = 0, i1 = A.g.nx - 1;
-        if (!all) {
-          double xa, xb;
-          double by0 = A.g.gy0 + j * A.g.cell - R, by1 = A.g.gy0 + (j + 1) * A.g.cell + R;
-          double dy = P1.y - P0.y;
-          if (fabs(dy) < 1e-300) {
-            xa = fmin(P0.x, P1.x);
-            xb = fmax(P0.x, P1.x);
-          } else {
-            double va = (by0 - P0.y) / dy, vb = (by1 - P0.y) / dy;
-            double w0 = fmax(0.0, fmin(va, vb) - 1e-12), w1 = fmin(1.0, fmax(va, vb) + 1e-12);
-            if (w0 > w1) continue;
-            double xa0 = P0.x + (P1.x - P0.x) * w0, xa1 = P0.x + (P1.x - P0.x) * w1;
-            xa = fmin(xa0, xa1);
-            xb = fmax(xa0, xa1);
-          }
-          i0 = (int)fmax(0.0, floor((xa - R - A.g.gx0) / A.g.cell));
-          i1 = (int)fmin((double)(A.g.nx - 1), floor((xb + R - A.g.gx0) / A.g.cell));
-        }
-        for (int ci = i0; ci <= i1; ci++) {
-          const int cell = j * A.g.nx + ci;
-          for (int q = cellStart[cell]; q < cellStart[cell + 1]; q++) {
-            const int32_t r = cellRx[q];
-            const V3 C = ld3(rx + 3 * (int64_t)r);
-            if (!sphere_hit(S, Tg, C, nd.plArg, nd.trueLen, model, rc)) continue;
-            bool blocked = false;
-            for (int k = depth - 1; k >= 0 && !blocked; k--) {
-              const TNode& a = tree[chain[k]];
-              blocked = sphere_hit(ld3(a.src), ld3(a.tgt), C, a.plArg, a.trueLen, model, rc);
-            }
-            if (blocked) continue;
-            myRec++;
-            unsigned long long hA = mix64(seqA ^ ((unsigned long long)(uint32_t)r * 0x9e3779b97f4a7c15ull));
-            unsigned long long hB = mix64(seqB + ((unsigned long long)(uint32_t)r << 7) + 0x632be59bd9b4e019ull);
-            if (hA == 0) hA = 1;
-            if (hB == 0) hB = 1;
-            // A record that cannot beat the key already stored for its group needs no correction:
-            // the stored key belongs to a corrected path and keys only decrease.
-            {
-              unsigned long long slot = hA & A.tabMask;
-              bool pruned = false;
-              for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
-                const unsigned long long a = __atomic_load_n(&tabA[slot], __ATOMIC_RELAXED);
-                if (a == 0ull) break;
-                if (a == hA) {
-                  pruned = __atomic_load_n(&tabOrder[slot], __ATOMIC_RELAXED) < orderKey;
-                  break;
-                }
-                slot = (slot + 1) & A.tabMask;
-              }
-              if (pruned) continue;
-            }
-            // Image correction (MultiPathCorrection / DirectPathCorrection).
-            bool ok;
-            if (depth == 0) {
-              ok = los[losBase + r] != 0;
-            } else if (!chainOk) {
-              ok = false;
-            } else {
-              ok = true;
-              V3 target = C;
-              for (int k = depth; k >= 1 && ok; k--) {
-                const DSurf& s = surf[isurf[k]];
-                V3 p;
-                double dist;
-                ok = tri_intersect(ld3(s.v1), ld3(s.e1), ld3(s.e2), img[k], vsub(target, img[k]), &p, &dist);
-                target = p;
-              }
-            }
-            if (!ok) continue;
-            myCor++;
-            unsigned long long slot = hA & A.tabMask;
-            bool placed = false;
-            for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
-              unsigned long long prev = atomicCAS(&tabA[slot], 0ull, hA);
-              if (prev == 0ull || prev == hA) {
-                if (prev == 0ull) { tabRx[slot] = (int32_t)(losBase + r); myUsed++; }
-                unsigned long long pb = atomicCAS(&tabB[slot], 0ull, hB);
-                if (pb != 0ull && pb != hB) atomicOr(&counters[3], 2ull);
-                atomicMin(&tabOrder[slot], orderKey);
-                placed = true;
-                break;
-              }
-              slot = (slot + 1) & A.tabMask;
-            }
-            if (!placed) atomicOr(&counters[3], 1ull);
+      if (cells > kHeavyCells) {
+        const unsigned long long h = atomicAdd(&B.counters[5], 1ull);
+        B.heavy[h] = (int32_t)n;
+      } else {
+        for (int j = c.j0; j <= c.j1; j++) {
+          int i0, i1;
+          if (!row_range(A, c, j, &i0, &i1)) continue;
+          for (int ci = i0; ci <= i1; ci++) {
+            const int cell = j * A.g.nx + ci;
+            for (int q = B.cellStart[cell]; q < B.cellStart[cell + 1]; q++) capture_one(A, B, c, B.cellRx[q], myRec, myCor, myUsed);
           }
         }
       }
     }
   }
-  if (myRec) atomicAdd(&sRec, myRec);
-  if (myCor) atomicAdd(&sCor, myCor);
-  if (myUsed) atomicAdd(&sUsed, myUsed);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (sRec) atomicAdd(&counters[0], sRec);
-    if (sCor) atomicAdd(&counters[1], sCor);
-    if (sUsed) atomicAdd(&counters[2], sUsed);
+  flush_counters(B, myRec, myCor, myUsed);
+}
+
+// Wave per heavy node: the node's candidate cells, flattened row by row, are dealt to the lanes.
+__global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBufs B, int64_t nHeavy) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  unsigned long long myRec = 0, myCor = 0, myUsed = 0;
+  for (int64_t h = wave; h < nHeavy; h += nWaves) {
+    NodeCtx c;
+    if (!node_setup(A, B, B.heavy[h], c)) continue;
+    int off = 0;   // flattened index of the row's first cell
+    for (int j = c.j0; j <= c.j1; j++) {
+      int i0, i1;
+      if (!row_range(A, c, j, &i0, &i1)) continue;
+      const int first = i0 + (((lane - off) % 64) + 64) % 64;
+      for (int ci = first; ci <= i1; ci += 64) {
+        const int cell = j * A.g.nx + ci;
+        for (int q = B.cellStart[cell]; q < B.cellStart[cell + 1]; q++) capture_one(A, B, c, B.cellRx[q], myRec, myCor, myUsed);
+      }
+      off += i1 - i0 + 1;
+    }
   }
+  flush_counters(B, myRec, myCor, myUsed);
 }
 
 __global__ void k_collect(const unsigned long long* __restrict__ tabA, const unsigned long long* __restrict__ tabOrder,
@@ -1170,6 +1243,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       ENSURE(dTabB, cap);
       ENSURE(dTabOrder, cap);
       ENSURE(dTabRx, cap);
+      ENSURE(dHeavy, std::max<int64_t>(nodes, 1));
       HIPCHK(hipMemsetAsync(dTabA.p, 0, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dTabB.p, 0, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dTabOrder.p, 0xFF, cap * sizeof(unsigned long long), stream));
@@ -1180,10 +1254,19 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       CA.g = g;
       CA.tabMask = cap - 1;
       HIPCHK(hipEventRecord(kev[0], stream));
-      k_capture<<<blocks_for(nodes, 256), 256, 0, stream>>>(CA, dTx.p, dTree.p, dTris.p, surf, dRx.p, dCellStart.p,
-                                                             dCellRx.p, dLos.p, dTabA.p, dTabB.p, dTabOrder.p, dTabRx.p,
-                                                             dCounters.p);
+      CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
+                     dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p};
+      k_capture<<<blocks_for(nodes, 256), 256, 0, stream>>>(CA, CB);
       HIPCHK(hipGetLastError());
+      unsigned long long nHeavy = 0;
+      HIPCHK(hipMemcpyAsync(&nHeavy, dCounters.p + 5, sizeof nHeavy, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      if (nHeavy > 0) {
+        const int64_t waves = std::min<int64_t>((int64_t)nHeavy, (int64_t)numCU * 32);
+        k_capture_heavy<<<blocks_for(waves * 64, 256), 256, 0, stream>>>(CA, CB, (int64_t)nHeavy);
+        HIPCHK(hipGetLastError());
+      }
+      stats.heavy_nodes = (int64_t)nHeavy;
       HIPCHK(hipEventRecord(kev[1], stream));
       unsigned long long ctr[4];
       HIPCHK(hipMemcpyAsync(ctr, dCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
