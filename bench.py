@@ -55,6 +55,22 @@ def cpu_baseline(cfg, trace_kw, stride):
                 seconds=dt, bounces=int(r.bounces))
 
 
+def calib_copy(nbytes):
+    """One hipMemcpy device-to-device of nbytes (a blit kernel of known traffic: nbytes read and
+    nbytes written), for scaling FETCH_SIZE / WRITE_SIZE counter readings."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] calibration copy of {nbytes} bytes done", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,9 +78,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--tess", type=int, default=9)
     ap.add_argument("--model", type=int, default=1, help="reception model: 1 = alpha*L/sqrt(3), 0 = reference")
-    ap.add_argument("--cpu-stride", type=int, default=512)
+    ap.add_argument("--cpu-stride", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--also-ref-model", action="store_true", help="also time the reference reception model")
+    ap.add_argument("--calib", action="store_true",
+                    help="run one plain 256 MiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only; multi-GPU sharding is a design note, not a run",
@@ -147,6 +165,8 @@ def main():
         out["reference_reception_model"] = dict(value=round(s0["bounces"] / d0 / 1e6, 3), ms_per_step=round(d0 * 1e3, 3),
                                                 paths_kept=int(s0["winners"]),
                                                 capture_records=int(s0["capture_records"]))
+    if args.calib:
+        calib_copy(256 << 20)
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(cfg, trace_kw, args.cpu_stride)

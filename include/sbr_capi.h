@@ -127,8 +127,9 @@ typedef struct {
 } sbr_stats;
 
 /* Launch configuration of the traversal kernel (no reference counterpart).  lds_scene: -1 stage
- * the scene in LDS when it fits, 0 never; blocks_per_cu: persistent blocks per compute unit, 0 =
- * the occupancy maximum.  Results do not depend on it. */
+ * the scene in LDS when it fits and does not lower the resident blocks per CU, 1 whenever it
+ * fits, 0 never; blocks_per_cu: persistent blocks per compute unit, 0 = the occupancy maximum.
+ * Results do not depend on it. */
 typedef struct {
   int32_t lds_scene;
   int32_t blocks_per_cu;

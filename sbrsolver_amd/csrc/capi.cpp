@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -94,6 +95,33 @@ int Engine::upload(const sbr_scene_desc* d) {
     }
   }
   if (height > kMaxStack) return fail(SBR_E_LIMIT, "BVH taller than the traversal stack");
+  // Single-precision pre-test boxes: exact FP64 bounds rounded outward and padded by delta.  With
+  // E = max |coordinate| + 1 and ray origins with |coordinates| <= 4E, the single-precision
+  // slab endpoints of a ray are off by at most ~1e-6 E in arc length (origin and direction rounding
+  // 8e-7 E, fused slab products 6e-7 E), so delta = 1e-3 + 4e-6 E keeps every point of the exact box
+  // strictly inside the computed interval.
+  double ext = 0.0;
+  for (int i = 0; i < 9 * n; i++) ext = std::max(ext, std::fabs(d->tri_v[i]));
+  ext += 1.0;
+  const double delta = 1e-3 + 4e-6 * ext;
+  auto box32 = [&](const double* lo, const double* hi) {
+    DBox32 b{};
+    for (int a = 0; a < 3; a++) {
+      b.lo[a] = std::nextafter((float)(lo[a] - delta), -INFINITY);
+      b.hi[a] = std::nextafter((float)(hi[a] + delta), INFINITY);
+    }
+    return b;
+  };
+  std::vector<DBox32> tb32(trisOut.size());
+  for (size_t k = 0; k < trisOut.size(); k++) {
+    double lo[3], hi[3];
+    const double* tv = d->tri_v + 9 * (size_t)trisOut[k].id;
+    for (int a = 0; a < 3; a++) {
+      lo[a] = std::min({tv[a], tv[3 + a], tv[6 + a]});
+      hi[a] = std::max({tv[a], tv[3 + a], tv[6 + a]});
+    }
+    tb32[k] = box32(lo, hi);
+  }
   auto surf_of = [&](int id) {
     DSurf s{};
     const double* v = d->tri_v + 9 * (size_t)id;
@@ -114,7 +142,8 @@ int Engine::upload(const sbr_scene_desc* d) {
   if (device >= 0) {
     HIPCHK(hipSetDevice(device));
     if (put(dNodes, nodesOut, stream) || put(dTris, trisOut, stream) || put(dTriN, triN, stream) ||
-        put(dSurfQ1, sq, stream) || put(dSurfId, si, stream))
+        put(dSurfQ1, sq, stream) || put(dSurfId, si, stream) ||
+        put(dTriBox32, tb32, stream))
       return fail(SBR_E_NOMEM, "scene upload");
     HIPCHK(hipStreamSynchronize(stream));
   }
@@ -131,7 +160,10 @@ int Engine::upload(const sbr_scene_desc* d) {
   ntri = n;
   nnode = nn;
   treeHeight = height;
-  sceneBytes = hNodes.size() * sizeof(DNode) + hTris.size() * sizeof(DTri);
+  hTriBox32.swap(tb32);
+  sceneExtent = (float)ext;
+  boxDelta = (float)delta;
+  sceneBytes = hNodes.size() * sizeof(DNode) + hTris.size() * sizeof(DBox32);   // LDS-staged part
   ncop = 0;
   for (int t = 0; t < n; t++) ncop = std::max(ncop, hCop[t] + 1);
   haveScene = true;

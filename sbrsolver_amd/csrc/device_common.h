@@ -36,10 +36,15 @@ struct Hit {
   int idx;       // DFS triangle index, INT_MAX if none
   double dist;
   double t;
-  V3 p;
   int nodes;     // BVH nodes entered
   int tests;     // ray-triangle tests
 };
+
+// Hit point of a closest hit: the same operations tri_intersect applies, so the value is
+// bit-identical to the point it computed (kept out of the traversal loop to save registers).
+__device__ __forceinline__ V3 hit_point(V3 o, V3 d, const Hit& h) {
+  return h.idx == INT_MAX ? v3(0, 0, 0) : vadd(o, vmul(d, h.t));
+}
 
 struct RayPre {
   V3 o, d, inv;
@@ -81,6 +86,48 @@ __device__ __forceinline__ bool box_test(const double* bmin, const double* bmax,
   return box_line(bmin, bmax, r.o, r.d, lo, hi);
 }
 
+// Single-precision pre-test in front of each ray-triangle test: the triangle's box, rounded outward
+// and padded by delta (set at upload), is tested against the ray in single precision.  For rays
+// with |origin coordinates| <= 4E (E = scene extent) and a direction whose largest component lies
+// in [1e-3, 1e3], every parameter t at which the ray meets the exact box lies inside the computed
+// interval, so a triangle is skipped only when the exact test misses it (interval empty, or wholly
+// at t < 0) or when every hit on it lies beyond the best hit so far (entry > t_best (1 + 1e-6)).
+// Other rays take the exact test on every triangle.
+struct RayF {
+  float ix, iy, iz;     // 1 / direction (components clamped away from 0)
+  float nx, ny, nz;     // -origin / direction
+  bool on;
+};
+
+__device__ __forceinline__ RayF rayf_pre(V3 o, V3 d, float ext) {
+  RayF f;
+  const double dm = fmax(fabs(d.x), fmax(fabs(d.y), fabs(d.z)));
+  const double lim = 4.0 * (double)ext;
+  f.on = fabs(o.x) <= lim && fabs(o.y) <= lim && fabs(o.z) <= lim && dm >= 1e-3 && dm <= 1e3;
+  auto inv = [](double c) {
+    float x = (float)c;
+    if (fabsf(x) < 1e-20f) x = x < 0.0f ? -1e-20f : 1e-20f;
+    return 1.0f / x;
+  };
+  f.ix = inv(d.x);
+  f.iy = inv(d.y);
+  f.iz = inv(d.z);
+  f.nx = -(float)o.x * f.ix;
+  f.ny = -(float)o.y * f.iy;
+  f.nz = -(float)o.z * f.iz;
+  return f;
+}
+
+// True if the triangle behind box b may still hold a hit at t <= clip.
+__device__ __forceinline__ bool box32_may_hit(const DBox32& b, const RayF& f, float clip) {
+  const float ax = fmaf(b.lo[0], f.ix, f.nx), bx = fmaf(b.hi[0], f.ix, f.nx);
+  const float ay = fmaf(b.lo[1], f.iy, f.ny), by = fmaf(b.hi[1], f.iy, f.ny);
+  const float az = fmaf(b.lo[2], f.iz, f.nz), bz = fmaf(b.hi[2], f.iz, f.nz);
+  const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+  const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+  return tn <= tf && tf >= 0.0f && tn <= clip;
+}
+
 template <bool EXACT>
 __device__ __forceinline__ bool keep_box(double lo, double hi, const Hit& h) {
   if (EXACT) return true;
@@ -92,15 +139,17 @@ __device__ __forceinline__ bool keep_box(double lo, double hi, const Hit& h) {
 // Traversal stack: entry e of the calling thread is at stkN[e * stride] / stkT[e * stride]
 // (stride = threads per block, so a wave's entries are contiguous).
 template <bool EXACT>
-__device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, const DTri* __restrict__ tris, V3 o, V3 d,
-                                            Hit& h, int* stkN, float* stkT, int stride) {
+__device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
+                                            const DBox32* __restrict__ tbox, float ext, V3 o, V3 d, Hit& h, int* stkN,
+                                            float* stkT, int stride) {
   h.idx = INT_MAX;
   h.dist = INFINITY;
   h.t = INFINITY;
-  h.p = v3(0, 0, 0);
   h.nodes = 0;
   h.tests = 0;
   const RayPre r = ray_pre(o, d);
+  const RayF f = rayf_pre(o, d, ext);
+  float clip = INFINITY;
   double lo, hi;
   if (!box_test(nodes[0].bmin, nodes[0].bmax, r, &lo, &hi) || !keep_box<EXACT>(lo, hi, h)) return false;
   int sp = 0;
@@ -109,8 +158,9 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
     const DNode nd = nodes[cur];
     h.nodes++;
     if (nd.count > 0) {
-      h.tests += nd.count;
       for (int k = nd.a; k < nd.a + nd.count; k++) {
+        if (f.on && !box32_may_hit(tbox[k], f, clip)) continue;
+        h.tests++;
         const DTri& tr = tris[k];
         V3 p;
         double dist, t;
@@ -119,7 +169,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
             h.idx = k;
             h.dist = dist;
             h.t = t;
-            h.p = p;
+            clip = __double2float_ru(t * (1.0 + 1e-6));
           }
         }
       }

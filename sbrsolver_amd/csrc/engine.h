@@ -39,6 +39,13 @@ struct alignas(32) DTri {
   int32_t pad;
   double pad2;
 };
+// Conservative single-precision box (rounded outward and padded), 32 B: a ray that provably misses
+// it misses the exact FP64 box, so it serves as a cheap pre-test in front of the exact tests.
+struct alignas(32) DBox32 {
+  float lo[3];
+  float hi[3];
+  float pad[2];
+};
 // Surface data addressed by a path's hitSurfaceID (image points, re-intersection, field frame).
 struct alignas(64) DSurf {
   double v1[3];
@@ -110,7 +117,7 @@ struct DBuf {
 
 // Launch configuration of the traversal kernel (sbr_set_tuning; the autotuner fills it).
 struct Tuning {
-  int ldsScene = -1;      // -1: stage the scene in LDS when it fits, 0: never
+  int ldsScene = -1;      // -1: LDS staging when it fits and keeps occupancy, 1: whenever it fits, 0: never
   int blocksPerCU = 0;    // persistent blocks per CU; 0: occupancy maximum
 };
 
@@ -123,6 +130,8 @@ struct Engine {
 
   // scene (host copies for queries)
   int ntri = 0, nnode = 0, ncop = 0, treeHeight = 0;
+  // Traversal stack entries per ray: one deferred sibling per internal node of the current path.
+  int stackDepth() const { return treeHeight > 1 ? treeHeight - 1 : 1; }
   size_t sceneBytes = 0;
   std::vector<DNode> hNodes;
   std::vector<DTri> hTris;
@@ -132,6 +141,9 @@ struct Engine {
   std::vector<double> hVerts, hNorms;       // by triangleId
   DBuf<DNode> dNodes;
   DBuf<DTri> dTris;
+  std::vector<DBox32> hTriBox32;   // single-precision pre-test box per DFS triangle
+  DBuf<DBox32> dTriBox32;
+  float sceneExtent = 1.0f, boxDelta = 1e-3f;
   DBuf<double> dTriN;
   DBuf<DSurf> dSurfQ1, dSurfId;
   std::vector<double> mats;                 // 6 per material
@@ -156,6 +168,7 @@ struct Engine {
   DBuf<unsigned long long> dTabA, dTabB, dTabOrder;
   DBuf<int32_t> dTabRx;
   DBuf<int32_t> dHeavy;          // nodes left to the wave-per-node capture
+  DBuf<unsigned int> dRootBits;  // roots owning a kept path
   DBuf<unsigned long long> dCounters;
   DBuf<unsigned long long> dWinKey, dWinKey2;
   DBuf<int32_t> dWinRx, dWinRx2, dWinIdx, dWinIdx2, dWinNode;

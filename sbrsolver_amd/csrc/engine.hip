@@ -7,7 +7,7 @@
 //   2. k_trace<EXACT, LDS>: the interaction trees of every launch (RayLaunch, RTSolver.cpp:128-257),
 //      one persistent launch per tree depth.  Each launch reads its item count from device memory and
 //      appends the next depth's items with wave-aggregated atomics, so the bounce loop has no host
-//      round trip.  Small scenes are staged in LDS; the traversal stack always lives in LDS.
+//      round trip.  The BVH nodes and pre-test boxes of small scenes are staged in LDS; the traversal stack always lives in LDS.
 //   3. k_los: direct-path line of sight per (transmitter, receiver) (DirectPathCorrection,
 //      RTSolver.cpp:375-394).
 //   4. k_capture: every tree node against the receivers binned in a uniform xy grid
@@ -65,10 +65,12 @@ template struct DBuf<unsigned long long>;
 template struct DBuf<int64_t>;
 template struct DBuf<DNode>;
 template struct DBuf<DTri>;
+template struct DBuf<DBox32>;
 template struct DBuf<DSurf>;
 template struct DBuf<TNode>;
 template struct DBuf<TxDesc>;
 template struct DBuf<LevelRec>;
+template struct DBuf<unsigned int>;
 
 #define ENSURE(buf, cnt)                                                                         \
   do {                                                                                           \
@@ -158,14 +160,16 @@ struct TraceArgs {
   int32_t depth;
   int32_t nTx;
   int32_t stackDepth;
-  int32_t nodeBytes, triBytes;   // scene staged in LDS (LDS variant)
+  int32_t nodeBytes, boxBytes;   // nodes and triangle pre-test boxes staged in LDS (LDS variant)
+  float ext;                                // scene extent for the single-precision pre-test
   unsigned long long nodeCap, queueCap;
 };
 
 // Flags: [0] bit 0 node overflow, bit 1 queue overflow; [6], [7] node visits / triangle tests.
 template <bool EXACT, bool LDS>
-__global__ void __launch_bounds__(kBlock) k_trace(TraceArgs A, const TxDesc* __restrict__ txs,
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_trace(TraceArgs A, const TxDesc* __restrict__ txs,
                                                   const DNode* __restrict__ gNodes, const DTri* __restrict__ gTris,
+                                                  const DBox32* __restrict__ gBox,
                                                   const double* __restrict__ triN, const int32_t* __restrict__ qIn,
                                                   int32_t* __restrict__ qOut, TNode* __restrict__ tree,
                                                   LevelRec* __restrict__ lv, unsigned long long* __restrict__ flags,
@@ -173,17 +177,18 @@ __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs A, const TxDesc* __r
   extern __shared__ __align__(16) unsigned char smem[];
   const DNode* nodes = gNodes;
   const DTri* tris = gTris;
+  const DBox32* tbox = gBox;
   size_t sceneB = 0;
   if (LDS) {
     const int4* s0 = reinterpret_cast<const int4*>(gNodes);
     int4* d0 = reinterpret_cast<int4*>(smem);
     for (int i = threadIdx.x; i < A.nodeBytes / 16; i += blockDim.x) d0[i] = s0[i];
-    const int4* s1 = reinterpret_cast<const int4*>(gTris);
-    int4* d1 = reinterpret_cast<int4*>(smem + A.nodeBytes);
-    for (int i = threadIdx.x; i < A.triBytes / 16; i += blockDim.x) d1[i] = s1[i];
-    sceneB = (size_t)A.nodeBytes + A.triBytes;
+    const int4* s2 = reinterpret_cast<const int4*>(gBox);
+    int4* d2 = reinterpret_cast<int4*>(smem + A.nodeBytes);
+    for (int i = threadIdx.x; i < A.boxBytes / 16; i += blockDim.x) d2[i] = s2[i];
+    sceneB = (size_t)A.nodeBytes + A.boxBytes;
     nodes = reinterpret_cast<const DNode*>(smem);
-    tris = reinterpret_cast<const DTri*>(smem + A.nodeBytes);
+    tbox = reinterpret_cast<const DBox32*>(smem + A.nodeBytes);
     __syncthreads();
   }
   int* stkN = reinterpret_cast<int*>(smem + sceneB) + threadIdx.x;
@@ -205,14 +210,12 @@ __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs A, const TxDesc* __r
     int mask = 0;
     if (i < n && nodeIdx >= A.nodeCap) atomicOr(&flags[0], 1ull);
     if (i < n && nodeIdx < A.nodeCap) {
-      // The part of the record known before the traversal is stored first, so that only the ray,
-      // the running path length and the child permissions stay live across the traversal.
-      TNode* o = &tree[nodeIdx];
-      V3 src, dir;
+      // The record is assembled in registers and written with one 128-byte store (a partial-line
+      // store before the traversal and another after it costs a second write-back of the line).
+      TNode nd;
       double totalPL;
       int allow;
       {
-        TNode nd;
         const TxDesc* T;
         if (A.depth == 0) {
           int lo = 0, hi = A.nTx - 1;   // transmitter owning root item i
@@ -223,8 +226,9 @@ __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs A, const TxDesc* __r
           }
           T = &txs[lo];
           const int64_t launch = T->launchOffset + ((int64_t)i - T->launchStart) * T->launchStride;
-          src = v3(T->tx[0], T->tx[1], T->tx[2]);
-          dir = ld3(T->rays + 3 * launch);
+          const V3 d = ld3(T->rays + 3 * launch);
+          nd.src[0] = T->tx[0]; nd.src[1] = T->tx[1]; nd.src[2] = T->tx[2];
+          nd.dir[0] = d.x; nd.dir[1] = d.y; nd.dir[2] = d.z;
           nd.parent = -1;
           nd.gl = (int32_t)i;
           nd.code = 0;
@@ -240,10 +244,11 @@ __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs A, const TxDesc* __r
           const bool refl = q & 1;
           const TNode& P = tree[pi];
           T = &txs[P.txi];
-          src = ld3(P.tgt);
           const V3 pd = ld3(P.dir);
           // Reflect about the hit triangle's stored normal, then normalize (RTSolver.cpp:186-203).
-          dir = refl ? vnormalized(vreflect(pd, ld3(triN + 3 * (int64_t)P.hitDfs))) : pd;
+          const V3 d = refl ? vnormalized(vreflect(pd, ld3(triN + 3 * (int64_t)P.hitDfs))) : pd;
+          nd.src[0] = P.tgt[0]; nd.src[1] = P.tgt[1]; nd.src[2] = P.tgt[2];
+          nd.dir[0] = d.x; nd.dir[1] = d.y; nd.dir[2] = d.z;
           nd.parent = pi;
           nd.gl = P.gl;
           nd.code = (P.code << 1) | (refl ? 1u : 0u);
@@ -257,38 +262,35 @@ __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs A, const TxDesc* __r
         }
         const bool depthOk = T->maxTotal <= 0 || (nd.rc + nd.tc) < T->maxTotal;
         allow = ((nd.tc < T->maxT && depthOk) ? 1 : 0) | ((nd.rc < T->maxR && depthOk) ? 2 : 0);
-        o->src[0] = src.x; o->src[1] = src.y; o->src[2] = src.z;
-        o->dir[0] = dir.x; o->dir[1] = dir.y; o->dir[2] = dir.z;
-        o->plArg = nd.plArg;
-        o->trueLen = nd.trueLen;
-        o->parent = nd.parent;
-        o->gl = nd.gl;
-        o->code = nd.code;
-        o->depth = (uint8_t)A.depth;
-        o->type = nd.type;
-        o->rc = nd.rc;
-        o->tc = nd.tc;
-        o->txi = nd.txi;
-        o->pad = 0;
       }
+      nd.depth = (uint8_t)A.depth;
+      nd.pad = 0;
+      const V3 src = ld3(nd.src), dir = ld3(nd.dir);
       Hit h;
-      if (closest_hit<EXACT>(nodes, tris, src, dir, h, stkN, stkT, blockDim.x)) {
+      if (closest_hit<EXACT>(nodes, tris, tbox, A.ext, src, dir, h, stkN, stkT, blockDim.x)) {
         const DTri& t = tris[h.idx];
-        o->tgt[0] = h.p.x; o->tgt[1] = h.p.y; o->tgt[2] = h.p.z;
-        o->pl = totalPL + h.dist;
-        o->tri = t.id;
-        o->cop = t.cop;
-        o->hitDfs = h.idx;
+        const V3 hp = hit_point(src, dir, h);
+        nd.tgt[0] = hp.x; nd.tgt[1] = hp.y; nd.tgt[2] = hp.z;
+        nd.pl = totalPL + h.dist;
+        nd.tri = t.id;
+        nd.cop = t.cop;
+        nd.hitDfs = h.idx;
         mask = allow;   // transmitted child (bit 0, BFS order: first), reflected child (bit 1)
       } else {
         // A miss leaves a 100 m stub below the root, a 1e10 m segment at the root (RTSolver.cpp:224-256).
         const double len = A.depth == 0 ? 1E10 : 1E2;
         const V3 t = vadd(src, vmul(dir, len));
-        o->tgt[0] = t.x; o->tgt[1] = t.y; o->tgt[2] = t.z;
-        o->pl = INFINITY;
-        o->tri = -1;
-        o->cop = -1;
-        o->hitDfs = -1;
+        nd.tgt[0] = t.x; nd.tgt[1] = t.y; nd.tgt[2] = t.z;
+        nd.pl = INFINITY;
+        nd.tri = -1;
+        nd.cop = -1;
+        nd.hitDfs = -1;
+      }
+      {
+        const int4* srcw = reinterpret_cast<const int4*>(&nd);
+        int4* dstw = reinterpret_cast<int4*>(&tree[nodeIdx]);
+#pragma unroll
+        for (int w = 0; w < 8; w++) dstw[w] = srcw[w];
       }
       cntNodes += h.nodes;
       cntTests += h.tests;
@@ -328,6 +330,7 @@ __global__ void __launch_bounds__(kBlock) k_trace(TraceArgs A, const TxDesc* __r
 template <bool EXACT>
 __global__ void __launch_bounds__(kBlock) k_los(const TxDesc* __restrict__ txs, int32_t nTx,
                                                 const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
+                                                const DBox32* __restrict__ tbox, float ext,
                                                 const double* __restrict__ rx, int32_t m, int32_t stackDepth,
                                                 uint8_t* __restrict__ los) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -339,12 +342,13 @@ __global__ void __launch_bounds__(kBlock) k_los(const TxDesc* __restrict__ txs, 
   V3 tx = v3(T.tx[0], T.tx[1], T.tx[2]);
   V3 d = vsub(ld3(rx + 3 * (k % m)), tx);
   Hit h;
-  bool hh = closest_hit<EXACT>(nodes, tris, tx, d, h, stkN, stkT, blockDim.x);
+  bool hh = closest_hit<EXACT>(nodes, tris, tbox, ext, tx, d, h, stkN, stkT, blockDim.x);
   los[k] = (!hh || vnorm(d) < h.dist) ? 1 : 0;
 }
 
 template <bool EXACT>
 __global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
+                                                          const DBox32* __restrict__ tbox, float ext,
                                                           const double* __restrict__ org, const double* __restrict__ dirs,
                                                           int32_t n, int32_t stackDepth, int32_t* __restrict__ triOut,
                                                           double* __restrict__ ptOut, double* __restrict__ distOut) {
@@ -354,10 +358,11 @@ __global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restric
   int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Hit h;
-  bool hh = closest_hit<EXACT>(nodes, tris, ld3(org + 3 * (int64_t)i), ld3(dirs + 3 * (int64_t)i), h, stkN, stkT,
-                               blockDim.x);
+  bool hh = closest_hit<EXACT>(nodes, tris, tbox, ext, ld3(org + 3 * (int64_t)i), ld3(dirs + 3 * (int64_t)i), h,
+                               stkN, stkT, blockDim.x);
   triOut[i] = hh ? tris[h.idx].id : -1;
-  ptOut[3 * i] = h.p.x; ptOut[3 * i + 1] = h.p.y; ptOut[3 * i + 2] = h.p.z;
+  const V3 hp = hit_point(ld3(org + 3 * (int64_t)i), ld3(dirs + 3 * (int64_t)i), h);
+  ptOut[3 * i] = hp.x; ptOut[3 * i + 1] = hp.y; ptOut[3 * i + 2] = hp.z;
   distOut[i] = h.dist;
 }
 
@@ -385,19 +390,18 @@ __device__ __forceinline__ unsigned long long order_key(const TNode& nd) {
          (unsigned long long)(nd.code & ((1u << 27) - 1));
 }
 
-// Per-node state of the capture: the segment, the conservative search region and the path data
-// shared by every receiver tested against the node.
+// Per-node state of the capture: the segment, the conservative search region and the dedup key of
+// the path's coplanar-id sequence.  No per-node arrays: ancestors are reached through parent links,
+// and image sources are rebuilt only for the (rare) records that need a correction.
 struct NodeCtx {
-  V3 S, Tg, D, P0, P1;
+  V3 S, Tg, P0, P1;
   double plArg, trueLen, rc, R;
-  int model, depth;
-  bool all, chainOk;
+  int model, depth, txi;
+  int32_t node;
+  bool all;
   unsigned long long seqA, seqB, orderKey;
   int64_t losBase;
   int j0, j1;
-  int32_t chain[kMaxDepth + 1];
-  int32_t isurf[kMaxDepth + 1];
-  V3 img[kMaxDepth + 1];
 };
 
 struct CaptureBufs {
@@ -418,16 +422,18 @@ struct CaptureBufs {
 
 // Returns false when no receiver can be captured by the node.
 __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n, NodeCtx& c) {
-  const TNode nd = B.tree[n];
+  const TNode& nd = B.tree[n];
   const TxDesc& T = B.txs[nd.txi];
+  c.node = (int32_t)n;
   c.depth = nd.depth;
+  c.txi = nd.txi;
   c.model = T.receptionModel;
   c.rc = T.rcoef;
   c.plArg = nd.plArg;
   c.trueLen = nd.trueLen;
   c.S = ld3(nd.src);
   c.Tg = ld3(nd.tgt);
-  c.D = vsub(c.Tg, c.S);
+  const V3 D = vsub(c.Tg, c.S);
   // Conservative search radius: a capture at C has its closest point P within
   // (base + |C - ref|) * rc / (1 - rc) of C, with base/ref per reception model.
   double far2 = 0.0;
@@ -444,7 +450,7 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
   double u0 = 0.0, u1 = 1.0;
   if (!c.all) {
     for (int a = 0; a < 3; a++) {
-      double lo = A.g.bmin[a] - c.R, hi = A.g.bmax[a] + c.R, s = vget(c.S, a), d = vget(c.D, a);
+      double lo = A.g.bmin[a] - c.R, hi = A.g.bmax[a] + c.R, s = vget(c.S, a), d = vget(D, a);
       if (fabs(d) < 1e-300) {
         if (s < lo || s > hi) return false;
       } else {
@@ -455,8 +461,8 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
       }
     }
   }
-  c.P0 = vadd(c.S, vmul(c.D, u0));
-  c.P1 = vadd(c.S, vmul(c.D, u1));
+  c.P0 = vadd(c.S, vmul(D, u0));
+  c.P1 = vadd(c.S, vmul(D, u1));
   c.j0 = 0;
   c.j1 = A.g.ny - 1;
   if (!c.all) {
@@ -464,31 +470,20 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
     c.j0 = (int)fmax(0.0, floor((ylo - A.g.gy0) / A.g.cell));
     c.j1 = (int)fmin((double)(A.g.ny - 1), floor((yhi - A.g.gy0) / A.g.cell));
   }
-  // Path data: image sources and surfaces (GetImageSources, RTSolver.cpp:396-433) and the hash of
-  // the coplanar-id sequence of the ancestors (the dedup key without the receiver).
+  // Hash of the ancestors' coplanar ids, root first (the dedup key without the receiver).
+  int32_t anc[kMaxDepth];
   {
-    int64_t q = n;
-    for (int k = c.depth; k >= 0; k--) { c.chain[k] = (int32_t)q; q = B.tree[q].parent; }
+    int32_t q = nd.parent;
+    for (int k = c.depth - 1; k >= 0; k--) { anc[k] = B.tree[q].cop; q = B.tree[q].parent; }
   }
-  c.seqA = 0x243f6a8885a308d3ull ^ ((unsigned long long)c.depth << 48) ^ (unsigned long long)nd.txi;
-  c.seqB = 0x13198a2e03707344ull + (unsigned long long)c.depth * 0x51ull + ((unsigned long long)nd.txi << 20);
-  c.chainOk = true;
-  V3 cur = v3(T.tx[0], T.tx[1], T.tx[2]);
-  c.img[0] = cur;
-  c.isurf[0] = -1;
+  c.seqA = 0x243f6a8885a308d3ull ^ ((unsigned long long)c.depth << 48) ^ (unsigned long long)c.txi;
+  c.seqB = 0x13198a2e03707344ull + (unsigned long long)c.depth * 0x51ull + ((unsigned long long)c.txi << 20);
   for (int k = 0; k < c.depth; k++) {
-    const TNode& a = B.tree[c.chain[k]];
-    const TNode& b = B.tree[c.chain[k + 1]];
-    c.seqA = mix64(c.seqA + (unsigned long long)(uint32_t)a.cop + 0x9e3779b97f4a7c15ull);
-    c.seqB = mix64(c.seqB ^ ((unsigned long long)(uint32_t)a.cop * 0xd6e8feb86659fd93ull + (unsigned long long)k));
-    if (a.tri < 0) { c.chainOk = false; break; }
-    if (b.type == 1) cur = image_point(B.surf[a.tri], cur);
-    else if (b.type != 2) { c.chainOk = false; break; }
-    c.img[k + 1] = cur;
-    c.isurf[k + 1] = a.tri;
+    c.seqA = mix64(c.seqA + (unsigned long long)(uint32_t)anc[k] + 0x9e3779b97f4a7c15ull);
+    c.seqB = mix64(c.seqB ^ ((unsigned long long)(uint32_t)anc[k] * 0xd6e8feb86659fd93ull + (unsigned long long)k));
   }
   c.orderKey = order_key(nd);
-  c.losBase = (int64_t)nd.txi * A.nrx;
+  c.losBase = (int64_t)c.txi * A.nrx;
   return true;
 }
 
@@ -518,14 +513,34 @@ __device__ __forceinline__ bool row_range(const CaptureArgs& A, const NodeCtx& c
   return *i0 <= *i1;
 }
 
+// Image source k of the path ending at `leaf` (GetImageSources, RTSolver.cpp:396-433): the
+// transmitter mirrored across the surfaces of its reflections among the first k interactions.
+__device__ V3 image_source(const CaptureBufs& B, const TxDesc& T, int32_t leaf, int depth, int k, int32_t* surfOut) {
+  V3 cur = v3(T.tx[0], T.tx[1], T.tx[2]);
+  for (int m = 1; m <= k; m++) {
+    // node at depth m - 1 (its hit surface) and node at depth m (its interaction type)
+    int32_t b = leaf;
+    for (int up = depth; up > m; up--) b = B.tree[b].parent;
+    const TNode& nb = B.tree[b];
+    const TNode& na = B.tree[nb.parent];
+    if (nb.type == 1) cur = image_point(B.surf[na.tri], cur);
+    if (m == k) *surfOut = na.tri;
+  }
+  return cur;
+}
+
 // One receiver against one node: reception sphere, ancestor rule, image correction, dedup.
 __device__ void capture_one(const CaptureArgs& A, const CaptureBufs& B, const NodeCtx& c, int32_t r,
                             unsigned long long& myRec, unsigned long long& myCor, unsigned long long& myUsed) {
   const V3 C = ld3(B.rx + 3 * (int64_t)r);
   if (!sphere_hit(c.S, c.Tg, C, c.plArg, c.trueLen, c.model, c.rc)) return;
-  for (int k = c.depth - 1; k >= 0; k--) {
-    const TNode& a = B.tree[c.chain[k]];
-    if (sphere_hit(ld3(a.src), ld3(a.tgt), C, a.plArg, a.trueLen, c.model, c.rc)) return;
+  {
+    int32_t q = B.tree[c.node].parent;
+    while (q >= 0) {
+      const TNode& a = B.tree[q];
+      if (sphere_hit(ld3(a.src), ld3(a.tgt), C, a.plArg, a.trueLen, c.model, c.rc)) return;
+      q = a.parent;
+    }
   }
   myRec++;
   unsigned long long hA = mix64(c.seqA ^ ((unsigned long long)(uint32_t)r * 0x9e3779b97f4a7c15ull));
@@ -546,20 +561,22 @@ __device__ void capture_one(const CaptureArgs& A, const CaptureBufs& B, const No
       slot = (slot + 1) & A.tabMask;
     }
   }
-  // Image correction (MultiPathCorrection / DirectPathCorrection).
-  bool ok;
+  // Image correction (MultiPathCorrection / DirectPathCorrection): walk back from the receiver,
+  // re-intersecting each reflection / transmission surface from its image source.
+  bool ok = true;
   if (c.depth == 0) {
     ok = B.los[c.losBase + r] != 0;
-  } else if (!c.chainOk) {
-    ok = false;
   } else {
-    ok = true;
+    const TxDesc& T = B.txs[c.txi];
     V3 target = C;
     for (int k = c.depth; k >= 1 && ok; k--) {
-      const DSurf& s = B.surf[c.isurf[k]];
+      int32_t sid = -1;
+      const V3 img = image_source(B, T, c.node, c.depth, k, &sid);
+      if (sid < 0) { ok = false; break; }
+      const DSurf& s = B.surf[sid];
       V3 p;
       double dist;
-      ok = tri_intersect(ld3(s.v1), ld3(s.e1), ld3(s.e2), c.img[k], vsub(target, c.img[k]), &p, &dist);
+      ok = tri_intersect(ld3(s.v1), ld3(s.e1), ld3(s.e2), img, vsub(target, img), &p, &dist);
       target = p;
     }
   }
@@ -664,11 +681,21 @@ __global__ void k_collect(const unsigned long long* __restrict__ tabA, const uns
   winRx[w] = tabRx[s];
 }
 
-// Maps every kept path's order key (sorted) to its tree node.
+// Marks the roots that own a kept path.
+__global__ void k_mark_roots(const unsigned long long* __restrict__ keys, int64_t W, unsigned int* __restrict__ bits) {
+  int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const unsigned int gl = (unsigned int)(keys[w] >> 32);
+  atomicOr(&bits[gl >> 5], 1u << (gl & 31));
+}
+
+// Maps every kept path's order key (sorted) to its tree node; nodes of unmarked roots leave at once.
 __global__ void k_resolve(const TNode* __restrict__ tree, int64_t nodes, const unsigned long long* __restrict__ keys,
-                          int64_t W, int32_t* __restrict__ winNode) {
+                          int64_t W, const unsigned int* __restrict__ bits, int32_t* __restrict__ winNode) {
   int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nodes) return;
+  const unsigned int gl = (unsigned int)tree[n].gl;
+  if (!((bits[gl >> 5] >> (gl & 31)) & 1u)) return;
   const unsigned long long key = order_key(tree[n]);
   int64_t lo = 0, hi = W;
   while (lo < hi) {
@@ -1002,13 +1029,13 @@ int Engine::closest_hit(const double* o, const double* d, int n, int exact, int3
   if (!dO.p || !dD.p || !dP.p || !dDist.p || !dT.p) return fail(SBR_E_NOMEM, "closest_hit buffers");
   HIPCHK(hipMemcpyAsync(dO.p, o, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(dD.p, d, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
-  const size_t sm = stack_bytes(treeHeight, kBlock);
+  const size_t sm = stack_bytes(stackDepth(), kBlock);
   if (exact)
-    k_closest_batch<true><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dTris.p, dO.p, dD.p, n, treeHeight,
-                                                                         dT.p, dP.p, dDist.p);
+    k_closest_batch<true><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dTris.p, dTriBox32.p, sceneExtent,
+                                                                         dO.p, dD.p, n, stackDepth(), dT.p, dP.p, dDist.p);
   else
-    k_closest_batch<false><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dTris.p, dO.p, dD.p, n, treeHeight,
-                                                                          dT.p, dP.p, dDist.p);
+    k_closest_batch<false><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dTris.p, dTriBox32.p, sceneExtent,
+                                                                          dO.p, dD.p, n, stackDepth(), dT.p, dP.p, dDist.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(tri, dT.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(pt, dP.p, 3 * sizeof(double) * n, hipMemcpyDeviceToHost, stream));
@@ -1024,21 +1051,28 @@ static double ms_since(std::chrono::steady_clock::time_point t) {
 // Bounce loop: one persistent k_trace launch per depth, no host synchronisation in between.  Returns
 // SBR_E_LIMIT (with nodeCap grown) when the node or queue buffers overflowed: run it again.
 int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t nRoot, int maxDepth, bool exact, bool count) {
-  bool lds = sceneBytes + stack_bytes(treeHeight, kBlock) <= kSceneLdsMax;
-  if (tune.ldsScene == 0) lds = false;
-  const size_t smem = (lds ? sceneBytes : 0) + stack_bytes(treeHeight, kBlock);
+  // Scene in LDS or read through the caches: automatic choice takes LDS staging unless it lowers the
+  // number of resident blocks per CU.
+  const size_t stk = stack_bytes(stackDepth(), kBlock);
+  const bool ldsFits = sceneBytes + stk <= kSceneLdsMax;
+  auto variant = [&](bool l) {
+    return exact ? (l ? k_trace<true, true> : k_trace<true, false>) : (l ? k_trace<false, true> : k_trace<false, false>);
+  };
+  int occGlobal = 1, occLds = 0;
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occGlobal, variant(false), kBlock, stk));
+  if (ldsFits) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occLds, variant(true), kBlock, sceneBytes + stk));
+  bool lds = ldsFits && (tune.ldsScene == 1 || (tune.ldsScene < 0 && occLds >= occGlobal));
+  const size_t smem = (lds ? sceneBytes : 0) + stk;
   TraceArgs A{};
   A.nTx = (int32_t)txs.size();
-  A.stackDepth = treeHeight;
+  A.stackDepth = stackDepth();
   A.nodeBytes = (int32_t)(hNodes.size() * sizeof(DNode));
-  A.triBytes = (int32_t)(hTris.size() * sizeof(DTri));
+  A.boxBytes = (int32_t)(hTris.size() * sizeof(DBox32));
+  A.ext = sceneExtent;
   A.nodeCap = nodeCap;
   A.queueCap = nodeCap;
-  auto kern = exact ? (lds ? k_trace<true, true> : k_trace<true, false>)
-                    : (lds ? k_trace<false, true> : k_trace<false, false>);
-  int perCU = 1;
-  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kern, kBlock, smem));
-  perCU = std::max(1, perCU);
+  auto kern = variant(lds);
+  int perCU = std::max(1, lds ? occLds : occGlobal);
   if (tune.blocksPerCU > 0) perCU = std::min(perCU, tune.blocksPerCU);
   const unsigned grid = (unsigned)std::min<int64_t>((int64_t)numCU * perCU, blocks_for(std::max<int64_t>(nRoot, 1), kBlock));
   stats.trace_lds = lds ? 1 : 0;
@@ -1056,8 +1090,8 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t nRoot, int maxDe
     A.depth = d;
     const int32_t* qin = dQueue[d & 1].p;
     int32_t* qout = dQueue[(d + 1) & 1].p;
-    kern<<<grid, kBlock, smem, stream>>>(A, dTx.p, dNodes.p, dTris.p, dTriN.p, qin, qout, dTree.p, dLevels.p,
-                                         dCounters.p, count ? 1 : 0);
+    kern<<<grid, kBlock, smem, stream>>>(A, dTx.p, dNodes.p, dTris.p, dTriBox32.p, dTriN.p, qin, qout, dTree.p,
+                                         dLevels.p, dCounters.p, count ? 1 : 0);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(kev[1], stream));
@@ -1221,13 +1255,13 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     HIPCHK(hipMemcpyAsync(dCellStart.p, cellCount.data(), sizeof(int32_t) * cellCount.size(), hipMemcpyHostToDevice,
                           stream));
     HIPCHK(hipMemcpyAsync(dCellRx.p, cellRx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, stream));
-    const size_t sm = stack_bytes(treeHeight, kBlock);
+    const size_t sm = stack_bytes(stackDepth(), kBlock);
     if (exact)
-      k_los<true><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dTris.p, dRx.p, m, treeHeight,
-                                                                     dLos.p);
+      k_los<true><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dTris.p, dTriBox32.p,
+                                                                     sceneExtent, dRx.p, m, stackDepth(), dLos.p);
     else
-      k_los<false><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dTris.p, dRx.p, m, treeHeight,
-                                                                      dLos.p);
+      k_los<false><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dTris.p, dTriBox32.p,
+                                                                      sceneExtent, dRx.p, m, stackDepth(), dLos.p);
     HIPCHK(hipGetLastError());
 
     const DSurf* surf = P[0].q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
@@ -1314,7 +1348,12 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       // by order key -> dWinKey2 (keys), dWinRx2 (tx * nrx + rx)
       HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb1, dWinKey.p, dWinKey2.p, dWinRx.p, dWinRx2.p,
                                                 (int)winners, 0, 64, stream));
-      k_resolve<<<blocks_for(nodes, 256), 256, 0, stream>>>(dTree.p, nodes, dWinKey2.p, winners, dWinNode.p);
+      const int64_t words = (stats.launches + 31) / 32 + 1;
+      ENSURE(dRootBits, words);
+      HIPCHK(hipMemsetAsync(dRootBits.p, 0, words * sizeof(unsigned int), stream));
+      k_mark_roots<<<blocks_for(winners, 256), 256, 0, stream>>>(dWinKey2.p, winners, dRootBits.p);
+      k_resolve<<<blocks_for(nodes, 256), 256, 0, stream>>>(dTree.p, nodes, dWinKey2.p, winners, dRootBits.p,
+                                                             dWinNode.p);
       k_iota<<<blocks_for(winners, 256), 256, 0, stream>>>(dWinIdx.p, winners);
       HIPCHK(hipGetLastError());
       // stable by tx * nrx + rx -> dWinRx (keys), dWinIdx2 (index into the key-sorted arrays)
