@@ -23,12 +23,19 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# Algorithmic bytes of the closest-hit kernel (k_trace_level): per bounce the parent record is read
-# and the new record written (128 B each) plus its queue entry and child mask (8 B); per BVH node
-# entered one 64 B node; per ray-triangle test one 96 B triangle record.
-B_PER_BOUNCE = 128 + 128 + 8
-B_PER_NODE = 64
-B_PER_TRI = 96
+# Algorithmic bytes of one bounce (one closest-hit query, the work of k_trace):
+#   B = 32 (ray in) + 32 (segment out) + 32 * nodes + 48 * triangles,
+# with nodes / triangles per bounce those of a canonical closest-first, t-clipped BVH2 traversal
+# of the same tree, counted by the oracle over the whole workload and committed as a fixture
+# (tests/golden/traversal_counts.json, written by tests/golden/make_traversal_counts.py).
+B_RAY_IN, B_SEG_OUT, B_PER_NODE, B_PER_TRI = 32, 32, 32, 48
+# VALU model of the same traversal: ~22 FP64 operations per box test, 2 box tests per node
+# entered, ~40 per ray-triangle test.
+OPS_PER_NODE, OPS_PER_TRI = 44, 40
+COUNTS = os.path.join(ROOT, "tests", "golden", "traversal_counts.json")
+# HBM traffic of one k_trace launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the
+# readings for a copy of known size (line "traffic_bytes_per_launch: N" in this summary).
+TRAFFIC = os.path.join(ROOT, "profiles", "k_trace_traffic_urban_L9_R6T1.md")
 
 
 def host_cores():
@@ -78,7 +85,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--tess", type=int, default=9)
     ap.add_argument("--model", type=int, default=1, help="reception model: 1 = alpha*L/sqrt(3), 0 = reference")
-    ap.add_argument("--cpu-stride", type=int, default=256)
+    ap.add_argument("--cpu-stride", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--also-ref-model", action="store_true", help="also time the reference reception model")
     ap.add_argument("--calib", action="store_true",
@@ -97,7 +104,7 @@ def main():
     ctx.materials(cfg["materials"])
     print(f"[bench] scene {cfg['name']} uploaded; warmup {args.warmup}", file=sys.stderr, flush=True)
 
-    # Traversal counts for the algorithmic bytes (one extra, untimed step).
+    # The kernel's own traversal counts (one extra, untimed step; reported beside the canonical ones).
     cnt = ctx.trace(make_params(**trace_kw, countTraversal=1), cfg["rx"]).stats
     for _ in range(args.warmup):
         ctx.trace(make_params(**trace_kw), cfg["rx"])
@@ -114,28 +121,38 @@ def main():
     value = bounces * args.steps / dt / 1e6
     ms_step = dt / args.steps * 1e3
 
-    # Dominant kernel and its roofline (kernel times from HIP events on the library's stream).
-    k_trace_ms = np.mean([s["ms_k_trace"] for s in stats])
-    k_trace_n = stats[-1]["n_k_trace"]
-    k_cap_ms = np.mean([s["ms_k_capture"] for s in stats])
-    trace_bytes = bounces * B_PER_BOUNCE + cnt["node_visits"] * B_PER_NODE + cnt["tri_tests"] * B_PER_TRI
+    # Dominant kernel (k_trace: one launch per tree depth) and its roofline.  Kernel time is the
+    # HIP-event time of the launches on the library's stream, averaged per launch.
+    k_trace_ms = float(np.mean([s["ms_k_trace"] for s in stats]))
+    k_trace_n = int(stats[-1]["n_k_trace"])
+    k_cap_ms = float(np.mean([s["ms_k_capture"] for s in stats]))
     stage = {k: float(np.mean([s[k] for s in stats])) for k in ("ms_raygen", "ms_trace", "ms_capture", "ms_field",
                                                                   "ms_total")}
-    if k_trace_ms >= k_cap_ms:
-        per_launch_bytes = trace_bytes / k_trace_n
-        avg_ms = k_trace_ms / k_trace_n
-        dom = "k_trace_level"
-    else:
-        per_launch_bytes = bounces * 128.0   # one tree-node record per node read by the capture kernel
-        avg_ms = k_cap_ms
-        dom = "k_capture"
+    with open(COUNTS) as f:
+        canon = json.load(f)[cfg["name"]]
+    assert canon["bounces"] == bounces, "committed traversal counts are for another workload"
+    b_bounce = B_RAY_IN + B_SEG_OUT + B_PER_NODE * canon["nodes_per_bounce"] + B_PER_TRI * canon["tris_per_bounce"]
+    per_launch_bytes = bounces * b_bounce / k_trace_n
+    avg_ms = k_trace_ms / k_trace_n
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC):
+        import re
+        m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(TRAFFIC).read())
+        traffic = float(m.group(1)) if m else None
+    ops = bounces * (OPS_PER_NODE * canon["nodes_per_bounce"] + OPS_PER_TRI * canon["tris_per_bounce"])
+    valu_tflops = ops / (k_trace_ms * 1e-3) / 1e12
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=None, kernel=dom,
-                    avg_launch_ms=round(avg_ms, 4), algorithmic_bytes_per_launch=int(per_launch_bytes),
-                    node_visits=int(cnt["node_visits"]), tri_tests=int(cnt["tri_tests"]),
-                    k_trace_ms_per_step=round(float(k_trace_ms), 3), k_trace_launches=int(k_trace_n),
-                    k_capture_ms_per_step=round(float(k_cap_ms), 3))
+                    frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic, kernel="k_trace",
+                    avg_launch_ms=round(avg_ms, 4), launches_per_step=k_trace_n,
+                    bytes_per_bounce=round(b_bounce, 1), algorithmic_bytes_per_launch=int(per_launch_bytes),
+                    canonical_nodes_per_bounce=round(canon["nodes_per_bounce"], 4),
+                    canonical_tris_per_bounce=round(canon["tris_per_bounce"], 4),
+                    kernel_nodes_per_bounce=round(cnt["node_visits"] / bounces, 4),
+                    kernel_exact_tri_tests_per_bounce=round(cnt["tri_tests"] / bounces, 4),
+                    valu_model=dict(achieved=round(valu_tflops, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s (FP64)",
+                                    frac=round(valu_tflops / FP64_PEAK_TFLOPS, 5)),
+                    k_capture_ms_per_step=round(k_cap_ms, 3))
 
     out = {
         "metric": "Mray-bounces/s",

@@ -21,7 +21,7 @@ class Params(C.Structure):
         ("freq", C.c_double), ("pt", C.c_double),
         ("useFresnel", C.c_int), ("tess", C.c_int), ("antennaType", C.c_int),
         ("receptionModel", C.c_int), ("launchStride", C.c_int), ("launchOffset", C.c_int),
-        ("q1MeshIndexing", C.c_int),
+        ("q1MeshIndexing", C.c_int), ("countCanonical", C.c_int),
     ]
 
 
@@ -47,7 +47,8 @@ def lib():
                   "orc_result_npaths", "orc_result_nsegs", "orc_trees_n"):
             getattr(L, f).argtypes = [vp]
             getattr(L, f).restype = C.c_int
-        for f in ("orc_result_bounces", "orc_result_node_visits", "orc_result_capture_records"):
+        for f in ("orc_result_bounces", "orc_result_node_visits", "orc_result_capture_records",
+                  "orc_result_canon_nodes", "orc_result_canon_tris"):
             getattr(L, f).argtypes = [vp]
             getattr(L, f).restype = C.c_longlong
         L.orc_result_seconds.argtypes = [vp]
@@ -73,7 +74,8 @@ def _p(a, t):
 
 
 def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None, maxTotal=0, useFresnel=True,
-                antennaType=1, receptionModel=0, launchStride=1, launchOffset=0, q1MeshIndexing=1):
+                antennaType=1, receptionModel=0, launchStride=1, launchOffset=0, q1MeshIndexing=1,
+                countCanonical=0):
     p = Params()
     for i in range(3):
         p.tx[i] = float(tx[i])
@@ -85,6 +87,7 @@ def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None
     p.useFresnel, p.tess, p.antennaType = int(bool(useFresnel)), int(tess), int(antennaType)
     p.receptionModel, p.launchStride, p.launchOffset = int(receptionModel), int(launchStride), int(launchOffset)
     p.q1MeshIndexing = int(q1MeshIndexing)
+    p.countCanonical = int(countCanonical)
     return p
 
 
@@ -175,6 +178,8 @@ class Result:
         L = lib()
         self.bounces = L.orc_result_bounces(h)
         self.node_visits = L.orc_result_node_visits(h)
+        self.canon_nodes = L.orc_result_canon_nodes(h)   # counting traversal (countCanonical=1)
+        self.canon_tris = L.orc_result_canon_tris(h)
         self.capture_records = L.orc_result_capture_records(h)
         self.seconds = L.orc_result_seconds(h)
         np_, ns = L.orc_result_npaths(h), L.orc_result_nsegs(h)

@@ -221,11 +221,26 @@ struct BNode {
     }
     return true;
   }
+  // The same slab test, also returning the final [t_min, t_max] (counting traversal only).
+  bool LineInterval(V3 o, V3 d, double* lo, double* hi) const {
+    double t_max = kINF, t_min = -kINF;
+    for (int i = 0; i < 3; i++) {
+      double t1 = (bmin[i] - o[i]) / d[i];
+      double t2 = (bmax[i] - o[i]) / d[i];
+      t_max = smin(t_max, smax(t1, t2));
+      t_min = smax(t_min, smin(t1, t2));
+      if (t_min > t_max) return false;
+    }
+    *lo = t_min;
+    *hi = t_max;
+    return true;
+  }
 };
 
 struct BVH {
   std::vector<BNode> nodes;
   long long nodeVisits = 0;  // counting mode (not in the reference)
+  long long canonNodes = 0, canonTris = 0;
   BVH() { nodes.resize(1); }  // BVH.h:39-43
 
   // BVH.h:247-267
@@ -289,6 +304,42 @@ struct BVH {
       std::vector<Tri*> right(objects.begin() + bestSplit, objects.end());
       Build(left, nodes[nodeIdx].child[0]);
       Build(right, nodes[nodeIdx].child[1]);
+    }
+  }
+
+  // Counting mode (not in the reference): a canonical closest-first, t-clipped traversal of the
+  // same tree -- boxes entered nearest entry first, boxes wholly behind the origin or entered beyond
+  // the best hit skipped -- counting boxes entered and triangles tested.  It defines the work of one
+  // bounce for the algorithmic byte count; its hit is not used.
+  void CanonicalCount(V3 o, V3 d) {
+    double best = kINF, lo, hi;
+    if (!nodes[0].LineInterval(o, d, &lo, &hi) || hi < 0.0) return;
+    std::vector<std::pair<int, double>> st{{0, lo}};
+    while (!st.empty()) {
+      const std::pair<int, double> top = st.back();
+      st.pop_back();
+      if (top.second > best) continue;
+      canonNodes++;
+      const BNode& n = nodes[top.first];
+      if (n.child[0] == -1) {
+        for (Tri* obj : n.objects) {
+          canonTris++;
+          HitInfo cur;
+          if (obj->RayIntersects(o, d, cur) && cur.distance < best) best = cur.distance;
+        }
+        continue;
+      }
+      double l0, h0, l1, h1;
+      const bool in0 = nodes[n.child[0]].LineInterval(o, d, &l0, &h0) && h0 >= 0.0 && l0 <= best;
+      const bool in1 = nodes[n.child[1]].LineInterval(o, d, &l1, &h1) && h1 >= 0.0 && l1 <= best;
+      if (in0 && in1) {
+        if (l1 < l0) { st.push_back({n.child[0], l0}); st.push_back({n.child[1], l1}); }
+        else { st.push_back({n.child[1], l1}); st.push_back({n.child[0], l0}); }
+      } else if (in0) {
+        st.push_back({n.child[0], l0});
+      } else if (in1) {
+        st.push_back({n.child[1], l1});
+      }
     }
   }
 
@@ -643,6 +694,7 @@ struct Params {
   int receptionModel;     // 0: reference (RTSolver.cpp:838-841), 1: alpha*L/sqrt(3), alpha = 69deg/2^n
   int launchStride, launchOffset;
   int q1MeshIndexing;     // 1: mesh[hitSurfaceID] on the BVH-sorted vector (reference), 0: by id
+  int countCanonical = 0; // 1: also run BVH::CanonicalCount per bounce (work count only)
 };
 
 struct Tracer {
@@ -663,6 +715,7 @@ struct Tracer {
     bounces++;
     HitInfo hit;
     bool hasHit = bvh.RayIntersects(0, src, dir, hit);
+    if (p.countCanonical) bvh.CanonicalCount(src, dir);
     if (hasHit) {
       double angle = AngleBetween(hit.normal, dir, false, false);
       PTN* nn = node;
@@ -905,6 +958,7 @@ struct Tracer {
 struct Result {
   long long bounces = 0;
   long long nodeVisits = 0;
+  long long canonNodes = 0, canonTris = 0;   // counting traversal (countCanonical)
   long long captureRecords = 0;
   std::vector<double> fields;   // 6 per receiver
   // kept paths, rx ascending, launch ascending, BFS order
@@ -940,7 +994,7 @@ static Result* Trace(Scene* sc, const Params& p, const double* rx, int nrx, cons
   const int nsel = (int)launches.size();
   // per selected launch: list of (rx, paths)
   std::vector<std::vector<std::pair<int, std::vector<Path>>>> perLaunch(nsel);
-  std::vector<long long> bl(nsel, 0), nv(nsel, 0);
+  std::vector<long long> bl(nsel, 0), nv(nsel, 0), cn(nsel, 0), ct(nsel, 0);
   V3 txp = mk(p.tx[0], p.tx[1], p.tx[2]);
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -955,7 +1009,7 @@ static Result* Trace(Scene* sc, const Params& p, const double* rx, int nrx, cons
       int i = launches[li];
       PTN root;
       long long b = 0;
-      long long v0 = bvh.nodeVisits;
+      long long v0 = bvh.nodeVisits, c0 = bvh.canonNodes, t0c = bvh.canonTris;
       tr.RayLaunch(bvh, &root, txp, dirs[i], -1, -1, 0, 0, 0, true, kINF, 0.0, b);
       bl[li] = b;
       for (int j = 0; j < nrx; j++) {
@@ -970,9 +1024,16 @@ static Result* Trace(Scene* sc, const Params& p, const double* rx, int nrx, cons
       DeleteChildNodes(root.childTransmit);
       DeleteChildNodes(root.childReflect);
       nv[li] = bvh.nodeVisits - v0;
+      cn[li] = bvh.canonNodes - c0;
+      ct[li] = bvh.canonTris - t0c;
     }
   }
-  for (int li = 0; li < nsel; li++) { res->bounces += bl[li]; res->nodeVisits += nv[li]; }
+  for (int li = 0; li < nsel; li++) {
+    res->bounces += bl[li];
+    res->nodeVisits += nv[li];
+    res->canonNodes += cn[li];
+    res->canonTris += ct[li];
+  }
   // per receiver: (launch, path) in launch order
   std::vector<std::vector<std::pair<int, Path>>> perRx(nrx);
   for (int li = 0; li < nsel; li++)
@@ -1097,6 +1158,7 @@ typedef struct {
   int maxR, maxT, maxTotal;
   double freq, pt;
   int useFresnel, tess, antennaType, receptionModel, launchStride, launchOffset, q1MeshIndexing;
+  int countCanonical;
 } orc_params;
 
 void* orc_scene_from_stl(const char* path) {
@@ -1225,11 +1287,14 @@ void* orc_trace(void* sp, const orc_params* pp, const double* rx, int nrx, const
   p.freq = pp->freq; p.pt = pp->pt; p.useFresnel = pp->useFresnel; p.tess = pp->tess;
   p.antennaType = pp->antennaType; p.receptionModel = pp->receptionModel;
   p.launchStride = pp->launchStride; p.launchOffset = pp->launchOffset; p.q1MeshIndexing = pp->q1MeshIndexing;
+  p.countCanonical = pp->countCanonical;
   return Trace((Scene*)sp, p, rx, nrx, mats, nmat, nthreads);
 }
 void orc_result_free(void* r) { delete (Result*)r; }
 long long orc_result_bounces(void* r) { return ((Result*)r)->bounces; }
 long long orc_result_node_visits(void* r) { return ((Result*)r)->nodeVisits; }
+long long orc_result_canon_nodes(void* r) { return ((Result*)r)->canonNodes; }
+long long orc_result_canon_tris(void* r) { return ((Result*)r)->canonTris; }
 long long orc_result_capture_records(void* r) { return ((Result*)r)->captureRecords; }
 double orc_result_seconds(void* r) { return ((Result*)r)->seconds; }
 void orc_result_fields(void* r, double* out) {

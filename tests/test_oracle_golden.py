@@ -222,3 +222,24 @@ def test_planted_mistake_is_caught():
     b = O.trace(sc, O.make_params(**tr), cfg["rx"], bad)
     with pytest.raises(AssertionError):
         _worst(b.fields, a.fields, "planted")
+
+
+def test_canonical_traversal_counts_fixture():
+    """tests/golden/traversal_counts.json (the per-bounce work behind bench.py's algorithmic bytes):
+    the counting pass leaves the trace unchanged, a sampled recount agrees with the committed
+    whole-workload figures, and the canonical traversal enters no more boxes than the reference's
+    visit-every-crossed-box DFS."""
+    import json
+    with open(os.path.join(GOLDEN, "traversal_counts.json")) as f:
+        fx = json.load(f)["urban_canyon_482tri_L9_R6T1"]
+    cfg = scenes.config2(tess=9)
+    sc = O.Scene.from_soup(*cfg["soup"])
+    kw = dict(cfg["trace"], receptionModel=1, launchStride=257)
+    a = O.trace(sc, O.make_params(**kw, countCanonical=1), cfg["rx"][:50], cfg["materials"], nthreads=8)
+    b = O.trace(sc, O.make_params(**kw), cfg["rx"][:50], cfg["materials"], nthreads=8)
+    assert a.bounces == b.bounces and b.canon_nodes == 0 and b.canon_tris == 0
+    assert np.array_equal(a.fields, b.fields, equal_nan=True)
+    assert abs(a.canon_nodes / a.bounces - fx["nodes_per_bounce"]) < 0.03 * fx["nodes_per_bounce"]
+    assert abs(a.canon_tris / a.bounces - fx["tris_per_bounce"]) < 0.03 * fx["tris_per_bounce"]
+    assert a.canon_nodes <= a.node_visits
+    assert fx["canon_nodes"] / fx["bounces"] == fx["nodes_per_bounce"]
