@@ -78,6 +78,50 @@ def calib_copy(nbytes):
     print(f"[bench] calibration copy of {nbytes} bytes done", file=sys.stderr, flush=True)
 
 
+def batched_run(ctx, make_params, scenes, root_budget=1 << 25):
+    """Batched multi-transmitter workload (256 transmitters on the urban scene, levels 6-9, 4-6
+    reflections, 10k shared receivers): sbr_trace_batch over groups of transmitters holding at most
+    root_budget launches each, against one sbr_trace per transmitter on the same inputs."""
+    cfg = scenes.config4()
+    plist = [make_params(**t) for t in cfg["txs"]]
+    groups, cur, roots = [], [], 0
+    for p in plist:
+        n = 10 * 4 ** p.tess + 2
+        if cur and roots + n > root_budget:
+            groups.append(cur)
+            cur, roots = [], 0
+        cur.append(p)
+        roots += n
+    if cur:
+        groups.append(cur)
+
+    def run_batched():
+        b = 0
+        for g in groups:
+            b += ctx.trace_batch(g, cfg["rx"]).stats["bounces"]
+        return b
+
+    def run_single():
+        b = 0
+        for p in plist:
+            b += ctx.trace(p, cfg["rx"]).stats["bounces"]
+        return b
+
+    run_batched()
+    t0 = time.perf_counter()
+    bb = run_batched()
+    tb = time.perf_counter() - t0
+    run_single()
+    t0 = time.perf_counter()
+    bs = run_single()
+    ts = time.perf_counter() - t0
+    assert bb == bs
+    return dict(workload=cfg["name"], transmitters=len(plist), batches=len(groups), launches=int(
+        sum(10 * 4 ** p.tess + 2 for p in plist)), bounces=int(bb), batched_mrays=round(bb / tb / 1e6, 3),
+        batched_s=round(tb, 4), single_tx_mrays=round(bs / ts / 1e6, 3), single_tx_s=round(ts, 4),
+        speedup=round(ts / tb, 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +132,8 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--also-ref-model", action="store_true", help="also time the reference reception model")
+    ap.add_argument("--batch", action="store_true",
+                    help="also run the 256-transmitter batched workload against per-transmitter traces")
     ap.add_argument("--calib", action="store_true",
                     help="run one plain 256 MiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -182,6 +228,9 @@ def main():
         out["reference_reception_model"] = dict(value=round(s0["bounces"] / d0 / 1e6, 3), ms_per_step=round(d0 * 1e3, 3),
                                                 paths_kept=int(s0["winners"]),
                                                 capture_records=int(s0["capture_records"]))
+    if args.batch:
+        print("[bench] batched multi-transmitter workload", file=sys.stderr, flush=True)
+        out["batched"] = batched_run(ctx, make_params, scenes)
     if args.calib:
         calib_copy(256 << 20)
     if not args.no_cpu:

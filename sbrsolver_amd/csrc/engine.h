@@ -183,6 +183,7 @@ struct Engine {
   int64_t nodes = 0;
   int64_t winners = 0;
   size_t nodeCap = 0;
+  size_t tabCapHint = 0;   // dedup table size of the last trace
   std::vector<double> hField;
   RxGrid grid{};
 

@@ -27,6 +27,7 @@
 #include <chrono>
 #include <cstring>
 
+#include "dedup_hash.h"
 #include "device_common.h"
 #include "engine.h"
 
@@ -376,15 +377,6 @@ struct CaptureArgs {
   unsigned long long tabMask;
 };
 
-__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
-  x ^= x >> 30;
-  x *= 0xbf58476d1ce4e5b9ull;
-  x ^= x >> 27;
-  x *= 0x94d049bb133111ebull;
-  x ^= x >> 31;
-  return x;
-}
-
 __device__ __forceinline__ unsigned long long order_key(const TNode& nd) {
   return ((unsigned long long)(uint32_t)nd.gl << 32) | ((unsigned long long)nd.depth << 27) |
          (unsigned long long)(nd.code & ((1u << 27) - 1));
@@ -476,12 +468,8 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
     int32_t q = nd.parent;
     for (int k = c.depth - 1; k >= 0; k--) { anc[k] = B.tree[q].cop; q = B.tree[q].parent; }
   }
-  c.seqA = 0x243f6a8885a308d3ull ^ ((unsigned long long)c.depth << 48) ^ (unsigned long long)c.txi;
-  c.seqB = 0x13198a2e03707344ull + (unsigned long long)c.depth * 0x51ull + ((unsigned long long)c.txi << 20);
-  for (int k = 0; k < c.depth; k++) {
-    c.seqA = mix64(c.seqA + (unsigned long long)(uint32_t)anc[k] + 0x9e3779b97f4a7c15ull);
-    c.seqB = mix64(c.seqB ^ ((unsigned long long)(uint32_t)anc[k] * 0xd6e8feb86659fd93ull + (unsigned long long)k));
-  }
+  seq_seed(c.txi, c.depth, &c.seqA, &c.seqB);
+  for (int k = 0; k < c.depth; k++) seq_fold(anc[k], k, &c.seqA, &c.seqB);
   c.orderKey = order_key(nd);
   c.losBase = (int64_t)c.txi * A.nrx;
   return true;
@@ -543,10 +531,8 @@ __device__ void capture_one(const CaptureArgs& A, const CaptureBufs& B, const No
     }
   }
   myRec++;
-  unsigned long long hA = mix64(c.seqA ^ ((unsigned long long)(uint32_t)r * 0x9e3779b97f4a7c15ull));
-  unsigned long long hB = mix64(c.seqB + ((unsigned long long)(uint32_t)r << 7) + 0x632be59bd9b4e019ull);
-  if (hA == 0) hA = 1;
-  if (hB == 0) hB = 1;
+  unsigned long long hA, hB;
+  group_hash(c.seqA, c.seqB, r, &hA, &hB);
   // A record that cannot beat the key already stored for its group needs no correction: the stored
   // key belongs to a corrected path and keys only decrease.  A stale read is only less selective.
   {
@@ -1265,8 +1251,10 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     HIPCHK(hipGetLastError());
 
     const DSurf* surf = P[0].q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
+    // Dedup table: power of two, at least 4 slots per (transmitter, receiver) pair and a quarter of
+    // the last trace's size; grown and the capture re-run when it fills beyond 70 %.
     size_t cap = 1;
-    while (cap < (size_t)std::max<int64_t>(1 << 16, 64 * nOut)) cap <<= 1;
+    while (cap < (size_t)std::max<int64_t>(std::max<int64_t>(1 << 16, 4 * nOut), (int64_t)(tabCapHint / 4))) cap <<= 1;
     hipEvent_t kev[2];
     HIPCHK(hipEventCreate(&kev[0]));
     HIPCHK(hipEventCreate(&kev[1]));
@@ -1313,6 +1301,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         cap <<= 2;
         continue;
       }
+      tabCapHint = cap;
       stats.capture_records = (int64_t)ctr[0];
       stats.corrected = (int64_t)ctr[1];
       winners = (int64_t)ctr[2];

@@ -88,27 +88,59 @@ def urban_buildings(seed, count=40, half=250.0, fmin=10.0, fmax=40.0, hmin=10.0,
     return boxes, rng
 
 
-def config2(tess=9, rx_n=100):
-    """Synthetic urban canyon at 28 GHz: 500x500 m ground (2 triangles) plus 40 seeded box buildings
-    (footprints U[10,40] m, heights U[10,60] m; 482 triangles), concrete; 1 Tx at 25 m in a street
-    cell; icosahedron level 9; 6 reflections plus one transmission; 100x100 receivers at z = 1.5."""
+def urban_scene():
+    """The urban-canyon scene of config2 (ground plus 40 seeded boxes) and the generator state after
+    the buildings were drawn."""
     boxes, rng = urban_buildings(SEED_BASE + 2)
     s = Soup()
     h = 250.0
     s.quad((-h, -h, 0.0), (h, -h, 0.0), (h, h, 0.0), (-h, h, 0.0), (0, 0, 1))
     for (x0, y0, x1, y1, bh) in boxes:
         s.box(x0, y0, 0.0, x1, y1, bh)
-    # Tx in a street cell: the first seeded point at least 3 m from every footprint.
+    return s, boxes, rng
+
+
+def street_point(rng, boxes, z, margin=3.0, half=200.0):
+    """First seeded point of [-half, half]^2 at least `margin` from every building footprint."""
     while True:
-        tx = (rng.uniform(-200, 200), rng.uniform(-200, 200), 25.0)
-        if all(not (x0 - 3 < tx[0] < x1 + 3 and y0 - 3 < tx[1] < y1 + 3) for (x0, y0, x1, y1, _) in boxes):
-            break
+        p = (rng.uniform(-half, half), rng.uniform(-half, half), z)
+        if all(not (x0 - margin < p[0] < x1 + margin and y0 - margin < p[1] < y1 + margin)
+               for (x0, y0, x1, y1, _) in boxes):
+            return p
+
+
+def config2(tess=9, rx_n=100):
+    """Synthetic urban canyon at 28 GHz: 500x500 m ground (2 triangles) plus 40 seeded box buildings
+    (footprints U[10,40] m, heights U[10,60] m; 482 triangles), concrete; 1 Tx at 25 m in a street
+    cell; icosahedron level 9; 6 reflections plus one transmission; 100x100 receivers at z = 1.5."""
+    s, boxes, rng = urban_scene()
+    tx = street_point(rng, boxes, 25.0)   # Tx in a street cell
     pitch = 500.0 / rx_n
     g = -250.0 + pitch / 2 + pitch * np.arange(rx_n)
     rx = np.array([[x, y, 1.5] for x in g for y in g])
     mats = np.array([[28e9] + CONCRETE[1:]])
     trace = dict(tx=tx, tess=tess, maxR=6, maxT=1, frequency=28e9, pt=1.0, antennaType=2)
     return dict(name=f"urban_canyon_482tri_L{tess}_R6T1", soup=s.arrays(), rx=rx, materials=mats, trace=trace)
+
+
+def config4(n_tx=256, rx_n=100):
+    """Batched multi-transmitter run on the urban-canyon scene: n_tx transmitters in street cells
+    at heights U[3, 30] m, icosahedron level U{6..9} and U{4..6} reflections each (no
+    transmission), 28 GHz, 100x100 shared receivers at z = 1.5 (seed 20261203 + 4); reception radius
+    alpha * L / sqrt(3)."""
+    s, boxes, _ = urban_scene()
+    rng = SplitMix64(SEED_BASE + 4)
+    txs = []
+    for _ in range(n_tx):
+        z = rng.uniform(3.0, 30.0)
+        tx = street_point(rng, boxes, z)
+        txs.append(dict(tx=tx, tess=rng.randint(6, 9), maxR=rng.randint(4, 6), maxT=0, frequency=28e9, pt=1.0,
+                        antennaType=2, receptionModel=1))
+    pitch = 500.0 / rx_n
+    g = -250.0 + pitch / 2 + pitch * np.arange(rx_n)
+    rx = np.array([[x, y, 1.5] for x in g for y in g])
+    mats = np.array([[28e9] + CONCRETE[1:]])
+    return dict(name=f"urban_canyon_482tri_{n_tx}tx_L6-9_R4-6", soup=s.arrays(), rx=rx, materials=mats, txs=txs)
 
 
 def ground_fixture():
