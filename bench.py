@@ -122,6 +122,58 @@ def batched_run(ctx, make_params, scenes, root_budget=1 << 25):
         speedup=round(ts / tb, 3))
 
 
+def serve_run(ctx, make_params, scenes, Server, n=48, slots=3, load=0.7):
+    """Serving loop on the urban scene: n coverage queries (levels 6-10, 4 reflections, 1..50k
+    receivers).  Phase 1 submits all at once to measure capacity (req/s when saturated); phase 2
+    submits the same queries with Poisson arrivals at `load` x capacity and reports the latency
+    (submit -> fields on the host) of that phase."""
+    import random
+    _, boxes, _ = scenes.urban_scene()
+    qs = scenes.coverage_queries(n, boxes)
+    params = [make_params(**kw) for kw, _ in qs]
+
+    def finish(srv, ids):
+        infos = []
+        for q in ids:
+            rc, info = srv.poll(q, wait_ms=-1)
+            if rc != 0:
+                raise RuntimeError(f"query failed: {srv.error(q)}")
+            infos.append(info)
+            srv.release(q)
+        return infos
+
+    with Server(ctx, slots=slots) as srv:
+        finish(srv, [srv.submit(p, rx) for p, (_, rx) in zip(params, qs)])      # warm engines and ray caches
+        t0 = time.perf_counter()
+        infos = finish(srv, [srv.submit(p, rx) for p, (_, rx) in zip(params, qs)])
+        t_sat = time.perf_counter() - t0
+        capacity = n / t_sat
+        rate = load * capacity
+        rnd = random.Random(20261203 + 5)
+        ids = []
+        t0 = time.perf_counter()
+        t_next = t0
+        for p, (_, rx) in zip(params, qs):
+            t_next += rnd.expovariate(rate)
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(p, rx))
+        infos = finish(srv, ids)
+        t_poisson = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.sort([i["ms_total"] for i in infos])
+    bounces = sum(i["bounces"] for i in infos)
+    nearest = lambda q: float(lat[min(len(lat), max(1, int(np.ceil(q * len(lat))))) - 1])
+    return dict(workload="urban_canyon_482tri_coverage_queries_L6-10_R4_rx1-50k", queries=n, slots=slots,
+                capacity_req_s=round(capacity, 2), capacity_mray_bounces_s=round(bounces / t_sat / 1e6, 2),
+                offered_req_s=round(rate, 2), req_per_s=round(n / t_poisson, 2),
+                mray_bounces_per_s=round(bounces / t_poisson / 1e6, 2), p50_ms=round(nearest(0.5), 3),
+                p99_ms=round(nearest(0.99), 3), mean_ms=round(float(lat.mean()), 3),
+                max_ms=round(float(lat[-1]), 3), mean_queued_ms=round(float(np.mean([i["ms_queued"] for i in infos])), 3),
+                peak_inflight_bytes=int(st["peak_inflight_bytes"]), budget_bytes=int(st["budget_bytes"]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -132,6 +184,7 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--also-ref-model", action="store_true", help="also time the reference reception model")
+    ap.add_argument("--serve", action="store_true", help="also run the serving loop on coverage queries")
     ap.add_argument("--batch", action="store_true",
                     help="also run the 256-transmitter batched workload against per-transmitter traces")
     ap.add_argument("--calib", action="store_true",
@@ -228,6 +281,10 @@ def main():
         out["reference_reception_model"] = dict(value=round(s0["bounces"] / d0 / 1e6, 3), ms_per_step=round(d0 * 1e3, 3),
                                                 paths_kept=int(s0["winners"]),
                                                 capture_records=int(s0["capture_records"]))
+    if args.serve:
+        print("[bench] serving loop", file=sys.stderr, flush=True)
+        from sbrsolver_amd import Server
+        out["serving"] = serve_run(ctx, make_params, scenes, Server)
     if args.batch:
         print("[bench] batched multi-transmitter workload", file=sys.stderr, flush=True)
         out["batched"] = batched_run(ctx, make_params, scenes)

@@ -186,6 +186,67 @@ int sbr_result_tree_count(sbr_ctx* ctx, int64_t* n_nodes);
 int sbr_result_trees(sbr_ctx* ctx, int32_t* launch, int32_t* depth, int64_t* code, int32_t* tri, int32_t* cop,
                      int32_t* type, double* src, double* tgt, double* pl);
 
+/* ---- Serving loop (no reference counterpart: the reference has no serving mode) -------------
+ * A server keeps the scene and material tables of a context resident on its device and runs
+ * coverage queries (one transmitter, its own receivers) submitted from any host thread.  It owns
+ * `slots` engines, each with its own stream and worker thread: a query waiting in the queue is
+ * admitted as soon as a slot is free and the estimated device memory of the queries in flight plus
+ * its own stays within the HBM budget, so queries that arrive while others run overlap with them on
+ * the device (continuous batching).  A query's result does not depend on what runs beside it.
+ * Threading: every sbr_server call is thread-safe. */
+#define SBR_PENDING 1      /* sbr_poll: the query has not finished yet */
+
+typedef struct sbr_server sbr_server;
+
+typedef struct {
+  int32_t slots;             /* concurrent queries on the device; 0 = 3 */
+  int32_t max_queued;        /* queries waiting for admission before sbr_submit refuses; 0 = 4096 */
+  int64_t hbm_budget_bytes;  /* admission budget for the queries in flight; 0 = 3/4 of the free memory */
+} sbr_server_config;
+
+typedef struct {
+  int32_t state;             /* 0 queued, 1 running, 2 done, 3 failed */
+  int32_t status;            /* SBR_OK or the SBR_E_* of a failed query */
+  int32_t n_rx;
+  int32_t slot;              /* engine that ran it, -1 before admission */
+  int64_t launches, bounces, paths;
+  int64_t est_bytes;         /* admission estimate */
+  double ms_queued;          /* submit -> admission */
+  double ms_run;             /* admission -> result on the host */
+  double ms_total;           /* submit -> result on the host */
+} sbr_request_info;
+
+typedef struct {
+  int64_t submitted, completed, failed, refused, queued, running;
+  int64_t bounces;           /* of completed queries */
+  int64_t budget_bytes, peak_inflight_bytes;
+  double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> result latency of completed queries */
+  double window_s;           /* first submission -> last completion */
+  double req_per_s;          /* completed / window_s */
+  double mray_bounces_per_s; /* bounces / window_s / 1e6 */
+} sbr_server_stats;
+
+/* Scene and materials are copied from `src` (which must hold both); the server is independent of
+ * `src` afterwards.  A host-only `src` (device -1) gives a server whose queries fail with
+ * SBR_E_STATE (queue, admission and error paths without a device). */
+int sbr_server_create(sbr_ctx* src, const sbr_server_config* cfg, sbr_server** out);
+/* Drains: waits for running queries, fails the queued ones with SBR_E_STATE, frees everything. */
+int sbr_server_destroy(sbr_server* s);
+const char* sbr_server_last_error(const sbr_server* s);
+/* Queues a query; rx[n_rx][3] is copied.  SBR_E_LIMIT when the queue is full, SBR_E_NOMEM when
+ * the query alone exceeds the HBM budget.  *id identifies it for the calls below. */
+int sbr_submit(sbr_server* s, const sbr_trace_params* params, const double* rx, int32_t n_rx, int64_t* id);
+/* Waits up to wait_ms (< 0: no limit) for the query.  Returns SBR_OK (done), SBR_PENDING, the
+ * query's SBR_E_* status if it failed, or SBR_E_ARG for an unknown id.  info may be NULL. */
+int sbr_poll(sbr_server* s, int64_t id, int32_t wait_ms, sbr_request_info* info);
+/* Error message of a failed query. */
+const char* sbr_request_error(sbr_server* s, int64_t id);
+/* Fields of a finished query: out[n_rx][3][2], as sbr_result_fields. */
+int sbr_request_fields(sbr_server* s, int64_t id, double* out, int32_t n_rx);
+/* Forgets a finished query (its result is freed; its latency stays in the statistics). */
+int sbr_request_release(sbr_server* s, int64_t id);
+int sbr_server_get_stats(sbr_server* s, sbr_server_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@ LIB = os.path.join(HERE, "libsbr_amd.so")
 SOURCES = [
     os.path.join(HERE, "csrc", "engine.hip"),
     os.path.join(HERE, "csrc", "capi.cpp"),
+    os.path.join(HERE, "csrc", "serve.cpp"),
     os.path.join(HERE, "csrc", "host", "preprocessor.cpp"),
     os.path.join(HERE, "csrc", "host", "rtsolver.cpp"),
 ]
@@ -25,15 +26,21 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    objs = []
+def build(verbose=False, jobs=4):
+    from concurrent.futures import ThreadPoolExecutor
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(HERE, "csrc", os.path.basename(src) + ".o")
-        cmd = [HIPCC] + FLAGS + ["-x", "hip", "-c", src, "-o", obj]
+        cmds.append([HIPCC] + FLAGS + ["-x", "hip", "-c", src, "-o", obj])
+        objs.append(obj)
+
+    def compile_one(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
-        objs.append(obj)
+
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(compile_one, cmds))
     tmp = LIB + ".tmp"
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)

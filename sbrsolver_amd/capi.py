@@ -54,6 +54,32 @@ class Material(C.Structure):
                 ("relConductivity", C.c_double)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("max_queued", C.c_int32), ("hbm_budget_bytes", C.c_int64)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("n_rx", C.c_int32), ("slot", C.c_int32),
+                ("launches", C.c_int64), ("bounces", C.c_int64), ("paths", C.c_int64), ("est_bytes", C.c_int64),
+                ("ms_queued", C.c_double), ("ms_run", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("refused", C.c_int64),
+                ("queued", C.c_int64), ("running", C.c_int64), ("bounces", C.c_int64), ("budget_bytes", C.c_int64),
+                ("peak_inflight_bytes", C.c_int64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("max_ms", C.c_double), ("window_s", C.c_double),
+                ("req_per_s", C.c_double), ("mray_bounces_per_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SBR_PENDING = 1
+
 # Every symbol the header declares, with its ctypes signature.
 _vp, _i32, _i64, _dp = C.c_void_p, C.c_int32, C.c_int64, C.POINTER(C.c_double)
 SIGNATURES = {
@@ -82,6 +108,15 @@ SIGNATURES = {
     "sbr_result_path_fields": ([_vp, _vp, _i64], C.c_int),
     "sbr_result_tree_count": ([_vp, _vp], C.c_int),
     "sbr_result_trees": ([_vp] + [_vp] * 9, C.c_int),
+    "sbr_server_create": ([_vp, _vp, C.POINTER(C.c_void_p)], C.c_int),
+    "sbr_server_destroy": ([_vp], C.c_int),
+    "sbr_server_last_error": ([_vp], C.c_char_p),
+    "sbr_submit": ([_vp, _vp, _vp, _i32, C.POINTER(C.c_int64)], C.c_int),
+    "sbr_poll": ([_vp, _i64, _i32, _vp], C.c_int),
+    "sbr_request_error": ([_vp, _i64], C.c_char_p),
+    "sbr_request_fields": ([_vp, _i64, _vp, _i32], C.c_int),
+    "sbr_request_release": ([_vp, _i64], C.c_int),
+    "sbr_server_get_stats": ([_vp, _vp], C.c_int),
 }
 
 
@@ -280,3 +315,68 @@ class Context:
         arr = (TraceParams * len(params_list))(*params_list)
         self._chk(lib().sbr_trace_batch(self.h, arr, len(params_list), _a(r), r.shape[0]))
         return Result(self, r.shape[0], len(params_list))
+
+
+class Server:
+    """sbr_server: coverage queries against the resident scene of `ctx` (copied at creation), run on
+    `slots` concurrent engines and admitted by HBM budget.  Thread-safe."""
+
+    def __init__(self, ctx, slots=3, hbm_budget_bytes=0, max_queued=0):
+        cfg = ServerConfig(int(slots), int(max_queued), int(hbm_budget_bytes))
+        h = C.c_void_p()
+        ctx._chk(lib().sbr_server_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sbr_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        if rc < 0:
+            msg = lib().sbr_server_last_error(self.h)
+            raise SbrError(f"sbr error {rc}: {msg.decode() if msg else ''}")
+        return rc
+
+    def submit(self, params, rx):
+        r = np.ascontiguousarray(rx, dtype=np.float64).reshape(-1, 3)
+        qid = C.c_int64()
+        self._chk(lib().sbr_submit(self.h, C.byref(params), _a(r), r.shape[0], C.byref(qid)))
+        return qid.value
+
+    def poll(self, qid, wait_ms=0):
+        """(status, info): status 0 done, SBR_PENDING, or the failed query's negative status."""
+        info = RequestInfo()
+        rc = lib().sbr_poll(self.h, int(qid), int(wait_ms), C.byref(info))
+        if rc < 0 and info.state != 3:   # the call failed (unknown id), not the query
+            self._chk(rc)
+        return rc, info.as_dict()
+
+    def error(self, qid):
+        m = lib().sbr_request_error(self.h, int(qid))
+        return m.decode() if m else ""
+
+    def fields(self, qid, n_rx):
+        f = np.zeros((n_rx, 3, 2))
+        self._chk(lib().sbr_request_fields(self.h, int(qid), _a(f), int(n_rx)))
+        return f[..., 0] + 1j * f[..., 1]
+
+    def release(self, qid):
+        self._chk(lib().sbr_request_release(self.h, int(qid)))
+
+    def stats(self):
+        st = ServerStats()
+        self._chk(lib().sbr_server_get_stats(self.h, C.byref(st)))
+        return st.as_dict()

@@ -13,9 +13,6 @@
 #include "sbrsolver/Preprocessor.h"
 #include "sbrsolver/SceneDesc.h"
 
-struct sbr_ctx {
-  sbr::Engine eng;
-};
 
 namespace sbr {
 
@@ -169,6 +166,39 @@ int Engine::upload(const sbr_scene_desc* d) {
   haveScene = true;
   haveResult = false;
   return SBR_OK;
+}
+
+int Engine::clone_scene(const Engine& s) {
+  if (!s.haveScene) return fail(SBR_E_STATE, "source has no scene");
+  if (s.mats.empty()) return fail(SBR_E_STATE, "source has no materials");
+  hNodes = s.hNodes;
+  hTris = s.hTris;
+  hTriN = s.hTriN;
+  hSurfQ1 = s.hSurfQ1;
+  hSurfId = s.hSurfId;
+  hCop = s.hCop;
+  hMat = s.hMat;
+  hMeshTriId = s.hMeshTriId;
+  hVerts = s.hVerts;
+  hNorms = s.hNorms;
+  hTriBox32 = s.hTriBox32;
+  ntri = s.ntri;
+  nnode = s.nnode;
+  ncop = s.ncop;
+  treeHeight = s.treeHeight;
+  sceneBytes = s.sceneBytes;
+  sceneExtent = s.sceneExtent;
+  boxDelta = s.boxDelta;
+  if (device >= 0) {
+    HIPCHK(hipSetDevice(device));
+    if (put(dNodes, hNodes, stream) || put(dTris, hTris, stream) || put(dTriN, hTriN, stream) ||
+        put(dSurfQ1, hSurfQ1, stream) || put(dSurfId, hSurfId, stream) || put(dTriBox32, hTriBox32, stream))
+      return fail(SBR_E_NOMEM, "scene upload");
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  haveScene = true;
+  haveResult = false;
+  return set_materials(reinterpret_cast<const sbr_material*>(s.mats.data()), (int)(s.mats.size() / 6));
 }
 
 }  // namespace sbr

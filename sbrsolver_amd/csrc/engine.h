@@ -191,6 +191,7 @@ struct Engine {
   int fail(int code, const std::string& msg) { err = msg; return code; }
   int init(int dev);
   int upload(const sbr_scene_desc* d);
+  int clone_scene(const Engine& src);   // scene and materials of another engine (device copies made here)
   int set_materials(const sbr_material* m, int n);
   int rays_for(int tess, const double** out, int64_t* nOut);   // cached launch directions
   int trace(const sbr_trace_params* p, int ntx, const double* rx, int m);
@@ -207,3 +208,8 @@ struct Engine {
 };
 
 }  // namespace sbr
+
+// Handle behind the C-ABI's sbr_ctx.
+struct sbr_ctx {
+  sbr::Engine eng;
+};

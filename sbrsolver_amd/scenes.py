@@ -143,6 +143,23 @@ def config4(n_tx=256, rx_n=100):
     return dict(name=f"urban_canyon_482tri_{n_tx}tx_L6-9_R4-6", soup=s.arrays(), rx=rx, materials=mats, txs=txs)
 
 
+def coverage_queries(n, boxes, seed=SEED_BASE + 5, half=250.0, tess=(6, 10), max_rx=50000, maxR=4):
+    """Coverage queries for the serving loop: each a transmitter in a street cell at U[3, 30] m, an
+    icosahedron level drawn from `tess`, maxR reflections, and a receiver count drawn log-uniform
+    from [1, max_rx] with receivers uniform over the area at z = 1.5."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(n):
+        z = rng.uniform(3.0, 30.0)
+        tx = street_point(rng, boxes, z, half=0.8 * half)
+        level = rng.randint(tess[0], tess[1])
+        nrx = max(1, min(max_rx, int(np.exp(rng.uniform(0.0, np.log(max_rx))))))
+        rx = np.array([[rng.uniform(-half, half), rng.uniform(-half, half), 1.5] for _ in range(nrx)])
+        out.append((dict(tx=tx, tess=level, maxR=maxR, maxT=0, frequency=28e9, pt=1.0, antennaType=2,
+                         receptionModel=1), rx))
+    return out
+
+
 def ground_fixture():
     """The reference's committed main() scenario (SBRSolver_rev2.cpp:18-144): ground.stl, 3000
     receivers on x = 0, z = 3, y = 0..2999, Tx (0, 0, 5), txCoordSys rows (0,1,0), (0,0,1), (1,0,0),
