@@ -81,7 +81,8 @@ def calib_copy(nbytes):
 def batched_run(ctx, make_params, scenes, root_budget=1 << 25):
     """Batched multi-transmitter workload (256 transmitters on the urban scene, levels 6-9, 4-6
     reflections, 10k shared receivers): sbr_trace_batch over groups of transmitters holding at most
-    root_budget launches each, against one sbr_trace per transmitter on the same inputs."""
+    root_budget launches each (the library splits a call further into launch windows if needed),
+    against one sbr_trace per transmitter on the same inputs."""
     cfg = scenes.config4()
     plist = [make_params(**t) for t in cfg["txs"]]
     groups, cur, roots = [], [], 0
@@ -95,10 +96,15 @@ def batched_run(ctx, make_params, scenes, root_budget=1 << 25):
     if cur:
         groups.append(cur)
 
+    windows = []
+
     def run_batched():
         b = 0
+        windows.clear()
         for g in groups:
-            b += ctx.trace_batch(g, cfg["rx"]).stats["bounces"]
+            st = ctx.trace_batch(g, cfg["rx"]).stats
+            b += st["bounces"]
+            windows.append(st["windows"])
         return b
 
     def run_single():
@@ -116,7 +122,8 @@ def batched_run(ctx, make_params, scenes, root_budget=1 << 25):
     bs = run_single()
     ts = time.perf_counter() - t0
     assert bb == bs
-    return dict(workload=cfg["name"], transmitters=len(plist), batches=len(groups), launches=int(
+    return dict(workload=cfg["name"], transmitters=len(plist), batches=len(groups), windows=int(sum(windows)),
+                launches=int(
         sum(10 * 4 ** p.tess + 2 for p in plist)), bounces=int(bb), batched_mrays=round(bb / tb / 1e6, 3),
         batched_s=round(tb, 4), single_tx_mrays=round(bs / ts / 1e6, 3), single_tx_s=round(ts, 4),
         speedup=round(ts / tb, 3))

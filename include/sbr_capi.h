@@ -124,15 +124,18 @@ typedef struct {
   int32_t trace_lds;        /* 1 if the traversal staged the scene in LDS */
   int32_t trace_blocks_per_cu;
   int64_t heavy_nodes;      /* tree nodes captured wave-cooperatively (large candidate regions) */
+  int64_t windows;          /* launch windows the trace ran in (tree buffers bounded by window_nodes) */
 } sbr_stats;
 
 /* Launch configuration of the traversal kernel (no reference counterpart).  lds_scene: -1 stage
  * the scene in LDS when it fits and does not lower the resident blocks per CU, 1 whenever it
- * fits, 0 never; blocks_per_cu: persistent blocks per compute unit, 0 = the occupancy maximum.
- * Results do not depend on it. */
+ * fits, 0 never; blocks_per_cu: persistent blocks per compute unit, 0 = the occupancy maximum;
+ * window_nodes: interaction-tree nodes held on the device at once (128 B each), 0 = 2^27.  A trace
+ * whose trees need more runs in several launch windows.  Results do not depend on any of them. */
 typedef struct {
   int32_t lds_scene;
   int32_t blocks_per_cu;
+  int64_t window_nodes;
 } sbr_tuning;
 
 /* device >= 0: HIP device ordinal.  device == -1: host-only context (scene preparation and

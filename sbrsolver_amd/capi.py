@@ -38,6 +38,7 @@ class Stats(C.Structure):
         ("ms_k_trace", C.c_double), ("n_k_trace", C.c_int64), ("ms_k_capture", C.c_double),
         ("node_visits", C.c_int64), ("tri_tests", C.c_int64),
         ("trace_lds", C.c_int32), ("trace_blocks_per_cu", C.c_int32), ("heavy_nodes", C.c_int64),
+        ("windows", C.c_int64),
     ]
 
     def as_dict(self):
@@ -45,7 +46,7 @@ class Stats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("lds_scene", C.c_int32), ("blocks_per_cu", C.c_int32)]
+    _fields_ = [("lds_scene", C.c_int32), ("blocks_per_cu", C.c_int32), ("window_nodes", C.c_int64)]
 
 
 class Material(C.Structure):
@@ -277,14 +278,14 @@ class Context:
         m = DEFAULT_MATERIALS if mats is None else np.ascontiguousarray(mats, dtype=np.float64).reshape(-1, 6)
         self._chk(lib().sbr_materials(self.h, _a(m), m.shape[0]))
 
-    def set_tuning(self, lds_scene=-1, blocks_per_cu=0):
-        t = Tuning(int(lds_scene), int(blocks_per_cu))
+    def set_tuning(self, lds_scene=-1, blocks_per_cu=0, window_nodes=0):
+        t = Tuning(int(lds_scene), int(blocks_per_cu), int(window_nodes))
         self._chk(lib().sbr_set_tuning(self.h, C.byref(t)))
 
     def get_tuning(self):
         t = Tuning()
         self._chk(lib().sbr_get_tuning(self.h, C.byref(t)))
-        return dict(lds_scene=t.lds_scene, blocks_per_cu=t.blocks_per_cu)
+        return dict(lds_scene=t.lds_scene, blocks_per_cu=t.blocks_per_cu, window_nodes=t.window_nodes)
 
     # ---- kernels
     def generate_rays(self, tess):

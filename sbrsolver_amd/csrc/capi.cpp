@@ -334,10 +334,12 @@ int sbr_materials(sbr_ctx* ctx, const sbr_material* mats, int32_t count) {
 
 int sbr_set_tuning(sbr_ctx* ctx, const sbr_tuning* t) {
   return guarded(ctx, [&] {
-    if (!t || t->lds_scene < -1 || t->lds_scene > 1 || t->blocks_per_cu < 0 || t->blocks_per_cu > 64)
+    if (!t || t->lds_scene < -1 || t->lds_scene > 1 || t->blocks_per_cu < 0 || t->blocks_per_cu > 64 ||
+        t->window_nodes < 0 || t->window_nodes > (int64_t(1) << 30))
       return ctx->eng.fail(SBR_E_ARG, "tuning out of range");
     ctx->eng.tune.ldsScene = t->lds_scene;
     ctx->eng.tune.blocksPerCU = t->blocks_per_cu;
+    ctx->eng.tune.windowNodes = t->window_nodes;
     return SBR_OK;
   });
 }
@@ -347,6 +349,7 @@ int sbr_get_tuning(sbr_ctx* ctx, sbr_tuning* t) {
     if (!t) return ctx->eng.fail(SBR_E_ARG, "null tuning");
     t->lds_scene = ctx->eng.tune.ldsScene;
     t->blocks_per_cu = ctx->eng.tune.blocksPerCU;
+    t->window_nodes = ctx->eng.tune.windowNodes;
     return SBR_OK;
   });
 }
@@ -428,33 +431,18 @@ int sbr_result_paths(sbr_ctx* ctx, int32_t* prx, int32_t* plaunch, int32_t* pnse
 }
 
 int sbr_result_path_tx(sbr_ctx* ctx, int32_t* out, int64_t n_paths) {
-  return guarded(ctx, [&] {
-    sbr::Engine& e = ctx->eng;
-    if (!e.haveResult) return e.fail(SBR_E_STATE, "no trace result");
-    if (n_paths != e.winners || (!out && n_paths > 0)) return e.fail(SBR_E_ARG, "path tx: count mismatch");
-    if (n_paths == 0) return SBR_OK;
-    std::vector<int32_t> txrx(n_paths);
-    if (hipMemcpy(txrx.data(), e.dWinRx.p, sizeof(int32_t) * n_paths, hipMemcpyDeviceToHost) != hipSuccess)
-      return e.fail(SBR_E_HIP, "copy path keys");
-    for (int64_t w = 0; w < n_paths; w++) out[w] = txrx[w] / e.nrx;
-    return SBR_OK;
-  });
+  return guarded(ctx, [&] { return ctx->eng.path_tx(out, n_paths); });
 }
 
 int sbr_result_path_fields(sbr_ctx* ctx, double* out, int64_t n_paths) {
-  return guarded(ctx, [&] {
-    sbr::Engine& e = ctx->eng;
-    if (!e.haveResult) return e.fail(SBR_E_STATE, "no trace result");
-    if (n_paths != e.winners || (!out && n_paths > 0)) return e.fail(SBR_E_ARG, "path fields: count mismatch");
-    if (n_paths > 0 && hipMemcpy(out, e.dWinField.p, 6 * sizeof(double) * n_paths, hipMemcpyDeviceToHost) != hipSuccess)
-      return e.fail(SBR_E_HIP, "copy path fields");
-    return SBR_OK;
-  });
+  return guarded(ctx, [&] { return ctx->eng.path_fields(out, n_paths); });
 }
 
 int sbr_result_tree_count(sbr_ctx* ctx, int64_t* n) {
   return guarded(ctx, [&] {
     if (!ctx->eng.haveResult) return ctx->eng.fail(SBR_E_STATE, "no trace result");
+    if (ctx->eng.multiWindow)
+      return ctx->eng.fail(SBR_E_STATE, "interaction trees are kept for single-window traces only");
     if (n) *n = ctx->eng.nodes;
     return SBR_OK;
   });

@@ -17,6 +17,7 @@ constexpr int kMaxDepth = 24;       // reflections + transmissions per path (tre
 constexpr int kMaxStack = 64;       // BVH traversal stack (tree height bound, checked on upload)
 constexpr int kBlock = 256;         // threads per block of the traversal kernels
 constexpr size_t kSceneLdsMax = 96 * 1024;   // scenes up to this size are staged in LDS
+constexpr size_t kWindowNodes = size_t(1) << 27;   // tree nodes per launch window (17 GB of records)
 
 // ---- scene, device layout ---------------------------------------------------------------------
 // BVH node in depth-first order (children[0] subtree first), 64 B.
@@ -119,6 +120,14 @@ struct DBuf {
 struct Tuning {
   int ldsScene = -1;      // -1: LDS staging when it fits and keeps occupancy, 1: whenever it fits, 0: never
   int blocksPerCU = 0;    // persistent blocks per CU; 0: occupancy maximum
+  int64_t windowNodes = 0;   // tree nodes per launch window; 0: kWindowNodes
+};
+
+// Kept paths gathered window by window when a trace needs several launch windows.
+struct PathCache {
+  std::vector<int32_t> ptx, prx, plaunch, pnseg, stri, scop, stype;
+  std::vector<int64_t> poff;
+  std::vector<double> ssrc, stgt, pfield;
 };
 
 struct Engine {
@@ -169,7 +178,8 @@ struct Engine {
   DBuf<int32_t> dTabRx;
   DBuf<int32_t> dHeavy;          // nodes left to the wave-per-node capture
   DBuf<unsigned int> dRootBits;  // roots owning a kept path
-  DBuf<unsigned long long> dCounters;
+  DBuf<unsigned long long> dCounters;        // capture: records, corrected, groups, flags, collected, heavy
+  DBuf<unsigned long long> dTraceCounters;   // bounce loop: overflow flags, node visits, triangle tests
   DBuf<unsigned long long> dWinKey, dWinKey2;
   DBuf<int32_t> dWinRx, dWinRx2, dWinIdx, dWinIdx2, dWinNode;
   DBuf<double> dWinField;
@@ -184,6 +194,7 @@ struct Engine {
   int64_t winners = 0;
   size_t nodeCap = 0;
   size_t tabCapHint = 0;   // dedup table size of the last trace
+  double perRootHint = 0;  // tree nodes per root item of the last trace (first window size)
   std::vector<double> hField;
   RxGrid grid{};
 
@@ -201,10 +212,21 @@ struct Engine {
             bool countOnly);
   int trees(int32_t* launch, int32_t* depth, int64_t* code, int32_t* tri, int32_t* cop, int32_t* type, double* src,
             double* tgt, double* pl);
+  int path_tx(int32_t* out, int64_t n);
+  int path_fields(double* out, int64_t n);
+
+  bool multiWindow = false;   // last trace ran in several launch windows (paths kept in `cache`)
+  PathCache cache;
 
  private:
   int generate_rays(int tess, DBuf<double>& out, int64_t* nOut);
-  int trace_stage(const std::vector<TxDesc>& txs, int64_t nRoot, int maxDepth, bool exact, bool count);
+  int trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_t nRoot, int maxDepth, bool exact,
+                  bool count, int64_t* need, float* ms);
+  int paths_window(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
+                   int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
+                   bool countOnly);
+  int cache_window_paths(int64_t w);
+  int finish_path_cache();
 };
 
 }  // namespace sbr

@@ -162,6 +162,7 @@ struct TraceArgs {
   int32_t nTx;
   int32_t stackDepth;
   int32_t nodeBytes, boxBytes;   // nodes and triangle pre-test boxes staged in LDS (LDS variant)
+  int64_t rootBase;              // global root index of the window's first root item
   float ext;                                // scene extent for the single-precision pre-test
   unsigned long long nodeCap, queueCap;
 };
@@ -219,19 +220,20 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       {
         const TxDesc* T;
         if (A.depth == 0) {
-          int lo = 0, hi = A.nTx - 1;   // transmitter owning root item i
+          const unsigned long long gi = (unsigned long long)A.rootBase + i;   // global root index
+          int lo = 0, hi = A.nTx - 1;   // transmitter owning root item gi
           while (lo < hi) {
             int mid = (lo + hi + 1) >> 1;
-            if ((unsigned long long)txs[mid].launchStart <= i) lo = mid;
+            if ((unsigned long long)txs[mid].launchStart <= gi) lo = mid;
             else hi = mid - 1;
           }
           T = &txs[lo];
-          const int64_t launch = T->launchOffset + ((int64_t)i - T->launchStart) * T->launchStride;
+          const int64_t launch = T->launchOffset + ((int64_t)gi - T->launchStart) * T->launchStride;
           const V3 d = ld3(T->rays + 3 * launch);
           nd.src[0] = T->tx[0]; nd.src[1] = T->tx[1]; nd.src[2] = T->tx[2];
           nd.dir[0] = d.x; nd.dir[1] = d.y; nd.dir[2] = d.z;
           nd.parent = -1;
-          nd.gl = (int32_t)i;
+          nd.gl = (int32_t)gi;
           nd.code = 0;
           nd.type = 0;
           nd.rc = nd.tc = 0;
@@ -655,13 +657,16 @@ __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBuf
   flush_counters(B, myRec, myCor, myUsed);
 }
 
+// Groups whose smallest order key has its root in [glLo, glHi) (those first seen in the window).
 __global__ void k_collect(const unsigned long long* __restrict__ tabA, const unsigned long long* __restrict__ tabOrder,
-                          const int32_t* __restrict__ tabRx, unsigned long long cap,
-                          unsigned long long* __restrict__ winKey, int32_t* __restrict__ winRx,
-                          unsigned long long* __restrict__ counter) {
+                          const int32_t* __restrict__ tabRx, unsigned long long cap, unsigned long long glLo,
+                          unsigned long long glHi, unsigned long long* __restrict__ winKey,
+                          int32_t* __restrict__ winRx, unsigned long long* __restrict__ counter) {
   unsigned long long s = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= cap) return;
   if (tabA[s] == 0ull) return;
+  const unsigned long long gl = tabOrder[s] >> 32;
+  if (gl < glLo || gl >= glHi) return;
   unsigned long long w = atomicAdd(counter, 1ull);
   winKey[w] = tabOrder[s];
   winRx[w] = tabRx[s];
@@ -826,7 +831,8 @@ __global__ void __launch_bounds__(256) k_field(FieldArgs F, const TxDesc* __rest
   for (int c = 0; c < 3; c++) { o[2 * c] = f[c].r; o[2 * c + 1] = f[c].i; }
 }
 
-// Per (transmitter, receiver): sum of its kept paths' fields in (launch, BFS) order.
+// Per (transmitter, receiver): running sum of its kept paths' fields in (launch, BFS) order,
+// continued from the value the previous launch windows left.
 __global__ void k_rx_sum(const int32_t* __restrict__ winTxRx, int64_t W, const double* __restrict__ winField,
                          int64_t nOut, double* __restrict__ field) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -837,7 +843,9 @@ __global__ void k_rx_sum(const int32_t* __restrict__ winTxRx, int64_t W, const d
     if (winTxRx[mid] < r) lo = mid + 1;
     else hi = mid;
   }
-  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (lo >= W || winTxRx[lo] != r) return;
+  double acc[6];
+  for (int c = 0; c < 6; c++) acc[c] = field[6 * r + c];
   for (int64_t w = lo; w < W && winTxRx[w] == r; w++)
     for (int c = 0; c < 6; c++) acc[c] += winField[6 * w + c];
   for (int c = 0; c < 6; c++) field[6 * r + c] = acc[c];
@@ -1034,9 +1042,17 @@ static double ms_since(std::chrono::steady_clock::time_point t) {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
 }
 
-// Bounce loop: one persistent k_trace launch per depth, no host synchronisation in between.  Returns
-// SBR_E_LIMIT (with nodeCap grown) when the node or queue buffers overflowed: run it again.
-int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t nRoot, int maxDepth, bool exact, bool count) {
+// Bounce loop of the root items [rootBase, rootBase + nRoot): one persistent k_trace launch per
+// depth, no host synchronisation in between.  Returns SBR_E_LIMIT when the node or queue buffers
+// overflowed, with *need a lower bound of the nodes the window needs; *ms is the kernels' time.
+int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_t nRoot, int maxDepth, bool exact,
+                        bool count, int64_t* need, float* ms) {
+  *need = 0;
+  *ms = 0;
+  if (nRoot == 0) {
+    nodes = 0;
+    return SBR_OK;
+  }
   // Scene in LDS or read through the caches: automatic choice takes LDS staging unless it lowers the
   // number of resident blocks per CU.
   const size_t stk = stack_bytes(stackDepth(), kBlock);
@@ -1055,6 +1071,7 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t nRoot, int maxDe
   A.nodeBytes = (int32_t)(hNodes.size() * sizeof(DNode));
   A.boxBytes = (int32_t)(hTris.size() * sizeof(DBox32));
   A.ext = sceneExtent;
+  A.rootBase = rootBase;
   A.nodeCap = nodeCap;
   A.queueCap = nodeCap;
   auto kern = variant(lds);
@@ -1066,7 +1083,7 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t nRoot, int maxDe
   std::vector<LevelRec> lv0(kMaxDepth + 2, LevelRec{0, 0});
   lv0[0].count = (unsigned long long)nRoot;
   HIPCHK(hipMemcpyAsync(dLevels.p, lv0.data(), sizeof(LevelRec) * lv0.size(), hipMemcpyHostToDevice, stream));
-  HIPCHK(hipMemsetAsync(dCounters.p, 0, 8 * sizeof(unsigned long long), stream));
+  HIPCHK(hipMemsetAsync(dTraceCounters.p, 0, 8 * sizeof(unsigned long long), stream));
   hipEvent_t kev[2];
   HIPCHK(hipEventCreate(&kev[0]));
   HIPCHK(hipEventCreate(&kev[1]));
@@ -1077,19 +1094,17 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t nRoot, int maxDe
     const int32_t* qin = dQueue[d & 1].p;
     int32_t* qout = dQueue[(d + 1) & 1].p;
     kern<<<grid, kBlock, smem, stream>>>(A, dTx.p, dNodes.p, dTris.p, dTriBox32.p, dTriN.p, qin, qout, dTree.p,
-                                         dLevels.p, dCounters.p, count ? 1 : 0);
+                                         dLevels.p, dTraceCounters.p, count ? 1 : 0);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(kev[1], stream));
   std::vector<LevelRec> lv(kMaxDepth + 2);
   unsigned long long ctr[8];
   HIPCHK(hipMemcpyAsync(lv.data(), dLevels.p, sizeof(LevelRec) * lv.size(), hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipMemcpyAsync(ctr, dCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(ctr, dTraceCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
   float kms = 0;
   HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
-  stats.ms_k_trace = kms;
-  stats.n_k_trace = maxDepth + 1;
   unsigned long long total = 0;
   int deepest = 0;
   for (int d = 0; d <= maxDepth; d++) {
@@ -1097,14 +1112,17 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t nRoot, int maxDe
     if (lv[d].count) deepest = d;
   }
   if ((ctr[0] & 3ull) || total > nodeCap) {
-    nodeCap = std::max<size_t>(nodeCap * 2, (size_t)(total + total / 4 + 1024));
+    *need = (int64_t)std::max<unsigned long long>(total, nodeCap + 1);
     return SBR_E_LIMIT;
   }
+  *ms = kms;
   nodes = (int64_t)total;
-  stats.bounces = nodes;
-  stats.max_depth = deepest;
-  stats.node_visits = count ? (int64_t)ctr[6] : 0;
-  stats.tri_tests = count ? (int64_t)ctr[7] : 0;
+  stats.ms_k_trace += kms;
+  stats.n_k_trace += maxDepth + 1;
+  stats.bounces += nodes;
+  stats.max_depth = std::max<int64_t>(stats.max_depth, deepest);
+  stats.node_visits += count ? (int64_t)ctr[6] : 0;
+  stats.tri_tests += count ? (int64_t)ctr[7] : 0;
   return SBR_OK;
 }
 
@@ -1182,30 +1200,14 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   HIPCHK(hipEventRecord(ev[1], stream));
   stats.launches = nRoot;
 
-  // ---- interaction trees
-  ENSURE(dLevels, kMaxDepth + 2);
-  ENSURE(dCounters, 8);
-  if (nodeCap < (size_t)nRoot * 4 + 1024) nodeCap = (size_t)nRoot * 4 + 1024;
-  for (int attempt = 0;; attempt++) {
-    if (nodeCap >= (size_t(1) << 30)) return fail(SBR_E_LIMIT, "more than 2^30 tree nodes");
-    ENSURE(dTree, nodeCap);
-    ENSURE(dQueue[0], nodeCap);
-    ENSURE(dQueue[1], nodeCap);
-    nodeCap = std::min(dTree.n, std::min(dQueue[0].n, dQueue[1].n));
-    int rc = trace_stage(txs, nRoot, maxDepth, exact, count);
-    if (rc == SBR_OK) break;
-    if (rc != SBR_E_LIMIT || attempt > 8) return rc == SBR_E_LIMIT ? fail(rc, "tree buffers could not be sized") : rc;
-  }
-  HIPCHK(hipEventRecord(ev[2], stream));
-
-  // ---- receivers: grid, line of sight, capture + correction + dedup
+  // ---- receivers: grid (host binning) and direct-path line of sight for every pair
   winners = 0;
   nrx = m;
   ntx = nt;
   const int64_t nOut = (int64_t)nt * m;
+  RxGrid g{};
   if (m > 0) {
     // Uniform xy grid with about one receiver per cell.
-    RxGrid g{};
     for (int a = 0; a < 3; a++) { g.bmin[a] = INFINITY; g.bmax[a] = -INFINITY; }
     for (int r = 0; r < m; r++)
       for (int a = 0; a < 3; a++) {
@@ -1249,27 +1251,92 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       k_los<false><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dTris.p, dTriBox32.p,
                                                                       sceneExtent, dRx.p, m, stackDepth(), dLos.p);
     HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(ev[2], stream));
 
-    const DSurf* surf = P[0].q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
-    // Dedup table: power of two, at least 4 slots per (transmitter, receiver) pair and a quarter of
-    // the last trace's size; grown and the capture re-run when it fills beyond 70 %.
-    size_t cap = 1;
-    while (cap < (size_t)std::max<int64_t>(std::max<int64_t>(1 << 16, 4 * nOut), (int64_t)(tabCapHint / 4))) cap <<= 1;
-    hipEvent_t kev[2];
-    HIPCHK(hipEventCreate(&kev[0]));
-    HIPCHK(hipEventCreate(&kev[1]));
-    struct KevGuard { hipEvent_t* e; ~KevGuard() { (void)hipEventDestroy(e[0]); (void)hipEventDestroy(e[1]); } } kguard{kev};
-    bool sized = false;
-    for (int attempt = 0; attempt < 6 && !sized; attempt++) {
+  // ---- launch windows: trees, capture, kept paths and field sums per window of root items.
+  // Order keys grow with the global root index, so a duplicate-removal group first seen in window w
+  // is final when w ends (later windows only hold larger keys): its path is resolved and its field
+  // added while w's trees are still resident.  Receiver sums continue in key order across windows.
+  const DSurf* surf = P[0].q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
+  const int64_t words = (nRoot + 31) / 32 + 1;
+  size_t cap = 1;
+  while (cap < (size_t)std::max<int64_t>(std::max<int64_t>(1 << 16, 4 * nOut), (int64_t)(tabCapHint / 4))) cap <<= 1;
+  ENSURE(dLevels, kMaxDepth + 2);
+  ENSURE(dCounters, 8);
+  ENSURE(dTraceCounters, 8);
+  const size_t budget = std::max<size_t>(tune.windowNodes > 0 ? (size_t)tune.windowNodes : kWindowNodes, 4096);
+  bool done = false;
+  for (int tabAttempt = 0; tabAttempt < 6 && !done; tabAttempt++) {
+    stats.bounces = stats.node_visits = stats.tri_tests = stats.capture_records = stats.corrected = 0;
+    stats.ms_k_trace = stats.ms_k_capture = stats.ms_trace = stats.ms_capture = stats.ms_field = 0;
+    stats.n_k_trace = 0;
+    stats.max_depth = 0;
+    stats.heavy_nodes = 0;
+    stats.windows = 0;
+    winners = 0;
+    multiWindow = false;
+    cache = PathCache{};
+    if (m > 0) {
       ENSURE(dTabA, cap);
       ENSURE(dTabB, cap);
       ENSURE(dTabOrder, cap);
       ENSURE(dTabRx, cap);
-      ENSURE(dHeavy, std::max<int64_t>(nodes, 1));
+      ENSURE(dField, 6 * (size_t)nOut);
       HIPCHK(hipMemsetAsync(dTabA.p, 0, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dTabB.p, 0, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dTabOrder.p, 0xFF, cap * sizeof(unsigned long long), stream));
-      HIPCHK(hipMemsetAsync(dCounters.p, 0, 6 * sizeof(unsigned long long), stream));
+      HIPCHK(hipMemsetAsync(dField.p, 0, 6 * sizeof(double) * nOut, stream));
+      HIPCHK(hipMemsetAsync(dCounters.p, 0, 8 * sizeof(unsigned long long), stream));
+    }
+    unsigned long long usedBefore = 0, recBefore = 0, corBefore = 0;
+    // nodes per root item: the last trace's figure (or one chain per root), refined by every window
+    double perRoot = perRootHint > 0 ? perRootHint : (double)(maxDepth + 1);
+    bool tableFull = false;
+    int64_t ws = 0;
+    while (ws < nRoot || (nRoot == 0 && ws == 0)) {
+      // trees of roots [ws, ws + W)
+      int64_t W = std::max<int64_t>(1, std::min<int64_t>(nRoot - ws, (int64_t)((double)budget / perRoot)));
+      if (nRoot == 0) W = 0;
+      float msT = 0;
+      for (int attempt = 0;; attempt++) {
+        const size_t want = std::min(budget, (size_t)((double)W * perRoot * 1.25) + 1024);
+        if (nodeCap < want) nodeCap = want;
+        if (nodeCap > budget) nodeCap = budget;
+        ENSURE(dTree, nodeCap);
+        ENSURE(dQueue[0], nodeCap);
+        ENSURE(dQueue[1], nodeCap);
+        nodeCap = std::min(dTree.n, std::min(dQueue[0].n, dQueue[1].n));
+        int64_t need = 0;
+        int rc = trace_stage(txs, ws, W, maxDepth, exact, count, &need, &msT);
+        if (rc == SBR_OK) break;
+        if (rc != SBR_E_LIMIT || attempt > 40) return rc == SBR_E_LIMIT ? fail(rc, "tree buffers could not be sized") : rc;
+        // Overflowed (need is only a lower bound): first use the whole window budget, then halve
+        // the window until its trees fit.
+        perRoot = std::max(perRoot, (double)need / (double)std::max<int64_t>(W, 1));
+        if (nodeCap < budget) {
+          nodeCap = std::max(nodeCap, std::min(budget, (size_t)need * 2 + 1024));
+        } else {
+          if (W == 1) return fail(SBR_E_LIMIT, "one launch needs more tree nodes than the window budget");
+          W = std::max<int64_t>(1, W / 2);
+        }
+      }
+      stats.ms_trace += msT;
+      stats.windows++;
+      if (W > 0) perRoot = std::max(1.0, (double)nodes / (double)W);
+      const bool last = ws + W >= nRoot;
+      if (ws == 0 && !last) multiWindow = true;
+      if (m == 0) {
+        ws += std::max<int64_t>(W, 1);
+        continue;
+      }
+
+      // capture + correction + duplicate removal for this window
+      hipEvent_t kev[4];
+      for (auto& e : kev) HIPCHK(hipEventCreate(&e));
+      struct KevGuard { hipEvent_t* e; ~KevGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } kguard{kev};
+      ENSURE(dHeavy, std::max<int64_t>(nodes, 1));
+      HIPCHK(hipMemsetAsync(dCounters.p + 5, 0, sizeof(unsigned long long), stream));
       CaptureArgs CA{};
       CA.nodes = nodes;
       CA.nrx = m;
@@ -1278,8 +1345,10 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       HIPCHK(hipEventRecord(kev[0], stream));
       CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
                      dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p};
-      k_capture<<<blocks_for(nodes, 256), 256, 0, stream>>>(CA, CB);
-      HIPCHK(hipGetLastError());
+      if (nodes > 0) {
+        k_capture<<<blocks_for(nodes, 256), 256, 0, stream>>>(CA, CB);
+        HIPCHK(hipGetLastError());
+      }
       unsigned long long nHeavy = 0;
       HIPCHK(hipMemcpyAsync(&nHeavy, dCounters.p + 5, sizeof nHeavy, hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
@@ -1288,75 +1357,97 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         k_capture_heavy<<<blocks_for(waves * 64, 256), 256, 0, stream>>>(CA, CB, (int64_t)nHeavy);
         HIPCHK(hipGetLastError());
       }
-      stats.heavy_nodes = (int64_t)nHeavy;
+      stats.heavy_nodes += (int64_t)nHeavy;
       HIPCHK(hipEventRecord(kev[1], stream));
       unsigned long long ctr[4];
       HIPCHK(hipMemcpyAsync(ctr, dCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
       float kms = 0;
       HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
-      stats.ms_k_capture = kms;
+      stats.ms_k_capture += kms;
+      stats.ms_capture += kms;
       if (ctr[3] & 2ull) return fail(SBR_E_INTERNAL, "dedup hash collision detected");
       if ((ctr[3] & 1ull) || ctr[2] * 10 > cap * 7) {
-        cap <<= 2;
-        continue;
+        tableFull = true;
+        break;
       }
-      tabCapHint = cap;
-      stats.capture_records = (int64_t)ctr[0];
-      stats.corrected = (int64_t)ctr[1];
-      winners = (int64_t)ctr[2];
-      sized = true;
-    }
-    if (!sized) return fail(SBR_E_LIMIT, "dedup table could not be sized");
-    HIPCHK(hipEventRecord(ev[3], stream));
+      stats.capture_records += (int64_t)(ctr[0] - recBefore);
+      stats.corrected += (int64_t)(ctr[1] - corBefore);
+      recBefore = ctr[0];
+      corBefore = ctr[1];
+      const int64_t Wn = (int64_t)(ctr[2] - usedBefore);   // groups first seen in this window
+      usedBefore = ctr[2];
 
-    // ---- kept paths: collect, sort by order key, resolve nodes, sort by (tx, rx), fields, sums
-    const int64_t Wc = std::max<int64_t>(winners, 1);
-    ENSURE(dWinKey, Wc);
-    ENSURE(dWinKey2, Wc);
-    ENSURE(dWinRx, Wc);
-    ENSURE(dWinRx2, Wc);
-    ENSURE(dWinIdx, Wc);
-    ENSURE(dWinIdx2, Wc);
-    ENSURE(dWinNode, Wc);
-    ENSURE(dWinField, 6 * Wc);
-    ENSURE(dField, 6 * (size_t)nOut);
-    HIPCHK(hipMemsetAsync(dCounters.p + 4, 0, sizeof(unsigned long long), stream));
-    k_collect<<<blocks_for((int64_t)cap, 256), 256, 0, stream>>>(dTabA.p, dTabOrder.p, dTabRx.p, cap, dWinKey.p,
-                                                                  dWinRx.p, dCounters.p + 4);
-    HIPCHK(hipGetLastError());
-    if (winners > 0) {
-      int txrxBits = 1;
-      while ((int64_t(1) << txrxBits) < nOut) txrxBits++;
-      size_t tb1 = 0, tb2 = 0;
-      HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, dWinKey.p, dWinKey2.p, dWinRx.p, dWinRx2.p,
-                                                (int)winners, 0, 64, stream));
-      HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dWinRx2.p, dWinRx.p, dWinIdx.p, dWinIdx2.p,
-                                                (int)winners, 0, txrxBits, stream));
-      ENSURE(dScratch, std::max(tb1, tb2));
-      // by order key -> dWinKey2 (keys), dWinRx2 (tx * nrx + rx)
-      HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb1, dWinKey.p, dWinKey2.p, dWinRx.p, dWinRx2.p,
-                                                (int)winners, 0, 64, stream));
-      const int64_t words = (stats.launches + 31) / 32 + 1;
-      ENSURE(dRootBits, words);
-      HIPCHK(hipMemsetAsync(dRootBits.p, 0, words * sizeof(unsigned int), stream));
-      k_mark_roots<<<blocks_for(winners, 256), 256, 0, stream>>>(dWinKey2.p, winners, dRootBits.p);
-      k_resolve<<<blocks_for(nodes, 256), 256, 0, stream>>>(dTree.p, nodes, dWinKey2.p, winners, dRootBits.p,
-                                                             dWinNode.p);
-      k_iota<<<blocks_for(winners, 256), 256, 0, stream>>>(dWinIdx.p, winners);
-      HIPCHK(hipGetLastError());
-      // stable by tx * nrx + rx -> dWinRx (keys), dWinIdx2 (index into the key-sorted arrays)
-      HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb2, dWinRx2.p, dWinRx.p, dWinIdx.p, dWinIdx2.p,
-                                                (int)winners, 0, txrxBits, stream));
-      FieldArgs FA{winners, m};
-      k_field<<<blocks_for(winners, 256), 256, 0, stream>>>(FA, dTx.p, dTree.p, dTris.p, surf, dMats.p, dRx.p, dWinRx.p,
-                                                             dWinIdx2.p, dWinNode.p, dWinField.p);
-      HIPCHK(hipGetLastError());
+      // kept paths of this window: collect, sort by order key, resolve nodes, sort by (tx, rx),
+      // fields, receiver sums
+      HIPCHK(hipEventRecord(kev[2], stream));
+      if (Wn > 0) {
+        const int64_t Wc = Wn;
+        ENSURE(dWinKey, Wc);
+        ENSURE(dWinKey2, Wc);
+        ENSURE(dWinRx, Wc);
+        ENSURE(dWinRx2, Wc);
+        ENSURE(dWinIdx, Wc);
+        ENSURE(dWinIdx2, Wc);
+        ENSURE(dWinNode, Wc);
+        ENSURE(dWinField, 6 * Wc);
+        HIPCHK(hipMemsetAsync(dCounters.p + 4, 0, sizeof(unsigned long long), stream));
+        k_collect<<<blocks_for((int64_t)cap, 256), 256, 0, stream>>>(dTabA.p, dTabOrder.p, dTabRx.p, cap,
+                                                                      (unsigned long long)ws,
+                                                                      (unsigned long long)(ws + W), dWinKey.p,
+                                                                      dWinRx.p, dCounters.p + 4);
+        HIPCHK(hipGetLastError());
+        int txrxBits = 1;
+        while ((int64_t(1) << txrxBits) < nOut) txrxBits++;
+        size_t tb1 = 0, tb2 = 0;
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, dWinKey.p, dWinKey2.p, dWinRx.p, dWinRx2.p, (int)Wn, 0,
+                                                  64, stream));
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb2, dWinRx2.p, dWinRx.p, dWinIdx.p, dWinIdx2.p, (int)Wn, 0,
+                                                  txrxBits, stream));
+        ENSURE(dScratch, std::max(tb1, tb2));
+        // by order key -> dWinKey2 (keys), dWinRx2 (tx * nrx + rx)
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb1, dWinKey.p, dWinKey2.p, dWinRx.p, dWinRx2.p, (int)Wn,
+                                                  0, 64, stream));
+        ENSURE(dRootBits, words);
+        HIPCHK(hipMemsetAsync(dRootBits.p, 0, words * sizeof(unsigned int), stream));
+        k_mark_roots<<<blocks_for(Wn, 256), 256, 0, stream>>>(dWinKey2.p, Wn, dRootBits.p);
+        k_resolve<<<blocks_for(nodes, 256), 256, 0, stream>>>(dTree.p, nodes, dWinKey2.p, Wn, dRootBits.p, dWinNode.p);
+        k_iota<<<blocks_for(Wn, 256), 256, 0, stream>>>(dWinIdx.p, Wn);
+        HIPCHK(hipGetLastError());
+        // stable by tx * nrx + rx -> dWinRx (keys), dWinIdx2 (index into the key-sorted arrays)
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb2, dWinRx2.p, dWinRx.p, dWinIdx.p, dWinIdx2.p, (int)Wn,
+                                                  0, txrxBits, stream));
+        FieldArgs FA{Wn, m};
+        k_field<<<blocks_for(Wn, 256), 256, 0, stream>>>(FA, dTx.p, dTree.p, dTris.p, surf, dMats.p, dRx.p, dWinRx.p,
+                                                          dWinIdx2.p, dWinNode.p, dWinField.p);
+        HIPCHK(hipGetLastError());
+        k_rx_sum<<<blocks_for(nOut, 256), 256, 0, stream>>>(dWinRx.p, Wn, dWinField.p, nOut, dField.p);
+        HIPCHK(hipGetLastError());
+      }
+      HIPCHK(hipEventRecord(kev[3], stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      HIPCHK(hipEventElapsedTime(&kms, kev[2], kev[3]));
+      stats.ms_field += kms;
+      winners = Wn;
+      if (multiWindow && Wn > 0) {
+        int rc = cache_window_paths(Wn);
+        if (rc) return rc;
+      }
+      ws += std::max<int64_t>(W, 1);
     }
-    k_rx_sum<<<blocks_for(nOut, 256), 256, 0, stream>>>(dWinRx.p, winners, dWinField.p, nOut, dField.p);
-    HIPCHK(hipGetLastError());
-  } else {
-    HIPCHK(hipEventRecord(ev[3], stream));
+    if (tableFull) {
+      cap <<= 2;
+      continue;
+    }
+    done = true;
+  }
+  if (!done) return fail(SBR_E_LIMIT, "dedup table could not be sized");
+  if (nRoot > 0) perRootHint = std::max(1.0, (double)stats.bounces / (double)nRoot);
+  if (m > 0) tabCapHint = cap;
+  if (multiWindow) {
+    int rc = finish_path_cache();
+    if (rc) return rc;
+    winners = (int64_t)cache.prx.size();
   }
   HIPCHK(hipEventRecord(ev[4], stream));
   HIPCHK(hipStreamSynchronize(stream));
@@ -1365,11 +1456,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   HIPCHK(hipEventElapsedTime(&a, ev[0], ev[1]));
   stats.ms_raygen = a;
   HIPCHK(hipEventElapsedTime(&a, ev[1], ev[2]));
-  stats.ms_trace = a;
-  HIPCHK(hipEventElapsedTime(&a, ev[2], ev[3]));
-  stats.ms_capture = a;
-  HIPCHK(hipEventElapsedTime(&a, ev[3], ev[4]));
-  stats.ms_field = a;
+  stats.ms_capture += a;   // receiver binning upload and line of sight
+  stats.launches = nRoot;
   hField.assign(6 * (size_t)nOut, 0.0);
   if (nOut > 0)
     HIPCHK(hipMemcpyAsync(hField.data(), dField.p, 6 * sizeof(double) * nOut, hipMemcpyDeviceToHost, stream));
@@ -1380,10 +1468,10 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   return SBR_OK;
 }
 
-int Engine::paths(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
-                  int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
-                  bool countOnly) {
-  if (!haveResult) return fail(SBR_E_STATE, "no trace result");
+// Kept paths of the window whose winners are in dWin* (winners of them).
+int Engine::paths_window(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
+                         int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
+                         bool countOnly) {
   HIPCHK(hipSetDevice(device));
   std::vector<int32_t> txrx(winners), perm(winners);
   std::vector<unsigned long long> key(winners);
@@ -1434,9 +1522,133 @@ int Engine::paths(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, 
   return SBR_OK;
 }
 
+int Engine::paths(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
+                  int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
+                  bool countOnly) {
+  if (!haveResult) return fail(SBR_E_STATE, "no trace result");
+  if (!multiWindow)
+    return paths_window(ptx, prx, plaunch, pnseg, poff, stri, scop, stype, ssrc, stgt, nPaths, nSegs, countOnly);
+  const int64_t np = (int64_t)cache.prx.size(), ns = (int64_t)cache.stri.size();
+  *nPaths = np;
+  *nSegs = ns;
+  if (countOnly) return SBR_OK;
+  if (ptx) std::copy(cache.ptx.begin(), cache.ptx.end(), ptx);
+  std::copy(cache.prx.begin(), cache.prx.end(), prx);
+  std::copy(cache.plaunch.begin(), cache.plaunch.end(), plaunch);
+  std::copy(cache.pnseg.begin(), cache.pnseg.end(), pnseg);
+  std::copy(cache.poff.begin(), cache.poff.end(), poff);
+  std::copy(cache.stri.begin(), cache.stri.end(), stri);
+  std::copy(cache.scop.begin(), cache.scop.end(), scop);
+  std::copy(cache.stype.begin(), cache.stype.end(), stype);
+  std::copy(cache.ssrc.begin(), cache.ssrc.end(), ssrc);
+  std::copy(cache.stgt.begin(), cache.stgt.end(), stgt);
+  return SBR_OK;
+}
+
+int Engine::path_tx(int32_t* out, int64_t n) {
+  if (!haveResult) return fail(SBR_E_STATE, "no trace result");
+  if (n != winners || (!out && n > 0)) return fail(SBR_E_ARG, "path tx: count mismatch");
+  if (n == 0) return SBR_OK;
+  if (multiWindow) {
+    std::copy(cache.ptx.begin(), cache.ptx.end(), out);
+    return SBR_OK;
+  }
+  std::vector<int32_t> txrx(n);
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipMemcpy(txrx.data(), dWinRx.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  for (int64_t w = 0; w < n; w++) out[w] = txrx[w] / nrx;
+  return SBR_OK;
+}
+
+int Engine::path_fields(double* out, int64_t n) {
+  if (!haveResult) return fail(SBR_E_STATE, "no trace result");
+  if (n != winners || (!out && n > 0)) return fail(SBR_E_ARG, "path fields: count mismatch");
+  if (n == 0) return SBR_OK;
+  if (multiWindow) {
+    std::copy(cache.pfield.begin(), cache.pfield.end(), out);
+    return SBR_OK;
+  }
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipMemcpy(out, dWinField.p, 6 * sizeof(double) * n, hipMemcpyDeviceToHost));
+  return SBR_OK;
+}
+
+// Appends the current window's kept paths (winners w in dWin*) to the host cache.
+int Engine::cache_window_paths(int64_t w) {
+  int64_t np = 0, ns = 0;
+  int rc = paths_window(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &np,
+                        &ns, true);
+  if (rc) return rc;
+  if (np != w) return fail(SBR_E_INTERNAL, "window path count mismatch");
+  std::vector<int32_t> ptx(np), prx(np), pl(np), pn(np), st(ns), sc(ns), sy(ns);
+  std::vector<int64_t> po(np);
+  std::vector<double> s0(3 * ns), s1(3 * ns), pf(6 * np);
+  rc = paths_window(ptx.data(), prx.data(), pl.data(), pn.data(), po.data(), st.data(), sc.data(), sy.data(), s0.data(),
+                    s1.data(), &np, &ns, false);
+  if (rc) return rc;
+  HIPCHK(hipMemcpy(pf.data(), dWinField.p, 6 * sizeof(double) * np, hipMemcpyDeviceToHost));
+  const int64_t segBase = (int64_t)cache.stri.size();
+  for (int64_t k = 0; k < np; k++) po[k] += segBase;
+  auto app = [](auto& dst, const auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
+  app(cache.ptx, ptx);
+  app(cache.prx, prx);
+  app(cache.plaunch, pl);
+  app(cache.pnseg, pn);
+  app(cache.poff, po);
+  app(cache.stri, st);
+  app(cache.scop, sc);
+  app(cache.stype, sy);
+  app(cache.ssrc, s0);
+  app(cache.stgt, s1);
+  app(cache.pfield, pf);
+  return SBR_OK;
+}
+
+// Windows come in launch order and each window's paths are (tx, rx) then launch ordered: a stable
+// sort by (tx, rx) gives the single-window order.
+int Engine::finish_path_cache() {
+  const int64_t np = (int64_t)cache.prx.size();
+  std::vector<int64_t> perm(np);
+  for (int64_t k = 0; k < np; k++) perm[k] = k;
+  std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
+    return cache.ptx[a] != cache.ptx[b] ? cache.ptx[a] < cache.ptx[b] : cache.prx[a] < cache.prx[b];
+  });
+  PathCache c;
+  c.ptx.resize(np);
+  c.prx.resize(np);
+  c.plaunch.resize(np);
+  c.pnseg.resize(np);
+  c.poff.resize(np);
+  c.pfield.resize(6 * np);
+  int64_t seg = 0;
+  for (int64_t k = 0; k < np; k++) {
+    const int64_t q = perm[k];
+    c.ptx[k] = cache.ptx[q];
+    c.prx[k] = cache.prx[q];
+    c.plaunch[k] = cache.plaunch[q];
+    c.pnseg[k] = cache.pnseg[q];
+    c.poff[k] = seg;
+    for (int j = 0; j < 6; j++) c.pfield[6 * k + j] = cache.pfield[6 * q + j];
+    for (int32_t j = 0; j < cache.pnseg[q]; j++) {
+      const int64_t from = cache.poff[q] + j;
+      c.stri.push_back(cache.stri[from]);
+      c.scop.push_back(cache.scop[from]);
+      c.stype.push_back(cache.stype[from]);
+      for (int a = 0; a < 3; a++) {
+        c.ssrc.push_back(cache.ssrc[3 * from + a]);
+        c.stgt.push_back(cache.stgt[3 * from + a]);
+      }
+    }
+    seg += cache.pnseg[q];
+  }
+  cache = std::move(c);
+  return SBR_OK;
+}
+
 int Engine::trees(int32_t* launch, int32_t* depth, int64_t* code, int32_t* tri, int32_t* cop, int32_t* type,
                   double* src, double* tgt, double* pl) {
   if (!haveResult) return fail(SBR_E_STATE, "no trace result");
+  if (multiWindow) return fail(SBR_E_STATE, "interaction trees are kept for single-window traces only");
   HIPCHK(hipSetDevice(device));
   std::vector<TNode> h(nodes);
   std::vector<TxDesc> txs(ntx);
