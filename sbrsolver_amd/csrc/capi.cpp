@@ -5,9 +5,11 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <new>
 
 #include "engine.h"
+#include "prefilter.h"
 #include "sbr_capi.h"
 #include "sbrsolver/BVH.h"
 #include "sbrsolver/Preprocessor.h"
@@ -100,13 +102,10 @@ int Engine::upload(const sbr_scene_desc* d) {
   double ext = 0.0;
   for (int i = 0; i < 9 * n; i++) ext = std::max(ext, std::fabs(d->tri_v[i]));
   ext += 1.0;
-  const double delta = 1e-3 + 4e-6 * ext;
+  const double delta = prefilter_delta(ext);
   auto box32 = [&](const double* lo, const double* hi) {
     DBox32 b{};
-    for (int a = 0; a < 3; a++) {
-      b.lo[a] = std::nextafter((float)(lo[a] - delta), -INFINITY);
-      b.hi[a] = std::nextafter((float)(hi[a] + delta), INFINITY);
-    }
+    box_grown(lo, hi, delta, b.lo, b.hi);
     return b;
   };
   std::vector<DBox32> tb32(trisOut.size());
@@ -118,6 +117,75 @@ int Engine::upload(const sbr_scene_desc* d) {
       hi[a] = std::max({tv[a], tv[3 + a], tv[6 + a]});
     }
     tb32[k] = box32(lo, hi);
+  }
+  // Leaf BVHs of the large leaves: median splits on the longest axis of the triangles' pre-test box
+  // centres, leaves of at most kSubLeafSize triangles, node boxes the union of the pre-test boxes.
+  std::vector<DSub> subs;
+  std::vector<int32_t> subTri;
+  int subInternal = 0;
+  {
+    std::vector<int32_t> ids;
+    std::function<void(int, int, int, int)> fill = [&](int node, int lo, int hi, int level) {
+      DSub dsub{};
+      for (int a = 0; a < 3; a++) { dsub.lo[a] = INFINITY; dsub.hi[a] = -INFINITY; }
+      float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (int i = lo; i < hi; i++) {
+        const DBox32& b = tb32[ids[i]];
+        for (int a = 0; a < 3; a++) {
+          dsub.lo[a] = std::min(dsub.lo[a], b.lo[a]);
+          dsub.hi[a] = std::max(dsub.hi[a], b.hi[a]);
+          const float c = 0.5f * (b.lo[a] + b.hi[a]);
+          clo[a] = std::min(clo[a], c);
+          chi[a] = std::max(chi[a], c);
+        }
+      }
+      if (hi - lo <= kSubLeafSize) {
+        dsub.first = (int32_t)subTri.size();
+        dsub.count = hi - lo;
+        subTri.insert(subTri.end(), ids.begin() + lo, ids.begin() + hi);
+        subs[node] = dsub;
+        return;
+      }
+      subInternal = std::max(subInternal, level + 1);
+      int axis = 0;
+      for (int a = 1; a < 3; a++)
+        if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+      const int mid = (lo + hi) / 2;
+      std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi, [&](int32_t x, int32_t y) {
+        const float cx = tb32[x].lo[axis] + tb32[x].hi[axis], cy = tb32[y].lo[axis] + tb32[y].hi[axis];
+        return cx != cy ? cx < cy : x < y;
+      });
+      const int c = (int)subs.size();
+      subs.resize(c + 2);
+      dsub.first = c;
+      dsub.count = 0;
+      subs[node] = dsub;
+      fill(c, lo, mid, level + 1);
+      fill(c + 1, mid, hi, level + 1);
+    };
+    for (auto& nd : nodesOut) {
+      nd.sub = -1;
+      if (nd.count > kSubLeaf) {
+        ids.resize(nd.count);
+        for (int i = 0; i < nd.count; i++) ids[i] = nd.a + i;
+        nd.sub = (int32_t)subs.size();
+        subs.resize(subs.size() + 1);
+        fill(nd.sub, 0, nd.count, 0);
+      }
+    }
+  }
+  std::vector<DNodeF> nf(nodesOut.size());
+  for (size_t k = 0; k < nodesOut.size(); k++) {
+    const DNode& n = nodesOut[k];
+    if (n.count > 0) continue;
+    const int32_t ch[2] = {n.a, n.b};
+    for (int c = 0; c < 2; c++) {
+      const DNode& cn = nodesOut[ch[c]];
+      box_grown(cn.bmin, cn.bmax, 2.0 * delta, &nf[k].box[c][0], &nf[k].box[c][3]);
+      const bool inner = box_shrunk(cn.bmin, cn.bmax, delta, &nf[k].box[c][6], &nf[k].box[c][9]);
+      nf[k].ch[c].item = ch[c] | (cn.count > 0 ? kLeafBit : 0);
+      nf[k].ch[c].inner = inner ? 1 : 0;
+    }
   }
   auto surf_of = [&](int id) {
     DSurf s{};
@@ -140,7 +208,8 @@ int Engine::upload(const sbr_scene_desc* d) {
     HIPCHK(hipSetDevice(device));
     if (put(dNodes, nodesOut, stream) || put(dTris, trisOut, stream) || put(dTriN, triN, stream) ||
         put(dSurfQ1, sq, stream) || put(dSurfId, si, stream) ||
-        put(dTriBox32, tb32, stream))
+        put(dTriBox32, tb32, stream) || put(dSub, subs, stream) || put(dSubTri, subTri, stream) ||
+        put(dNodesF, nf, stream))
       return fail(SBR_E_NOMEM, "scene upload");
     HIPCHK(hipStreamSynchronize(stream));
   }
@@ -158,9 +227,13 @@ int Engine::upload(const sbr_scene_desc* d) {
   nnode = nn;
   treeHeight = height;
   hTriBox32.swap(tb32);
+  hNodesF.swap(nf);
+  hSub.swap(subs);
+  hSubTri.swap(subTri);
+  subHeight = subInternal;
   sceneExtent = (float)ext;
   boxDelta = (float)delta;
-  sceneBytes = hNodes.size() * sizeof(DNode) + hTris.size() * sizeof(DBox32);   // LDS-staged part
+  sceneBytes = hNodesF.size() * sizeof(DNodeF);   // LDS-staged part
   ncop = 0;
   for (int t = 0; t < n; t++) ncop = std::max(ncop, hCop[t] + 1);
   haveScene = true;
@@ -182,6 +255,10 @@ int Engine::clone_scene(const Engine& s) {
   hVerts = s.hVerts;
   hNorms = s.hNorms;
   hTriBox32 = s.hTriBox32;
+  hNodesF = s.hNodesF;
+  hSub = s.hSub;
+  hSubTri = s.hSubTri;
+  subHeight = s.subHeight;
   ntri = s.ntri;
   nnode = s.nnode;
   ncop = s.ncop;
@@ -192,7 +269,8 @@ int Engine::clone_scene(const Engine& s) {
   if (device >= 0) {
     HIPCHK(hipSetDevice(device));
     if (put(dNodes, hNodes, stream) || put(dTris, hTris, stream) || put(dTriN, hTriN, stream) ||
-        put(dSurfQ1, hSurfQ1, stream) || put(dSurfId, hSurfId, stream) || put(dTriBox32, hTriBox32, stream))
+        put(dSurfQ1, hSurfQ1, stream) || put(dSurfId, hSurfId, stream) || put(dTriBox32, hTriBox32, stream) ||
+        put(dSub, hSub, stream) || put(dSubTri, hSubTri, stream) || put(dNodesF, hNodesF, stream))
       return fail(SBR_E_NOMEM, "scene upload");
     HIPCHK(hipStreamSynchronize(stream));
   }

@@ -7,6 +7,7 @@
 #include <climits>
 
 #include "engine.h"
+#include "prefilter.h"
 
 namespace sbr {
 
@@ -86,48 +87,6 @@ __device__ __forceinline__ bool box_test(const double* bmin, const double* bmax,
   return box_line(bmin, bmax, r.o, r.d, lo, hi);
 }
 
-// Single-precision pre-test in front of each ray-triangle test: the triangle's box, rounded outward
-// and padded by delta (set at upload), is tested against the ray in single precision.  For rays
-// with |origin coordinates| <= 4E (E = scene extent) and a direction whose largest component lies
-// in [1e-3, 1e3], every parameter t at which the ray meets the exact box lies inside the computed
-// interval, so a triangle is skipped only when the exact test misses it (interval empty, or wholly
-// at t < 0) or when every hit on it lies beyond the best hit so far (entry > t_best (1 + 1e-6)).
-// Other rays take the exact test on every triangle.
-struct RayF {
-  float ix, iy, iz;     // 1 / direction (components clamped away from 0)
-  float nx, ny, nz;     // -origin / direction
-  bool on;
-};
-
-__device__ __forceinline__ RayF rayf_pre(V3 o, V3 d, float ext) {
-  RayF f;
-  const double dm = fmax(fabs(d.x), fmax(fabs(d.y), fabs(d.z)));
-  const double lim = 4.0 * (double)ext;
-  f.on = fabs(o.x) <= lim && fabs(o.y) <= lim && fabs(o.z) <= lim && dm >= 1e-3 && dm <= 1e3;
-  auto inv = [](double c) {
-    float x = (float)c;
-    if (fabsf(x) < 1e-20f) x = x < 0.0f ? -1e-20f : 1e-20f;
-    return 1.0f / x;
-  };
-  f.ix = inv(d.x);
-  f.iy = inv(d.y);
-  f.iz = inv(d.z);
-  f.nx = -(float)o.x * f.ix;
-  f.ny = -(float)o.y * f.iy;
-  f.nz = -(float)o.z * f.iz;
-  return f;
-}
-
-// True if the triangle behind box b may still hold a hit at t <= clip.
-__device__ __forceinline__ bool box32_may_hit(const DBox32& b, const RayF& f, float clip) {
-  const float ax = fmaf(b.lo[0], f.ix, f.nx), bx = fmaf(b.hi[0], f.ix, f.nx);
-  const float ay = fmaf(b.lo[1], f.iy, f.ny), by = fmaf(b.hi[1], f.iy, f.ny);
-  const float az = fmaf(b.lo[2], f.iz, f.nz), bz = fmaf(b.hi[2], f.iz, f.nz);
-  const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
-  const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
-  return tn <= tf && tf >= 0.0f && tn <= clip;
-}
-
 template <bool EXACT>
 __device__ __forceinline__ bool keep_box(double lo, double hi, const Hit& h) {
   if (EXACT) return true;
@@ -136,12 +95,45 @@ __device__ __forceinline__ bool keep_box(double lo, double hi, const Hit& h) {
   return true;
 }
 
-// Traversal stack: entry e of the calling thread is at stkN[e * stride] / stkT[e * stride]
-// (stride = threads per block, so a wave's entries are contiguous).
+// Whether the traversal enters a child, single-precision path: the grown box decides a miss (or a
+// child wholly behind the origin or beyond the best hit), the shrunk box decides a pass, and a ray
+// between the two takes the reference's FP64 slab test on the exact box.  *entry is a lower bound
+// of the t of every hit inside the child.
+__device__ __forceinline__ bool child_fast(const float* b, const DChildF& c, const DNode* __restrict__ nodes,
+                                           const RayPre& r, const RayF& f, float clip, const Hit& h, float* entry) {
+  if (!box32_may_hit(b, b + 3, f, clip, entry)) return false;
+  if (c.inner && box32_crossed(b + 6, b + 9, f)) return true;
+  double lo, hi;
+  const DNode& n = nodes[c.item & ~kLeafBit];
+  return box_test(n.bmin, n.bmax, r, &lo, &hi) && keep_box<false>(lo, hi, h);
+}
+
+// FP64 path (EXACT, or a ray the single-precision tests do not cover).
 template <bool EXACT>
-__device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
-                                            const DBox32* __restrict__ tbox, float ext, V3 o, V3 d, Hit& h, int* stkN,
-                                            float* stkT, int stride) {
+__device__ __forceinline__ bool child_exact(const DChildF& c, const DNode* __restrict__ nodes, const RayPre& r,
+                                            const Hit& h, float* entry) {
+  double lo, hi;
+  const DNode& n = nodes[c.item & ~kLeafBit];
+  if (!box_test(n.bmin, n.bmax, r, &lo, &hi) || !keep_box<EXACT>(lo, hi, h)) return false;
+  *entry = __double2float_rd(lo);
+  return true;
+}
+
+// Traversal stack: entry e of the calling thread is at stkN[e * stride] / stkT[e * stride]
+// (stride = threads per block, so a wave's entries are contiguous).  Entries are work items --
+// inner node n, leaf n | kLeafBit, leaf-BVH node ~s -- with a lower bound of their entry parameter.
+//
+// SUB: the scene has leaf BVHs (otherwise that code is left out of the kernel).
+// A leaf with a leaf BVH is searched through it (EXACT=false, pre-test usable for the ray): the
+// leaf-BVH boxes are unions of the triangles' padded pre-test boxes, so a subtree is skipped only
+// when the ray provably misses all its triangles or every hit in it lies beyond the best hit,
+// i.e. exactly when the per-triangle pre-test would reject each of its triangles.  The leaf's
+// result, the minimum (distance, DFS position) over its hit triangles, is unchanged.
+template <bool EXACT, bool SUB>
+__device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, const DNodeF* __restrict__ nodesF,
+                                            const DTri* __restrict__ tris, const DBox32* __restrict__ tbox,
+                                            const DSub* __restrict__ subs, const int32_t* __restrict__ subTri,
+                                            float ext, V3 o, V3 d, Hit& h, int* stkN, float* stkT, int stride) {
   h.idx = INT_MAX;
   h.dist = INFINITY;
   h.t = INFINITY;
@@ -149,46 +141,82 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
   h.tests = 0;
   const RayPre r = ray_pre(o, d);
   const RayF f = rayf_pre(o, d, ext);
+  const bool fast = !EXACT && f.on;
   float clip = INFINITY;
-  double lo, hi;
-  if (!box_test(nodes[0].bmin, nodes[0].bmax, r, &lo, &hi) || !keep_box<EXACT>(lo, hi, h)) return false;
+  {
+    double lo, hi;
+    if (!box_test(nodes[0].bmin, nodes[0].bmax, r, &lo, &hi) || !keep_box<EXACT>(lo, hi, h)) return false;
+  }
   int sp = 0;
-  int cur = 0;
+  int cur = nodes[0].count > 0 ? kLeafBit : 0;
+  // Exact test of DFS triangle k after its pre-test, keeping the minimum (distance, position).
+  auto test_tri = [&](int k) {
+    float e;
+    if (f.on && !box32_may_hit(tbox[k].lo, tbox[k].hi, f, clip, &e)) return;
+    h.tests++;
+    const DTri& tr = tris[k];
+    V3 p;
+    double dist, t;
+    if (tri_intersect(ld3(tr.v1), ld3(tr.e1), ld3(tr.e2), o, d, &p, &dist, &t)) {
+      if (dist < h.dist || (dist == h.dist && k < h.idx)) {
+        h.idx = k;
+        h.dist = dist;
+        h.t = t;
+        clip = __double2float_ru(t * (1.0 + 1e-6));
+      }
+    }
+  };
   while (true) {
-    const DNode nd = nodes[cur];
     h.nodes++;
-    if (nd.count > 0) {
-      for (int k = nd.a; k < nd.a + nd.count; k++) {
-        if (f.on && !box32_may_hit(tbox[k], f, clip)) continue;
-        h.tests++;
-        const DTri& tr = tris[k];
-        V3 p;
-        double dist, t;
-        if (tri_intersect(ld3(tr.v1), ld3(tr.e1), ld3(tr.e2), o, d, &p, &dist, &t)) {
-          if (dist < h.dist || (dist == h.dist && k < h.idx)) {
-            h.idx = k;
-            h.dist = dist;
-            h.t = t;
-            clip = __double2float_ru(t * (1.0 + 1e-6));
-          }
+    if (SUB && cur < 0) {
+      const DSub sn = subs[~cur];
+      if (sn.count > 0) {
+        for (int j = sn.first; j < sn.first + sn.count; j++) test_tri(subTri[j]);
+      } else {
+        float ta, tb;
+        const bool ina = box32_may_hit(subs[sn.first].lo, subs[sn.first].hi, f, clip, &ta);
+        const bool inb = box32_may_hit(subs[sn.first + 1].lo, subs[sn.first + 1].hi, f, clip, &tb);
+        if (ina && inb) {
+          const bool aFirst = ta <= tb;
+          stkN[sp * stride] = ~(aFirst ? sn.first + 1 : sn.first);
+          stkT[sp * stride] = aFirst ? tb : ta;
+          sp++;
+          cur = ~(aFirst ? sn.first : sn.first + 1);
+          continue;
+        }
+        if (ina || inb) {
+          cur = ~(ina ? sn.first : sn.first + 1);
+          continue;
         }
       }
+    } else if (cur & kLeafBit) {
+      const int4 li = reinterpret_cast<const int4*>(&nodes[cur & ~kLeafBit])[3];   // a, b, count, sub
+      if (SUB && fast && li.w >= 0) {
+        cur = ~li.w;   // search the leaf through its leaf BVH
+        continue;
+      }
+      for (int k = li.x; k < li.x + li.z; k++) test_tri(k);
     } else {
-      double la, ha, lb, hb;
-      bool ina = box_test(nodes[nd.a].bmin, nodes[nd.a].bmax, r, &la, &ha) && keep_box<EXACT>(la, ha, h);
-      bool inb = box_test(nodes[nd.b].bmin, nodes[nd.b].bmax, r, &lb, &hb) && keep_box<EXACT>(lb, hb, h);
+      const DNodeF& nf = nodesF[cur];
+      float ta, tb;
+      bool ina, inb;
+      if (fast) {
+        ina = child_fast(nf.box[0], nf.ch[0], nodes, r, f, clip, h, &ta);
+        inb = child_fast(nf.box[1], nf.ch[1], nodes, r, f, clip, h, &tb);
+      } else {
+        ina = child_exact<EXACT>(nf.ch[0], nodes, r, h, &ta);
+        inb = child_exact<EXACT>(nf.ch[1], nodes, r, h, &tb);
+      }
       if (ina && inb) {
-        int nearN = nd.a, farN = nd.b;
-        double farT = lb;
-        if (!EXACT && lb < la) { nearN = nd.b; farN = nd.a; farT = la; }
-        stkN[sp * stride] = farN;
-        stkT[sp * stride] = __double2float_rd(farT);
+        const bool bFirst = !EXACT && tb < ta;   // EXACT: the reference's DFS order
+        stkN[sp * stride] = bFirst ? nf.ch[0].item : nf.ch[1].item;
+        stkT[sp * stride] = bFirst ? ta : tb;
         sp++;
-        cur = nearN;
+        cur = bFirst ? nf.ch[1].item : nf.ch[0].item;
         continue;
       }
       if (ina || inb) {
-        cur = ina ? nd.a : nd.b;
+        cur = ina ? nf.ch[0].item : nf.ch[1].item;
         continue;
       }
     }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -27,8 +28,36 @@ struct alignas(64) DNode {
   int32_t a;      // inner: first child (DFS index); leaf: first triangle (DFS index)
   int32_t b;      // inner: second child; leaf: unused
   int32_t count;  // leaf: triangle count (>0); inner: 0
-  int32_t pad;
+  int32_t sub;    // leaf with more than kSubLeaf triangles: root of its leaf BVH (DSub index), else -1
 };
+// Node of a leaf BVH, 32 B.  Large leaves of the reference tree (a flat, tessellated floor ends as
+// one leaf of thousands of triangles: its box has no surface area, so the SAH never splits it) get
+// a BVH of their own over single-precision pre-test boxes.  It only prunes what provably cannot hit
+// or cannot beat the best hit, so the leaf's result is unchanged.
+struct alignas(32) DSub {
+  float lo[3];
+  float hi[3];
+  int32_t first;  // inner: first child (the second follows); leaf: first entry of subTri
+  int32_t count;  // leaf: triangle count (>0); inner: 0
+};
+// Reference-tree node as the single-precision traversal reads it, 128 B: the boxes of both children
+// stored in the parent, each as the exact box grown by 2 delta (a ray whose computed interval of it
+// is empty misses the exact box by more than delta, where the reference's FP64 slab test cannot
+// pass) and shrunk by delta (a non-empty computed interval puts the line at depth delta / 2 inside
+// the exact box, where the FP64 test cannot fail).  Only rays in between take the FP64 test.
+constexpr int32_t kLeafBit = 1 << 30;   // work item of a leaf: DFS node index | kLeafBit
+struct DChildF {
+  int32_t item;    // child's work item (node index, | kLeafBit for a leaf)
+  int32_t inner;   // 1: the shrunk box is non-empty
+};
+struct alignas(128) DNodeF {
+  float box[2][12];   // per child: grown lo[3], hi[3], shrunk lo[3], hi[3]
+  DChildF ch[2];
+  int32_t pad[4];
+};
+static_assert(sizeof(DNodeF) == 128, "DNodeF layout");
+constexpr int kSubLeaf = 64;      // leaves above this size get a leaf BVH
+constexpr int kSubLeafSize = 4;   // triangles per leaf-BVH leaf
 // Triangle as the traversal reads it, in leaf (DFS) order, 96 B.
 struct alignas(32) DTri {
   double v1[3];
@@ -140,7 +169,8 @@ struct Engine {
   // scene (host copies for queries)
   int ntri = 0, nnode = 0, ncop = 0, treeHeight = 0;
   // Traversal stack entries per ray: one deferred sibling per internal node of the current path.
-  int stackDepth() const { return treeHeight > 1 ? treeHeight - 1 : 1; }
+  // A leaf BVH continues the path below its leaf, adding its own internal levels.
+  int stackDepth() const { return std::max(1, treeHeight - 1 + subHeight); }
   size_t sceneBytes = 0;
   std::vector<DNode> hNodes;
   std::vector<DTri> hTris;
@@ -149,9 +179,16 @@ struct Engine {
   std::vector<int32_t> hCop, hMat, hMeshTriId;
   std::vector<double> hVerts, hNorms;       // by triangleId
   DBuf<DNode> dNodes;
+  std::vector<DNodeF> hNodesF;
+  DBuf<DNodeF> dNodesF;
   DBuf<DTri> dTris;
   std::vector<DBox32> hTriBox32;   // single-precision pre-test box per DFS triangle
   DBuf<DBox32> dTriBox32;
+  std::vector<DSub> hSub;          // leaf BVHs of large leaves
+  std::vector<int32_t> hSubTri;    // their triangles (DFS indices)
+  DBuf<DSub> dSub;
+  DBuf<int32_t> dSubTri;
+  int subHeight = 0;               // height of the tallest leaf BVH
   float sceneExtent = 1.0f, boxDelta = 1e-3f;
   DBuf<double> dTriN;
   DBuf<DSurf> dSurfQ1, dSurfId;

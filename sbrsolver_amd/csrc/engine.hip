@@ -67,6 +67,8 @@ template struct DBuf<int64_t>;
 template struct DBuf<DNode>;
 template struct DBuf<DTri>;
 template struct DBuf<DBox32>;
+template struct DBuf<DSub>;
+template struct DBuf<DNodeF>;
 template struct DBuf<DSurf>;
 template struct DBuf<TNode>;
 template struct DBuf<TxDesc>;
@@ -161,36 +163,34 @@ struct TraceArgs {
   int32_t depth;
   int32_t nTx;
   int32_t stackDepth;
-  int32_t nodeBytes, boxBytes;   // nodes and triangle pre-test boxes staged in LDS (LDS variant)
+  int32_t nodeBytes;             // single-precision node records staged in LDS (LDS variant)
   int64_t rootBase;              // global root index of the window's first root item
   float ext;                                // scene extent for the single-precision pre-test
   unsigned long long nodeCap, queueCap;
 };
 
 // Flags: [0] bit 0 node overflow, bit 1 queue overflow; [6], [7] node visits / triangle tests.
-template <bool EXACT, bool LDS>
+template <bool EXACT, bool LDS, bool SUB>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_trace(TraceArgs A, const TxDesc* __restrict__ txs,
-                                                  const DNode* __restrict__ gNodes, const DTri* __restrict__ gTris,
-                                                  const DBox32* __restrict__ gBox,
+                                                  const DNode* __restrict__ nodes, const DNodeF* __restrict__ gNodesF,
+                                                  const DTri* __restrict__ gTris,
+                                                  const DBox32* __restrict__ gBox, const DSub* __restrict__ subs,
+                                                  const int32_t* __restrict__ subTri,
                                                   const double* __restrict__ triN, const int32_t* __restrict__ qIn,
                                                   int32_t* __restrict__ qOut, TNode* __restrict__ tree,
                                                   LevelRec* __restrict__ lv, unsigned long long* __restrict__ flags,
                                                   int countTraversal) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const DNode* nodes = gNodes;
+  const DNodeF* nodesF = gNodesF;
   const DTri* tris = gTris;
   const DBox32* tbox = gBox;
   size_t sceneB = 0;
-  if (LDS) {
-    const int4* s0 = reinterpret_cast<const int4*>(gNodes);
+  if (LDS) {   // single-precision node records in LDS; FP64 boxes, triangles and pre-test boxes via the caches
+    const int4* s0 = reinterpret_cast<const int4*>(gNodesF);
     int4* d0 = reinterpret_cast<int4*>(smem);
     for (int i = threadIdx.x; i < A.nodeBytes / 16; i += blockDim.x) d0[i] = s0[i];
-    const int4* s2 = reinterpret_cast<const int4*>(gBox);
-    int4* d2 = reinterpret_cast<int4*>(smem + A.nodeBytes);
-    for (int i = threadIdx.x; i < A.boxBytes / 16; i += blockDim.x) d2[i] = s2[i];
-    sceneB = (size_t)A.nodeBytes + A.boxBytes;
-    nodes = reinterpret_cast<const DNode*>(smem);
-    tbox = reinterpret_cast<const DBox32*>(smem + A.nodeBytes);
+    sceneB = (size_t)A.nodeBytes;
+    nodesF = reinterpret_cast<const DNodeF*>(smem);
     __syncthreads();
   }
   int* stkN = reinterpret_cast<int*>(smem + sceneB) + threadIdx.x;
@@ -270,7 +270,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       nd.pad = 0;
       const V3 src = ld3(nd.src), dir = ld3(nd.dir);
       Hit h;
-      if (closest_hit<EXACT>(nodes, tris, tbox, A.ext, src, dir, h, stkN, stkT, blockDim.x)) {
+      if (closest_hit<EXACT, SUB>(nodes, nodesF, tris, tbox, subs, subTri, A.ext, src, dir, h, stkN, stkT,
+                                  blockDim.x)) {
         const DTri& t = tris[h.idx];
         const V3 hp = hit_point(src, dir, h);
         nd.tgt[0] = hp.x; nd.tgt[1] = hp.y; nd.tgt[2] = hp.z;
@@ -332,8 +333,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
 // =============================================================================================
 template <bool EXACT>
 __global__ void __launch_bounds__(kBlock) k_los(const TxDesc* __restrict__ txs, int32_t nTx,
-                                                const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
-                                                const DBox32* __restrict__ tbox, float ext,
+                                                const DNode* __restrict__ nodes, const DNodeF* __restrict__ nodesF,
+                                                const DTri* __restrict__ tris,
+                                                const DBox32* __restrict__ tbox, const DSub* __restrict__ subs,
+                                                const int32_t* __restrict__ subTri, float ext,
                                                 const double* __restrict__ rx, int32_t m, int32_t stackDepth,
                                                 uint8_t* __restrict__ los) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -345,13 +348,17 @@ __global__ void __launch_bounds__(kBlock) k_los(const TxDesc* __restrict__ txs, 
   V3 tx = v3(T.tx[0], T.tx[1], T.tx[2]);
   V3 d = vsub(ld3(rx + 3 * (k % m)), tx);
   Hit h;
-  bool hh = closest_hit<EXACT>(nodes, tris, tbox, ext, tx, d, h, stkN, stkT, blockDim.x);
+  bool hh = closest_hit<EXACT, true>(nodes, nodesF, tris, tbox, subs, subTri, ext, tx, d, h, stkN, stkT, blockDim.x);
   los[k] = (!hh || vnorm(d) < h.dist) ? 1 : 0;
 }
 
 template <bool EXACT>
-__global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restrict__ nodes, const DTri* __restrict__ tris,
-                                                          const DBox32* __restrict__ tbox, float ext,
+__global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restrict__ nodes,
+                                                          const DNodeF* __restrict__ nodesF,
+                                                          const DTri* __restrict__ tris,
+                                                          const DBox32* __restrict__ tbox,
+                                                          const DSub* __restrict__ subs,
+                                                          const int32_t* __restrict__ subTri, float ext,
                                                           const double* __restrict__ org, const double* __restrict__ dirs,
                                                           int32_t n, int32_t stackDepth, int32_t* __restrict__ triOut,
                                                           double* __restrict__ ptOut, double* __restrict__ distOut) {
@@ -361,8 +368,8 @@ __global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restric
   int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Hit h;
-  bool hh = closest_hit<EXACT>(nodes, tris, tbox, ext, ld3(org + 3 * (int64_t)i), ld3(dirs + 3 * (int64_t)i), h,
-                               stkN, stkT, blockDim.x);
+  bool hh = closest_hit<EXACT, true>(nodes, nodesF, tris, tbox, subs, subTri, ext, ld3(org + 3 * (int64_t)i),
+                               ld3(dirs + 3 * (int64_t)i), h, stkN, stkT, blockDim.x);
   triOut[i] = hh ? tris[h.idx].id : -1;
   const V3 hp = hit_point(ld3(org + 3 * (int64_t)i), ld3(dirs + 3 * (int64_t)i), h);
   ptOut[3 * i] = hp.x; ptOut[3 * i + 1] = hp.y; ptOut[3 * i + 2] = hp.z;
@@ -1025,11 +1032,13 @@ int Engine::closest_hit(const double* o, const double* d, int n, int exact, int3
   HIPCHK(hipMemcpyAsync(dD.p, d, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
   const size_t sm = stack_bytes(stackDepth(), kBlock);
   if (exact)
-    k_closest_batch<true><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dTris.p, dTriBox32.p, sceneExtent,
-                                                                         dO.p, dD.p, n, stackDepth(), dT.p, dP.p, dDist.p);
+    k_closest_batch<true><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
+                                                                         dSubTri.p, sceneExtent, dO.p, dD.p, n,
+                                                                         stackDepth(), dT.p, dP.p, dDist.p);
   else
-    k_closest_batch<false><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dTris.p, dTriBox32.p, sceneExtent,
-                                                                          dO.p, dD.p, n, stackDepth(), dT.p, dP.p, dDist.p);
+    k_closest_batch<false><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
+                                                                          dSubTri.p, sceneExtent, dO.p, dD.p, n,
+                                                                          stackDepth(), dT.p, dP.p, dDist.p);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(tri, dT.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(pt, dP.p, 3 * sizeof(double) * n, hipMemcpyDeviceToHost, stream));
@@ -1057,8 +1066,11 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   // number of resident blocks per CU.
   const size_t stk = stack_bytes(stackDepth(), kBlock);
   const bool ldsFits = sceneBytes + stk <= kSceneLdsMax;
+  const bool sub = !hSub.empty();
   auto variant = [&](bool l) {
-    return exact ? (l ? k_trace<true, true> : k_trace<true, false>) : (l ? k_trace<false, true> : k_trace<false, false>);
+    if (exact) return l ? k_trace<true, true, false> : k_trace<true, false, false>;
+    if (sub) return l ? k_trace<false, true, true> : k_trace<false, false, true>;
+    return l ? k_trace<false, true, false> : k_trace<false, false, false>;
   };
   int occGlobal = 1, occLds = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occGlobal, variant(false), kBlock, stk));
@@ -1068,8 +1080,7 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   TraceArgs A{};
   A.nTx = (int32_t)txs.size();
   A.stackDepth = stackDepth();
-  A.nodeBytes = (int32_t)(hNodes.size() * sizeof(DNode));
-  A.boxBytes = (int32_t)(hTris.size() * sizeof(DBox32));
+  A.nodeBytes = (int32_t)(hNodesF.size() * sizeof(DNodeF));
   A.ext = sceneExtent;
   A.rootBase = rootBase;
   A.nodeCap = nodeCap;
@@ -1093,8 +1104,8 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
     A.depth = d;
     const int32_t* qin = dQueue[d & 1].p;
     int32_t* qout = dQueue[(d + 1) & 1].p;
-    kern<<<grid, kBlock, smem, stream>>>(A, dTx.p, dNodes.p, dTris.p, dTriBox32.p, dTriN.p, qin, qout, dTree.p,
-                                         dLevels.p, dTraceCounters.p, count ? 1 : 0);
+    kern<<<grid, kBlock, smem, stream>>>(A, dTx.p, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p, dSubTri.p, dTriN.p, qin,
+                                         qout, dTree.p, dLevels.p, dTraceCounters.p, count ? 1 : 0);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(kev[1], stream));
@@ -1245,11 +1256,13 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     HIPCHK(hipMemcpyAsync(dCellRx.p, cellRx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, stream));
     const size_t sm = stack_bytes(stackDepth(), kBlock);
     if (exact)
-      k_los<true><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dTris.p, dTriBox32.p,
-                                                                     sceneExtent, dRx.p, m, stackDepth(), dLos.p);
+      k_los<true><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
+                                                                     dSubTri.p, sceneExtent, dRx.p, m, stackDepth(),
+                                                                     dLos.p);
     else
-      k_los<false><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dTris.p, dTriBox32.p,
-                                                                      sceneExtent, dRx.p, m, stackDepth(), dLos.p);
+      k_los<false><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p,
+                                                                      dSub.p, dSubTri.p, sceneExtent, dRx.p, m,
+                                                                      stackDepth(), dLos.p);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(ev[2], stream));

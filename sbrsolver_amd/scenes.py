@@ -56,6 +56,32 @@ class Soup:
         self.quad((x0, y0, z0), (x0, y0, z1), (x0, y1, z1), (x0, y1, z0), (-s, 0, 0), mat)   # x = x0
         self.quad((x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1), (s, 0, 0), mat)    # x = x1
 
+    def grid(self, o, u, v, nu, nv, n, mat=0):
+        """Rectangle o + [0,1] u + [0,1] v split into nu x nv cells of two triangles each; the
+        triangles wind (o, o + u, o + u + v) so that n should be u x v normalised."""
+        o, u, v = np.asarray(o, float), np.asarray(u, float), np.asarray(v, float)
+        for i in range(nu):
+            for j in range(nv):
+                a = o + u * (i / nu) + v * (j / nv)
+                b = o + u * ((i + 1) / nu) + v * (j / nv)
+                c = o + u * ((i + 1) / nu) + v * ((j + 1) / nv)
+                d = o + u * (i / nu) + v * ((j + 1) / nv)
+                self.quad(a, b, c, d, n, mat)
+
+    def tessellated_box(self, x0, y0, z0, x1, y1, z1, cell, mat=0, bottom=False, top=True):
+        """Outward box whose faces are split into cells of about `cell` metres."""
+        nx, ny, nz = (max(1, int(round((x1 - x0) / cell))), max(1, int(round((y1 - y0) / cell))),
+                      max(1, int(round((z1 - z0) / cell))))
+        dx, dy, dz = (x1 - x0, 0, 0), (0, y1 - y0, 0), (0, 0, z1 - z0)
+        if bottom:
+            self.grid((x0, y0, z0), dy, dx, ny, nx, (0, 0, -1), mat)
+        if top:
+            self.grid((x0, y0, z1), dx, dy, nx, ny, (0, 0, 1), mat)
+        self.grid((x0, y0, z0), dx, dz, nx, nz, (0, -1, 0), mat)     # y = y0
+        self.grid((x0, y1, z0), dz, dx, nz, nx, (0, 1, 0), mat)      # y = y1
+        self.grid((x0, y0, z0), dz, dy, nz, ny, (-1, 0, 0), mat)     # x = x0
+        self.grid((x1, y0, z0), dy, dz, ny, nz, (1, 0, 0), mat)      # x = x1
+
     def arrays(self):
         return (np.asarray(self.v, np.float32).reshape(-1, 9), np.asarray(self.n, np.float32).reshape(-1, 3),
                 np.asarray(self.m, np.int32))
@@ -70,6 +96,63 @@ def config1():
     mats = np.array([[2.4e9] + CONCRETE[1:]])
     trace = dict(tx=(2.0, 2.0, 1.5), tess=5, maxR=4, maxT=0, frequency=2.4e9, pt=1.0, antennaType=2)
     return dict(name="box_room_12tri_L5_R4", soup=s.arrays(), rx=rx, materials=mats, trace=trace)
+
+
+def config3(tess=10, floor_cell=0.25, wall_cell=0.5, n_walls=40, n_furniture=300, rx_grid=(200, 100),
+            maxR=8, maxT=8, maxTotal=8):
+    """Dense indoor triangle soup (seed 20261203 + 3): floor, ceiling and perimeter of a 50x50x4 m
+    room, n_walls seeded interior walls (0.15 m thick, 3-15 m long, full height) and n_furniture
+    boxes (0.4-2 m), all faces tessellated (about 200k triangles at the defaults); 16 materials with
+    er ~ U[2, 7] and sigma ~ logU[1e-3, 1e-1] S/m at 2.4 GHz, one per object; Tx at 2 m in free
+    space; icosahedron level 10; reflections and transmissions up to 8 each, at most 8 in total;
+    200x100 receivers at z = 1 (pitch 0.25 x 0.5 m)."""
+    rng = SplitMix64(SEED_BASE + 3)
+    L, H = 50.0, 4.0
+    mats = np.array([[2.4e9, -1.0, -1.0, rng.uniform(2.0, 7.0), 0.0, float(np.exp(rng.uniform(np.log(1e-3), np.log(1e-1))))]
+                     for _ in range(16)])
+    s = Soup()
+    nf = int(round(L / floor_cell))
+    s.grid((0, 0, 0), (L, 0, 0), (0, L, 0), nf, nf, (0, 0, 1), rng.randint(0, 15))          # floor, facing up
+    s.grid((0, 0, H), (0, L, 0), (L, 0, 0), nf, nf, (0, 0, -1), rng.randint(0, 15))         # ceiling, facing down
+    nw, nh = int(round(L / wall_cell)), int(round(H / wall_cell))
+    pm = rng.randint(0, 15)
+    s.grid((0, 0, 0), (0, 0, H), (L, 0, 0), nh, nw, (0, 1, 0), pm)                           # y = 0, inward
+    s.grid((0, L, 0), (L, 0, 0), (0, 0, H), nw, nh, (0, -1, 0), pm)                          # y = L
+    s.grid((0, 0, 0), (0, L, 0), (0, 0, H), nw, nh, (1, 0, 0), pm)                           # x = 0
+    s.grid((L, 0, 0), (0, 0, H), (0, L, 0), nh, nw, (-1, 0, 0), pm)                          # x = L
+    solids = []
+    for _ in range(n_walls):
+        length, t = rng.uniform(3.0, 15.0), 0.15
+        if rng.uniform() < 0.5:
+            x0, y0 = rng.uniform(1.0, L - 1.0 - length), rng.uniform(1.0, L - 1.0)
+            b = (x0, y0, x0 + length, y0 + t)
+        else:
+            x0, y0 = rng.uniform(1.0, L - 1.0), rng.uniform(1.0, L - 1.0 - length)
+            b = (x0, y0, x0 + t, y0 + length)
+        solids.append(b)
+        s.tessellated_box(b[0], b[1], 0.0, b[2], b[3], H, wall_cell, rng.randint(0, 15), top=False)
+    placed = 0
+    while placed < n_furniture:
+        w, d, h = rng.uniform(0.4, 2.0), rng.uniform(0.4, 2.0), rng.uniform(0.4, 2.0)
+        x0, y0 = rng.uniform(0.5, L - 0.5 - w), rng.uniform(0.5, L - 0.5 - d)
+        b = (x0, y0, x0 + w, y0 + d)
+        if any(b[0] < o[2] + 0.1 and o[0] < b[2] + 0.1 and b[1] < o[3] + 0.1 and o[1] < b[3] + 0.1 for o in solids):
+            continue
+        solids.append(b)
+        s.tessellated_box(b[0], b[1], 0.0, b[2], b[3], h, wall_cell, rng.randint(0, 15))
+        placed += 1
+    while True:
+        tx = (rng.uniform(2.0, L - 2.0), rng.uniform(2.0, L - 2.0), 2.0)
+        if all(not (o[0] - 0.3 < tx[0] < o[2] + 0.3 and o[1] - 0.3 < tx[1] < o[3] + 0.3) for o in solids):
+            break
+    nx, ny = rx_grid
+    px, py = L / nx, L / ny
+    rx = np.array([[px / 2 + px * i, py / 2 + py * j, 1.0] for i in range(nx) for j in range(ny)])
+    trace = dict(tx=tx, tess=tess, maxR=maxR, maxT=maxT, maxTotal=maxTotal, frequency=2.4e9, pt=1.0, antennaType=2,
+                 receptionModel=1)
+    v, n, m = s.arrays()
+    return dict(name=f"indoor_{len(m)}tri_L{tess}_R{maxR}T{maxT}max{maxTotal}", soup=(v, n, m), rx=rx, materials=mats,
+                trace=trace)
 
 
 def urban_buildings(seed, count=40, half=250.0, fmin=10.0, fmax=40.0, hmin=10.0, hmax=60.0, gap=2.0):
