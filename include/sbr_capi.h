@@ -125,6 +125,7 @@ typedef struct {
   int32_t trace_blocks_per_cu;
   int64_t heavy_nodes;      /* tree nodes captured wave-cooperatively (large candidate regions) */
   int64_t windows;          /* launch windows the trace ran in (tree buffers bounded by window_nodes) */
+  int64_t capture_tests;    /* reception-sphere tests of the capture's candidate pass (countTraversal) */
 } sbr_stats;
 
 /* Launch configuration of the traversal kernel (no reference counterpart).  lds_scene: -1 stage

@@ -236,13 +236,14 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
 // ---------------------------------------------------------------------------------------------
 // Reception sphere: HitReceptionSphere (RTSolver.cpp:805-863), and the alpha*L/sqrt(3) model.
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ bool sphere_hit(V3 S, V3 T, V3 C, double plArg, double trueLen, int model, double rcoef) {
+// base: the totalPathLength handed to the node (model 0) or the true length to S (model 1).
+__device__ __forceinline__ bool sphere_hit(V3 S, V3 T, V3 C, double base, int model, double rcoef) {
   V3 rayDir = vsub(T, S);
   double t0 = vdot(rayDir, vsub(C, S)) / vsqnorm(rayDir);
   if (t0 < 0) return false;
   if (t0 >= 1) t0 = 1;
   V3 cp = vadd(S, vmul(rayDir, t0));
-  double unfolded = model == 0 ? plArg + vnorm(cp) : trueLen + vnorm(vmul(rayDir, t0));
+  double unfolded = model == 0 ? base + vnorm(cp) : base + vnorm(vmul(rayDir, t0));
   double radius = unfolded * rcoef;
   double distance = vnorm(vsub(C, cp));
   return distance <= radius;

@@ -101,26 +101,33 @@ struct TxDesc {
 };
 
 // ---- interaction tree ---------------------------------------------------------------------------
-// One RayLaunch call (one bounce), 128 B.  The hit triangle's material is tris[hitDfs].mat; the
-// launch index within the transmitter follows from gl (see TxDesc).
-struct alignas(64) TNode {
+// One RayLaunch call (one bounce), 128 B in two 64 B halves.  The first half is all that the
+// receiver capture reads for a node and its ancestors; the second serves the bounce loop, the dedup
+// key, the path export and the field.  The hit triangle's material is tris[hitDfs].mat; the launch
+// index within the transmitter follows from gl (see TxDesc).
+struct alignas(128) TNode {
+  // capture half
   double src[3];
   double tgt[3];
+  double base;       // length the reception radius grows with: the totalPathLength RayCapture hands
+                     // this node (reference sphere) or the true length from the transmitter to src
+  int32_t parent;    // node index of the parent, -1 for a root
+  uint8_t depth;
+  uint8_t pad0;
+  int16_t txi;       // transmitter index in the batch
+  // bounce half
   double dir[3];     // direction the segment was traced with
   double pl;         // pathLength as stored by the reference (cumulative; INF for a miss)
-  double plArg;      // totalPathLength that RayCapture hands to this node (sum of ancestors' pl)
-  double trueLen;    // geometric length from the transmitter to src
-  int32_t parent;    // node index of the parent, -1 for a root
   int32_t gl;        // global root index of the tree (transmitters' root items concatenated)
   int32_t tri;       // hitSurfaceID, -1 on a miss
   int32_t cop;       // hitCoplanarId, -1 on a miss
   int32_t hitDfs;    // DFS index of the hit triangle, -1 on a miss
   uint32_t code;     // branch code: bit per level, transmit = 0, reflect = 1, first level most significant
-  uint8_t depth;
   uint8_t type;      // 0 from the transmitter, 1 reflected, 2 transmitted
   uint8_t rc, tc;    // reflections / transmissions so far
-  int16_t txi;       // transmitter index in the batch
-  int16_t pad;
+  uint8_t mask;      // children created: bit 0 transmitted, bit 1 reflected
+  int32_t firstChild;   // node index of the first child created (the transmitted one if any), -1 if none
+  int32_t pad1;
 };
 static_assert(sizeof(TNode) == 128, "TNode layout");
 
@@ -214,7 +221,8 @@ struct Engine {
   DBuf<unsigned long long> dTabA, dTabB, dTabOrder;
   DBuf<int32_t> dTabRx;
   DBuf<int32_t> dHeavy;          // nodes left to the wave-per-node capture
-  DBuf<unsigned int> dRootBits;  // roots owning a kept path
+  DBuf<int2> dRec;               // (node, receiver) records of the capture's candidate pass
+  size_t recCap = 0;
   DBuf<unsigned long long> dCounters;        // capture: records, corrected, groups, flags, collected, heavy
   DBuf<unsigned long long> dTraceCounters;   // bounce loop: overflow flags, node visits, triangle tests
   DBuf<unsigned long long> dWinKey, dWinKey2;

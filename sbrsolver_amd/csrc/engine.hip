@@ -69,6 +69,7 @@ template struct DBuf<DTri>;
 template struct DBuf<DBox32>;
 template struct DBuf<DSub>;
 template struct DBuf<DNodeF>;
+template struct DBuf<int2>;
 template struct DBuf<DSurf>;
 template struct DBuf<TNode>;
 template struct DBuf<TxDesc>;
@@ -211,10 +212,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     const unsigned long long nodeIdx = L.base + i;
     int mask = 0;
     if (i < n && nodeIdx >= A.nodeCap) atomicOr(&flags[0], 1ull);
-    if (i < n && nodeIdx < A.nodeCap) {
-      // The record is assembled in registers and written with one 128-byte store (a partial-line
-      // store before the traversal and another after it costs a second write-back of the line).
-      TNode nd;
+    // The record is assembled in registers and written with one 128-byte store once the children's
+    // queue positions are known (partial-line stores before and after the traversal would cost a
+    // second write-back of the line).
+    TNode nd;
+    const bool live = i < n && nodeIdx < A.nodeCap;
+    if (live) {
       double totalPL;
       int allow;
       {
@@ -238,8 +241,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
           nd.type = 0;
           nd.rc = nd.tc = 0;
           nd.txi = (int16_t)lo;
-          nd.plArg = 0.0;
-          nd.trueLen = 0.0;
+          nd.base = 0.0;
           totalPL = 0.0;
         } else {
           const int32_t q = qIn[i];
@@ -259,15 +261,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
           nd.rc = P.rc + (refl ? 1 : 0);
           nd.tc = P.tc + (refl ? 0 : 1);
           nd.txi = P.txi;
-          nd.plArg = P.plArg + P.pl;
-          nd.trueLen = P.trueLen + vnorm(vsub(ld3(P.tgt), ld3(P.src)));
+          // reference sphere: the totalPathLength RayCapture hands on (RTSolver.cpp:288, 293);
+          // alpha * L / sqrt(3): the true length travelled
+          nd.base = T->receptionModel == 0 ? P.base + P.pl : P.base + vnorm(vsub(ld3(P.tgt), ld3(P.src)));
           totalPL = P.pl;
         }
         const bool depthOk = T->maxTotal <= 0 || (nd.rc + nd.tc) < T->maxTotal;
         allow = ((nd.tc < T->maxT && depthOk) ? 1 : 0) | ((nd.rc < T->maxR && depthOk) ? 2 : 0);
       }
       nd.depth = (uint8_t)A.depth;
-      nd.pad = 0;
+      nd.pad0 = 0;
+      nd.pad1 = 0;
       const V3 src = ld3(nd.src), dir = ld3(nd.dir);
       Hit h;
       if (closest_hit<EXACT, SUB>(nodes, nodesF, tris, tbox, subs, subTri, A.ext, src, dir, h, stkN, stkT,
@@ -290,16 +294,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
         nd.cop = -1;
         nd.hitDfs = -1;
       }
-      {
-        const int4* srcw = reinterpret_cast<const int4*>(&nd);
-        int4* dstw = reinterpret_cast<int4*>(&tree[nodeIdx]);
-#pragma unroll
-        for (int w = 0; w < 8; w++) dstw[w] = srcw[w];
-      }
+      nd.mask = (uint8_t)mask;
       cntNodes += h.nodes;
       cntTests += h.tests;
     }
-    // Wave-aggregated append of the children to the next depth's queue.
+    // Wave-aggregated append of the children to the next depth's queue; child q of that queue
+    // becomes node L.base + n + q.
     const int c = (mask & 1) + ((mask >> 1) & 1);
     int incl = c;
 #pragma unroll
@@ -311,15 +311,24 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     unsigned long long wbase = 0;
     if (lane == 63 && tot > 0) wbase = atomicAdd(&lv[A.depth + 1].count, (unsigned long long)tot);
     wbase = __shfl(wbase, 63, 64);
+    int32_t firstChild = -1;
     if (c > 0) {
       unsigned long long pos = wbase + (unsigned long long)(incl - c);
       if (pos + c > A.queueCap) {
         atomicOr(&flags[0], 2ull);
       } else {
+        firstChild = (int32_t)(L.base + n + pos);
         const int32_t nodeI = (int32_t)nodeIdx;
         if (mask & 1) qOut[pos++] = nodeI << 1;
         if (mask & 2) qOut[pos] = (nodeI << 1) | 1;
       }
+    }
+    if (live) {
+      nd.firstChild = firstChild;
+      const int4* srcw = reinterpret_cast<const int4*>(&nd);
+      int4* dstw = reinterpret_cast<int4*>(&tree[nodeIdx]);
+#pragma unroll
+      for (int w = 0; w < 8; w++) dstw[w] = srcw[w];
     }
   }
   if (countTraversal) {
@@ -391,15 +400,17 @@ __device__ __forceinline__ unsigned long long order_key(const TNode& nd) {
          (unsigned long long)(nd.code & ((1u << 27) - 1));
 }
 
-// Per-node state of the capture: the segment, the conservative search region and the dedup key of
-// the path's coplanar-id sequence.  No per-node arrays: ancestors are reached through parent links,
-// and image sources are rebuilt only for the (rare) records that need a correction.
+// Per-node state of the capture: the segment and the conservative search region.  No per-node
+// arrays: ancestors are reached through parent links.  Only the first (capture) half of the records
+// is read for a node that captures nothing; the dedup key of the node's coplanar-id sequence and its
+// order key are computed at its first capture, and image sources only for records that need a
+// correction.
 struct NodeCtx {
   V3 S, Tg, P0, P1;
-  double plArg, trueLen, rc, R;
+  double base, rc, R;
   int model, depth, txi;
-  int32_t node;
-  bool all;
+  int32_t node, parent;
+  bool all, keyed;
   unsigned long long seqA, seqB, orderKey;
   int64_t losBase;
   int j0, j1;
@@ -417,21 +428,27 @@ struct CaptureBufs {
   unsigned long long* tabB;
   unsigned long long* tabOrder;
   int32_t* tabRx;
-  unsigned long long* counters;   // [0] records [1] corrected [2] slots used [3] flags [5] heavy nodes
+  // [0] records [1] corrected [2] slots used [3] flags (bit 0 table full, bit 1 hash collision,
+  // bit 2 record buffer full) [4] collected [5] heavy nodes [6] candidate sphere tests (count mode)
+  unsigned long long* counters;
   int32_t* heavy;
+  int2* rec;                      // (node, receiver) records of the candidate pass
+  unsigned long long recCap;
+  int countTests;
 };
 
 // Returns false when no receiver can be captured by the node.
 __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n, NodeCtx& c) {
-  const TNode& nd = B.tree[n];
+  const TNode& nd = B.tree[n];   // capture half only
   const TxDesc& T = B.txs[nd.txi];
   c.node = (int32_t)n;
+  c.parent = nd.parent;
   c.depth = nd.depth;
   c.txi = nd.txi;
   c.model = T.receptionModel;
   c.rc = T.rcoef;
-  c.plArg = nd.plArg;
-  c.trueLen = nd.trueLen;
+  c.base = nd.base;
+  c.keyed = false;
   c.S = ld3(nd.src);
   c.Tg = ld3(nd.tgt);
   const V3 D = vsub(c.Tg, c.S);
@@ -443,8 +460,7 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
     double e = fmax(fabs(A.g.bmin[a] - sa), fabs(A.g.bmax[a] - sa));
     far2 += e * e;
   }
-  const double base = c.model == 0 ? c.plArg : c.trueLen;
-  double R = (base + sqrt(far2)) * c.rc / (1.0 - c.rc);
+  double R = (c.base + sqrt(far2)) * c.rc / (1.0 - c.rc);
   c.R = R * (1.0 + 1e-6) + 1e-6;
   c.all = !(c.R < 1e300) || c.rc >= 1.0;
   // Clip the segment to the receivers' box grown by R.
@@ -471,17 +487,20 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
     c.j0 = (int)fmax(0.0, floor((ylo - A.g.gy0) / A.g.cell));
     c.j1 = (int)fmin((double)(A.g.ny - 1), floor((yhi - A.g.gy0) / A.g.cell));
   }
-  // Hash of the ancestors' coplanar ids, root first (the dedup key without the receiver).
-  int32_t anc[kMaxDepth];
-  {
-    int32_t q = nd.parent;
-    for (int k = c.depth - 1; k >= 0; k--) { anc[k] = B.tree[q].cop; q = B.tree[q].parent; }
-  }
-  seq_seed(c.txi, c.depth, &c.seqA, &c.seqB);
-  for (int k = 0; k < c.depth; k++) seq_fold(anc[k], k, &c.seqA, &c.seqB);
-  c.orderKey = order_key(nd);
   c.losBase = (int64_t)c.txi * A.nrx;
   return true;
+}
+
+// Dedup key of the node's path (hash of the ancestors' coplanar ids, root first, without the
+// receiver) and its order key; computed once, at the node's first capture.
+__device__ void node_key(const CaptureBufs& B, NodeCtx& c) {
+  int32_t anc[kMaxDepth];
+  int32_t q = c.parent;
+  for (int k = c.depth - 1; k >= 0; k--) { anc[k] = B.tree[q].cop; q = B.tree[q].parent; }
+  seq_seed(c.txi, c.depth, &c.seqA, &c.seqB);
+  for (int k = 0; k < c.depth; k++) seq_fold(anc[k], k, &c.seqA, &c.seqB);
+  c.orderKey = order_key(B.tree[c.node]);
+  c.keyed = true;
 }
 
 // Grid cells [i0, i1] of row j that can hold a receiver within R of the clipped segment (xy).
@@ -526,85 +545,125 @@ __device__ V3 image_source(const CaptureBufs& B, const TxDesc& T, int32_t leaf, 
   return cur;
 }
 
-// One receiver against one node: reception sphere, ancestor rule, image correction, dedup.
-__device__ void capture_one(const CaptureArgs& A, const CaptureBufs& B, const NodeCtx& c, int32_t r,
-                            unsigned long long& myRec, unsigned long long& myCor, unsigned long long& myUsed) {
-  const V3 C = ld3(B.rx + 3 * (int64_t)r);
-  if (!sphere_hit(c.S, c.Tg, C, c.plArg, c.trueLen, c.model, c.rc)) return;
-  {
-    int32_t q = B.tree[c.node].parent;
-    while (q >= 0) {
-      const TNode& a = B.tree[q];
-      if (sphere_hit(ld3(a.src), ld3(a.tgt), C, a.plArg, a.trueLen, c.model, c.rc)) return;
-      q = a.parent;
-    }
+// Image correction (MultiPathCorrection / DirectPathCorrection): walk back from the receiver,
+// re-intersecting each reflection / transmission surface from its image source.  Out of line: it
+// runs for few records and would otherwise hold registers across the candidate loop.
+__device__ bool corrected(const CaptureBufs& B, const NodeCtx& c, int32_t r, V3 C) {
+  if (c.depth == 0) return B.los[c.losBase + r] != 0;
+  const TxDesc& T = B.txs[c.txi];
+  V3 target = C;
+  for (int k = c.depth; k >= 1; k--) {
+    int32_t sid = -1;
+    const V3 img = image_source(B, T, c.node, c.depth, k, &sid);
+    if (sid < 0) return false;
+    const DSurf& s = B.surf[sid];
+    V3 p;
+    double dist;
+    if (!tri_intersect(ld3(s.v1), ld3(s.e1), ld3(s.e2), img, vsub(target, img), &p, &dist)) return false;
+    target = p;
   }
-  myRec++;
-  unsigned long long hA, hB;
-  group_hash(c.seqA, c.seqB, r, &hA, &hB);
-  // A record that cannot beat the key already stored for its group needs no correction: the stored
-  // key belongs to a corrected path and keys only decrease.  A stale read is only less selective.
-  {
-    unsigned long long slot = hA & A.tabMask;
-    for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
-      const unsigned long long a = B.tabA[slot];
-      if (a == 0ull) break;
-      if (a == hA) {
-        if (B.tabOrder[slot] < c.orderKey) return;
-        break;
-      }
-      slot = (slot + 1) & A.tabMask;
-    }
-  }
-  // Image correction (MultiPathCorrection / DirectPathCorrection): walk back from the receiver,
-  // re-intersecting each reflection / transmission surface from its image source.
-  bool ok = true;
-  if (c.depth == 0) {
-    ok = B.los[c.losBase + r] != 0;
-  } else {
-    const TxDesc& T = B.txs[c.txi];
-    V3 target = C;
-    for (int k = c.depth; k >= 1 && ok; k--) {
-      int32_t sid = -1;
-      const V3 img = image_source(B, T, c.node, c.depth, k, &sid);
-      if (sid < 0) { ok = false; break; }
-      const DSurf& s = B.surf[sid];
-      V3 p;
-      double dist;
-      ok = tri_intersect(ld3(s.v1), ld3(s.e1), ld3(s.e2), img, vsub(target, img), &p, &dist);
-      target = p;
-    }
-  }
-  if (!ok) return;
-  myCor++;
-  unsigned long long slot = hA & A.tabMask;
-  for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
-    unsigned long long prev = atomicCAS(&B.tabA[slot], 0ull, hA);
-    if (prev == 0ull || prev == hA) {
-      if (prev == 0ull) { B.tabRx[slot] = (int32_t)(c.losBase + r); myUsed++; }
-      unsigned long long pb = atomicCAS(&B.tabB[slot], 0ull, hB);
-      if (pb != 0ull && pb != hB) atomicOr(&B.counters[3], 2ull);
-      atomicMin(&B.tabOrder[slot], c.orderKey);
-      return;
-    }
-    slot = (slot + 1) & A.tabMask;
-  }
-  atomicOr(&B.counters[3], 1ull);
+  return true;
 }
 
-__device__ __forceinline__ void flush_counters(const CaptureBufs& B, unsigned long long myRec, unsigned long long myCor,
-                                               unsigned long long myUsed) {
-  // wave-level reduction, then one atomic per wave and counter
+// Appends a (node, receiver) record; called by the lanes that have one (wave-aggregated atomic
+// over the active lanes).
+__device__ __forceinline__ void append_record(const CaptureBufs& B, int32_t node, int32_t r) {
+  const unsigned long long active = __ballot(1);
+  const int leader = __ffsll((unsigned long long)active) - 1;
+  const int lane = (int)(threadIdx.x & 63);
+  const int rank = __popcll(active & ((1ull << lane) - 1ull));
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(&B.counters[0], (unsigned long long)__popcll(active));
+  base = __shfl(base, leader, 64);
+  const unsigned long long pos = base + (unsigned long long)rank;
+  if (pos < B.recCap) B.rec[pos] = make_int2(node, r);
+  else atomicOr(&B.counters[3], 4ull);
+}
+
+// Candidate pass, one receiver against one node: reception sphere and the ancestor rule (a node
+// yields a path iff it captures the receiver and no proper ancestor does).  Only the capture halves
+// of the records are read.
+__device__ __forceinline__ void capture_one(const CaptureBufs& B, const NodeCtx& c, int32_t r,
+                                            unsigned long long& myTests) {
+  myTests++;
+  const V3 C = ld3(B.rx + 3 * (int64_t)r);
+  if (!sphere_hit(c.S, c.Tg, C, c.base, c.model, c.rc)) return;
+  int32_t q = c.parent;
+  while (q >= 0) {
+    const TNode& a = B.tree[q];
+    if (sphere_hit(ld3(a.src), ld3(a.tgt), C, a.base, c.model, c.rc)) return;
+    q = a.parent;
+  }
+  append_record(B, c.node, r);
+}
+
+// Dedup pass, one thread per record: the group key of the node's coplanar-id sequence, the prune
+// against the key its group already holds, image correction, insertion.
+__global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int64_t nRec) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long myCor = 0, myUsed = 0;
+  if (k < nRec) {
+    const int2 rr = B.rec[k];
+    const int32_t r = rr.y;
+    NodeCtx c;
+    const TNode& nd = B.tree[rr.x];
+    c.node = rr.x;
+    c.parent = nd.parent;
+    c.depth = nd.depth;
+    c.txi = nd.txi;
+    c.losBase = (int64_t)c.txi * A.nrx;
+    node_key(B, c);
+    unsigned long long hA, hB;
+    group_hash(c.seqA, c.seqB, r, &hA, &hB);
+    bool keep = true;
+    // A record that cannot beat the key already stored for its group needs no correction: the
+    // stored key belongs to a corrected path and keys only decrease.  A stale read is only less
+    // selective.
+    {
+      unsigned long long slot = hA & A.tabMask;
+      for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
+        const unsigned long long a = B.tabA[slot];
+        if (a == 0ull) break;
+        if (a == hA) {
+          if (B.tabOrder[slot] < c.orderKey) keep = false;
+          break;
+        }
+        slot = (slot + 1) & A.tabMask;
+      }
+    }
+    if (keep && corrected(B, c, r, ld3(B.rx + 3 * (int64_t)r))) {
+      myCor++;
+      unsigned long long slot = hA & A.tabMask;
+      bool placed = false;
+      for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
+        unsigned long long prev = atomicCAS(&B.tabA[slot], 0ull, hA);
+        if (prev == 0ull || prev == hA) {
+          if (prev == 0ull) { B.tabRx[slot] = (int32_t)(c.losBase + r); myUsed++; }
+          unsigned long long pb = atomicCAS(&B.tabB[slot], 0ull, hB);
+          if (pb != 0ull && pb != hB) atomicOr(&B.counters[3], 2ull);
+          atomicMin(&B.tabOrder[slot], c.orderKey);
+          placed = true;
+          break;
+        }
+        slot = (slot + 1) & A.tabMask;
+      }
+      if (!placed) atomicOr(&B.counters[3], 1ull);
+    }
+  }
   for (int off = 32; off > 0; off >>= 1) {
-    myRec += __shfl_xor(myRec, off, 64);
     myCor += __shfl_xor(myCor, off, 64);
     myUsed += __shfl_xor(myUsed, off, 64);
   }
   if ((threadIdx.x & 63) == 0) {
-    if (myRec) atomicAdd(&B.counters[0], myRec);
     if (myCor) atomicAdd(&B.counters[1], myCor);
     if (myUsed) atomicAdd(&B.counters[2], myUsed);
   }
+}
+
+__device__ __forceinline__ void flush_tests(const CaptureBufs& B, unsigned long long myTests) {
+  if (!B.countTests) return;
+  for (int off = 32; off > 0; off >>= 1) myTests += __shfl_xor(myTests, off, 64);
+  if ((threadIdx.x & 63) == 0 && myTests) atomicAdd(&B.counters[6], myTests);
 }
 
 // Thread per node.  Nodes whose candidate region spans more than kHeavyCells grid cells are left to
@@ -613,7 +672,7 @@ constexpr int kHeavyCells = 48;
 
 __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
   const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long myRec = 0, myCor = 0, myUsed = 0;
+  unsigned long long myTests = 0;
   if (n < A.nodes) {
     NodeCtx c;
     if (node_setup(A, B, n, c)) {
@@ -631,13 +690,13 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
           if (!row_range(A, c, j, &i0, &i1)) continue;
           for (int ci = i0; ci <= i1; ci++) {
             const int cell = j * A.g.nx + ci;
-            for (int q = B.cellStart[cell]; q < B.cellStart[cell + 1]; q++) capture_one(A, B, c, B.cellRx[q], myRec, myCor, myUsed);
+            for (int q = B.cellStart[cell]; q < B.cellStart[cell + 1]; q++) capture_one(B, c, B.cellRx[q], myTests);
           }
         }
       }
     }
   }
-  flush_counters(B, myRec, myCor, myUsed);
+  flush_tests(B, myTests);
 }
 
 // Wave per heavy node: the node's candidate cells, flattened row by row, are dealt to the lanes.
@@ -645,7 +704,7 @@ __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBuf
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  unsigned long long myRec = 0, myCor = 0, myUsed = 0;
+  unsigned long long myTests = 0;
   for (int64_t h = wave; h < nHeavy; h += nWaves) {
     NodeCtx c;
     if (!node_setup(A, B, B.heavy[h], c)) continue;
@@ -656,12 +715,12 @@ __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBuf
       const int first = i0 + (((lane - off) % 64) + 64) % 64;
       for (int ci = first; ci <= i1; ci += 64) {
         const int cell = j * A.g.nx + ci;
-        for (int q = B.cellStart[cell]; q < B.cellStart[cell + 1]; q++) capture_one(A, B, c, B.cellRx[q], myRec, myCor, myUsed);
+        for (int q = B.cellStart[cell]; q < B.cellStart[cell + 1]; q++) capture_one(B, c, B.cellRx[q], myTests);
       }
       off += i1 - i0 + 1;
     }
   }
-  flush_counters(B, myRec, myCor, myUsed);
+  flush_tests(B, myTests);
 }
 
 // Groups whose smallest order key has its root in [glLo, glHi) (those first seen in the window).
@@ -679,29 +738,21 @@ __global__ void k_collect(const unsigned long long* __restrict__ tabA, const uns
   winRx[w] = tabRx[s];
 }
 
-// Marks the roots that own a kept path.
-__global__ void k_mark_roots(const unsigned long long* __restrict__ keys, int64_t W, unsigned int* __restrict__ bits) {
+// Tree node of every kept path (order keys, window roots from rootBase): from the root, follow the
+// branch code's bits through firstChild (the transmitted child comes first when it exists).
+__global__ void k_resolve(const TNode* __restrict__ tree, const unsigned long long* __restrict__ keys, int64_t W,
+                          int64_t rootBase, int32_t* __restrict__ winNode) {
   int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= W) return;
-  const unsigned int gl = (unsigned int)(keys[w] >> 32);
-  atomicOr(&bits[gl >> 5], 1u << (gl & 31));
-}
-
-// Maps every kept path's order key (sorted) to its tree node; nodes of unmarked roots leave at once.
-__global__ void k_resolve(const TNode* __restrict__ tree, int64_t nodes, const unsigned long long* __restrict__ keys,
-                          int64_t W, const unsigned int* __restrict__ bits, int32_t* __restrict__ winNode) {
-  int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nodes) return;
-  const unsigned int gl = (unsigned int)tree[n].gl;
-  if (!((bits[gl >> 5] >> (gl & 31)) & 1u)) return;
-  const unsigned long long key = order_key(tree[n]);
-  int64_t lo = 0, hi = W;
-  while (lo < hi) {
-    int64_t mid = (lo + hi) >> 1;
-    if (keys[mid] < key) lo = mid + 1;
-    else hi = mid;
+  const unsigned long long key = keys[w];
+  const int depth = (int)((key >> 27) & 31ull);
+  const uint32_t code = (uint32_t)(key & ((1ull << 27) - 1));
+  int32_t n = (int32_t)((int64_t)(key >> 32) - rootBase);
+  for (int l = depth - 1; l >= 0; l--) {
+    const TNode& t = tree[n];
+    n = t.firstChild + (((code >> l) & 1u) && (t.mask & 1) ? 1 : 0);
   }
-  for (int64_t j = lo; j < W && keys[j] == key; j++) winNode[j] = (int32_t)n;
+  winNode[w] = n;
 }
 
 __global__ void k_iota(int32_t* __restrict__ a, int64_t n) {
@@ -1272,7 +1323,6 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   // is final when w ends (later windows only hold larger keys): its path is resolved and its field
   // added while w's trees are still resident.  Receiver sums continue in key order across windows.
   const DSurf* surf = P[0].q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
-  const int64_t words = (nRoot + 31) / 32 + 1;
   size_t cap = 1;
   while (cap < (size_t)std::max<int64_t>(std::max<int64_t>(1 << 16, 4 * nOut), (int64_t)(tabCapHint / 4))) cap <<= 1;
   ENSURE(dLevels, kMaxDepth + 2);
@@ -1286,6 +1336,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     stats.n_k_trace = 0;
     stats.max_depth = 0;
     stats.heavy_nodes = 0;
+    stats.capture_tests = 0;
     stats.windows = 0;
     winners = 0;
     multiWindow = false;
@@ -1302,7 +1353,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       HIPCHK(hipMemsetAsync(dField.p, 0, 6 * sizeof(double) * nOut, stream));
       HIPCHK(hipMemsetAsync(dCounters.p, 0, 8 * sizeof(unsigned long long), stream));
     }
-    unsigned long long usedBefore = 0, recBefore = 0, corBefore = 0;
+    unsigned long long usedBefore = 0, corBefore = 0;
     // nodes per root item: the last trace's figure (or one chain per root), refined by every window
     double perRoot = perRootHint > 0 ? perRootHint : (double)(maxDepth + 1);
     bool tableFull = false;
@@ -1349,28 +1400,54 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       for (auto& e : kev) HIPCHK(hipEventCreate(&e));
       struct KevGuard { hipEvent_t* e; ~KevGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } kguard{kev};
       ENSURE(dHeavy, std::max<int64_t>(nodes, 1));
-      HIPCHK(hipMemsetAsync(dCounters.p + 5, 0, sizeof(unsigned long long), stream));
       CaptureArgs CA{};
       CA.nodes = nodes;
       CA.nrx = m;
       CA.g = g;
       CA.tabMask = cap - 1;
       HIPCHK(hipEventRecord(kev[0], stream));
-      CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
-                     dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p};
-      if (nodes > 0) {
-        k_capture<<<blocks_for(nodes, 256), 256, 0, stream>>>(CA, CB);
-        HIPCHK(hipGetLastError());
-      }
-      unsigned long long nHeavy = 0;
-      HIPCHK(hipMemcpyAsync(&nHeavy, dCounters.p + 5, sizeof nHeavy, hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipStreamSynchronize(stream));
-      if (nHeavy > 0) {
-        const int64_t waves = std::min<int64_t>((int64_t)nHeavy, (int64_t)numCU * 32);
-        k_capture_heavy<<<blocks_for(waves * 64, 256), 256, 0, stream>>>(CA, CB, (int64_t)nHeavy);
-        HIPCHK(hipGetLastError());
+      // candidate pass: (node, receiver) records; redone with a larger record buffer if it fills
+      unsigned long long nRec = 0, nHeavy = 0;
+      for (int recAttempt = 0;; recAttempt++) {
+        if (recCap < 1024) recCap = std::max<size_t>(1 << 20, (size_t)nodes);
+        ENSURE(dRec, recCap);
+        recCap = dRec.n;
+        HIPCHK(hipMemsetAsync(dCounters.p, 0, sizeof(unsigned long long), stream));
+        HIPCHK(hipMemsetAsync(dCounters.p + 3, 0, sizeof(unsigned long long), stream));
+        HIPCHK(hipMemsetAsync(dCounters.p + 5, 0, 2 * sizeof(unsigned long long), stream));
+        CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
+                       dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, (unsigned long long)recCap,
+                       count ? 1 : 0};
+        if (nodes > 0) {
+          k_capture<<<blocks_for(nodes, 256), 256, 0, stream>>>(CA, CB);
+          HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipMemcpyAsync(&nHeavy, dCounters.p + 5, sizeof nHeavy, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (nHeavy > 0) {
+          const int64_t waves = std::min<int64_t>((int64_t)nHeavy, (int64_t)numCU * 32);
+          k_capture_heavy<<<blocks_for(waves * 64, 256), 256, 0, stream>>>(CA, CB, (int64_t)nHeavy);
+          HIPCHK(hipGetLastError());
+        }
+        unsigned long long c4[7];
+        HIPCHK(hipMemcpyAsync(c4, dCounters.p, sizeof c4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        nRec = c4[0];
+        if (!(c4[3] & 4ull)) {
+          stats.capture_tests += (int64_t)c4[6];
+          break;
+        }
+        if (recAttempt > 4) return fail(SBR_E_LIMIT, "capture record buffer could not be sized");
+        recCap = std::max<size_t>(recCap * 2, (size_t)(nRec + nRec / 4 + 1024));
       }
       stats.heavy_nodes += (int64_t)nHeavy;
+      // dedup pass over the records
+      if (nRec > 0) {
+        CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
+                       dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, (unsigned long long)recCap, 0};
+        k_dedup<<<blocks_for((int64_t)nRec, 256), 256, 0, stream>>>(CA, CB, (int64_t)nRec);
+        HIPCHK(hipGetLastError());
+      }
       HIPCHK(hipEventRecord(kev[1], stream));
       unsigned long long ctr[4];
       HIPCHK(hipMemcpyAsync(ctr, dCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
@@ -1384,9 +1461,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         tableFull = true;
         break;
       }
-      stats.capture_records += (int64_t)(ctr[0] - recBefore);
+      stats.capture_records += (int64_t)nRec;
       stats.corrected += (int64_t)(ctr[1] - corBefore);
-      recBefore = ctr[0];
       corBefore = ctr[1];
       const int64_t Wn = (int64_t)(ctr[2] - usedBefore);   // groups first seen in this window
       usedBefore = ctr[2];
@@ -1421,10 +1497,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         // by order key -> dWinKey2 (keys), dWinRx2 (tx * nrx + rx)
         HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb1, dWinKey.p, dWinKey2.p, dWinRx.p, dWinRx2.p, (int)Wn,
                                                   0, 64, stream));
-        ENSURE(dRootBits, words);
-        HIPCHK(hipMemsetAsync(dRootBits.p, 0, words * sizeof(unsigned int), stream));
-        k_mark_roots<<<blocks_for(Wn, 256), 256, 0, stream>>>(dWinKey2.p, Wn, dRootBits.p);
-        k_resolve<<<blocks_for(nodes, 256), 256, 0, stream>>>(dTree.p, nodes, dWinKey2.p, Wn, dRootBits.p, dWinNode.p);
+        k_resolve<<<blocks_for(Wn, 256), 256, 0, stream>>>(dTree.p, dWinKey2.p, Wn, ws, dWinNode.p);
         k_iota<<<blocks_for(Wn, 256), 256, 0, stream>>>(dWinIdx.p, Wn);
         HIPCHK(hipGetLastError());
         // stable by tx * nrx + rx -> dWinRx (keys), dWinIdx2 (index into the key-sorted arrays)

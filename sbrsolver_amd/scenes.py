@@ -226,6 +226,30 @@ def config4(n_tx=256, rx_n=100):
     return dict(name=f"urban_canyon_482tri_{n_tx}tx_L6-9_R4-6", soup=s.arrays(), rx=rx, materials=mats, txs=txs)
 
 
+def config5_scene(n_buildings=2000, lot=60.0, cell=5.0):
+    """Seeded synthetic city for the serving stream (seed 20261203 + 5): a square grid of lots of
+    `lot` metres holding n_buildings box buildings (footprints U[15, 45] m inside their lot, heights
+    U[10, 80] m), facades and roofs tessellated into cells of about `cell` metres, on a 2-triangle
+    ground; concrete at 28 GHz.  About 1M triangles at the defaults.  Returns (name, soup, boxes,
+    half extent, materials)."""
+    rng = SplitMix64(SEED_BASE + 5)
+    side = int(np.ceil(np.sqrt(n_buildings)))
+    half = side * lot / 2.0
+    s = Soup()
+    s.quad((-half, -half, 0.0), (half, -half, 0.0), (half, half, 0.0), (-half, half, 0.0), (0, 0, 1))
+    boxes = []
+    for k in range(n_buildings):
+        gx, gy = k % side, k // side
+        w, d, h = rng.uniform(15.0, 45.0), rng.uniform(15.0, 45.0), rng.uniform(10.0, 80.0)
+        x0 = -half + gx * lot + rng.uniform(5.0, lot - 5.0 - w)
+        y0 = -half + gy * lot + rng.uniform(5.0, lot - 5.0 - d)
+        boxes.append((x0, y0, x0 + w, y0 + d, h))
+        s.tessellated_box(x0, y0, 0.0, x0 + w, y0 + d, h, cell)
+    v, n, m = s.arrays()
+    mats = np.array([[28e9] + CONCRETE[1:]])
+    return f"city_{len(m)}tri_{n_buildings}bldg", (v, n, m), boxes, half, mats
+
+
 def coverage_queries(n, boxes, seed=SEED_BASE + 5, half=250.0, tess=(6, 10), max_rx=50000, maxR=4):
     """Coverage queries for the serving loop: each a transmitter in a street cell at U[3, 30] m, an
     icosahedron level drawn from `tess`, maxR reflections, and a receiver count drawn log-uniform
