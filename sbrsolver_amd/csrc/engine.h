@@ -57,6 +57,8 @@ struct alignas(128) DNodeF {
 };
 static_assert(sizeof(DNodeF) == 128, "DNodeF layout");
 constexpr int kSubLeaf = 64;      // leaves above this size get a leaf BVH
+constexpr int kRecParts = 1024;   // partitions of the capture record buffer (one append counter each)
+constexpr int kRecStride = 16;    // counters 128 B apart
 constexpr int kSubLeafSize = 4;   // triangles per leaf-BVH leaf
 // Triangle as the traversal reads it, in leaf (DFS) order, 96 B.
 struct alignas(32) DTri {
@@ -222,6 +224,8 @@ struct Engine {
   DBuf<int32_t> dTabRx;
   DBuf<int32_t> dHeavy;          // nodes left to the wave-per-node capture
   DBuf<int2> dRec;               // (node, receiver) records of the capture's candidate pass
+  DBuf<unsigned long long> dRecCount, dRecPrefix;   // per-partition record counts / prefix
+  DBuf<unsigned long long> dNodeKey;   // dedup and order keys of the nodes with records
   size_t recCap = 0;
   DBuf<unsigned long long> dCounters;        // capture: records, corrected, groups, flags, collected, heavy
   DBuf<unsigned long long> dTraceCounters;   // bounce loop: overflow flags, node visits, triangle tests

@@ -410,7 +410,7 @@ struct NodeCtx {
   double base, rc, R;
   int model, depth, txi;
   int32_t node, parent;
-  bool all, keyed;
+  bool all, keyed, emitted;
   unsigned long long seqA, seqB, orderKey;
   int64_t losBase;
   int j0, j1;
@@ -432,9 +432,14 @@ struct CaptureBufs {
   // bit 2 record buffer full) [4] collected [5] heavy nodes [6] candidate sphere tests (count mode)
   unsigned long long* counters;
   int32_t* heavy;
-  int2* rec;                      // (node, receiver) records of the candidate pass
-  unsigned long long recCap;
+  // (node, receiver) records of the candidate pass, in kRecParts partitions of recPart slots each
+  // (appends spread over that many counters, kRecStride words apart, instead of contending on one)
+  int2* rec;
+  unsigned long long recPart;
+  unsigned long long* recCount;
+  const unsigned long long* recPrefix;   // dedup pass: records before each partition
   int countTests;
+  unsigned long long* keys;       // per node with records: seqA, seqB, order key (3 words)
 };
 
 // Returns false when no receiver can be captured by the node.
@@ -449,6 +454,7 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
   c.rc = T.rcoef;
   c.base = nd.base;
   c.keyed = false;
+  c.emitted = false;
   c.S = ld3(nd.src);
   c.Tg = ld3(nd.tgt);
   const V3 D = vsub(c.Tg, c.S);
@@ -566,24 +572,25 @@ __device__ bool corrected(const CaptureBufs& B, const NodeCtx& c, int32_t r, V3 
 }
 
 // Appends a (node, receiver) record; called by the lanes that have one (wave-aggregated atomic
-// over the active lanes).
+// over the active lanes, on the partition of the calling wave).
 __device__ __forceinline__ void append_record(const CaptureBufs& B, int32_t node, int32_t r) {
   const unsigned long long active = __ballot(1);
   const int leader = __ffsll((unsigned long long)active) - 1;
   const int lane = (int)(threadIdx.x & 63);
   const int rank = __popcll(active & ((1ull << lane) - 1ull));
+  const unsigned part = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kRecParts;
   unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(&B.counters[0], (unsigned long long)__popcll(active));
+  if (lane == leader) base = atomicAdd(&B.recCount[part * kRecStride], (unsigned long long)__popcll(active));
   base = __shfl(base, leader, 64);
   const unsigned long long pos = base + (unsigned long long)rank;
-  if (pos < B.recCap) B.rec[pos] = make_int2(node, r);
+  if (pos < B.recPart) B.rec[part * B.recPart + pos] = make_int2(node, r);
   else atomicOr(&B.counters[3], 4ull);
 }
 
 // Candidate pass, one receiver against one node: reception sphere and the ancestor rule (a node
 // yields a path iff it captures the receiver and no proper ancestor does).  Only the capture halves
 // of the records are read.
-__device__ __forceinline__ void capture_one(const CaptureBufs& B, const NodeCtx& c, int32_t r,
+__device__ __forceinline__ void capture_one(const CaptureBufs& B, NodeCtx& c, int32_t r,
                                             unsigned long long& myTests) {
   myTests++;
   const V3 C = ld3(B.rx + 3 * (int64_t)r);
@@ -595,6 +602,18 @@ __device__ __forceinline__ void capture_one(const CaptureBufs& B, const NodeCtx&
     q = a.parent;
   }
   append_record(B, c.node, r);
+  c.emitted = true;
+}
+
+// After a node's candidates: the first lane of the wave that emitted records for it stores its key
+// (all lanes of the wave-per-node kernel share the node; in the thread-per-node kernel each lane
+// has its own).
+__device__ __forceinline__ void store_key(const CaptureBufs& B, NodeCtx& c) {
+  node_key(B, c);
+  unsigned long long* k = B.keys + 3 * (int64_t)c.node;
+  k[0] = c.seqA;
+  k[1] = c.seqB;
+  k[2] = c.orderKey;
 }
 
 // Dedup pass, one thread per record: the group key of the node's coplanar-id sequence, the prune
@@ -603,7 +622,13 @@ __global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long myCor = 0, myUsed = 0;
   if (k < nRec) {
-    const int2 rr = B.rec[k];
+    int lo = 0, hi = kRecParts - 1;   // partition holding record k
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (B.recPrefix[mid] <= (unsigned long long)k) lo = mid;
+      else hi = mid - 1;
+    }
+    const int2 rr = B.rec[lo * B.recPart + ((unsigned long long)k - B.recPrefix[lo])];
     const int32_t r = rr.y;
     NodeCtx c;
     const TNode& nd = B.tree[rr.x];
@@ -612,7 +637,11 @@ __global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int
     c.depth = nd.depth;
     c.txi = nd.txi;
     c.losBase = (int64_t)c.txi * A.nrx;
-    node_key(B, c);
+    const unsigned long long* key = B.keys + 3 * (int64_t)c.node;
+    c.seqA = key[0];
+    c.seqB = key[1];
+    c.orderKey = key[2];
+    c.keyed = true;
     unsigned long long hA, hB;
     group_hash(c.seqA, c.seqB, r, &hA, &hB);
     bool keep = true;
@@ -693,6 +722,7 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
             for (int q = B.cellStart[cell]; q < B.cellStart[cell + 1]; q++) capture_one(B, c, B.cellRx[q], myTests);
           }
         }
+        if (c.emitted) store_key(B, c);
       }
     }
   }
@@ -719,6 +749,8 @@ __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBuf
       }
       off += i1 - i0 + 1;
     }
+    const unsigned long long em = __ballot(c.emitted);
+    if (em && lane == __ffsll(em) - 1) store_key(B, c);
   }
   flush_tests(B, myTests);
 }
@@ -1400,6 +1432,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       for (auto& e : kev) HIPCHK(hipEventCreate(&e));
       struct KevGuard { hipEvent_t* e; ~KevGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } kguard{kev};
       ENSURE(dHeavy, std::max<int64_t>(nodes, 1));
+      ENSURE(dNodeKey, 3 * (size_t)std::max<int64_t>(nodes, 1));
       CaptureArgs CA{};
       CA.nodes = nodes;
       CA.nrx = m;
@@ -1408,16 +1441,20 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       HIPCHK(hipEventRecord(kev[0], stream));
       // candidate pass: (node, receiver) records; redone with a larger record buffer if it fills
       unsigned long long nRec = 0, nHeavy = 0;
+      std::vector<unsigned long long> partCount(kRecParts * kRecStride), prefix(kRecParts + 1, 0);
+      ENSURE(dRecCount, kRecParts * kRecStride);
+      ENSURE(dRecPrefix, kRecParts + 1);
       for (int recAttempt = 0;; recAttempt++) {
-        if (recCap < 1024) recCap = std::max<size_t>(1 << 20, (size_t)nodes);
+        if (recCap < (size_t)kRecParts * 1024) recCap = std::max<size_t>((size_t)kRecParts * 2048, (size_t)nodes);
         ENSURE(dRec, recCap);
         recCap = dRec.n;
-        HIPCHK(hipMemsetAsync(dCounters.p, 0, sizeof(unsigned long long), stream));
+        const unsigned long long part = recCap / kRecParts;
+        HIPCHK(hipMemsetAsync(dRecCount.p, 0, sizeof(unsigned long long) * kRecParts * kRecStride, stream));
         HIPCHK(hipMemsetAsync(dCounters.p + 3, 0, sizeof(unsigned long long), stream));
         HIPCHK(hipMemsetAsync(dCounters.p + 5, 0, 2 * sizeof(unsigned long long), stream));
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
-                       dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, (unsigned long long)recCap,
-                       count ? 1 : 0};
+                       dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p, nullptr,
+                       count ? 1 : 0, dNodeKey.p};
         if (nodes > 0) {
           k_capture<<<blocks_for(nodes, 256), 256, 0, stream>>>(CA, CB);
           HIPCHK(hipGetLastError());
@@ -1431,20 +1468,31 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         }
         unsigned long long c4[7];
         HIPCHK(hipMemcpyAsync(c4, dCounters.p, sizeof c4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(partCount.data(), dRecCount.p, sizeof(unsigned long long) * partCount.size(),
+                              hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        nRec = c4[0];
+        unsigned long long maxPart = 0;
+        for (int p = 0; p < kRecParts; p++) {
+          const unsigned long long cnt = partCount[(size_t)p * kRecStride];
+          prefix[p + 1] = prefix[p] + std::min(cnt, part);
+          maxPart = std::max(maxPart, cnt);
+        }
+        nRec = prefix[kRecParts];
         if (!(c4[3] & 4ull)) {
           stats.capture_tests += (int64_t)c4[6];
           break;
         }
         if (recAttempt > 4) return fail(SBR_E_LIMIT, "capture record buffer could not be sized");
-        recCap = std::max<size_t>(recCap * 2, (size_t)(nRec + nRec / 4 + 1024));
+        recCap = std::max<size_t>(recCap * 2, (size_t)(maxPart + maxPart / 4 + 1024) * kRecParts);
       }
+      HIPCHK(hipMemcpyAsync(dRecPrefix.p, prefix.data(), sizeof(unsigned long long) * prefix.size(),
+                            hipMemcpyHostToDevice, stream));
       stats.heavy_nodes += (int64_t)nHeavy;
       // dedup pass over the records
       if (nRec > 0) {
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
-                       dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, (unsigned long long)recCap, 0};
+                       dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, recCap / kRecParts, dRecCount.p,
+                       dRecPrefix.p, 0, dNodeKey.p};
         k_dedup<<<blocks_for((int64_t)nRec, 256), 256, 0, stream>>>(CA, CB, (int64_t)nRec);
         HIPCHK(hipGetLastError());
       }
