@@ -126,6 +126,7 @@ typedef struct {
   int64_t heavy_nodes;      /* tree nodes captured wave-cooperatively (large candidate regions) */
   int64_t windows;          /* launch windows the trace ran in (tree buffers bounded by window_nodes) */
   int64_t capture_tests;    /* reception-sphere tests of the capture's candidate pass (countTraversal) */
+  int64_t box_tests_fp64;   /* child boxes the traversal decided with the FP64 slab test (countTraversal) */
 } sbr_stats;
 
 /* Launch configuration of the traversal kernel (no reference counterpart).  lds_scene: -1 stage

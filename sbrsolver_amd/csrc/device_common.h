@@ -39,6 +39,7 @@ struct Hit {
   double t;
   int nodes;     // BVH nodes entered
   int tests;     // ray-triangle tests
+  int exact64;   // child boxes decided by the FP64 slab test (grazing or no single-precision path)
 };
 
 // Hit point of a closest hit: the same operations tri_intersect applies, so the value is
@@ -100,9 +101,10 @@ __device__ __forceinline__ bool keep_box(double lo, double hi, const Hit& h) {
 // between the two takes the reference's FP64 slab test on the exact box.  *entry is a lower bound
 // of the t of every hit inside the child.
 __device__ __forceinline__ bool child_fast(const float* b, const DChildF& c, const DNode* __restrict__ nodes,
-                                           const RayPre& r, const RayF& f, float clip, const Hit& h, float* entry) {
+                                           const RayPre& r, const RayF& f, float clip, Hit& h, float* entry) {
   if (!box32_may_hit(b, b + 3, f, clip, entry)) return false;
   if (c.inner && box32_crossed(b + 6, b + 9, f)) return true;
+  h.exact64++;
   double lo, hi;
   const DNode& n = nodes[c.item & ~kLeafBit];
   return box_test(n.bmin, n.bmax, r, &lo, &hi) && keep_box<false>(lo, hi, h);
@@ -139,6 +141,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
   h.t = INFINITY;
   h.nodes = 0;
   h.tests = 0;
+  h.exact64 = 0;
   const RayPre r = ray_pre(o, d);
   const RayF f = rayf_pre(o, d, ext);
   const bool fast = !EXACT && f.on;

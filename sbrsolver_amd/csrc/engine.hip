@@ -170,7 +170,8 @@ struct TraceArgs {
   unsigned long long nodeCap, queueCap;
 };
 
-// Flags: [0] bit 0 node overflow, bit 1 queue overflow; [6], [7] node visits / triangle tests.
+// Flags: [0] bit 0 node overflow, bit 1 queue overflow; [5] FP64 box tests, [6] node visits, [7] triangle
+// tests (count mode).
 template <bool EXACT, bool LDS, bool SUB>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_trace(TraceArgs A, const TxDesc* __restrict__ txs,
                                                   const DNode* __restrict__ nodes, const DNodeF* __restrict__ gNodesF,
@@ -204,7 +205,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     n = A.queueCap;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) lv[A.depth + 1].base = L.base + n;
-  unsigned long long cntNodes = 0, cntTests = 0;
+  unsigned long long cntNodes = 0, cntTests = 0, cntExact = 0;
   const int lane = threadIdx.x & 63;
   for (unsigned long long b0 = (unsigned long long)blockIdx.x * blockDim.x; b0 < n;
        b0 += (unsigned long long)gridDim.x * blockDim.x) {
@@ -297,6 +298,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       nd.mask = (uint8_t)mask;
       cntNodes += h.nodes;
       cntTests += h.tests;
+      cntExact += h.exact64;
     }
     // Wave-aggregated append of the children to the next depth's queue; child q of that queue
     // becomes node L.base + n + q.
@@ -334,6 +336,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
   if (countTraversal) {
     atomicAdd(&flags[6], cntNodes);
     atomicAdd(&flags[7], cntTests);
+    atomicAdd(&flags[5], cntExact);
   }
 }
 
@@ -1217,6 +1220,7 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   stats.max_depth = std::max<int64_t>(stats.max_depth, deepest);
   stats.node_visits += count ? (int64_t)ctr[6] : 0;
   stats.tri_tests += count ? (int64_t)ctr[7] : 0;
+  stats.box_tests_fp64 += count ? (int64_t)ctr[5] : 0;
   return SBR_OK;
 }
 
@@ -1364,6 +1368,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   bool done = false;
   for (int tabAttempt = 0; tabAttempt < 6 && !done; tabAttempt++) {
     stats.bounces = stats.node_visits = stats.tri_tests = stats.capture_records = stats.corrected = 0;
+    stats.box_tests_fp64 = 0;
     stats.ms_k_trace = stats.ms_k_capture = stats.ms_trace = stats.ms_capture = stats.ms_field = 0;
     stats.n_k_trace = 0;
     stats.max_depth = 0;
