@@ -140,6 +140,13 @@ typedef struct {
   int64_t window_nodes;
 } sbr_tuning;
 
+/* Autotuner (no reference counterpart): times the traversal kernel on a sample of the given trace
+ * (every launch_stride-th launch, chosen so that about 2^18 launches run) for each launch
+ * configuration -- scene staged in LDS or not, persistent blocks per CU from the occupancy maximum
+ * down -- keeps the fastest, sets it on the context and returns it in *chosen.  The last trace
+ * result of the context is replaced by the last trial. */
+int sbr_autotune(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx, int32_t n_rx, sbr_tuning* chosen);
+
 /* device >= 0: HIP device ordinal.  device == -1: host-only context (scene preparation and
  * scene queries work; device entry points return SBR_E_STATE). */
 int sbr_create(int device, sbr_ctx** out);

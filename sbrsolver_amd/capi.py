@@ -97,6 +97,7 @@ SIGNATURES = {
     "sbr_materials": ([_vp, _vp, _i32], C.c_int),
     "sbr_set_tuning": ([_vp, _vp], C.c_int),
     "sbr_get_tuning": ([_vp, _vp], C.c_int),
+    "sbr_autotune": ([_vp, _vp, _vp, _i32, _vp], C.c_int),
     "sbr_generate_rays": ([_vp, _i32, _vp, _i64, _vp], C.c_int),
     "sbr_closest_hit": ([_vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp], C.c_int),
     "sbr_trace": ([_vp, _vp, _vp, _i32], C.c_int),
@@ -281,6 +282,14 @@ class Context:
     def set_tuning(self, lds_scene=-1, blocks_per_cu=0, window_nodes=0):
         t = Tuning(int(lds_scene), int(blocks_per_cu), int(window_nodes))
         self._chk(lib().sbr_set_tuning(self.h, C.byref(t)))
+
+    def autotune(self, params, rx):
+        """Times the traversal on a sample of this trace for each launch configuration and keeps
+        the fastest (see sbr_autotune); returns the chosen tuning."""
+        r = np.ascontiguousarray(rx, dtype=np.float64).reshape(-1, 3)
+        t = Tuning()
+        self._chk(lib().sbr_autotune(self.h, C.byref(params), _a(r), r.shape[0], C.byref(t)))
+        return dict(lds_scene=t.lds_scene, blocks_per_cu=t.blocks_per_cu, window_nodes=t.window_nodes)
 
     def get_tuning(self):
         t = Tuning()

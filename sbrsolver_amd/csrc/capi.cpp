@@ -422,6 +422,51 @@ int sbr_set_tuning(sbr_ctx* ctx, const sbr_tuning* t) {
   });
 }
 
+int sbr_autotune(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx, int32_t n_rx, sbr_tuning* chosen) {
+  return guarded(ctx, [&] {
+    sbr::Engine& e = ctx->eng;
+    if (!params || !chosen) return e.fail(SBR_E_ARG, "autotune: null argument");
+    if (e.device < 0) return e.fail(SBR_E_STATE, "host-only context has no device");
+    sbr_trace_params p = *params;
+    const int64_t launches = 10 * (int64_t(1) << (2 * std::max(0, p.tess))) + 2;
+    p.launchStride = (int32_t)std::max<int64_t>(1, launches >> 18);
+    p.launchOffset = 0;
+    p.countTraversal = 0;
+    const sbr::Tuning saved = e.tune;
+    sbr::Tuning best = saved;
+    double bestMs = 1e300;
+    // first trial: automatic choice, which also reports the occupancy maximum
+    std::vector<sbr::Tuning> trials{{-1, 0, saved.windowNodes}};
+    for (int lds = 0; lds <= 1; lds++)
+      for (int bpc = 1; bpc <= 8; bpc++) trials.push_back({lds, bpc, saved.windowNodes});
+    int maxBpc = 8;
+    for (size_t k = 0; k < trials.size(); k++) {
+      if (k > 0 && trials[k].blocksPerCU > maxBpc) continue;
+      e.tune = trials[k];
+      double ms = 0;
+      for (int rep = 0; rep < 3; rep++) {   // first run warms buffers and caches
+        int rc = e.trace(&p, 1, rx, n_rx);
+        if (rc) {
+          e.tune = saved;
+          return rc;
+        }
+        if (rep > 0) ms += e.stats.ms_k_trace;
+      }
+      if (k == 0) maxBpc = std::max(1, (int)e.stats.trace_blocks_per_cu);
+      if (k > 0 && e.stats.trace_lds != trials[k].ldsScene) continue;   // staging did not fit
+      if (ms < bestMs) {
+        bestMs = ms;
+        best = trials[k];
+      }
+    }
+    e.tune = best;
+    chosen->lds_scene = best.ldsScene;
+    chosen->blocks_per_cu = best.blocksPerCU;
+    chosen->window_nodes = best.windowNodes;
+    return SBR_OK;
+  });
+}
+
 int sbr_get_tuning(sbr_ctx* ctx, sbr_tuning* t) {
   return guarded(ctx, [&] {
     if (!t) return ctx->eng.fail(SBR_E_ARG, "null tuning");

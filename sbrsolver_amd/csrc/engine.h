@@ -154,7 +154,7 @@ struct DBuf {
   ~DBuf() { release(); }
 };
 
-// Launch configuration of the traversal kernel (sbr_set_tuning; the autotuner fills it).
+// Launch configuration of the traversal kernel (sbr_set_tuning, or chosen by sbr_autotune).
 struct Tuning {
   int ldsScene = -1;      // -1: LDS staging when it fits and keeps occupancy, 1: whenever it fits, 0: never
   int blocksPerCU = 0;    // persistent blocks per CU; 0: occupancy maximum

@@ -198,6 +198,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
   int* stkN = reinterpret_cast<int*>(smem + sceneB) + threadIdx.x;
   float* stkT = reinterpret_cast<float*>(smem + sceneB + (size_t)A.stackDepth * blockDim.x * 4) + threadIdx.x;
 
+  // An overflow at an earlier depth (flags[0] != 0) means the trace is redone with more room: this
+  // depth's queue may hold unwritten entries, so nothing is read from it.
+  if (A.depth > 0 && flags[0] != 0ull) return;
   const LevelRec L = lv[A.depth];
   unsigned long long n = L.count;
   if (n > A.queueCap) {

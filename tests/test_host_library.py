@@ -40,6 +40,12 @@ def test_host_only_context_rejects_device_work():
         c.materials()
         with pytest.raises(SbrError):
             c.trace(make_params(tx=(0, 0, 1), tess=1, maxR=1), np.zeros((1, 3)))
+        with pytest.raises(SbrError, match="no device"):
+            c.autotune(make_params(tx=(0, 0, 1), tess=1, maxR=1), np.zeros((1, 3)))
+        with pytest.raises(SbrError, match="out of range"):
+            c.set_tuning(window_nodes=-5)
+        c.set_tuning(1, 2, 1 << 20)
+        assert c.get_tuning() == dict(lds_scene=1, blocks_per_cu=2, window_nodes=1 << 20)
         with pytest.raises(SbrError):
             c.scene_from_stl("/nonexistent.stl")
 
