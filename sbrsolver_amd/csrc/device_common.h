@@ -152,10 +152,8 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
   }
   int sp = 0;
   int cur = nodes[0].count > 0 ? kLeafBit : 0;
-  // Exact test of DFS triangle k after its pre-test, keeping the minimum (distance, position).
-  auto test_tri = [&](int k) {
-    float e;
-    if (f.on && !box32_may_hit(tbox[k].lo, tbox[k].hi, f, clip, &e)) return;
+  // Exact test of DFS triangle k, keeping the minimum (distance, position).
+  auto test_exact = [&](int k) {
     h.tests++;
     const DTri& tr = tris[k];
     V3 p;
@@ -168,6 +166,12 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
         clip = __double2float_ru(t * (1.0 + 1e-6));
       }
     }
+  };
+  // ... after its pre-test.
+  auto test_tri = [&](int k) {
+    float e;
+    if (f.on && !box32_may_hit(tbox[k].lo, tbox[k].hi, f, clip, &e)) return;
+    test_exact(k);
   };
   while (true) {
     h.nodes++;
@@ -198,7 +202,21 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
         cur = ~li.w;   // search the leaf through its leaf BVH
         continue;
       }
-      for (int k = li.x; k < li.x + li.z; k++) test_tri(k);
+      if (f.on) {
+        // Pre-test boxes two at a time, so that their loads are in flight together.
+        const int end = li.x + li.z;
+        for (int k = li.x; k < end; k += 2) {
+          const DBox32 b0 = tbox[k];
+          const bool two = k + 1 < end;
+          DBox32 b1 = b0;
+          if (two) b1 = tbox[k + 1];
+          float e;
+          if (box32_may_hit(b0.lo, b0.hi, f, clip, &e)) test_exact(k);
+          if (two && box32_may_hit(b1.lo, b1.hi, f, clip, &e)) test_exact(k + 1);
+        }
+      } else {
+        for (int k = li.x; k < li.x + li.z; k++) test_exact(k);
+      }
     } else {
       const DNodeF& nf = nodesF[cur];
       float ta, tb;
