@@ -452,7 +452,11 @@ int sbr_autotune(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx,
         }
         if (rep > 0) ms += e.stats.ms_k_trace;
       }
-      if (k == 0) maxBpc = std::max(1, (int)e.stats.trace_blocks_per_cu);
+      if (k == 0) {   // the automatic choice, recorded as the configuration it resolved to
+        maxBpc = std::max(1, (int)e.stats.trace_blocks_per_cu);
+        trials[0].ldsScene = (int32_t)e.stats.trace_lds;
+        trials[0].blocksPerCU = maxBpc;
+      }
       if (k > 0 && e.stats.trace_lds != trials[k].ldsScene) continue;   // staging did not fit
       if (ms < bestMs) {
         bestMs = ms;

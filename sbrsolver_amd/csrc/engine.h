@@ -223,6 +223,7 @@ struct Engine {
   DBuf<unsigned long long> dTabA, dTabB, dTabOrder;
   DBuf<int32_t> dTabRx;
   DBuf<int32_t> dHeavy;          // nodes left to the wave-per-node capture
+  DBuf<int32_t> dCand;           // nodes whose segment reaches the receivers' capture region
   DBuf<int2> dRec;               // (node, receiver) records of the capture's candidate pass
   DBuf<unsigned long long> dRecCount, dRecPrefix;   // per-partition record counts / prefix
   DBuf<unsigned long long> dNodeKey;   // dedup and order keys of the nodes with records
@@ -240,6 +241,7 @@ struct Engine {
   sbr_stats stats{};
   int nrx = 0, ntx = 0;
   int64_t nodes = 0;
+  int64_t capNodes = 0;          // of them listed for the capture
   int64_t winners = 0;
   size_t nodeCap = 0;
   size_t tabCapHint = 0;   // dedup table size of the last trace
@@ -270,7 +272,7 @@ struct Engine {
  private:
   int generate_rays(int tess, DBuf<double>& out, int64_t* nOut);
   int trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_t nRoot, int maxDepth, bool exact,
-                  bool count, int64_t* need, float* ms);
+                  bool count, const RxGrid* capG, int64_t* need, float* ms);
   int paths_window(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
                    int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
                    bool countOnly);

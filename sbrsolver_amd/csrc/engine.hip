@@ -160,6 +160,48 @@ __global__ void k_ico_dirs(const double* __restrict__ verts, int64_t nv, double*
 // =============================================================================================
 // 2. Interaction trees, one persistent launch per depth (RayLaunch, RTSolver.cpp:128-257)
 // =============================================================================================
+
+// Capture region of a segment S->Tg: a capture at C has its closest point P within
+// (base + |C - ref|) * rc / (1 - rc) of C, with base/ref per reception model, so only the part of
+// the segment inside the receivers' box grown by that bound R can capture.  Returns false when no
+// part is; otherwise the clipped segment P0->P1 and R.  One function for the bounce kernel (which
+// lists the nodes that can capture) and the capture pass (which scans them), so both decide alike.
+struct SegClip {
+  V3 P0, P1;
+  double R;
+  bool all;
+};
+__device__ __forceinline__ bool seg_clip(const RxGrid& g, V3 S, V3 Tg, double base, int model, double rc,
+                                         SegClip& o) {
+  const V3 D = vsub(Tg, S);
+  double far2 = 0.0;
+  for (int a = 0; a < 3; a++) {
+    double sa = model == 0 ? 0.0 : vget(S, a);
+    double e = fmax(fabs(g.bmin[a] - sa), fabs(g.bmax[a] - sa));
+    far2 += e * e;
+  }
+  double R = (base + sqrt(far2)) * rc / (1.0 - rc);
+  o.R = R * (1.0 + 1e-6) + 1e-6;
+  o.all = !(o.R < 1e300) || rc >= 1.0;
+  double u0 = 0.0, u1 = 1.0;
+  if (!o.all) {
+    for (int a = 0; a < 3; a++) {
+      double lo = g.bmin[a] - o.R, hi = g.bmax[a] + o.R, s = vget(S, a), d = vget(D, a);
+      if (fabs(d) < 1e-300) {
+        if (s < lo || s > hi) return false;
+      } else {
+        double ua = (lo - s) / d, ub = (hi - s) / d;
+        u0 = fmax(u0, fmin(ua, ub) - 1e-12);
+        u1 = fmin(u1, fmax(ua, ub) + 1e-12);
+        if (u0 > u1) return false;
+      }
+    }
+  }
+  o.P0 = vadd(S, vmul(D, u0));
+  o.P1 = vadd(S, vmul(D, u1));
+  return true;
+}
+
 struct TraceArgs {
   int32_t depth;
   int32_t nTx;
@@ -167,11 +209,13 @@ struct TraceArgs {
   int32_t nodeBytes;             // single-precision node records staged in LDS (LDS variant)
   int64_t rootBase;              // global root index of the window's first root item
   float ext;                                // scene extent for the single-precision pre-test
+  int32_t capture;               // 1: list the nodes whose segment can capture (receivers' box in g)
   unsigned long long nodeCap, queueCap;
+  RxGrid g;
 };
 
-// Flags: [0] bit 0 node overflow, bit 1 queue overflow; [5] FP64 box tests, [6] node visits, [7] triangle
-// tests (count mode).
+// Flags: [0] bit 0 node overflow, bit 1 queue overflow; [1] nodes listed for the capture; [5] FP64
+// box tests, [6] node visits, [7] triangle tests (count mode).
 template <bool EXACT, bool LDS, bool SUB>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_trace(TraceArgs A, const TxDesc* __restrict__ txs,
                                                   const DNode* __restrict__ nodes, const DNodeF* __restrict__ gNodesF,
@@ -181,7 +225,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
                                                   const double* __restrict__ triN, const int32_t* __restrict__ qIn,
                                                   int32_t* __restrict__ qOut, TNode* __restrict__ tree,
                                                   LevelRec* __restrict__ lv, unsigned long long* __restrict__ flags,
-                                                  int countTraversal) {
+                                                  int32_t* __restrict__ cand, int countTraversal) {
   extern __shared__ __align__(16) unsigned char smem[];
   const DNodeF* nodesF = gNodesF;
   const DTri* tris = gTris;
@@ -335,6 +379,24 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
       for (int w = 0; w < 8; w++) dstw[w] = srcw[w];
     }
+    // Nodes whose segment reaches the receivers' capture region, listed (wave-aggregated append)
+    // so that the capture pass reads only those records.
+    if (A.capture) {
+      bool capt = false;
+      if (live) {
+        const TxDesc& T = txs[nd.txi];
+        SegClip sc;
+        capt = seg_clip(A.g, ld3(nd.src), ld3(nd.tgt), nd.base, T.receptionModel, T.rcoef, sc);
+      }
+      const unsigned long long cm = __ballot(capt);
+      if (cm) {
+        const int leader = __ffsll(cm) - 1;
+        unsigned long long cb = 0;
+        if (lane == leader) cb = atomicAdd(&flags[1], (unsigned long long)__popcll(cm));
+        cb = __shfl(cb, leader, 64);
+        if (capt) cand[cb + __popcll(cm & ((1ull << lane) - 1ull))] = (int32_t)nodeIdx;
+      }
+    }
   }
   if (countTraversal) {
     atomicAdd(&flags[6], cntNodes);
@@ -395,7 +457,7 @@ __global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restric
 // 4. Capture, image correction, duplicate removal
 // =============================================================================================
 struct CaptureArgs {
-  int64_t nodes;
+  int64_t nodes;   // listed nodes
   int32_t nrx;
   RxGrid g;
   unsigned long long tabMask;
@@ -446,6 +508,7 @@ struct CaptureBufs {
   const unsigned long long* recPrefix;   // dedup pass: records before each partition
   int countTests;
   unsigned long long* keys;       // per node with records: seqA, seqB, order key (3 words)
+  const int32_t* cand;            // nodes listed by the bounce kernel (CaptureArgs::nodes of them)
 };
 
 // Returns false when no receiver can be captured by the node.
@@ -463,35 +526,12 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
   c.emitted = false;
   c.S = ld3(nd.src);
   c.Tg = ld3(nd.tgt);
-  const V3 D = vsub(c.Tg, c.S);
-  // Conservative search radius: a capture at C has its closest point P within
-  // (base + |C - ref|) * rc / (1 - rc) of C, with base/ref per reception model.
-  double far2 = 0.0;
-  for (int a = 0; a < 3; a++) {
-    double sa = c.model == 0 ? 0.0 : vget(c.S, a);
-    double e = fmax(fabs(A.g.bmin[a] - sa), fabs(A.g.bmax[a] - sa));
-    far2 += e * e;
-  }
-  double R = (c.base + sqrt(far2)) * c.rc / (1.0 - c.rc);
-  c.R = R * (1.0 + 1e-6) + 1e-6;
-  c.all = !(c.R < 1e300) || c.rc >= 1.0;
-  // Clip the segment to the receivers' box grown by R.
-  double u0 = 0.0, u1 = 1.0;
-  if (!c.all) {
-    for (int a = 0; a < 3; a++) {
-      double lo = A.g.bmin[a] - c.R, hi = A.g.bmax[a] + c.R, s = vget(c.S, a), d = vget(D, a);
-      if (fabs(d) < 1e-300) {
-        if (s < lo || s > hi) return false;
-      } else {
-        double ua = (lo - s) / d, ub = (hi - s) / d;
-        u0 = fmax(u0, fmin(ua, ub) - 1e-12);
-        u1 = fmin(u1, fmax(ua, ub) + 1e-12);
-        if (u0 > u1) return false;
-      }
-    }
-  }
-  c.P0 = vadd(c.S, vmul(D, u0));
-  c.P1 = vadd(c.S, vmul(D, u1));
+  SegClip sc;
+  if (!seg_clip(A.g, c.S, c.Tg, c.base, c.model, c.rc, sc)) return false;
+  c.P0 = sc.P0;
+  c.P1 = sc.P1;
+  c.R = sc.R;
+  c.all = sc.all;
   c.j0 = 0;
   c.j1 = A.g.ny - 1;
   if (!c.all) {
@@ -701,14 +741,16 @@ __device__ __forceinline__ void flush_tests(const CaptureBufs& B, unsigned long 
   if ((threadIdx.x & 63) == 0 && myTests) atomicAdd(&B.counters[6], myTests);
 }
 
-// Thread per node.  Nodes whose candidate region spans more than kHeavyCells grid cells are left to
-// k_capture_heavy, which spreads one node over a wave.
+// Thread per listed node (those whose segment reaches the receivers' capture region).  Nodes whose
+// candidate region spans more than kHeavyCells grid cells are left to k_capture_heavy, which spreads
+// one node over a wave.
 constexpr int kHeavyCells = 48;
 
 __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long myTests = 0;
-  if (n < A.nodes) {
+  if (k < A.nodes) {
+    const int32_t n = B.cand[k];
     NodeCtx c;
     if (node_setup(A, B, n, c)) {
       int cells = 0;
@@ -1144,11 +1186,11 @@ static double ms_since(std::chrono::steady_clock::time_point t) {
 // depth, no host synchronisation in between.  Returns SBR_E_LIMIT when the node or queue buffers
 // overflowed, with *need a lower bound of the nodes the window needs; *ms is the kernels' time.
 int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_t nRoot, int maxDepth, bool exact,
-                        bool count, int64_t* need, float* ms) {
+                        bool count, const RxGrid* capG, int64_t* need, float* ms) {
   *need = 0;
   *ms = 0;
   if (nRoot == 0) {
-    nodes = 0;
+    nodes = capNodes = 0;
     return SBR_OK;
   }
   // Scene in LDS or read through the caches: automatic choice takes LDS staging unless it lowers the
@@ -1174,6 +1216,8 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   A.rootBase = rootBase;
   A.nodeCap = nodeCap;
   A.queueCap = nodeCap;
+  A.capture = capG ? 1 : 0;
+  if (capG) A.g = *capG;
   auto kern = variant(lds);
   int perCU = std::max(1, lds ? occLds : occGlobal);
   if (tune.blocksPerCU > 0) perCU = std::min(perCU, tune.blocksPerCU);
@@ -1194,7 +1238,7 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
     const int32_t* qin = dQueue[d & 1].p;
     int32_t* qout = dQueue[(d + 1) & 1].p;
     kern<<<grid, kBlock, smem, stream>>>(A, dTx.p, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p, dSubTri.p, dTriN.p, qin,
-                                         qout, dTree.p, dLevels.p, dTraceCounters.p, count ? 1 : 0);
+                                         qout, dTree.p, dLevels.p, dTraceCounters.p, dCand.p, count ? 1 : 0);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(kev[1], stream));
@@ -1217,6 +1261,7 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   }
   *ms = kms;
   nodes = (int64_t)total;
+  capNodes = capG ? (int64_t)ctr[1] : 0;
   stats.ms_k_trace += kms;
   stats.n_k_trace += maxDepth + 1;
   stats.bounces += nodes;
@@ -1410,9 +1455,11 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         ENSURE(dTree, nodeCap);
         ENSURE(dQueue[0], nodeCap);
         ENSURE(dQueue[1], nodeCap);
+        if (m > 0) ENSURE(dCand, nodeCap);
         nodeCap = std::min(dTree.n, std::min(dQueue[0].n, dQueue[1].n));
+        if (m > 0) nodeCap = std::min(nodeCap, dCand.n);
         int64_t need = 0;
-        int rc = trace_stage(txs, ws, W, maxDepth, exact, count, &need, &msT);
+        int rc = trace_stage(txs, ws, W, maxDepth, exact, count, m > 0 ? &g : nullptr, &need, &msT);
         if (rc == SBR_OK) break;
         if (rc != SBR_E_LIMIT || attempt > 40) return rc == SBR_E_LIMIT ? fail(rc, "tree buffers could not be sized") : rc;
         // Overflowed (need is only a lower bound): first use the whole window budget, then halve
@@ -1439,10 +1486,10 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       hipEvent_t kev[4];
       for (auto& e : kev) HIPCHK(hipEventCreate(&e));
       struct KevGuard { hipEvent_t* e; ~KevGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } kguard{kev};
-      ENSURE(dHeavy, std::max<int64_t>(nodes, 1));
+      ENSURE(dHeavy, std::max<int64_t>(capNodes, 1));
       ENSURE(dNodeKey, 3 * (size_t)std::max<int64_t>(nodes, 1));
       CaptureArgs CA{};
-      CA.nodes = nodes;
+      CA.nodes = capNodes;
       CA.nrx = m;
       CA.g = g;
       CA.tabMask = cap - 1;
@@ -1462,9 +1509,9 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         HIPCHK(hipMemsetAsync(dCounters.p + 5, 0, 2 * sizeof(unsigned long long), stream));
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
                        dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p, nullptr,
-                       count ? 1 : 0, dNodeKey.p};
-        if (nodes > 0) {
-          k_capture<<<blocks_for(nodes, 256), 256, 0, stream>>>(CA, CB);
+                       count ? 1 : 0, dNodeKey.p, dCand.p};
+        if (capNodes > 0) {
+          k_capture<<<blocks_for(capNodes, 256), 256, 0, stream>>>(CA, CB);
           HIPCHK(hipGetLastError());
         }
         HIPCHK(hipMemcpyAsync(&nHeavy, dCounters.p + 5, sizeof nHeavy, hipMemcpyDeviceToHost, stream));
@@ -1500,7 +1547,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       if (nRec > 0) {
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
                        dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, recCap / kRecParts, dRecCount.p,
-                       dRecPrefix.p, 0, dNodeKey.p};
+                       dRecPrefix.p, 0, dNodeKey.p, dCand.p};
         k_dedup<<<blocks_for((int64_t)nRec, 256), 256, 0, stream>>>(CA, CB, (int64_t)nRec);
         HIPCHK(hipGetLastError());
       }
