@@ -128,8 +128,8 @@ struct alignas(128) TNode {
   uint8_t type;      // 0 from the transmitter, 1 reflected, 2 transmitted
   uint8_t rc, tc;    // reflections / transmissions so far
   uint8_t mask;      // children created: bit 0 transmitted, bit 1 reflected
-  int32_t firstChild;   // node index of the first child created (the transmitted one if any), -1 if none
-  int32_t pad1;
+  int32_t childT;    // node index of the transmitted child, -1 if none
+  int32_t childR;    // node index of the reflected child, -1 if none
 };
 static_assert(sizeof(TNode) == 128, "TNode layout");
 

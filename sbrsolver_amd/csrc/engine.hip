@@ -319,7 +319,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       }
       nd.depth = (uint8_t)A.depth;
       nd.pad0 = 0;
-      nd.pad1 = 0;
       const V3 src = ld3(nd.src), dir = ld3(nd.dir);
       Hit h;
       if (closest_hit<EXACT, SUB>(nodes, nodesF, tris, tbox, subs, subTri, A.ext, src, dir, h, stkN, stkT,
@@ -348,32 +347,42 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       cntExact += h.exact64;
     }
     // Wave-aggregated append of the children to the next depth's queue; child q of that queue
-    // becomes node L.base + n + q.
-    const int c = (mask & 1) + ((mask >> 1) & 1);
-    int incl = c;
+    // becomes node L.base + n + q.  The wave's transmitted children go first and its reflected ones
+    // after them, so that the next depth's waves hold rays of one kind (transmitted rays continue
+    // into the wall, reflected ones leave it).
+    const int cT = mask & 1, cR = (mask >> 1) & 1;
+    int incl = cT | (cR << 16);   // two 16-bit inclusive counts
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
       int v = __shfl_up(incl, off, 64);
       if (lane >= off) incl += v;
     }
     const int tot = __shfl(incl, 63, 64);
+    const int totT = tot & 0xffff, totR = tot >> 16;
     unsigned long long wbase = 0;
-    if (lane == 63 && tot > 0) wbase = atomicAdd(&lv[A.depth + 1].count, (unsigned long long)tot);
+    if (lane == 63 && tot > 0) wbase = atomicAdd(&lv[A.depth + 1].count, (unsigned long long)(totT + totR));
     wbase = __shfl(wbase, 63, 64);
-    int32_t firstChild = -1;
-    if (c > 0) {
-      unsigned long long pos = wbase + (unsigned long long)(incl - c);
-      if (pos + c > A.queueCap) {
+    int32_t childT = -1, childR = -1;
+    if (cT + cR > 0) {
+      if (wbase + (unsigned long long)(totT + totR) > A.queueCap) {
         atomicOr(&flags[0], 2ull);
       } else {
-        firstChild = (int32_t)(L.base + n + pos);
         const int32_t nodeI = (int32_t)nodeIdx;
-        if (mask & 1) qOut[pos++] = nodeI << 1;
-        if (mask & 2) qOut[pos] = (nodeI << 1) | 1;
+        if (cT) {
+          const unsigned long long pos = wbase + (unsigned long long)((incl & 0xffff) - 1);
+          childT = (int32_t)(L.base + n + pos);
+          qOut[pos] = nodeI << 1;
+        }
+        if (cR) {
+          const unsigned long long pos = wbase + (unsigned long long)(totT + (incl >> 16) - 1);
+          childR = (int32_t)(L.base + n + pos);
+          qOut[pos] = (nodeI << 1) | 1;
+        }
       }
     }
     if (live) {
-      nd.firstChild = firstChild;
+      nd.childT = childT;
+      nd.childR = childR;
       const int4* srcw = reinterpret_cast<const int4*>(&nd);
       int4* dstw = reinterpret_cast<int4*>(&tree[nodeIdx]);
 #pragma unroll
@@ -819,7 +828,7 @@ __global__ void k_collect(const unsigned long long* __restrict__ tabA, const uns
 }
 
 // Tree node of every kept path (order keys, window roots from rootBase): from the root, follow the
-// branch code's bits through firstChild (the transmitted child comes first when it exists).
+// branch code's bits through the child links.
 __global__ void k_resolve(const TNode* __restrict__ tree, const unsigned long long* __restrict__ keys, int64_t W,
                           int64_t rootBase, int32_t* __restrict__ winNode) {
   int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -830,7 +839,7 @@ __global__ void k_resolve(const TNode* __restrict__ tree, const unsigned long lo
   int32_t n = (int32_t)((int64_t)(key >> 32) - rootBase);
   for (int l = depth - 1; l >= 0; l--) {
     const TNode& t = tree[n];
-    n = t.firstChild + (((code >> l) & 1u) && (t.mask & 1) ? 1 : 0);
+    n = ((code >> l) & 1u) ? t.childR : t.childT;
   }
   winNode[w] = n;
 }
