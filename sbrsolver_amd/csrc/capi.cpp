@@ -177,7 +177,11 @@ int Engine::upload(const sbr_scene_desc* d) {
   std::vector<DNodeF> nf(nodesOut.size());
   for (size_t k = 0; k < nodesOut.size(); k++) {
     const DNode& n = nodesOut[k];
-    if (n.count > 0) continue;
+    if (n.count > 0) {   // a leaf's own slot holds its triangle range: a, b, count, sub
+      nf[k].ch[0] = DChildF{n.a, n.b};
+      nf[k].ch[1] = DChildF{n.count, n.sub};
+      continue;
+    }
     const int32_t ch[2] = {n.a, n.b};
     for (int c = 0; c < 2; c++) {
       const DNode& cn = nodesOut[ch[c]];

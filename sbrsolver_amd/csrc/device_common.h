@@ -197,7 +197,8 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
         }
       }
     } else if (cur & kLeafBit) {
-      const int4 li = reinterpret_cast<const int4*>(&nodes[cur & ~kLeafBit])[3];   // a, b, count, sub
+      // a, b, count, sub from the leaf's slot of the single-precision records (LDS when staged)
+      const int4 li = *reinterpret_cast<const int4*>(&nodesF[cur & ~kLeafBit].ch[0]);
       if (SUB && fast && li.w >= 0) {
         cur = ~li.w;   // search the leaf through its leaf BVH
         continue;

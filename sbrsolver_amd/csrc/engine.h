@@ -52,7 +52,7 @@ struct DChildF {
 };
 struct alignas(128) DNodeF {
   float box[2][12];   // per child: grown lo[3], hi[3], shrunk lo[3], hi[3]
-  DChildF ch[2];
+  DChildF ch[2];      // a leaf's slot: {a, b}, {count, sub} of its DNode (no boxes)
   int32_t pad[4];
 };
 static_assert(sizeof(DNodeF) == 128, "DNodeF layout");
