@@ -173,7 +173,47 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
     if (f.on && !box32_may_hit(tbox[k].lo, tbox[k].hi, f, clip, &e)) return;
     test_exact(k);
   };
+  // Next work item from the stack, skipping entries that cannot beat the best hit; false when the
+  // stack is empty.
+  auto pop = [&]() {
+    while (sp > 0) {
+      --sp;
+      if (!EXACT && h.idx != INT_MAX && (double)stkT[sp * stride] > h.t * (1.0 + 1e-7) + 1e-7) continue;
+      cur = stkN[sp * stride];
+      return true;
+    }
+    return false;
+  };
+  bool live = true;
+  // While-while traversal: each lane descends through inner nodes until it holds a leaf (or a
+  // leaf-BVH node) or runs out of work, then the wave tests its lanes' leaves together.  Every lane
+  // visits the same items in the same order as a one-item-per-step loop would.
   while (true) {
+    while (live && cur >= 0 && !(cur & kLeafBit)) {
+      h.nodes++;
+      const DNodeF& nf = nodesF[cur];
+      float ta, tb;
+      bool ina, inb;
+      if (fast) {
+        ina = child_fast(nf.box[0], nf.ch[0], nodes, r, f, clip, h, &ta);
+        inb = child_fast(nf.box[1], nf.ch[1], nodes, r, f, clip, h, &tb);
+      } else {
+        ina = child_exact<EXACT>(nf.ch[0], nodes, r, h, &ta);
+        inb = child_exact<EXACT>(nf.ch[1], nodes, r, h, &tb);
+      }
+      if (ina && inb) {
+        const bool bFirst = !EXACT && tb < ta;   // EXACT: the reference's DFS order
+        stkN[sp * stride] = bFirst ? nf.ch[0].item : nf.ch[1].item;
+        stkT[sp * stride] = bFirst ? ta : tb;
+        sp++;
+        cur = bFirst ? nf.ch[1].item : nf.ch[0].item;
+      } else if (ina || inb) {
+        cur = ina ? nf.ch[0].item : nf.ch[1].item;
+      } else {
+        live = pop();
+      }
+    }
+    if (!live) break;
     h.nodes++;
     if (SUB && cur < 0) {
       const DSub sn = subs[~cur];
@@ -196,7 +236,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
           continue;
         }
       }
-    } else if (cur & kLeafBit) {
+    } else {
       // a, b, count, sub from the leaf's slot of the single-precision records (LDS when staged)
       const int4 li = *reinterpret_cast<const int4*>(&nodesF[cur & ~kLeafBit].ch[0]);
       if (SUB && fast && li.w >= 0) {
@@ -218,39 +258,9 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
       } else {
         for (int k = li.x; k < li.x + li.z; k++) test_exact(k);
       }
-    } else {
-      const DNodeF& nf = nodesF[cur];
-      float ta, tb;
-      bool ina, inb;
-      if (fast) {
-        ina = child_fast(nf.box[0], nf.ch[0], nodes, r, f, clip, h, &ta);
-        inb = child_fast(nf.box[1], nf.ch[1], nodes, r, f, clip, h, &tb);
-      } else {
-        ina = child_exact<EXACT>(nf.ch[0], nodes, r, h, &ta);
-        inb = child_exact<EXACT>(nf.ch[1], nodes, r, h, &tb);
-      }
-      if (ina && inb) {
-        const bool bFirst = !EXACT && tb < ta;   // EXACT: the reference's DFS order
-        stkN[sp * stride] = bFirst ? nf.ch[0].item : nf.ch[1].item;
-        stkT[sp * stride] = bFirst ? ta : tb;
-        sp++;
-        cur = bFirst ? nf.ch[1].item : nf.ch[0].item;
-        continue;
-      }
-      if (ina || inb) {
-        cur = ina ? nf.ch[0].item : nf.ch[1].item;
-        continue;
-      }
     }
-    bool popped = false;
-    while (sp > 0) {
-      --sp;
-      if (!EXACT && h.idx != INT_MAX && (double)stkT[sp * stride] > h.t * (1.0 + 1e-7) + 1e-7) continue;
-      cur = stkN[sp * stride];
-      popped = true;
-      break;
-    }
-    if (!popped) break;
+    live = pop();
+    if (!live) break;
   }
   return h.idx != INT_MAX;
 }
