@@ -244,16 +244,27 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
         continue;
       }
       if (f.on) {
-        // Pre-test boxes two at a time, so that their loads are in flight together.
+        // Pre-test the leaf's boxes, two loads in flight at a time, into a mask per 64 triangles;
+        // then the exact tests of the surviving triangles, so that the wave runs as many exact
+        // tests as its busiest lane needs rather than one wherever any lane's box passed.
         const int end = li.x + li.z;
-        for (int k = li.x; k < end; k += 2) {
-          const DBox32 b0 = tbox[k];
-          const bool two = k + 1 < end;
-          DBox32 b1 = b0;
-          if (two) b1 = tbox[k + 1];
-          float e;
-          if (box32_may_hit(b0.lo, b0.hi, f, clip, &e)) test_exact(k);
-          if (two && box32_may_hit(b1.lo, b1.hi, f, clip, &e)) test_exact(k + 1);
+        for (int k0 = li.x; k0 < end; k0 += 64) {
+          const int m = min(64, end - k0);
+          unsigned long long pass = 0;
+          for (int j = 0; j < m; j += 2) {
+            const DBox32 b0 = tbox[k0 + j];
+            const bool two = j + 1 < m;
+            DBox32 b1 = b0;
+            if (two) b1 = tbox[k0 + j + 1];
+            float e;
+            if (box32_may_hit(b0.lo, b0.hi, f, clip, &e)) pass |= 1ull << j;
+            if (two && box32_may_hit(b1.lo, b1.hi, f, clip, &e)) pass |= 2ull << j;
+          }
+          while (pass) {
+            const int j = __builtin_ctzll(pass);
+            pass &= pass - 1;
+            test_exact(k0 + j);
+          }
         }
       } else {
         for (int k = li.x; k < li.x + li.z; k++) test_exact(k);
