@@ -257,6 +257,7 @@ def main():
                     kernel_nodes_per_bounce=round(cnt["node_visits"] / bounces, 4),
                     kernel_exact_tri_tests_per_bounce=round(cnt["tri_tests"] / bounces, 4),
                     capture_sphere_tests_per_bounce=round(cnt["capture_tests"] / bounces, 4),
+                    capture_nodes_per_bounce=round(cnt["capture_nodes"] / bounces, 4),
                     fp64_box_tests_per_bounce=round(cnt["box_tests_fp64"] / bounces, 4),
                     valu_model=dict(achieved=round(valu_tflops, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s (FP64)",
                                     frac=round(valu_tflops / FP64_PEAK_TFLOPS, 5)),

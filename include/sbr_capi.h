@@ -127,6 +127,7 @@ typedef struct {
   int64_t windows;          /* launch windows the trace ran in (tree buffers bounded by window_nodes) */
   int64_t capture_tests;    /* reception-sphere tests of the capture's candidate pass (countTraversal) */
   int64_t box_tests_fp64;   /* child boxes the traversal decided with the FP64 slab test (countTraversal) */
+  int64_t capture_nodes;    /* tree nodes whose segment reaches the receivers' capture region */
 } sbr_stats;
 
 /* Launch configuration of the traversal kernel (no reference counterpart).  lds_scene: -1 stage

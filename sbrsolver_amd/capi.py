@@ -39,6 +39,7 @@ class Stats(C.Structure):
         ("node_visits", C.c_int64), ("tri_tests", C.c_int64),
         ("trace_lds", C.c_int32), ("trace_blocks_per_cu", C.c_int32), ("heavy_nodes", C.c_int64),
         ("windows", C.c_int64), ("capture_tests", C.c_int64), ("box_tests_fp64", C.c_int64),
+        ("capture_nodes", C.c_int64),
     ]
 
     def as_dict(self):

@@ -139,6 +139,13 @@ struct LevelRec {
   unsigned long long count;  // items (nodes) of the level
 };
 
+// Receiver in cell order of the receivers' grid, 32 B: position and index.
+struct alignas(32) RxCell {
+  double p[3];
+  int32_t r;
+  int32_t pad;
+};
+
 struct RxGrid {
   double bmin[3], bmax[3];
   double gx0, gy0, cell;
@@ -218,7 +225,8 @@ struct Engine {
   DBuf<LevelRec> dLevels;
   DBuf<uint8_t> dScratch;        // library temp storage (scan / sort)
   DBuf<double> dRx;
-  DBuf<int32_t> dCellStart, dCellRx;
+  DBuf<int32_t> dCellStart;
+  DBuf<RxCell> dCellRx;
   DBuf<uint8_t> dLos;
   DBuf<unsigned long long> dTabA, dTabB, dTabOrder;
   DBuf<int32_t> dTabRx;

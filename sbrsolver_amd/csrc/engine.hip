@@ -74,6 +74,7 @@ template struct DBuf<DSurf>;
 template struct DBuf<TNode>;
 template struct DBuf<TxDesc>;
 template struct DBuf<LevelRec>;
+template struct DBuf<RxCell>;
 template struct DBuf<unsigned int>;
 
 #define ENSURE(buf, cnt)                                                                         \
@@ -499,7 +500,7 @@ struct CaptureBufs {
   const DSurf* surf;
   const double* rx;
   const int32_t* cellStart;
-  const int32_t* cellRx;
+  const RxCell* cellRx;           // receivers in cell order (cellStart: first of each cell)
   const uint8_t* los;
   unsigned long long* tabA;
   unsigned long long* tabB;
@@ -645,10 +646,11 @@ __device__ __forceinline__ void append_record(const CaptureBufs& B, int32_t node
 // Candidate pass, one receiver against one node: reception sphere and the ancestor rule (a node
 // yields a path iff it captures the receiver and no proper ancestor does).  Only the capture halves
 // of the records are read.
-__device__ __forceinline__ void capture_one(const CaptureBufs& B, NodeCtx& c, int32_t r,
+__device__ __forceinline__ void capture_one(const CaptureBufs& B, NodeCtx& c, const RxCell& x,
                                             unsigned long long& myTests) {
   myTests++;
-  const V3 C = ld3(B.rx + 3 * (int64_t)r);
+  const V3 C = ld3(x.p);
+  const int32_t r = x.r;
   if (!sphere_hit(c.S, c.Tg, C, c.base, c.model, c.rc)) return;
   int32_t q = c.parent;
   while (q >= 0) {
@@ -771,13 +773,11 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
         const unsigned long long h = atomicAdd(&B.counters[5], 1ull);
         B.heavy[h] = (int32_t)n;
       } else {
-        for (int j = c.j0; j <= c.j1; j++) {
+        for (int j = c.j0; j <= c.j1; j++) {   // receivers of a row's cells i0..i1 are contiguous
           int i0, i1;
           if (!row_range(A, c, j, &i0, &i1)) continue;
-          for (int ci = i0; ci <= i1; ci++) {
-            const int cell = j * A.g.nx + ci;
-            for (int q = B.cellStart[cell]; q < B.cellStart[cell + 1]; q++) capture_one(B, c, B.cellRx[q], myTests);
-          }
+          const int q1 = B.cellStart[j * A.g.nx + i1 + 1];
+          for (int q = B.cellStart[j * A.g.nx + i0]; q < q1; q++) capture_one(B, c, B.cellRx[q], myTests);
         }
         if (c.emitted) store_key(B, c);
       }
@@ -1380,7 +1380,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     g.nx = std::max(1, (int)std::floor(ex / cell) + 1);
     g.ny = std::max(1, (int)std::floor(ey / cell) + 1);
     grid = g;
-    std::vector<int32_t> cellCount((size_t)g.nx * g.ny + 1, 0), cellOf(m), cellRx(m);
+    std::vector<int32_t> cellCount((size_t)g.nx * g.ny + 1, 0), cellOf(m);
+    std::vector<RxCell> cellRx(m);
     for (int r = 0; r < m; r++) {
       int ix = std::min(g.nx - 1, std::max(0, (int)std::floor((rx[3 * r] - g.gx0) / cell)));
       int iy = std::min(g.ny - 1, std::max(0, (int)std::floor((rx[3 * r + 1] - g.gy0) / cell)));
@@ -1389,7 +1390,12 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     }
     for (size_t c = 1; c < cellCount.size(); c++) cellCount[c] += cellCount[c - 1];
     std::vector<int32_t> fillPos(cellCount.begin(), cellCount.end() - 1);
-    for (int r = 0; r < m; r++) cellRx[fillPos[cellOf[r]]++] = r;
+    for (int r = 0; r < m; r++) {
+      RxCell& x = cellRx[fillPos[cellOf[r]]++];
+      for (int a = 0; a < 3; a++) x.p[a] = rx[3 * r + a];
+      x.r = r;
+      x.pad = 0;
+    }
     ENSURE(dRx, 3 * (size_t)m);
     ENSURE(dCellStart, cellCount.size());
     ENSURE(dCellRx, m);
@@ -1397,7 +1403,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     HIPCHK(hipMemcpyAsync(dRx.p, rx, 3 * sizeof(double) * m, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(dCellStart.p, cellCount.data(), sizeof(int32_t) * cellCount.size(), hipMemcpyHostToDevice,
                           stream));
-    HIPCHK(hipMemcpyAsync(dCellRx.p, cellRx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dCellRx.p, cellRx.data(), sizeof(RxCell) * m, hipMemcpyHostToDevice, stream));
     const size_t sm = stack_bytes(stackDepth(), kBlock);
     if (exact)
       k_los<true><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
@@ -1430,6 +1436,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     stats.n_k_trace = 0;
     stats.max_depth = 0;
     stats.heavy_nodes = 0;
+    stats.capture_nodes = 0;
     stats.capture_tests = 0;
     stats.windows = 0;
     winners = 0;
@@ -1552,6 +1559,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       HIPCHK(hipMemcpyAsync(dRecPrefix.p, prefix.data(), sizeof(unsigned long long) * prefix.size(),
                             hipMemcpyHostToDevice, stream));
       stats.heavy_nodes += (int64_t)nHeavy;
+      stats.capture_nodes += capNodes;
       // dedup pass over the records
       if (nRec > 0) {
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
