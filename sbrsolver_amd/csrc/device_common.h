@@ -96,20 +96,6 @@ __device__ __forceinline__ bool keep_box(double lo, double hi, const Hit& h) {
   return true;
 }
 
-// Whether the traversal enters a child, single-precision path: the grown box decides a miss (or a
-// child wholly behind the origin or beyond the best hit), the shrunk box decides a pass, and a ray
-// between the two takes the reference's FP64 slab test on the exact box.  *entry is a lower bound
-// of the t of every hit inside the child.
-__device__ __forceinline__ bool child_fast(const float* b, const DChildF& c, const DNode* __restrict__ nodes,
-                                           const RayPre& r, const RayF& f, float clip, Hit& h, float* entry) {
-  if (!box32_may_hit(b, b + 3, f, clip, entry)) return false;
-  if (c.inner && box32_crossed(b + 6, b + 9, f)) return true;
-  h.exact64++;
-  double lo, hi;
-  const DNode& n = nodes[c.item & ~kLeafBit];
-  return box_test(n.bmin, n.bmax, r, &lo, &hi) && keep_box<false>(lo, hi, h);
-}
-
 // FP64 path (EXACT, or a ray the single-precision tests do not cover).
 template <bool EXACT>
 __device__ __forceinline__ bool child_exact(const DChildF& c, const DNode* __restrict__ nodes, const RayPre& r,
@@ -195,8 +181,32 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
       float ta, tb;
       bool ina, inb;
       if (fast) {
-        ina = child_fast(nf.box[0], nf.ch[0], nodes, r, f, clip, h, &ta);
-        inb = child_fast(nf.box[1], nf.ch[1], nodes, r, f, clip, h, &tb);
+        // Whether the traversal enters a child, single-precision path: the grown box decides a
+        // miss (or a child wholly behind the origin or beyond the best hit), the shrunk box decides
+        // a pass, and a ray between the two takes the reference's FP64 slab test on the exact box.
+        // ta / tb: lower bounds of the t of every hit inside the child.
+        // Single-precision tests of both children first; then the FP64 slab test of each child
+        // whose ray lies between its grown and shrunk boxes, one child per pass, so that a wave
+        // runs the FP64 test once for all its lanes' first undecided child and a second time only
+        // where a lane has two.
+        ina = box32_may_hit(nf.box[0], nf.box[0] + 3, f, clip, &ta);
+        inb = box32_may_hit(nf.box[1], nf.box[1] + 3, f, clip, &tb);
+        bool und0 = ina && !(nf.ch[0].inner && box32_crossed(nf.box[0] + 6, nf.box[0] + 9, f));
+        bool und1 = inb && !(nf.ch[1].inner && box32_crossed(nf.box[1] + 6, nf.box[1] + 9, f));
+        while (und0 || und1) {
+          const bool first = und0;
+          h.exact64++;
+          double lo, hi;
+          const DNode& n = nodes[(first ? nf.ch[0].item : nf.ch[1].item) & ~kLeafBit];
+          const bool in = box_test(n.bmin, n.bmax, r, &lo, &hi) && keep_box<false>(lo, hi, h);
+          if (first) {
+            ina = in;
+            und0 = false;
+          } else {
+            inb = in;
+            und1 = false;
+          }
+        }
       } else {
         ina = child_exact<EXACT>(nf.ch[0], nodes, r, h, &ta);
         inb = child_exact<EXACT>(nf.ch[1], nodes, r, h, &tb);
