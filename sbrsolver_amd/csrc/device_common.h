@@ -262,13 +262,16 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
           const int m = min(64, end - k0);
           unsigned long long pass = 0;
           for (int j = 0; j < m; j += 2) {
+            // Both boxes loaded and tested unconditionally (the second index clamped), so that the
+            // two loads are in flight together.
+            const int j1 = min(j + 1, m - 1);
             const DBox32 b0 = tbox[k0 + j];
-            const bool two = j + 1 < m;
-            DBox32 b1 = b0;
-            if (two) b1 = tbox[k0 + j + 1];
-            float e;
-            if (box32_may_hit(b0.lo, b0.hi, f, clip, &e)) pass |= 1ull << j;
-            if (two && box32_may_hit(b1.lo, b1.hi, f, clip, &e)) pass |= 2ull << j;
+            const DBox32 b1 = tbox[k0 + j1];
+            float e0, e1;
+            const unsigned h0 = box32_may_hit(b0.lo, b0.hi, f, clip, &e0) ? 1u : 0u;
+            const unsigned h1 = box32_may_hit(b1.lo, b1.hi, f, clip, &e1) ? 1u : 0u;
+            pass |= (unsigned long long)h0 << j;
+            pass |= (unsigned long long)h1 << j1;
           }
           while (pass) {
             const int j = __builtin_ctzll(pass);
