@@ -261,17 +261,19 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
         for (int k0 = li.x; k0 < end; k0 += 64) {
           const int m = min(64, end - k0);
           unsigned long long pass = 0;
-          for (int j = 0; j < m; j += 2) {
-            // Both boxes loaded and tested unconditionally (the second index clamped), so that the
-            // two loads are in flight together.
-            const int j1 = min(j + 1, m - 1);
+          for (int j = 0; j < m; j += 4) {
+            // Four boxes loaded and tested unconditionally (indices clamped), so that their loads
+            // are in flight together.
+            const int j1 = min(j + 1, m - 1), j2 = min(j + 2, m - 1), j3 = min(j + 3, m - 1);
             const DBox32 b0 = tbox[k0 + j];
             const DBox32 b1 = tbox[k0 + j1];
-            float e0, e1;
-            const unsigned h0 = box32_may_hit(b0.lo, b0.hi, f, clip, &e0) ? 1u : 0u;
-            const unsigned h1 = box32_may_hit(b1.lo, b1.hi, f, clip, &e1) ? 1u : 0u;
-            pass |= (unsigned long long)h0 << j;
-            pass |= (unsigned long long)h1 << j1;
+            const DBox32 b2 = tbox[k0 + j2];
+            const DBox32 b3 = tbox[k0 + j3];
+            float e;
+            pass |= (unsigned long long)(box32_may_hit(b0.lo, b0.hi, f, clip, &e) ? 1u : 0u) << j;
+            pass |= (unsigned long long)(box32_may_hit(b1.lo, b1.hi, f, clip, &e) ? 1u : 0u) << j1;
+            pass |= (unsigned long long)(box32_may_hit(b2.lo, b2.hi, f, clip, &e) ? 1u : 0u) << j2;
+            pass |= (unsigned long long)(box32_may_hit(b3.lo, b3.hi, f, clip, &e) ? 1u : 0u) << j3;
           }
           while (pass) {
             const int j = __builtin_ctzll(pass);
