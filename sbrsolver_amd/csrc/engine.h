@@ -99,7 +99,7 @@ struct TxDesc {
   int32_t launchStride, launchOffset;
   int32_t maxR, maxT, maxTotal;
   int32_t useFresnel, antennaType, receptionModel;
-  int32_t pad[2];
+  double capK;               // rcoef / (1 - rcoef) rounded up (INFINITY for rcoef >= 1): capture prefilter
 };
 
 // ---- interaction tree ---------------------------------------------------------------------------

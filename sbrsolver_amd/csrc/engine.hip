@@ -165,8 +165,7 @@ __global__ void k_ico_dirs(const double* __restrict__ verts, int64_t nv, double*
 // Capture region of a segment S->Tg: a capture at C has its closest point P within
 // (base + |C - ref|) * rc / (1 - rc) of C, with base/ref per reception model, so only the part of
 // the segment inside the receivers' box grown by that bound R can capture.  Returns false when no
-// part is; otherwise the clipped segment P0->P1 and R.  One function for the bounce kernel (which
-// lists the nodes that can capture) and the capture pass (which scans them), so both decide alike.
+// part is; otherwise the clipped segment P0->P1 and R.
 struct SegClip {
   V3 P0, P1;
   double R;
@@ -200,6 +199,28 @@ __device__ __forceinline__ bool seg_clip(const RxGrid& g, V3 S, V3 Tg, double ba
   }
   o.P0 = vadd(S, vmul(D, u0));
   o.P1 = vadd(S, vmul(D, u1));
+  return true;
+}
+
+// Necessary condition for seg_clip to pass, cheap enough for every bounce: the segment's bounding
+// box meets the receivers' box grown by an upper bound of seg_clip's radius, with margins above
+// seg_clip's rounding and its 1e-12 widening of the segment parameter.  capK = rc / (1 - rc) rounded
+// up (INFINITY for rc >= 1, where seg_clip passes everything).
+__device__ __forceinline__ bool may_capture(const RxGrid& g, V3 S, V3 Tg, double base, int model, double capK) {
+  double far2 = 0.0;
+  for (int a = 0; a < 3; a++) {
+    const double sa = model == 0 ? 0.0 : vget(S, a);
+    const double e = fmax(fabs(g.bmin[a] - sa), fabs(g.bmax[a] - sa));
+    far2 += e * e;
+  }
+  const double R = (base + sqrt(far2)) * capK * (1.0 + 2e-6) + 2e-6;
+  if (!(R < 1e300)) return true;
+  for (int a = 0; a < 3; a++) {
+    const double s = vget(S, a), t = vget(Tg, a);
+    const double lo = fmin(s, t), hi = fmax(s, t);
+    const double m = 1e-9 * (fabs(lo) + fabs(hi) + fabs(g.bmin[a]) + fabs(g.bmax[a]) + R) + 1e-12;
+    if (hi < g.bmin[a] - R - m || lo > g.bmax[a] + R + m) return false;
+  }
   return true;
 }
 
@@ -389,14 +410,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
       for (int w = 0; w < 8; w++) dstw[w] = srcw[w];
     }
-    // Nodes whose segment reaches the receivers' capture region, listed (wave-aggregated append)
-    // so that the capture pass reads only those records.
+    // Nodes whose segment may reach the receivers' capture region, listed (wave-aggregated append)
+    // so that the capture pass reads only those records (it applies seg_clip itself).
     if (A.capture) {
       bool capt = false;
       if (live) {
         const TxDesc& T = txs[nd.txi];
-        SegClip sc;
-        capt = seg_clip(A.g, ld3(nd.src), ld3(nd.tgt), nd.base, T.receptionModel, T.rcoef, sc);
+        capt = may_capture(A.g, ld3(nd.src), ld3(nd.tgt), nd.base, T.receptionModel, T.capK);
       }
       const unsigned long long cm = __ballot(capt);
       if (cm) {
@@ -1334,6 +1354,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       double alpha = (69.0 * 3.141592653589793 / 180.0) / std::ldexp(1.0, p.tess);
       T.rcoef = alpha / std::sqrt(3.0);
     }
+    T.capK = T.rcoef < 1.0 ? T.rcoef / (1.0 - T.rcoef) * (1.0 + 1e-12) : INFINITY;
     int64_t nl = 0;
     int rc = rays_for(p.tess, &T.rays, &nl);
     if (rc) return rc;
