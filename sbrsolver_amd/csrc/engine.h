@@ -161,6 +161,29 @@ struct DBuf {
   ~DBuf() { release(); }
 };
 
+// Page-locked host memory (hipHostMalloc) for the per-trace copies, so that they go by DMA without a
+// pageable bounce.  Grown only while the stream is idle (at the start of a trace).
+struct HostPinned {
+  unsigned char* p = nullptr;
+  size_t n = 0;  // bytes
+  HostPinned() = default;
+  HostPinned(const HostPinned&) = delete;
+  HostPinned& operator=(const HostPinned&) = delete;
+  bool ensure(size_t bytes);
+  void release();
+  ~HostPinned() { release(); }
+  template <typename T>
+  T* at(size_t off) { return reinterpret_cast<T*>(p + off); }
+};
+// Offsets in the small staging area: bookkeeping read back or sent per launch window.
+constexpr size_t kPinLevels = 0;         // LevelRec[kMaxDepth + 2] read back
+constexpr size_t kPinLevels0 = 1024;     // LevelRec[kMaxDepth + 2] sent
+constexpr size_t kPinTraceCtr = 2048;    // 8 counters of the bounce kernel
+constexpr size_t kPinCapCtr = 2176;      // capture counters
+constexpr size_t kPinParts = 4096;       // record partition counters (kRecParts * kRecStride words)
+constexpr size_t kPinPrefix = kPinParts + 8 * kRecParts * kRecStride;   // kRecParts + 1 prefix words
+constexpr size_t kPinSmall = kPinPrefix + 8 * (kRecParts + 1) + 64;
+
 // Launch configuration of the traversal kernel (sbr_set_tuning, or chosen by sbr_autotune).
 struct Tuning {
   int ldsScene = -1;      // -1: LDS staging when it fits and keeps occupancy, 1: whenever it fits, 0: never
@@ -225,6 +248,7 @@ struct Engine {
   DBuf<LevelRec> dLevels;
   DBuf<uint8_t> dScratch;        // library temp storage (scan / sort)
   DBuf<double> dRx;
+  HostPinned pinSmall, pinIn, pinOut;   // bookkeeping; receivers in; fields out
   DBuf<int32_t> dCellStart;
   DBuf<RxCell> dCellRx;
   DBuf<uint8_t> dLos;

@@ -59,6 +59,25 @@ void DBuf<T>::release() {
   n = 0;
 }
 
+bool HostPinned::ensure(size_t bytes) {
+  if (bytes <= n && p) return true;
+  release();
+  const size_t cap = bytes + bytes / 4 + 4096;
+  if (hipHostMalloc((void**)&p, cap, hipHostMallocDefault) != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    return false;
+  }
+  n = cap;
+  return true;
+}
+void HostPinned::release() {
+  if (p) (void)hipHostFree(p);
+  p = nullptr;
+  n = 0;
+}
+static_assert(sizeof(LevelRec) * (kMaxDepth + 2) <= 1024, "level records fit their staging slot");
+
 template struct DBuf<double>;
 template struct DBuf<int32_t>;
 template struct DBuf<uint8_t>;
@@ -807,7 +826,8 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
 }
 
 // Wave per heavy node: the node's candidate cells, flattened row by row, are dealt to the lanes.
-__global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBufs B, int64_t nHeavy) {
+__global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBufs B) {
+  const int64_t nHeavy = (int64_t)B.counters[5];   // appended by k_capture
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -1253,9 +1273,10 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   const unsigned grid = (unsigned)std::min<int64_t>((int64_t)numCU * perCU, blocks_for(std::max<int64_t>(nRoot, 1), kBlock));
   stats.trace_lds = lds ? 1 : 0;
   stats.trace_blocks_per_cu = perCU;
-  std::vector<LevelRec> lv0(kMaxDepth + 2, LevelRec{0, 0});
+  LevelRec* lv0 = pinSmall.at<LevelRec>(kPinLevels0);
+  for (int d = 0; d < kMaxDepth + 2; d++) lv0[d] = LevelRec{0, 0};
   lv0[0].count = (unsigned long long)nRoot;
-  HIPCHK(hipMemcpyAsync(dLevels.p, lv0.data(), sizeof(LevelRec) * lv0.size(), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dLevels.p, lv0, sizeof(LevelRec) * (kMaxDepth + 2), hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemsetAsync(dTraceCounters.p, 0, 8 * sizeof(unsigned long long), stream));
   hipEvent_t kev[2];
   HIPCHK(hipEventCreate(&kev[0]));
@@ -1271,10 +1292,12 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(kev[1], stream));
-  std::vector<LevelRec> lv(kMaxDepth + 2);
-  unsigned long long ctr[8];
-  HIPCHK(hipMemcpyAsync(lv.data(), dLevels.p, sizeof(LevelRec) * lv.size(), hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipMemcpyAsync(ctr, dTraceCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
+  const LevelRec* lv = pinSmall.at<LevelRec>(kPinLevels);
+  const unsigned long long* ctr = pinSmall.at<unsigned long long>(kPinTraceCtr);
+  HIPCHK(hipMemcpyAsync(pinSmall.at<LevelRec>(kPinLevels), dLevels.p, sizeof(LevelRec) * (kMaxDepth + 2),
+                        hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinTraceCtr), dTraceCounters.p, 8 * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
   float kms = 0;
   HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
@@ -1313,6 +1336,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     if ((P[t].q1MeshIndexing != 0) != (P[0].q1MeshIndexing != 0))
       return fail(SBR_E_ARG, "q1MeshIndexing must be the same for every transmitter of a batch");
   HIPCHK(hipSetDevice(device));
+  if (!pinSmall.ensure(kPinSmall)) return fail(SBR_E_NOMEM, "pinned staging");
   haveResult = false;
   stats = sbr_stats{};
   auto t0 = std::chrono::steady_clock::now();
@@ -1401,16 +1425,22 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     g.nx = std::max(1, (int)std::floor(ex / cell) + 1);
     g.ny = std::max(1, (int)std::floor(ey / cell) + 1);
     grid = g;
-    std::vector<int32_t> cellCount((size_t)g.nx * g.ny + 1, 0), cellOf(m);
-    std::vector<RxCell> cellRx(m);
+    // Staged in page-locked memory: receivers in cell order, cell starts, receivers as given.
+    const size_t nCell = (size_t)g.nx * g.ny + 1;
+    const size_t offStart = sizeof(RxCell) * (size_t)m, offRx = offStart + ((sizeof(int32_t) * nCell + 63) & ~size_t(63));
+    if (!pinIn.ensure(offRx + 3 * sizeof(double) * (size_t)m)) return fail(SBR_E_NOMEM, "pinned receiver staging");
+    RxCell* cellRx = pinIn.at<RxCell>(0);
+    int32_t* cellCount = pinIn.at<int32_t>(offStart);
+    std::fill(cellCount, cellCount + nCell, 0);
+    std::vector<int32_t> cellOf(m);
     for (int r = 0; r < m; r++) {
       int ix = std::min(g.nx - 1, std::max(0, (int)std::floor((rx[3 * r] - g.gx0) / cell)));
       int iy = std::min(g.ny - 1, std::max(0, (int)std::floor((rx[3 * r + 1] - g.gy0) / cell)));
       cellOf[r] = iy * g.nx + ix;
       cellCount[cellOf[r] + 1]++;
     }
-    for (size_t c = 1; c < cellCount.size(); c++) cellCount[c] += cellCount[c - 1];
-    std::vector<int32_t> fillPos(cellCount.begin(), cellCount.end() - 1);
+    for (size_t c = 1; c < nCell; c++) cellCount[c] += cellCount[c - 1];
+    std::vector<int32_t> fillPos(cellCount, cellCount + nCell - 1);
     for (int r = 0; r < m; r++) {
       RxCell& x = cellRx[fillPos[cellOf[r]]++];
       for (int a = 0; a < 3; a++) x.p[a] = rx[3 * r + a];
@@ -1418,13 +1448,13 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       x.pad = 0;
     }
     ENSURE(dRx, 3 * (size_t)m);
-    ENSURE(dCellStart, cellCount.size());
+    ENSURE(dCellStart, nCell);
     ENSURE(dCellRx, m);
     ENSURE(dLos, nOut);
-    HIPCHK(hipMemcpyAsync(dRx.p, rx, 3 * sizeof(double) * m, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(dCellStart.p, cellCount.data(), sizeof(int32_t) * cellCount.size(), hipMemcpyHostToDevice,
-                          stream));
-    HIPCHK(hipMemcpyAsync(dCellRx.p, cellRx.data(), sizeof(RxCell) * m, hipMemcpyHostToDevice, stream));
+    memcpy(pinIn.at<double>(offRx), rx, 3 * sizeof(double) * m);
+    HIPCHK(hipMemcpyAsync(dRx.p, pinIn.at<double>(offRx), 3 * sizeof(double) * m, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dCellStart.p, cellCount, sizeof(int32_t) * nCell, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(dCellRx.p, cellRx, sizeof(RxCell) * m, hipMemcpyHostToDevice, stream));
     const size_t sm = stack_bytes(stackDepth(), kBlock);
     if (exact)
       k_los<true><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
@@ -1533,7 +1563,9 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       HIPCHK(hipEventRecord(kev[0], stream));
       // candidate pass: (node, receiver) records; redone with a larger record buffer if it fills
       unsigned long long nRec = 0, nHeavy = 0;
-      std::vector<unsigned long long> partCount(kRecParts * kRecStride), prefix(kRecParts + 1, 0);
+      const unsigned long long* partCount = pinSmall.at<unsigned long long>(kPinParts);
+      unsigned long long* prefix = pinSmall.at<unsigned long long>(kPinPrefix);
+      const unsigned long long* c4 = pinSmall.at<unsigned long long>(kPinCapCtr);
       ENSURE(dRecCount, kRecParts * kRecStride);
       ENSURE(dRecPrefix, kRecParts + 1);
       for (int recAttempt = 0;; recAttempt++) {
@@ -1551,18 +1583,18 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
           k_capture<<<blocks_for(capNodes, 256), 256, 0, stream>>>(CA, CB);
           HIPCHK(hipGetLastError());
         }
-        HIPCHK(hipMemcpyAsync(&nHeavy, dCounters.p + 5, sizeof nHeavy, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        if (nHeavy > 0) {
-          const int64_t waves = std::min<int64_t>((int64_t)nHeavy, (int64_t)numCU * 32);
-          k_capture_heavy<<<blocks_for(waves * 64, 256), 256, 0, stream>>>(CA, CB, (int64_t)nHeavy);
+        // wave per heavy node; the count is read on the device (no host round trip)
+        if (capNodes > 0) {
+          k_capture_heavy<<<blocks_for((int64_t)numCU * 32 * 64, 256), 256, 0, stream>>>(CA, CB);
           HIPCHK(hipGetLastError());
         }
-        unsigned long long c4[7];
-        HIPCHK(hipMemcpyAsync(c4, dCounters.p, sizeof c4, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipMemcpyAsync(partCount.data(), dRecCount.p, sizeof(unsigned long long) * partCount.size(),
+        HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinCapCtr), dCounters.p, 7 * sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinParts), dRecCount.p,
+                              sizeof(unsigned long long) * kRecParts * kRecStride, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
+        nHeavy = c4[5];
+        prefix[0] = 0;
         unsigned long long maxPart = 0;
         for (int p = 0; p < kRecParts; p++) {
           const unsigned long long cnt = partCount[(size_t)p * kRecStride];
@@ -1577,7 +1609,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         if (recAttempt > 4) return fail(SBR_E_LIMIT, "capture record buffer could not be sized");
         recCap = std::max<size_t>(recCap * 2, (size_t)(maxPart + maxPart / 4 + 1024) * kRecParts);
       }
-      HIPCHK(hipMemcpyAsync(dRecPrefix.p, prefix.data(), sizeof(unsigned long long) * prefix.size(),
+      HIPCHK(hipMemcpyAsync(dRecPrefix.p, prefix, sizeof(unsigned long long) * (kRecParts + 1),
                             hipMemcpyHostToDevice, stream));
       stats.heavy_nodes += (int64_t)nHeavy;
       stats.capture_nodes += capNodes;
@@ -1590,8 +1622,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         HIPCHK(hipGetLastError());
       }
       HIPCHK(hipEventRecord(kev[1], stream));
-      unsigned long long ctr[4];
-      HIPCHK(hipMemcpyAsync(ctr, dCounters.p, sizeof ctr, hipMemcpyDeviceToHost, stream));
+      unsigned long long* ctr = pinSmall.at<unsigned long long>(kPinCapCtr);
+      HIPCHK(hipMemcpyAsync(ctr, dCounters.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
       float kms = 0;
       HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
@@ -1686,9 +1718,12 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   stats.ms_capture += a;   // receiver binning upload and line of sight
   stats.launches = nRoot;
   hField.assign(6 * (size_t)nOut, 0.0);
-  if (nOut > 0)
-    HIPCHK(hipMemcpyAsync(hField.data(), dField.p, 6 * sizeof(double) * nOut, hipMemcpyDeviceToHost, stream));
+  if (nOut > 0) {
+    if (!pinOut.ensure(6 * sizeof(double) * nOut)) return fail(SBR_E_NOMEM, "pinned field staging");
+    HIPCHK(hipMemcpyAsync(pinOut.p, dField.p, 6 * sizeof(double) * nOut, hipMemcpyDeviceToHost, stream));
+  }
   HIPCHK(hipStreamSynchronize(stream));
+  if (nOut > 0) memcpy(hField.data(), pinOut.p, 6 * sizeof(double) * nOut);
   stats.ms_total = ms_since(t0);
   lastParams.assign(P, P + nt);
   haveResult = true;
