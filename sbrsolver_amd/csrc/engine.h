@@ -255,7 +255,8 @@ struct Engine {
   DBuf<unsigned long long> dTabA, dTabB, dTabOrder;
   DBuf<int32_t> dTabRx;
   DBuf<int32_t> dHeavy;          // nodes left to the wave-per-node capture
-  DBuf<int32_t> dCand;           // nodes whose segment reaches the receivers' capture region
+  DBuf<int32_t> dCand;           // nodes whose segment may reach the receivers' capture region
+  DBuf<uint8_t> dCandFlag;       // per node: 1 if listed (written by the bounce kernel)
   DBuf<int2> dRec;               // (node, receiver) records of the capture's candidate pass
   DBuf<unsigned long long> dRecCount, dRecPrefix;   // per-partition record counts / prefix
   DBuf<unsigned long long> dNodeKey;   // dedup and order keys of the nodes with records
@@ -273,7 +274,7 @@ struct Engine {
   sbr_stats stats{};
   int nrx = 0, ntx = 0;
   int64_t nodes = 0;
-  int64_t capNodes = 0;          // of them listed for the capture
+  int64_t capNodes = 0;          // of them listed for the capture (known after the capture pass)
   int64_t winners = 0;
   size_t nodeCap = 0;
   size_t tabCapHint = 0;   // dedup table size of the last trace

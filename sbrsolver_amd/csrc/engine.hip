@@ -255,8 +255,8 @@ struct TraceArgs {
   RxGrid g;
 };
 
-// Flags: [0] bit 0 node overflow, bit 1 queue overflow; [1] nodes listed for the capture; [5] FP64
-// box tests, [6] node visits, [7] triangle tests (count mode).
+// Flags: [0] bit 0 node overflow, bit 1 queue overflow; [5] FP64 box tests, [6] node visits, [7]
+// triangle tests (count mode).  [1] receives the number of listed nodes after the trace.
 template <bool EXACT, bool LDS, bool SUB>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_trace(TraceArgs A, const TxDesc* __restrict__ txs,
                                                   const DNode* __restrict__ nodes, const DNodeF* __restrict__ gNodesF,
@@ -266,7 +266,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
                                                   const double* __restrict__ triN, const int32_t* __restrict__ qIn,
                                                   int32_t* __restrict__ qOut, TNode* __restrict__ tree,
                                                   LevelRec* __restrict__ lv, unsigned long long* __restrict__ flags,
-                                                  int32_t* __restrict__ cand, int countTraversal) {
+                                                  uint8_t* __restrict__ candFlag, int countTraversal) {
   extern __shared__ __align__(16) unsigned char smem[];
   const DNodeF* nodesF = gNodesF;
   const DTri* tris = gTris;
@@ -429,22 +429,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
 #pragma unroll
       for (int w = 0; w < 8; w++) dstw[w] = srcw[w];
     }
-    // Nodes whose segment may reach the receivers' capture region, listed (wave-aggregated append)
-    // so that the capture pass reads only those records (it applies seg_clip itself).
-    if (A.capture) {
-      bool capt = false;
-      if (live) {
-        const TxDesc& T = txs[nd.txi];
-        capt = may_capture(A.g, ld3(nd.src), ld3(nd.tgt), nd.base, T.receptionModel, T.capK);
-      }
-      const unsigned long long cm = __ballot(capt);
-      if (cm) {
-        const int leader = __ffsll(cm) - 1;
-        unsigned long long cb = 0;
-        if (lane == leader) cb = atomicAdd(&flags[1], (unsigned long long)__popcll(cm));
-        cb = __shfl(cb, leader, 64);
-        if (capt) cand[cb + __popcll(cm & ((1ull << lane) - 1ull))] = (int32_t)nodeIdx;
-      }
+    // Whether the node's segment may reach the receivers' capture region: a flag per node (no
+    // atomics here), compacted into the capture pass's list after the last depth.
+    if (A.capture && live) {
+      const TxDesc& T = txs[nd.txi];
+      candFlag[nodeIdx] = may_capture(A.g, ld3(nd.src), ld3(nd.tgt), nd.base, T.receptionModel, T.capK) ? 1 : 0;
     }
   }
   if (countTraversal) {
@@ -506,7 +495,7 @@ __global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restric
 // 4. Capture, image correction, duplicate removal
 // =============================================================================================
 struct CaptureArgs {
-  int64_t nodes;   // listed nodes
+  int64_t nodes;   // tree nodes (bound of the listed ones)
   int32_t nrx;
   RxGrid g;
   unsigned long long tabMask;
@@ -557,7 +546,8 @@ struct CaptureBufs {
   const unsigned long long* recPrefix;   // dedup pass: records before each partition
   int countTests;
   unsigned long long* keys;       // per node with records: seqA, seqB, order key (3 words)
-  const int32_t* cand;            // nodes listed by the bounce kernel (CaptureArgs::nodes of them)
+  const int32_t* cand;            // nodes listed by the bounce kernel
+  const unsigned long long* nCand;   // their number
 };
 
 // Returns false when no receiver can be captured by the node.
@@ -797,9 +787,9 @@ __device__ __forceinline__ void flush_tests(const CaptureBufs& B, unsigned long 
 constexpr int kHeavyCells = 48;
 
 __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nc = (int64_t)B.nCand[0];   // listed nodes (written after the bounce loop)
   unsigned long long myTests = 0;
-  if (k < A.nodes) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += (int64_t)gridDim.x * blockDim.x) {
     const int32_t n = B.cand[k];
     NodeCtx c;
     if (node_setup(A, B, n, c)) {
@@ -1288,7 +1278,7 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
     const int32_t* qin = dQueue[d & 1].p;
     int32_t* qout = dQueue[(d + 1) & 1].p;
     kern<<<grid, kBlock, smem, stream>>>(A, dTx.p, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p, dSubTri.p, dTriN.p, qin,
-                                         qout, dTree.p, dLevels.p, dTraceCounters.p, dCand.p, count ? 1 : 0);
+                                         qout, dTree.p, dLevels.p, dTraceCounters.p, dCandFlag.p, count ? 1 : 0);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(kev[1], stream));
@@ -1313,7 +1303,16 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   }
   *ms = kms;
   nodes = (int64_t)total;
-  capNodes = capG ? (int64_t)ctr[1] : 0;
+  capNodes = 0;
+  if (capG && nodes > 0) {   // listed nodes, in node order; the count stays on the device
+    size_t tb = 0;
+    hipcub::CountingInputIterator<int32_t> idx(0);
+    HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, tb, idx, dCandFlag.p, dCand.p, dTraceCounters.p + 1, (int)nodes,
+                                         stream));
+    ENSURE(dScratch, tb);
+    HIPCHK(hipcub::DeviceSelect::Flagged(dScratch.p, tb, idx, dCandFlag.p, dCand.p, dTraceCounters.p + 1, (int)nodes,
+                                         stream));
+  }
   stats.ms_k_trace += kms;
   stats.n_k_trace += maxDepth + 1;
   stats.bounces += nodes;
@@ -1522,9 +1521,12 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         ENSURE(dTree, nodeCap);
         ENSURE(dQueue[0], nodeCap);
         ENSURE(dQueue[1], nodeCap);
-        if (m > 0) ENSURE(dCand, nodeCap);
+        if (m > 0) {
+          ENSURE(dCand, nodeCap);
+          ENSURE(dCandFlag, nodeCap);
+        }
         nodeCap = std::min(dTree.n, std::min(dQueue[0].n, dQueue[1].n));
-        if (m > 0) nodeCap = std::min(nodeCap, dCand.n);
+        if (m > 0) nodeCap = std::min(nodeCap, std::min(dCand.n, dCandFlag.n));
         int64_t need = 0;
         int rc = trace_stage(txs, ws, W, maxDepth, exact, count, m > 0 ? &g : nullptr, &need, &msT);
         if (rc == SBR_OK) break;
@@ -1553,10 +1555,10 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       hipEvent_t kev[4];
       for (auto& e : kev) HIPCHK(hipEventCreate(&e));
       struct KevGuard { hipEvent_t* e; ~KevGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } kguard{kev};
-      ENSURE(dHeavy, std::max<int64_t>(capNodes, 1));
+      ENSURE(dHeavy, std::max<int64_t>(nodes, 1));
       ENSURE(dNodeKey, 3 * (size_t)std::max<int64_t>(nodes, 1));
       CaptureArgs CA{};
-      CA.nodes = capNodes;
+      CA.nodes = nodes;   // bound of the listed nodes; their count is read on the device
       CA.nrx = m;
       CA.g = g;
       CA.tabMask = cap - 1;
@@ -1578,22 +1580,25 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         HIPCHK(hipMemsetAsync(dCounters.p + 5, 0, 2 * sizeof(unsigned long long), stream));
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
                        dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p, nullptr,
-                       count ? 1 : 0, dNodeKey.p, dCand.p};
-        if (capNodes > 0) {
-          k_capture<<<blocks_for(capNodes, 256), 256, 0, stream>>>(CA, CB);
+                       count ? 1 : 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
+        if (nodes > 0) {
+          k_capture<<<(unsigned)std::min<int64_t>(blocks_for(nodes, 256), (int64_t)numCU * 16), 256, 0, stream>>>(CA, CB);
           HIPCHK(hipGetLastError());
         }
         // wave per heavy node; the count is read on the device (no host round trip)
-        if (capNodes > 0) {
+        if (nodes > 0) {
           k_capture_heavy<<<blocks_for((int64_t)numCU * 32 * 64, 256), 256, 0, stream>>>(CA, CB);
           HIPCHK(hipGetLastError());
         }
         HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinCapCtr), dCounters.p, 7 * sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinTraceCtr) + 1, dTraceCounters.p + 1,
+                              sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinParts), dRecCount.p,
                               sizeof(unsigned long long) * kRecParts * kRecStride, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         nHeavy = c4[5];
+        capNodes = (int64_t)pinSmall.at<unsigned long long>(kPinTraceCtr)[1];
         prefix[0] = 0;
         unsigned long long maxPart = 0;
         for (int p = 0; p < kRecParts; p++) {
@@ -1617,7 +1622,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       if (nRec > 0) {
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
                        dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, recCap / kRecParts, dRecCount.p,
-                       dRecPrefix.p, 0, dNodeKey.p, dCand.p};
+                       dRecPrefix.p, 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
         k_dedup<<<blocks_for((int64_t)nRec, 256), 256, 0, stream>>>(CA, CB, (int64_t)nRec);
         HIPCHK(hipGetLastError());
       }
