@@ -1337,6 +1337,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   HIPCHK(hipSetDevice(device));
   if (!pinSmall.ensure(kPinSmall)) return fail(SBR_E_NOMEM, "pinned staging");
   haveResult = false;
+  lastParams.assign(P, P + nt);   // read by the path export of every window (multi-window traces)
   stats = sbr_stats{};
   auto t0 = std::chrono::steady_clock::now();
   hipEvent_t ev[5];
@@ -1730,7 +1731,6 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   HIPCHK(hipStreamSynchronize(stream));
   if (nOut > 0) memcpy(hField.data(), pinOut.p, 6 * sizeof(double) * nOut);
   stats.ms_total = ms_since(t0);
-  lastParams.assign(P, P + nt);
   haveResult = true;
   return SBR_OK;
 }

@@ -51,3 +51,25 @@ def test_batch_windows_equal_single_window():
         pb = b.paths()
         for k in pa:
             assert np.array_equal(pa[k], pb[k]), k
+
+
+def test_first_trace_of_a_context_in_windows():
+    """A context whose first trace runs in several launch windows (the path export of each window
+    reads the trace's own parameters) equals a single-window trace on another context."""
+    cfg = scenes.config4(n_tx=3, rx_n=30)
+    plist = [make_params(**dict(t, tess=min(t["tess"], 5))) for t in cfg["txs"]]
+    with Context(0) as ref:
+        ref.scene_from_soup(*cfg["soup"])
+        ref.materials(cfg["materials"])
+        a = ref.trace_batch(plist, cfg["rx"])
+        fa, pa = a.fields, a.paths()
+    with Context(0) as ctx:
+        ctx.scene_from_soup(*cfg["soup"])
+        ctx.materials(cfg["materials"])
+        ctx.set_tuning(window_nodes=30_000)
+        b = ctx.trace_batch(plist, cfg["rx"])
+        assert b.stats["windows"] >= 2
+        assert np.array_equal(fa, b.fields, equal_nan=True)
+        pb = b.paths()
+        for k in pa:
+            assert np.array_equal(pa[k], pb[k]), k
