@@ -199,9 +199,15 @@ __device__ __forceinline__ bool seg_clip(const RxGrid& g, V3 S, V3 Tg, double ba
     double e = fmax(fabs(g.bmin[a] - sa), fabs(g.bmax[a] - sa));
     far2 += e * e;
   }
-  double R = (base + sqrt(far2)) * rc / (1.0 - rc);
+  double R = rc < 1.0 ? (base + sqrt(far2)) * rc / (1.0 - rc) : INFINITY;
+  // A second bound through the segment itself: the unfolded length is base + |cp| <= base +
+  // max(|S|, |Tg|) (model 0, cp on the segment) or base + t0 |Tg - S| <= base + |Tg - S| (model 1),
+  // so no capture is farther than (base + that) * rc from the segment.
+  const double seg = model == 0 ? fmax(vnorm(S), vnorm(Tg)) : vnorm(D);
+  const double Rs = (base + seg) * rc;
+  if (Rs < R) R = Rs;
   o.R = R * (1.0 + 1e-6) + 1e-6;
-  o.all = !(o.R < 1e300) || rc >= 1.0;
+  o.all = !(o.R < 1e300);
   double u0 = 0.0, u1 = 1.0;
   if (!o.all) {
     for (int a = 0; a < 3; a++) {
@@ -224,15 +230,20 @@ __device__ __forceinline__ bool seg_clip(const RxGrid& g, V3 S, V3 Tg, double ba
 // Necessary condition for seg_clip to pass, cheap enough for every bounce: the segment's bounding
 // box meets the receivers' box grown by an upper bound of seg_clip's radius, with margins above
 // seg_clip's rounding and its 1e-12 widening of the segment parameter.  capK = rc / (1 - rc) rounded
-// up (INFINITY for rc >= 1, where seg_clip passes everything).
-__device__ __forceinline__ bool may_capture(const RxGrid& g, V3 S, V3 Tg, double base, int model, double capK) {
+// up (INFINITY for rc >= 1).
+__device__ __forceinline__ bool may_capture(const RxGrid& g, V3 S, V3 Tg, double base, int model, double rc,
+                                            double capK) {
   double far2 = 0.0;
   for (int a = 0; a < 3; a++) {
     const double sa = model == 0 ? 0.0 : vget(S, a);
     const double e = fmax(fabs(g.bmin[a] - sa), fabs(g.bmax[a] - sa));
     far2 += e * e;
   }
-  const double R = (base + sqrt(far2)) * capK * (1.0 + 2e-6) + 2e-6;
+  double R = (base + sqrt(far2)) * capK;
+  const double seg = model == 0 ? fmax(vnorm(S), vnorm(Tg)) : vnorm(vsub(Tg, S));
+  const double Rs = (base + seg) * rc;
+  if (Rs < R) R = Rs;
+  R = R * (1.0 + 2e-6) + 2e-6;
   if (!(R < 1e300)) return true;
   for (int a = 0; a < 3; a++) {
     const double s = vget(S, a), t = vget(Tg, a);
@@ -433,7 +444,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     // atomics here), compacted into the capture pass's list after the last depth.
     if (A.capture && live) {
       const TxDesc& T = txs[nd.txi];
-      candFlag[nodeIdx] = may_capture(A.g, ld3(nd.src), ld3(nd.tgt), nd.base, T.receptionModel, T.capK) ? 1 : 0;
+      candFlag[nodeIdx] =
+          may_capture(A.g, ld3(nd.src), ld3(nd.tgt), nd.base, T.receptionModel, T.rcoef, T.capK) ? 1 : 0;
     }
   }
   if (countTraversal) {
