@@ -164,24 +164,43 @@ DEFAULT_MATERIALS = np.array([[1.8e9, 221.13, 0.05, 1.0, 0.0, 1e6]], dtype=np.fl
 
 
 class Result:
+    """The context's last trace.  Statistics are read at once; fields, paths and trees when first
+    asked for, which must be before the next trace on the same context (SbrError otherwise)."""
+
     def __init__(self, ctx, nrx, ntx=1):
         self.ctx, self.nrx, self.ntx = ctx, nrx, ntx
+        self._gen = ctx._gen
         st = Stats()
         ctx._chk(lib().sbr_result_stats(ctx.h, C.byref(st)))
         self.stats = st.as_dict()
         self.bounces = st.bounces
-        f = np.zeros((ntx * nrx, 3, 2))
-        ctx._chk(lib().sbr_result_fields(ctx.h, _a(f), ntx * nrx))
-        f = f[..., 0] + 1j * f[..., 1]
-        self.fields = f if ntx == 1 else f.reshape(ntx, nrx, 3)
+        self._fields = None
+
+    def _live(self):
+        if self.ctx._gen != self._gen:
+            raise SbrError("result superseded: a later trace ran on this context; read results before the next trace")
+
+    @property
+    def fields(self):
+        """Complex field per (transmitter,) receiver and component, shape (nrx, 3) or (ntx, nrx, 3)."""
+        if self._fields is None:
+            self._live()
+            n = self.ntx * self.nrx
+            f = np.empty((n, 3, 2))
+            self.ctx._chk(lib().sbr_result_fields(self.ctx.h, _a(f), n))
+            c = f.view(np.complex128).reshape(n, 3)   # (re, im) pairs
+            self._fields = c if self.ntx == 1 else c.reshape(self.ntx, self.nrx, 3)
+        return self._fields
 
     def path_tx(self):
+        self._live()
         n = self.stats["winners"]
         out = np.zeros(n, np.int32)
         self.ctx._chk(lib().sbr_result_path_tx(self.ctx.h, _a(out), n))
         return out
 
     def paths(self):
+        self._live()
         L, h = lib(), self.ctx.h
         npth, nseg = C.c_int64(), C.c_int64()
         self.ctx._chk(L.sbr_result_path_counts(h, C.byref(npth), C.byref(nseg)))
@@ -195,12 +214,14 @@ class Result:
         return out
 
     def path_fields(self):
+        self._live()
         n = self.stats["winners"]
         f = np.zeros((n, 3, 2))
         self.ctx._chk(lib().sbr_result_path_fields(self.ctx.h, _a(f), n))
         return f[..., 0] + 1j * f[..., 1]
 
     def trees(self):
+        self._live()
         L, h = lib(), self.ctx.h
         nn = C.c_int64()
         self.ctx._chk(L.sbr_result_tree_count(h, C.byref(nn)))
@@ -223,6 +244,7 @@ class Context:
         if rc != 0:
             raise SbrError(f"sbr_create failed ({rc})")
         self.h = h
+        self._gen = 0   # traces run on this context (a Result reads only the latest one)
 
     def close(self):
         if getattr(self, "h", None):
@@ -289,6 +311,7 @@ class Context:
         the fastest (see sbr_autotune); returns the chosen tuning."""
         r = np.ascontiguousarray(rx, dtype=np.float64).reshape(-1, 3)
         t = Tuning()
+        self._gen += 1   # the trials replace the context's last result
         self._chk(lib().sbr_autotune(self.h, C.byref(params), _a(r), r.shape[0], C.byref(t)))
         return dict(lds_scene=t.lds_scene, blocks_per_cu=t.blocks_per_cu, window_nodes=t.window_nodes)
 
@@ -318,12 +341,14 @@ class Context:
 
     def trace(self, params, rx):
         r = np.ascontiguousarray(rx, dtype=np.float64).reshape(-1, 3)
+        self._gen += 1
         self._chk(lib().sbr_trace(self.h, C.byref(params), _a(r), r.shape[0]))
         return Result(self, r.shape[0])
 
     def trace_batch(self, params_list, rx):
         r = np.ascontiguousarray(rx, dtype=np.float64).reshape(-1, 3)
         arr = (TraceParams * len(params_list))(*params_list)
+        self._gen += 1
         self._chk(lib().sbr_trace_batch(self.h, arr, len(params_list), _a(r), r.shape[0]))
         return Result(self, r.shape[0], len(params_list))
 

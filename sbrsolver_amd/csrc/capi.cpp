@@ -535,7 +535,7 @@ int sbr_result_fields(sbr_ctx* ctx, double* out, int32_t n) {
     if (!e.haveResult) return e.fail(SBR_E_STATE, "no trace result");
     if ((int64_t)n != (int64_t)e.nrx * e.ntx || (!out && n > 0))
       return e.fail(SBR_E_ARG, "fields: count must be transmitters x receivers");
-    if (n > 0) memcpy(out, e.hField.data(), 6 * sizeof(double) * (size_t)n);
+    if (n > 0) memcpy(out, e.field_host(), 6 * sizeof(double) * (size_t)n);
     return SBR_OK;
   });
 }

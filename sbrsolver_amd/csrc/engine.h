@@ -257,6 +257,7 @@ struct Engine {
   DBuf<int32_t> dHeavy;          // nodes left to the wave-per-node capture
   DBuf<int32_t> dCand;           // nodes whose segment may reach the receivers' capture region
   DBuf<uint8_t> dCandFlag;       // per node: 1 if listed (written by the bounce kernel)
+  DBuf<int32_t> dTileCount, dTileOffset;   // compaction of the flags: per-tile counts and offsets
   DBuf<int2> dRec;               // (node, receiver) records of the capture's candidate pass
   DBuf<unsigned long long> dRecCount, dRecPrefix;   // per-partition record counts / prefix
   DBuf<unsigned long long> dNodeKey;   // dedup and order keys of the nodes with records
@@ -279,7 +280,8 @@ struct Engine {
   size_t nodeCap = 0;
   size_t tabCapHint = 0;   // dedup table size of the last trace
   double perRootHint = 0;  // tree nodes per root item of the last trace (first window size)
-  std::vector<double> hField;
+  size_t hFieldN = 0;   // doubles of the last trace's receiver fields, held in pinOut
+  const double* field_host() const { return reinterpret_cast<const double*>(pinOut.p); }
   RxGrid grid{};
 
   ~Engine();

@@ -855,18 +855,93 @@ __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBuf
 }
 
 // Groups whose smallest order key has its root in [glLo, glHi) (those first seen in the window).
+// Indices of the flagged nodes in node order, in two passes over tiles of kFlagTile flags (16 per
+// thread): per-tile counts, then (after a scan of the counts) each tile writes its indices.
+constexpr int kFlagTile = 4096;
+
+__device__ __forceinline__ int flags16(const uint8_t* __restrict__ flags, int64_t n, int64_t at, uint4* v) {
+  if (at + 16 <= n) {
+    *v = *reinterpret_cast<const uint4*>(flags + at);
+  } else {
+    unsigned w[4] = {0u, 0u, 0u, 0u};
+    for (int k = 0; k < 16; k++)
+      if (at + k < n && flags[at + k]) w[k >> 2] |= 1u << (8 * (k & 3));
+    *v = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+  return __popc(v->x) + __popc(v->y) + __popc(v->z) + __popc(v->w);   // flags are 0 or 1
+}
+
+// Sum over the block (256 threads); every thread gets the total and its exclusive prefix.
+__device__ __forceinline__ int block_scan256(int c, int* total) {
+  __shared__ int waveSum[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = c;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += v;
+  }
+  if (lane == 63) waveSum[wave] = incl;
+  __syncthreads();
+  int before = 0, all = 0;
+  for (int w = 0; w < 4; w++) {
+    if (w < wave) before += waveSum[w];
+    all += waveSum[w];
+  }
+  *total = all;
+  return before + incl - c;
+}
+
+__global__ void __launch_bounds__(256) k_flag_count(const uint8_t* __restrict__ flags, int64_t n,
+                                                    int32_t* __restrict__ tileCount) {
+  uint4 v;
+  const int c = flags16(flags, n, (int64_t)blockIdx.x * kFlagTile + 16 * threadIdx.x, &v);
+  int total;
+  (void)block_scan256(c, &total);
+  if (threadIdx.x == 0) tileCount[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(256) k_flag_scatter(const uint8_t* __restrict__ flags, int64_t n,
+                                                      const int32_t* __restrict__ tileCount,
+                                                      const int32_t* __restrict__ tileOffset, int32_t nTiles,
+                                                      int32_t* __restrict__ out, unsigned long long* __restrict__ nOut) {
+  const int64_t at = (int64_t)blockIdx.x * kFlagTile + 16 * threadIdx.x;
+  uint4 v;
+  const int c = flags16(flags, n, at, &v);
+  int total;
+  int pos = tileOffset[blockIdx.x] + block_scan256(c, &total);
+  const unsigned w[4] = {v.x, v.y, v.z, v.w};
+  for (int k = 0; k < 16; k++)
+    if ((w[k >> 2] >> (8 * (k & 3))) & 1u) out[pos++] = (int32_t)(at + k);
+  if (blockIdx.x == nTiles - 1 && threadIdx.x == 0)
+    *nOut = (unsigned long long)tileOffset[nTiles - 1] + (unsigned long long)tileCount[nTiles - 1];
+}
+
 __global__ void k_collect(const unsigned long long* __restrict__ tabA, const unsigned long long* __restrict__ tabOrder,
                           const int32_t* __restrict__ tabRx, unsigned long long cap, unsigned long long glLo,
                           unsigned long long glHi, unsigned long long* __restrict__ winKey,
                           int32_t* __restrict__ winRx, unsigned long long* __restrict__ counter) {
-  unsigned long long s = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= cap) return;
-  if (tabA[s] == 0ull) return;
-  const unsigned long long gl = tabOrder[s] >> 32;
-  if (gl < glLo || gl >= glHi) return;
-  unsigned long long w = atomicAdd(counter, 1ull);
-  winKey[w] = tabOrder[s];
-  winRx[w] = tabRx[s];
+  const unsigned long long s = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool take = false;
+  unsigned long long key = 0;
+  if (s < cap && tabA[s] != 0ull) {
+    key = tabOrder[s];
+    const unsigned long long gl = key >> 32;
+    take = gl >= glLo && gl < glHi;
+  }
+  // wave-aggregated append
+  const unsigned long long m = __ballot(take);
+  if (m == 0ull) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int leader = __ffsll(m) - 1;
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader, 64);
+  if (take) {
+    const unsigned long long w = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+    winKey[w] = key;
+    winRx[w] = tabRx[s];
+  }
 }
 
 // Tree node of every kept path (order keys, window roots from rootBase): from the root, follow the
@@ -1317,13 +1392,18 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   nodes = (int64_t)total;
   capNodes = 0;
   if (capG && nodes > 0) {   // listed nodes, in node order; the count stays on the device
+    const int32_t nTiles = (int32_t)((nodes + kFlagTile - 1) / kFlagTile);
+    ENSURE(dTileCount, nTiles);
+    ENSURE(dTileOffset, nTiles);
+    k_flag_count<<<nTiles, 256, 0, stream>>>(dCandFlag.p, nodes, dTileCount.p);
+    HIPCHK(hipGetLastError());
     size_t tb = 0;
-    hipcub::CountingInputIterator<int32_t> idx(0);
-    HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, tb, idx, dCandFlag.p, dCand.p, dTraceCounters.p + 1, (int)nodes,
-                                         stream));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, dTileCount.p, dTileOffset.p, nTiles, stream));
     ENSURE(dScratch, tb);
-    HIPCHK(hipcub::DeviceSelect::Flagged(dScratch.p, tb, idx, dCandFlag.p, dCand.p, dTraceCounters.p + 1, (int)nodes,
-                                         stream));
+    HIPCHK(hipcub::DeviceScan::ExclusiveSum(dScratch.p, tb, dTileCount.p, dTileOffset.p, nTiles, stream));
+    k_flag_scatter<<<nTiles, 256, 0, stream>>>(dCandFlag.p, nodes, dTileCount.p, dTileOffset.p, nTiles, dCand.p,
+                                               dTraceCounters.p + 1);
+    HIPCHK(hipGetLastError());
   }
   stats.ms_k_trace += kms;
   stats.n_k_trace += maxDepth + 1;
@@ -1735,13 +1815,14 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   HIPCHK(hipEventElapsedTime(&a, ev[1], ev[2]));
   stats.ms_capture += a;   // receiver binning upload and line of sight
   stats.launches = nRoot;
-  hField.assign(6 * (size_t)nOut, 0.0);
+  // Receiver fields read back into page-locked memory, where the result accessors find them.
+  hFieldN = 0;
   if (nOut > 0) {
     if (!pinOut.ensure(6 * sizeof(double) * nOut)) return fail(SBR_E_NOMEM, "pinned field staging");
     HIPCHK(hipMemcpyAsync(pinOut.p, dField.p, 6 * sizeof(double) * nOut, hipMemcpyDeviceToHost, stream));
   }
   HIPCHK(hipStreamSynchronize(stream));
-  if (nOut > 0) memcpy(hField.data(), pinOut.p, 6 * sizeof(double) * nOut);
+  hFieldN = 6 * (size_t)nOut;
   stats.ms_total = ms_since(t0);
   haveResult = true;
   return SBR_OK;

@@ -135,7 +135,7 @@ struct sbr_server {
       try {
         rc = e.trace(&r->p, 1, r->nrx > 0 ? r->rx.data() : nullptr, r->nrx);
         if (rc == SBR_OK) {
-          f = e.hField;
+          f.assign(e.field_host(), e.field_host() + e.hFieldN);
           launches = e.stats.launches;
           nb = e.stats.bounces;
           np = e.stats.winners;
