@@ -796,7 +796,7 @@ __device__ __forceinline__ void flush_tests(const CaptureBufs& B, unsigned long 
 // Thread per listed node (those whose segment reaches the receivers' capture region).  Nodes whose
 // candidate region spans more than kHeavyCells grid cells are left to k_capture_heavy, which spreads
 // one node over a wave.
-constexpr int kHeavyCells = 48;
+constexpr int kHeavyCells = 12;
 
 __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
   const int64_t nc = (int64_t)B.nCand[0];   // listed nodes (written after the bounce loop)
