@@ -129,6 +129,25 @@ def batched_run(ctx, make_params, scenes, root_budget=1 << 25):
         speedup=round(ts / tb, 3))
 
 
+def indoor_run(Context, make_params, scenes, tess=10):
+    """Dense indoor workload (about 207k triangles, icosahedron level 10, reflections and
+    transmissions up to 8 each and 8 in total, 20k receivers) on a context of its own: one warm-up
+    trace, then one timed."""
+    cfg = scenes.config3(tess=tess)
+    with Context(0) as ictx:
+        ictx.scene_from_soup(*cfg["soup"])
+        ictx.materials(cfg["materials"])
+        p = make_params(**cfg["trace"])
+        ictx.trace(p, cfg["rx"])
+        t0 = time.perf_counter()
+        s = ictx.trace(p, cfg["rx"]).stats
+        dt = time.perf_counter() - t0
+    return dict(workload=cfg["name"], triangles=int(len(cfg["soup"][2])), launches=int(s["launches"]),
+                bounces=int(s["bounces"]), windows=int(s["windows"]), value=round(s["bounces"] / dt / 1e6, 3),
+                unit="Mray-bounces/s", s_per_trace=round(dt, 3), ms_trace=round(s["ms_trace"], 1),
+                ms_capture=round(s["ms_capture"], 1), paths_kept=int(s["winners"]))
+
+
 def serve_run(ctx, make_params, scenes, Server, n=48, slots=3, load=0.7):
     """Serving loop on the urban scene: n coverage queries (levels 6-10, 4 reflections, 1..50k
     receivers).  Phase 1 submits all at once to measure capacity (req/s when saturated); phase 2
@@ -193,6 +212,7 @@ def main():
     ap.add_argument("--no-ref-model", action="store_true", help="skip the reference reception model line")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-transmitter line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop line")
+    ap.add_argument("--no-indoor", action="store_true", help="skip the dense indoor line")
 
     ap.add_argument("--calib", action="store_true",
                     help="run one plain 256 MiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
@@ -305,6 +325,8 @@ def main():
     if not args.no_serve:
         from sbrsolver_amd import Server
         extra("serving", lambda: serve_run(ctx, make_params, scenes, Server))
+    if not args.no_indoor:
+        extra("indoor", lambda: indoor_run(Context, make_params, scenes))
 
     if args.calib:
         calib_copy(256 << 20)

@@ -1603,8 +1603,9 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     bool tableFull = false;
     int64_t ws = 0;
     while (ws < nRoot || (nRoot == 0 && ws == 0)) {
-      // trees of roots [ws, ws + W)
-      int64_t W = std::max<int64_t>(1, std::min<int64_t>(nRoot - ws, (int64_t)((double)budget / perRoot)));
+      // trees of roots [ws, ws + W), sized for 1.25x the expected nodes so that a window denser than
+      // the estimate seldom overflows (an overflow reruns the window)
+      int64_t W = std::max<int64_t>(1, std::min<int64_t>(nRoot - ws, (int64_t)((double)budget / (1.25 * perRoot))));
       if (nRoot == 0) W = 0;
       float msT = 0;
       for (int attempt = 0;; attempt++) {
