@@ -202,6 +202,8 @@ struct Engine {
   int device = 0;
   Tuning tune;
   hipStream_t stream = nullptr;
+  hipStream_t sideStream = nullptr;   // line of sight, concurrent with the bounce loop
+  hipEvent_t evSide[2] = {nullptr, nullptr};   // receivers uploaded; line of sight done
   std::string err;
   int numCU = 256;
 

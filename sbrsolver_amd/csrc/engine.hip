@@ -1155,6 +1155,9 @@ __global__ void k_paths(int64_t W, int32_t nrx, const TxDesc* __restrict__ txs, 
 // =============================================================================================
 Engine::~Engine() {
   for (auto& kv : rayCache) delete kv.second;
+  for (auto& e : evSide)
+    if (e) (void)hipEventDestroy(e);
+  if (sideStream) (void)hipStreamDestroy(sideStream);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1169,6 +1172,8 @@ int Engine::init(int dev) {
   device = dev;
   HIPCHK(hipSetDevice(dev));
   HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&sideStream, hipStreamNonBlocking));
+  for (auto& e : evSide) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, dev));
   numCU = prop.multiProcessorCount;
@@ -1497,6 +1502,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   nrx = m;
   ntx = nt;
   const int64_t nOut = (int64_t)nt * m;
+  bool losWait = false;   // the capture waits for the side stream's line of sight once
   RxGrid g{};
   if (m > 0) {
     // Uniform xy grid with about one receiver per cell.
@@ -1547,16 +1553,29 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     HIPCHK(hipMemcpyAsync(dRx.p, pinIn.at<double>(offRx), 3 * sizeof(double) * m, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(dCellStart.p, cellCount, sizeof(int32_t) * nCell, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(dCellRx.p, cellRx, sizeof(RxCell) * m, hipMemcpyHostToDevice, stream));
+    // Line of sight for every pair.  A small one (a few blocks per CU at most) runs on the side
+    // stream beside the bounce loop, whose launch tails it fills, and the capture waits for it; a
+    // large one would hold CUs that the persistent bounce kernel needs, so it runs in line.
     const size_t sm = stack_bytes(stackDepth(), kBlock);
+    const bool side = blocks_for(nOut, kBlock) <= (int64_t)numCU / 2;
+    hipStream_t ls = side ? sideStream : stream;
+    if (side) {
+      HIPCHK(hipEventRecord(evSide[0], stream));
+      HIPCHK(hipStreamWaitEvent(sideStream, evSide[0], 0));
+    }
     if (exact)
-      k_los<true><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
-                                                                     dSubTri.p, sceneExtent, dRx.p, m, stackDepth(),
-                                                                     dLos.p);
+      k_los<true><<<blocks_for(nOut, kBlock), kBlock, sm, ls>>>(dTx.p, nt, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p,
+                                                                 dSub.p, dSubTri.p, sceneExtent, dRx.p, m, stackDepth(),
+                                                                 dLos.p);
     else
-      k_los<false><<<blocks_for(nOut, kBlock), kBlock, sm, stream>>>(dTx.p, nt, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p,
-                                                                      dSub.p, dSubTri.p, sceneExtent, dRx.p, m,
-                                                                      stackDepth(), dLos.p);
+      k_los<false><<<blocks_for(nOut, kBlock), kBlock, sm, ls>>>(dTx.p, nt, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p,
+                                                                  dSub.p, dSubTri.p, sceneExtent, dRx.p, m, stackDepth(),
+                                                                  dLos.p);
     HIPCHK(hipGetLastError());
+    if (side) {
+      HIPCHK(hipEventRecord(evSide[1], sideStream));
+      losWait = true;
+    }
   }
   HIPCHK(hipEventRecord(ev[2], stream));
 
@@ -1656,6 +1675,10 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       CA.nrx = m;
       CA.g = g;
       CA.tabMask = cap - 1;
+      if (losWait) {
+        HIPCHK(hipStreamWaitEvent(stream, evSide[1], 0));
+        losWait = false;
+      }
       HIPCHK(hipEventRecord(kev[0], stream));
       // candidate pass: (node, receiver) records; redone with a larger record buffer if it fills
       unsigned long long nRec = 0, nHeavy = 0;
