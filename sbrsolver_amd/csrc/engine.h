@@ -281,7 +281,7 @@ struct Engine {
   int64_t winners = 0;
   size_t nodeCap = 0;
   size_t tabCapHint = 0;   // dedup table size of the last trace
-  double perRootHint = 0;  // tree nodes per root item of the last trace (first window size)
+  double perRootHint = 0;  // tree nodes per root item of the last trace's densest window (first window size)
   size_t hFieldN = 0;   // doubles of the last trace's receiver fields, held in pinOut
   const double* field_host() const { return reinterpret_cast<const double*>(pinOut.p); }
   RxGrid grid{};

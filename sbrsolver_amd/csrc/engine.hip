@@ -1590,6 +1590,21 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   ENSURE(dCounters, 8);
   ENSURE(dTraceCounters, 8);
   const size_t budget = std::max<size_t>(tune.windowNodes > 0 ? (size_t)tune.windowNodes : kWindowNodes, 4096);
+  // Bound of the tree size per root item: one node per interaction sequence, i.e. sum over the
+  // allowed (reflections r, transmissions t) of C(r + t, r), for the largest transmitter.
+  double perRootBound = 1.0;
+  for (int t = 0; t < nt; t++) {
+    double b = 0.0;
+    for (int r = 0; r <= P[t].maxR; r++)
+      for (int k = 0; k <= P[t].maxT; k++) {
+        if (P[t].maxTotal > 0 && r + k > P[t].maxTotal) continue;
+        double c = 1.0;   // C(r + k, r)
+        for (int j = 1; j <= k; j++) c = c * (double)(r + j) / (double)j;
+        b += c;
+      }
+    perRootBound = std::max(perRootBound, b);
+  }
+  double perRootMax = 0;   // densest window of this trace (nodes per root item)
   bool done = false;
   for (int tabAttempt = 0; tabAttempt < 6 && !done; tabAttempt++) {
     stats.bounces = stats.node_visits = stats.tri_tests = stats.capture_records = stats.corrected = 0;
@@ -1617,8 +1632,9 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       HIPCHK(hipMemsetAsync(dCounters.p, 0, 8 * sizeof(unsigned long long), stream));
     }
     unsigned long long usedBefore = 0, corBefore = 0;
-    // nodes per root item: the last trace's figure (or one chain per root), refined by every window
-    double perRoot = perRootHint > 0 ? perRootHint : (double)(maxDepth + 1);
+    // Nodes per root item for sizing windows: the last trace's densest window, then the densest
+    // window of this trace so far, never above the tree-size bound (with which no window overflows).
+    double perRoot = perRootHint > 0 ? std::min(perRootHint, perRootBound) : perRootBound;
     bool tableFull = false;
     int64_t ws = 0;
     while (ws < nRoot || (nRoot == 0 && ws == 0)) {
@@ -1646,7 +1662,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         if (rc != SBR_E_LIMIT || attempt > 40) return rc == SBR_E_LIMIT ? fail(rc, "tree buffers could not be sized") : rc;
         // Overflowed (need is only a lower bound): first use the whole window budget, then halve
         // the window until its trees fit.
-        perRoot = std::max(perRoot, (double)need / (double)std::max<int64_t>(W, 1));
+        perRoot = std::min(perRootBound, std::max(perRoot, (double)need / (double)std::max<int64_t>(W, 1)));
         if (nodeCap < budget) {
           nodeCap = std::max(nodeCap, std::min(budget, (size_t)need * 2 + 1024));
         } else {
@@ -1656,7 +1672,10 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       }
       stats.ms_trace += msT;
       stats.windows++;
-      if (W > 0) perRoot = std::max(1.0, (double)nodes / (double)W);
+      if (W > 0) {
+        perRootMax = std::max(perRootMax, (double)nodes / (double)W);
+        perRoot = std::min(perRootBound, std::max(1.0, perRootMax));
+      }
       const bool last = ws + W >= nRoot;
       if (ws == 0 && !last) multiWindow = true;
       if (m == 0) {
@@ -1823,7 +1842,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     done = true;
   }
   if (!done) return fail(SBR_E_LIMIT, "dedup table could not be sized");
-  if (nRoot > 0) perRootHint = std::max(1.0, (double)stats.bounces / (double)nRoot);
+  if (nRoot > 0 && perRootMax > 0) perRootHint = perRootMax;
   if (m > 0) tabCapHint = cap;
   if (multiWindow) {
     int rc = finish_path_cache();
