@@ -184,3 +184,29 @@ def test_batch_equals_individual_traces(gpu_ctx):
         assert np.array_equal(bf[t], single.fields)
         assert np.array_equal(bpf[sel], single.path_fields())
     assert batch.bounces == total
+
+
+def test_full_level_urban_sample_matches_oracle(gpu_ctx):
+    """The headline workload at its full size (urban canyon, icosahedron level 9, R6 T1, 10k
+    receivers): every launch traced on the GPU gives the oracle's bounce count (committed fixture),
+    and every 512th launch (5120 of the 2.6 M level-9 directions) gives the oracle's trees, kept
+    paths and fields, with the reference's reception sphere (thousands of paths) and with
+    alpha*L/sqrt(3) (a handful: its radius is that of the full level)."""
+    import json
+    say("urban level 9: full bounce count, sampled parity")
+    cfg = scenes.config2(tess=9)
+    gpu_ctx.scene_from_soup(*cfg["soup"])
+    gpu_ctx.materials(cfg["materials"])
+    with open(os.path.join(GOLDEN, "traversal_counts.json")) as fh:
+        fixture = json.load(fh)["urban_canyon_482tri_L9_R6T1"]
+    full = gpu_ctx.trace(make_params(**dict(cfg["trace"], receptionModel=1)), cfg["rx"])
+    assert full.bounces == fixture["bounces"]
+    sc = O.Scene.from_soup(*cfg["soup"])
+    for model in (0, 1):
+        tr = dict(cfg["trace"], receptionModel=model, launchStride=512, launchOffset=7)
+        g = gpu_ctx.trace(make_params(**tr), cfg["rx"])
+        if model == 0:
+            _trees_equal(g, O.trees(sc, O.make_params(**tr)), "urban L9 sample")
+        o = O.trace(sc, O.make_params(**tr), cfg["rx"], cfg["materials"], nthreads=16)
+        worst = compare(o, g, f"urban L9 sample model {model}")
+        say("urban L9 sample model", model, "paths", len(o.prx), "bounces", o.bounces, "worst field rel err", worst)
