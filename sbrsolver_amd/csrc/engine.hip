@@ -254,6 +254,8 @@ __device__ __forceinline__ bool may_capture(const RxGrid& g, V3 S, V3 Tg, double
   return true;
 }
 
+constexpr int kTraceOversub = 16;   // bounce-kernel blocks per resident block slot
+
 struct TraceArgs {
   int32_t depth;
   int32_t nTx;
@@ -279,6 +281,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
                                                   LevelRec* __restrict__ lv, unsigned long long* __restrict__ flags,
                                                   uint8_t* __restrict__ candFlag, int countTraversal) {
   extern __shared__ __align__(16) unsigned char smem[];
+  // The depth's item count is final (written by the previous launch); blocks beyond it leave at
+  // once (the grid is sized for the largest depth and oversubscribes the CUs, so that the
+  // dispatcher balances the launch's end).
+  const LevelRec L = lv[A.depth];
+  unsigned long long n = L.count;
+  if (n > A.queueCap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&flags[0], 2ull);
+    n = A.queueCap;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) lv[A.depth + 1].base = L.base + n;
+  if ((unsigned long long)blockIdx.x * blockDim.x >= n) return;
   const DNodeF* nodesF = gNodesF;
   const DTri* tris = gTris;
   const DBox32* tbox = gBox;
@@ -297,13 +310,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
   // An overflow at an earlier depth (flags[0] != 0) means the trace is redone with more room: this
   // depth's queue may hold unwritten entries, so nothing is read from it.
   if (A.depth > 0 && flags[0] != 0ull) return;
-  const LevelRec L = lv[A.depth];
-  unsigned long long n = L.count;
-  if (n > A.queueCap) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&flags[0], 2ull);
-    n = A.queueCap;
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) lv[A.depth + 1].base = L.base + n;
   unsigned long long cntNodes = 0, cntTests = 0, cntExact = 0;
   const int lane = threadIdx.x & 63;
   for (unsigned long long b0 = (unsigned long long)blockIdx.x * blockDim.x; b0 < n;
@@ -1352,7 +1358,10 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   auto kern = variant(lds);
   int perCU = std::max(1, lds ? occLds : occGlobal);
   if (tune.blocksPerCU > 0) perCU = std::min(perCU, tune.blocksPerCU);
-  const unsigned grid = (unsigned)std::min<int64_t>((int64_t)numCU * perCU, blocks_for(std::max<int64_t>(nRoot, 1), kBlock));
+  // Resident blocks times kTraceOversub (capped by twice the roots' blocks): blocks that finish early
+  // are replaced by waiting ones, so the launch ends evenly.
+  const unsigned grid = (unsigned)std::max<int64_t>(
+      1, std::min<int64_t>((int64_t)numCU * perCU * kTraceOversub, blocks_for(std::max<int64_t>(2 * nRoot, 1), kBlock)));
   stats.trace_lds = lds ? 1 : 0;
   stats.trace_blocks_per_cu = perCU;
   LevelRec* lv0 = pinSmall.at<LevelRec>(kPinLevels0);
