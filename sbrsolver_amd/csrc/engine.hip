@@ -515,6 +515,7 @@ __global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restric
 struct CaptureArgs {
   int64_t nodes;   // tree nodes (bound of the listed ones)
   int32_t nrx;
+  int32_t cellLds;   // > 0: the cell starts (that many, all below 65536) are staged in LDS as uint16
   RxGrid g;
   unsigned long long tabMask;
 };
@@ -805,6 +806,12 @@ __device__ __forceinline__ void flush_tests(const CaptureBufs& B, unsigned long 
 constexpr int kHeavyCells = 12;
 
 __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
+  extern __shared__ uint16_t sCell[];
+  if (A.cellLds > 0) {   // cell starts in LDS: one global round trip less per row of cells
+    for (int k = threadIdx.x; k < A.cellLds; k += blockDim.x) sCell[k] = (uint16_t)B.cellStart[k];
+    __syncthreads();
+  }
+  auto cellStart = [&](int k) { return A.cellLds > 0 ? (int)sCell[k] : B.cellStart[k]; };
   const int64_t nc = (int64_t)B.nCand[0];   // listed nodes (written after the bounce loop)
   unsigned long long myTests = 0;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += (int64_t)gridDim.x * blockDim.x) {
@@ -823,8 +830,8 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
         for (int j = c.j0; j <= c.j1; j++) {   // receivers of a row's cells i0..i1 are contiguous
           int i0, i1;
           if (!row_range(A, c, j, &i0, &i1)) continue;
-          const int q1 = B.cellStart[j * A.g.nx + i1 + 1];
-          for (int q = B.cellStart[j * A.g.nx + i0]; q < q1; q++) capture_one(B, c, B.cellRx[q], myTests);
+          const int q1 = cellStart(j * A.g.nx + i1 + 1);
+          for (int q = cellStart(j * A.g.nx + i0); q < q1; q++) capture_one(B, c, B.cellRx[q], myTests);
         }
         if (c.emitted) store_key(B, c);
       }
@@ -1700,6 +1707,10 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       ENSURE(dNodeKey, 3 * (size_t)std::max<int64_t>(nodes, 1));
       CaptureArgs CA{};
       CA.nodes = nodes;   // bound of the listed nodes; their count is read on the device
+      {
+        const int64_t nCell = (int64_t)g.nx * g.ny + 1;
+        CA.cellLds = (m < 65536 && nCell <= 16384) ? (int32_t)nCell : 0;
+      }
       CA.nrx = m;
       CA.g = g;
       CA.tabMask = cap - 1;
@@ -1727,7 +1738,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
                        dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p, nullptr,
                        count ? 1 : 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
         if (nodes > 0) {
-          k_capture<<<(unsigned)std::min<int64_t>(blocks_for(nodes, 256), (int64_t)numCU * 16), 256, 0, stream>>>(CA, CB);
+          k_capture<<<(unsigned)std::min<int64_t>(blocks_for(nodes, 256), (int64_t)numCU * 16), 256,
+                      sizeof(uint16_t) * (size_t)CA.cellLds, stream>>>(CA, CB);
           HIPCHK(hipGetLastError());
         }
         // wave per heavy node; the count is read on the device (no host round trip)
