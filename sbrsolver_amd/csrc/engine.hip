@@ -818,12 +818,20 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
     const int32_t n = B.cand[k];
     NodeCtx c;
     if (node_setup(A, B, n, c)) {
+      // The decision is the loop's own exit flag: a `cells > kHeavyCells` test after the loop was
+      // merged by the compiler with the identical in-loop test, whose lane mask only holds the lanes
+      // still looping in the last iteration (lanes that left earlier were then taken for light ones).
       int cells = 0;
-      for (int j = c.j0; j <= c.j1 && cells <= kHeavyCells; j++) {
+      bool heavy = false;
+      for (int j = c.j0; j <= c.j1; j++) {
         int i0, i1;
         if (row_range(A, c, j, &i0, &i1)) cells += i1 - i0 + 1;
+        if (cells > kHeavyCells) {
+          heavy = true;
+          break;
+        }
       }
-      if (cells > kHeavyCells) {
+      if (heavy) {
         const unsigned long long h = atomicAdd(&B.counters[5], 1ull);
         B.heavy[h] = (int32_t)n;
       } else {
