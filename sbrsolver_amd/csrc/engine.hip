@@ -215,7 +215,9 @@ __device__ __forceinline__ bool seg_clip(const RxGrid& g, V3 S, V3 Tg, double ba
       if (fabs(d) < 1e-300) {
         if (s < lo || s > hi) return false;
       } else {
-        double ua = (lo - s) / d, ub = (hi - s) / d;
+        // the reciprocal's rounding (a few ulps of u) is far inside the 1e-12 widening below
+        const double inv = 1.0 / d;
+        double ua = (lo - s) * inv, ub = (hi - s) * inv;
         u0 = fmax(u0, fmin(ua, ub) - 1e-12);
         u1 = fmin(u1, fmax(ua, ub) + 1e-12);
         if (u0 > u1) return false;
@@ -517,6 +519,10 @@ struct CaptureArgs {
   int32_t nrx;
   int32_t cellLds;   // > 0: the cell starts (that many, all below 65536) are staged in LDS as uint16
   RxGrid g;
+  // 1 / g.cell.  Cell indices of the search region are taken with it; the region's radius carries
+  // a margin (1e-6 relative and absolute) far above the rounding that this adds, so a receiver
+  // within the capture radius still falls in a cell of the region.
+  double invCell;
   unsigned long long tabMask;
 };
 
@@ -533,6 +539,7 @@ __device__ __forceinline__ unsigned long long order_key(const TNode& nd) {
 struct NodeCtx {
   V3 S, Tg, P0, P1;
   double base, rc, R;
+  double invDy;   // 1 / (P1.y - P0.y), 0 when that is below 1e-300 in magnitude
   int model, depth, txi;
   int32_t node, parent;
   bool all, keyed, emitted;
@@ -592,10 +599,12 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
   c.all = sc.all;
   c.j0 = 0;
   c.j1 = A.g.ny - 1;
+  const double dy = c.P1.y - c.P0.y;
+  c.invDy = fabs(dy) < 1e-300 ? 0.0 : 1.0 / dy;
   if (!c.all) {
     double ylo = fmin(c.P0.y, c.P1.y) - c.R, yhi = fmax(c.P0.y, c.P1.y) + c.R;
-    c.j0 = (int)fmax(0.0, floor((ylo - A.g.gy0) / A.g.cell));
-    c.j1 = (int)fmin((double)(A.g.ny - 1), floor((yhi - A.g.gy0) / A.g.cell));
+    c.j0 = (int)fmax(0.0, floor((ylo - A.g.gy0) * A.invCell));
+    c.j1 = (int)fmin((double)(A.g.ny - 1), floor((yhi - A.g.gy0) * A.invCell));
   }
   c.losBase = (int64_t)c.txi * A.nrx;
   return true;
@@ -622,21 +631,34 @@ __device__ __forceinline__ bool row_range(const CaptureArgs& A, const NodeCtx& c
   }
   double xa, xb;
   const double by0 = A.g.gy0 + j * A.g.cell - c.R, by1 = A.g.gy0 + (j + 1) * A.g.cell + c.R;
-  const double dy = c.P1.y - c.P0.y;
-  if (fabs(dy) < 1e-300) {
+  if (c.invDy == 0.0) {
     xa = fmin(c.P0.x, c.P1.x);
     xb = fmax(c.P0.x, c.P1.x);
   } else {
-    double va = (by0 - c.P0.y) / dy, vb = (by1 - c.P0.y) / dy;
+    // segment parameters of the row's (grown) y band; the reciprocal's rounding is far inside the
+    // 1e-12 widening
+    double va = (by0 - c.P0.y) * c.invDy, vb = (by1 - c.P0.y) * c.invDy;
     double w0 = fmax(0.0, fmin(va, vb) - 1e-12), w1 = fmin(1.0, fmax(va, vb) + 1e-12);
     if (w0 > w1) return false;
     double xa0 = c.P0.x + (c.P1.x - c.P0.x) * w0, xa1 = c.P0.x + (c.P1.x - c.P0.x) * w1;
     xa = fmin(xa0, xa1);
     xb = fmax(xa0, xa1);
   }
-  *i0 = (int)fmax(0.0, floor((xa - c.R - A.g.gx0) / A.g.cell));
-  *i1 = (int)fmin((double)(A.g.nx - 1), floor((xb + c.R - A.g.gx0) / A.g.cell));
+  *i0 = (int)fmax(0.0, floor((xa - c.R - A.g.gx0) * A.invCell));
+  *i1 = (int)fmin((double)(A.g.nx - 1), floor((xb + c.R - A.g.gx0) * A.invCell));
   return *i0 <= *i1;
+}
+
+// Cells of the region's bounding box (rows j0..j1 by the columns of the clipped segment's x extent
+// grown by R): an upper bound of the cells the rows hold, up to rounding at a cell edge.
+__device__ __forceinline__ int64_t box_cells(const CaptureArgs& A, const NodeCtx& c) {
+  if (c.j0 > c.j1) return 0;
+  int i0 = 0, i1 = A.g.nx - 1;
+  if (!c.all) {
+    i0 = (int)fmax(0.0, floor((fmin(c.P0.x, c.P1.x) - c.R - A.g.gx0) * A.invCell));
+    i1 = (int)fmin((double)(A.g.nx - 1), floor((fmax(c.P0.x, c.P1.x) + c.R - A.g.gx0) * A.invCell));
+  }
+  return i0 > i1 ? 0 : (int64_t)(c.j1 - c.j0 + 1) * (i1 - i0 + 1);
 }
 
 // Image source k of the path ending at `leaf` (GetImageSources, RTSolver.cpp:396-433): the
@@ -821,9 +843,12 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
       // The decision is the loop's own exit flag: a `cells > kHeavyCells` test after the loop was
       // merged by the compiler with the identical in-loop test, whose lane mask only holds the lanes
       // still looping in the last iteration (lanes that left earlier were then taken for light ones).
+      // Rows are counted only when the bounding box of the region could hold more than
+      // kHeavyCells cells.
       int cells = 0;
       bool heavy = false;
-      for (int j = c.j0; j <= c.j1; j++) {
+      const int jEnd = box_cells(A, c) > kHeavyCells ? c.j1 : c.j0 - 1;
+      for (int j = c.j0; j <= jEnd; j++) {
         int i0, i1;
         if (row_range(A, c, j, &i0, &i1)) cells += i1 - i0 + 1;
         if (cells > kHeavyCells) {
@@ -848,26 +873,59 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
   flush_tests(B, myTests);
 }
 
-// Wave per heavy node: the node's candidate cells, flattened row by row, are dealt to the lanes.
+// Wave per heavy node.  The node's rows are taken 64 at a time, one row's cell range per lane; a
+// wave scan of the ranges' lengths flattens their cells, which are then dealt to the lanes one
+// each (a lane finds its row by a binary search over the scan), so a node costs about
+// cells / 64 steps whatever the shape of its region.
 __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBufs B) {
+  extern __shared__ uint16_t sCell[];
+  if (A.cellLds > 0) {
+    for (int k = threadIdx.x; k < A.cellLds; k += blockDim.x) sCell[k] = (uint16_t)B.cellStart[k];
+    __syncthreads();
+  }
+  auto cellStart = [&](int k) { return A.cellLds > 0 ? (int)sCell[k] : B.cellStart[k]; };
   const int64_t nHeavy = (int64_t)B.counters[5];   // appended by k_capture
   const int lane = threadIdx.x & 63;
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   unsigned long long myTests = 0;
-  for (int64_t h = wave; h < nHeavy; h += nWaves) {
+  for (int64_t h = wave; h < nHeavy; h += nWaves) {   // wave-uniform
     NodeCtx c;
-    if (!node_setup(A, B, B.heavy[h], c)) continue;
-    int off = 0;   // flattened index of the row's first cell
-    for (int j = c.j0; j <= c.j1; j++) {
-      int i0, i1;
-      if (!row_range(A, c, j, &i0, &i1)) continue;
-      const int first = i0 + (((lane - off) % 64) + 64) % 64;
-      for (int ci = first; ci <= i1; ci += 64) {
-        const int cell = j * A.g.nx + ci;
-        for (int q = B.cellStart[cell]; q < B.cellStart[cell + 1]; q++) capture_one(B, c, B.cellRx[q], myTests);
+    if (!node_setup(A, B, B.heavy[h], c)) continue;    // same outcome on every lane
+    for (int jb = c.j0; jb <= c.j1; jb += 64) {
+      const int j = jb + lane;
+      int i0 = 0, i1 = -1;
+      if (j > c.j1 || !row_range(A, c, j, &i0, &i1)) {
+        i0 = 0;
+        i1 = -1;
       }
-      off += i1 - i0 + 1;
+      const int cnt = i1 - i0 + 1;
+      int incl = cnt;   // inclusive scan of the row lengths over the lanes (all lanes active)
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+      }
+      const int excl = incl - cnt;
+      const int total = __shfl(incl, 63, 64);
+      for (int base = 0; base < total; base += 64) {   // wave-uniform trip count
+        const int f = base + lane;
+        // the row holding flattened cell f: the last lane whose exclusive prefix is <= f (rows
+        // with no cells share their prefix with the next row, so they are never the last)
+        int r = 0;
+#pragma unroll
+        for (int step = 32; step > 0; step >>= 1) {
+          const int cand = r + step;
+          const int e = __shfl(excl, cand & 63, 64);
+          if (cand < 64 && e <= f) r = cand;
+        }
+        const int rowFirst = __shfl(i0, r, 64), rowExcl = __shfl(excl, r, 64);
+        if (f < total) {
+          const int cell = (jb + r) * A.g.nx + rowFirst + (f - rowExcl);
+          const int q1 = cellStart(cell + 1);
+          for (int q = cellStart(cell); q < q1; q++) capture_one(B, c, B.cellRx[q], myTests);
+        }
+      }
     }
     const unsigned long long em = __ballot(c.emitted);
     if (em && lane == __ffsll(em) - 1) store_key(B, c);
@@ -1721,6 +1779,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       }
       CA.nrx = m;
       CA.g = g;
+      CA.invCell = 1.0 / g.cell;
       CA.tabMask = cap - 1;
       if (losWait) {
         HIPCHK(hipStreamWaitEvent(stream, evSide[1], 0));
@@ -1752,7 +1811,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         }
         // wave per heavy node; the count is read on the device (no host round trip)
         if (nodes > 0) {
-          k_capture_heavy<<<blocks_for((int64_t)numCU * 32 * 64, 256), 256, 0, stream>>>(CA, CB);
+          k_capture_heavy<<<blocks_for((int64_t)numCU * 32 * 64, 256), 256, sizeof(uint16_t) * (size_t)CA.cellLds,
+                            stream>>>(CA, CB);
           HIPCHK(hipGetLastError());
         }
         HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinCapCtr), dCounters.p, 7 * sizeof(unsigned long long),
