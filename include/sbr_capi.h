@@ -98,7 +98,8 @@ typedef struct {
   double pt;
   int32_t useFresnel;    /* useFresnelCoeff */
   int32_t tess;          /* txTesslation: icosahedron subdivision level */
-  int32_t antennaType;   /* 0 isotropic, 1 Hertzian (reference default), 2 half-wave dipole */
+  int32_t antennaType;   /* 0 isotropic, 1 Hertzian (reference default), 2 half-wave dipole,
+                            3 the context's tabulated pattern (sbr_set_tx_pattern) */
   int32_t receptionModel;/* 0 reference sphere (RTSolver.cpp:838-841); 1 alpha*L/sqrt(3) */
   int32_t launchStride;  /* trace launches i with i % stride == offset (1 = all) */
   int32_t launchOffset;
@@ -169,6 +170,15 @@ int sbr_scene_bvh_dfs(sbr_ctx* ctx, double* node_box, int32_t* node_a, int32_t* 
                       int32_t* tri_dfs_id);
 
 int sbr_materials(sbr_ctx* ctx, const sbr_material* mats, int32_t count);
+/* Tabulated transmitter pattern for antennaType 3 (extension: the reference reads TXPattern.dat
+ * into a GainMap, Preprocessor.cpp:3-35, but FieldCompute.cpp:117 always uses the analytic
+ * pattern).  gain_info holds n_theta * n_phi rows of GainInfo (DataStructures.h:32-37): theta gain
+ * dB, theta phase deg, phi gain dB, phi phase deg; row i * n_phi + j is theta = i * 180 / (n_theta - 1)
+ * and phi = j * 360 / n_phi degrees (TXPattern.dat: 181 x 360 at 1 degree).  A row gives the field
+ * coefficients j sqrt(60 Pt G) e^{j psi} (FieldCompute.cpp:114-116: sqrt(eta Pt G / 2 pi)); between
+ * rows they are interpolated bilinearly in (theta, phi), periodic in phi.  Copied to the device;
+ * replaces any earlier table; servers created from the context take it over. */
+int sbr_set_tx_pattern(sbr_ctx* ctx, const float* gain_info, int32_t n_theta, int32_t n_phi);
 int sbr_set_tuning(sbr_ctx* ctx, const sbr_tuning* tuning);
 int sbr_get_tuning(sbr_ctx* ctx, sbr_tuning* tuning);
 

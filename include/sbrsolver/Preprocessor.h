@@ -1,13 +1,29 @@
 // Host scene preparation of the reference call surface (Preprocessor.h:18-76).
 #pragma once
 
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "Triangle.h"
 #include "VecMatDef.h"
 
+// Antenna gain information (DataStructures.h:32-37): theta gain dB, theta phase deg, phi gain dB,
+// phi phase deg, keyed by (theta, phi) in degrees.
+struct GainInfo {
+  float data[4];
+};
+using ThetaPhiAngle = std::pair<float, float>;
+using GainMap = std::map<ThetaPhiAngle, GainInfo>;
+
 namespace Preprocessor {
+
+// TXPattern.dat reader (Preprocessor.cpp:3-35): every line of exactly six numbers (theta, phi,
+// theta gain, theta phase, phi gain, phi phase) becomes one entry; other lines (the header) are
+// skipped, as are lines whose six tokens are not all numbers (the reference's std::stof would
+// throw there).  Returns false when the file cannot be opened.
+bool ReadPatternFile(std::string fileName, GainMap& output);
 
 // STL (ASCII or binary) -> triangles with the file normals and coplanar ids
 // (Preprocessor.cpp:69-114).  Returns false when the file cannot be read.

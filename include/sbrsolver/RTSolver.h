@@ -13,6 +13,7 @@
 
 #include "../sbr_capi.h"
 #include "BVH.h"
+#include "Preprocessor.h"
 #include "Triangle.h"
 #include "VecMatDef.h"
 
@@ -33,7 +34,7 @@ class Paths {
 };
 
 struct RTSolverOptions {
-  int antennaType = 1;        // FieldCompute.cpp:117 hard-codes 1 (Hertzian dipole)
+  int antennaType = 1;        // FieldCompute.cpp:117 hard-codes 1 (Hertzian dipole); 3: SetTxPattern's table
   int receptionModel = 0;     // 0: reference sphere (RTSolver.cpp:838-841); 1: alpha*L/sqrt(3)
   int maxTotal = 0;           // cap on reflections + transmissions, 0 = none (reference)
   int q1MeshIndexing = 1;     // 1: surface data of mesh[hitSurfaceID] (reference); 0: by id
@@ -62,6 +63,10 @@ class RTSolver {
   const std::vector<std::vector<std::pair<int, std::vector<Ray>>>>& ReceiverPaths() const { return _kept; }
   const sbr_stats& Stats() const { return _stats; }
   const char* LastError() const;
+  // Transmitter pattern from a TXPattern.dat map (Preprocessor::ReadPatternFile), used by the
+  // following traces (sets options.antennaType = 3).  SBR_E_ARG unless the map is a full, evenly
+  // spaced (theta, phi) grid.
+  int SetTxPattern(const GainMap& pattern);
 
  protected:
   MaterialProperties* _materialProperties = nullptr;
@@ -79,6 +84,8 @@ class RTSolver {
 
  private:
   sbr_ctx* _ctx = nullptr;
+  std::vector<float> _pattern;   // GainInfo rows, theta-major
+  int _patternRows = 0, _patternCols = 0, _patternVersion = 0, _patternSent = -1;
   const void* _uploadedMesh = nullptr;
   const void* _uploadedBvh = nullptr;
   size_t _uploadedSize = 0;

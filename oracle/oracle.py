@@ -233,6 +233,24 @@ def trees(scene, params):
     return out
 
 
+def set_pattern(gain_info):
+    """Tabulated transmitter pattern for antennaType 3: gain_info of shape (n_theta, n_phi, 4)."""
+    g = np.ascontiguousarray(gain_info, dtype=np.float32)
+    L = lib()
+    L.orc_set_pattern.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    if L.orc_set_pattern(g.ctypes.data, g.shape[0], g.shape[1]) != 0:
+        raise ValueError("bad pattern table")
+
+
+def pattern_eval(theta, phi, pt=1.0):
+    """(E_theta, E_phi) of the tabulated pattern at (theta, phi) radians."""
+    out = np.zeros(4)
+    L = lib()
+    L.orc_pattern_eval.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p]
+    L.orc_pattern_eval(float(theta), float(phi), float(pt), out.ctypes.data)
+    return complex(out[0], out[1]), complex(out[2], out[3])
+
+
 def fresnel(theta_i, rel_perm, sigma, freq, width=0.0, inf_wall=True):
     out = np.zeros(8)
     lib().orc_fresnel(theta_i, rel_perm, sigma, freq, width, int(bool(inf_wall)), _p(out, C.c_double))

@@ -16,6 +16,10 @@
 //     data/output/RayPath.vtk, RxLocations.vtk, ElectricField.csv (ground.stl, tess 2).
 //   * BVH closest hit vs brute force over all triangles, Fresnel closed forms at normal
 //     incidence, image-point reflection, direct field closed form: tests/test_oracle_*.py.
+//   * Tabulated transmitter pattern (antennaType 3, an extension: the reference reads
+//     TXPattern.dat, Preprocessor.cpp:3-35, but never uses it): bilinear interpolation pinned by
+//     tables sampled from functions it reproduces exactly, and by the Hertzian dipole at grid angles
+//     (tests/test_pattern.py).
 //   * Multi-bounce (R>=2), transmission, oblique lossy Fresnel, half-wave / isotropic patterns:
 //     parity unpinned by reference output (no reference vector covers them); they rest on this
 //     restatement and on the invariant / closed-form tests.
@@ -57,6 +61,41 @@ static const double kEPS = (double)std::numeric_limits<float>::epsilon();  // Co
 static const double kINF = std::numeric_limits<double>::infinity();       // Constants.h:16
 typedef std::complex<double> cd;
 static const cd kJ(0.0, 1.0);                                  // Constants.h:22
+
+// Tabulated transmitter pattern (no reference counterpart beyond the GainInfo rows it reads,
+// DataStructures.h:32-37): grid theta_i = i * 180 / (nt - 1), phi_j = j * 360 / np degrees, per point
+// the coefficients j sqrt(60 G) e^{j psi} of the theta and phi fields (G from dB, psi in degrees,
+// FieldCompute.cpp:114-116), interpolated bilinearly in degrees, periodic in phi.
+static std::vector<cd> gPatTheta, gPatPhi;
+static int gPatNT = 0, gPatNP = 0;
+
+static void SetTablePattern(const float* gi, int nt, int np) {
+  gPatTheta.assign((size_t)nt * np, cd(0, 0));
+  gPatPhi.assign((size_t)nt * np, cd(0, 0));
+  for (size_t k = 0; k < (size_t)nt * np; k++) {
+    for (int h = 0; h < 2; h++) {
+      double gdb = gi[4 * k + 2 * h], psi = gi[4 * k + 2 * h + 1] * (kPI / 180.0);
+      cd c = kJ * std::sqrt(60.0 * std::pow(10.0, gdb / 10.0)) * cd(std::cos(psi), std::sin(psi));
+      (h == 0 ? gPatTheta : gPatPhi)[k] = c;
+    }
+  }
+  gPatNT = nt;
+  gPatNP = np;
+}
+
+static void TablePattern(double theta, double phi, cd* eth, cd* eph) {
+  double u = theta * (180.0 / kPI) * (gPatNT - 1) / 180.0;
+  int i0 = std::min(std::max((int)std::floor(u), 0), gPatNT - 2);
+  double a = u - i0;
+  double v = phi * (180.0 / kPI) * gPatNP / 360.0;
+  double fv = std::floor(v), b = v - fv;
+  int j0 = (((int)fv % gPatNP) + gPatNP) % gPatNP, j1 = (j0 + 1) % gPatNP;
+  size_t k00 = (size_t)i0 * gPatNP + j0, k01 = (size_t)i0 * gPatNP + j1;
+  size_t k10 = (size_t)(i0 + 1) * gPatNP + j0, k11 = (size_t)(i0 + 1) * gPatNP + j1;
+  double w00 = (1 - a) * (1 - b), w01 = (1 - a) * b, w10 = a * (1 - b), w11 = a * b;
+  *eth = w00 * gPatTheta[k00] + w01 * gPatTheta[k01] + w10 * gPatTheta[k10] + w11 * gPatTheta[k11];
+  *eph = w00 * gPatPhi[k00] + w01 * gPatPhi[k01] + w10 * gPatPhi[k10] + w11 * gPatPhi[k11];
+}
 
 struct V3 {
   double x, y, z;
@@ -860,8 +899,15 @@ struct Tracer {
   }
 
   // FieldCompute.cpp:153-183
-  V3c Pattern(int type, double theta, double /*phi*/, double pt) const {
+  V3c Pattern(int type, double theta, double phi, double pt) const {
     V3c e = {cd(0, 0), cd(0, 0), cd(0, 0)};
+    if (type == 3) {   // tabulated pattern (extension, see TablePattern)
+      cd eth, eph;
+      TablePattern(theta, phi, &eth, &eph);
+      e.y = std::sqrt(pt) * eth;
+      e.z = std::sqrt(pt) * eph;
+      return e;
+    }
     if (type == 0) { e.x = cd(0, std::sqrt(60 * pt)); e.y = 0; e.z = 0; }
     if (type == 1) { e.x = 0; e.y = cd(0, std::sqrt(60 * pt * 1.5) * std::sin(theta)); e.z = 0; }
     else if (type == 2) {
@@ -1341,6 +1387,22 @@ void orc_trees_get(void* dp, int* launch, int* depth, long long* code, int* tri,
   memcpy(pl, d->pl.data(), n * sizeof(double));
 }
 void orc_trees_free(void* d) { delete (TreeDump*)d; }
+
+// Tabulated pattern for antennaType 3 (rows as in sbr_set_tx_pattern); not thread-safe against a
+// running trace.
+int orc_set_pattern(const float* gain_info, int n_theta, int n_phi) {
+  if (!gain_info || n_theta < 2 || n_phi < 1) return 1;
+  orc::SetTablePattern(gain_info, n_theta, n_phi);
+  return 0;
+}
+// Field of the tabulated pattern at (theta, phi) radians for power pt: out = theta re, im, phi re, im.
+void orc_pattern_eval(double theta, double phi, double pt, double* out) {
+  std::complex<double> a, b;
+  orc::TablePattern(theta, phi, &a, &b);
+  a *= std::sqrt(pt);
+  b *= std::sqrt(pt);
+  out[0] = a.real(); out[1] = a.imag(); out[2] = b.real(); out[3] = b.imag();
+}
 
 // Fresnel coefficients for tests (FieldCompute.cpp:237-302): out = [rTM, tTM, rTE, tTE] as re,im pairs.
 void orc_fresnel(double thetaI, double relPerm, double sigma, double freq, double width, int infWall, double* out) {

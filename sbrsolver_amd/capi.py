@@ -96,6 +96,7 @@ SIGNATURES = {
     "sbr_scene_triangles": ([_vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "sbr_scene_bvh_dfs": ([_vp, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "sbr_materials": ([_vp, _vp, _i32], C.c_int),
+    "sbr_set_tx_pattern": ([_vp, _vp, _i32, _i32], C.c_int),
     "sbr_set_tuning": ([_vp, _vp], C.c_int),
     "sbr_get_tuning": ([_vp, _vp], C.c_int),
     "sbr_autotune": ([_vp, _vp, _vp, _i32, _vp], C.c_int),
@@ -301,6 +302,15 @@ class Context:
     def materials(self, mats=None):
         m = DEFAULT_MATERIALS if mats is None else np.ascontiguousarray(mats, dtype=np.float64).reshape(-1, 6)
         self._chk(lib().sbr_materials(self.h, _a(m), m.shape[0]))
+
+    def set_tx_pattern(self, gain_info):
+        """Tabulated transmitter pattern for antennaType 3 (sbr_set_tx_pattern): gain_info of shape
+        (n_theta, n_phi, 4) = (theta gain dB, theta phase deg, phi gain dB, phi phase deg) at
+        theta = i * 180 / (n_theta - 1), phi = j * 360 / n_phi degrees."""
+        g = np.ascontiguousarray(gain_info, dtype=np.float32)
+        if g.ndim != 3 or g.shape[2] != 4:
+            raise ValueError("gain_info must have shape (n_theta, n_phi, 4)")
+        self._chk(lib().sbr_set_tx_pattern(self.h, _a(g), g.shape[0], g.shape[1]))
 
     def set_tuning(self, lds_scene=-1, blocks_per_cu=0, window_nodes=0):
         t = Tuning(int(lds_scene), int(blocks_per_cu), int(window_nodes))

@@ -280,6 +280,10 @@ int Engine::clone_scene(const Engine& s) {
   }
   haveScene = true;
   haveResult = false;
+  if (s.patNT > 0) {
+    const int rc = set_pattern(s.hPattern.data(), s.patNT, s.patNP);
+    if (rc != SBR_OK) return rc;
+  }
   return set_materials(reinterpret_cast<const sbr_material*>(s.mats.data()), (int)(s.mats.size() / 6));
 }
 
@@ -412,6 +416,10 @@ int sbr_scene_bvh_dfs(sbr_ctx* ctx, double* box, int32_t* a, int32_t* b, int32_t
 
 int sbr_materials(sbr_ctx* ctx, const sbr_material* mats, int32_t count) {
   return guarded(ctx, [&] { return ctx->eng.set_materials(mats, count); });
+}
+
+int sbr_set_tx_pattern(sbr_ctx* ctx, const float* gain_info, int32_t n_theta, int32_t n_phi) {
+  return guarded(ctx, [&] { return ctx->eng.set_pattern(gain_info, n_theta, n_phi); });
 }
 
 int sbr_set_tuning(sbr_ctx* ctx, const sbr_tuning* t) {

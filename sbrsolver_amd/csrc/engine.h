@@ -235,6 +235,11 @@ struct Engine {
   DBuf<DSurf> dSurfQ1, dSurfId;
   std::vector<double> mats;                 // 6 per material
   DBuf<double> dMats;
+  // tabulated transmitter pattern (antennaType 3): the rows as given (4 floats per grid point) and
+  // the device table of field coefficients (4 doubles per grid point)
+  std::vector<float> hPattern;
+  DBuf<double> dPattern;
+  int32_t patNT = 0, patNP = 0;
   bool haveScene = false;
 
   // launch directions per icosahedron level (generated once, kept)
@@ -292,6 +297,7 @@ struct Engine {
   int upload(const sbr_scene_desc* d);
   int clone_scene(const Engine& src);   // scene and materials of another engine (device copies made here)
   int set_materials(const sbr_material* m, int n);
+  int set_pattern(const float* gainInfo, int nTheta, int nPhi);
   int rays_for(int tess, const double** out, int64_t* nOut);   // cached launch directions
   int trace(const sbr_trace_params* p, int ntx, const double* rx, int m);
   int closest_hit(const double* o, const double* d, int n, int exact, int32_t* tri, double* pt, double* dist);

@@ -1092,7 +1092,34 @@ __device__ __forceinline__ void cart_to_sph(V3 v, double* r, double* th, double*
 struct FieldArgs {
   int64_t winners;
   int32_t nrx;
+  // Tabulated transmitter pattern (antennaType 3): patNT x patNP grid points, theta-major, theta
+  // from 0 to 180 degrees and phi from 0 to 360 - 360 / patNP degrees; 4 doubles per point, the
+  // theta and phi field coefficients j sqrt(60 G) e^{j psi} as (re, im) pairs.
+  const double* pat;
+  int32_t patNT, patNP;
 };
+
+// Field of the tabulated pattern in the transmitter's spherical frame (per unit power): the
+// coefficients bilinearly interpolated in (theta, phi) degrees, periodic in phi.
+__device__ __forceinline__ void pattern_lookup(const FieldArgs& F, double th, double ph, cx* eth, cx* eph) {
+  const double u = th * (180.0 / SBR_PI) * (double)(F.patNT - 1) / 180.0;
+  int i0 = (int)floor(u);
+  i0 = i0 < 0 ? 0 : (i0 > F.patNT - 2 ? F.patNT - 2 : i0);
+  const double a = u - (double)i0;
+  const double v = ph * (180.0 / SBR_PI) * (double)F.patNP / 360.0;
+  const double fv = floor(v);
+  const double b = v - fv;
+  const int j0 = (((int)fv % F.patNP) + F.patNP) % F.patNP, j1 = (j0 + 1) % F.patNP;
+  const double w00 = (1.0 - a) * (1.0 - b), w01 = (1.0 - a) * b, w10 = a * (1.0 - b), w11 = a * b;
+  const double* c00 = F.pat + 4 * ((int64_t)i0 * F.patNP + j0);
+  const double* c01 = F.pat + 4 * ((int64_t)i0 * F.patNP + j1);
+  const double* c10 = F.pat + 4 * ((int64_t)(i0 + 1) * F.patNP + j0);
+  const double* c11 = F.pat + 4 * ((int64_t)(i0 + 1) * F.patNP + j1);
+  double o[4];
+  for (int q = 0; q < 4; q++) o[q] = w00 * c00[q] + w01 * c01[q] + w10 * c10[q] + w11 * c11[q];
+  *eth = mkc(o[0], o[1]);
+  *eph = mkc(o[2], o[3]);
+}
 
 // winTxRx / winPerm: kept paths sorted by (transmitter * nrx + receiver), each pointing into the
 // order-key-sorted arrays (winNode).
@@ -1146,6 +1173,12 @@ __global__ void __launch_bounds__(256) k_field(FieldArgs F, const TxDesc* __rest
       if (K.antennaType == 0) inc.x = mkc(0, sqrt(60 * K.pt));
       else if (K.antennaType == 1) inc.y = mkc(0, sqrt(60 * K.pt * 1.5) * sin(th));
       else if (K.antennaType == 2) inc.y = mkc(0, sqrt(60 * K.pt) * (cos(SBR_PI * cos(th) / 2.0) / sin(th)));
+      else if (K.antennaType == 3) {
+        cx eth, eph;
+        pattern_lookup(F, th, ph, &eth, &eph);
+        inc.y = cscale(eth, sqrt(K.pt));
+        inc.z = cscale(eph, sqrt(K.pt));
+      }
     } else {
       inc = matT_vec_c(cur, tot);
       double st = sin(th), sp = sin(ph), ct = cos(th), cp = cos(ph);
@@ -1266,6 +1299,30 @@ int Engine::set_materials(const sbr_material* m, int n) {
   HIPCHK(hipSetDevice(device));
   ENSURE(dMats, mats.size());
   HIPCHK(hipMemcpyAsync(dMats.p, mats.data(), mats.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SBR_OK;
+}
+
+int Engine::set_pattern(const float* gi, int nt, int np) {
+  if (!gi || nt < 2 || np < 1 || (int64_t)nt * np > (int64_t(1) << 24))
+    return fail(SBR_E_ARG, "pattern: need n_theta >= 2, n_phi >= 1 and a table of n_theta * n_phi rows");
+  std::vector<double> c(4 * (size_t)nt * np);
+  for (size_t k = 0; k < (size_t)nt * np; k++) {
+    for (int h = 0; h < 2; h++) {   // theta, phi: gain dB, phase degrees -> j sqrt(60 G) e^{j psi}
+      const double g = gi[4 * k + 2 * h], psi = gi[4 * k + 2 * h + 1];
+      if (!std::isfinite(g) || !std::isfinite(psi)) return fail(SBR_E_ARG, "pattern: values must be finite");
+      const double amp = std::sqrt(60.0 * std::pow(10.0, g / 10.0)), ang = psi * (3.141592653589793 / 180.0);
+      c[4 * k + 2 * h] = -amp * std::sin(ang);
+      c[4 * k + 2 * h + 1] = amp * std::cos(ang);
+    }
+  }
+  hPattern.assign(gi, gi + 4 * (size_t)nt * np);
+  patNT = nt;
+  patNP = np;
+  if (device < 0) return SBR_OK;
+  HIPCHK(hipSetDevice(device));
+  ENSURE(dPattern, c.size());
+  HIPCHK(hipMemcpyAsync(dPattern.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice, stream));
   HIPCHK(hipStreamSynchronize(stream));
   return SBR_OK;
 }
@@ -1538,7 +1595,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     maxDepth = std::max(maxDepth, md);
     const int stride = p.launchStride > 0 ? p.launchStride : 1;
     if (p.launchOffset < 0 || p.launchOffset >= stride) return fail(SBR_E_ARG, "launchOffset must be in [0, stride)");
-    if (p.antennaType < 0 || p.antennaType > 2) return fail(SBR_E_ARG, "antennaType must be 0, 1 or 2");
+    if (p.antennaType < 0 || p.antennaType > 3) return fail(SBR_E_ARG, "antennaType must be 0, 1, 2 or 3");
+    if (p.antennaType == 3 && patNT == 0) return fail(SBR_E_STATE, "antennaType 3 needs a pattern (sbr_set_tx_pattern)");
     if (p.receptionModel < 0 || p.receptionModel > 1) return fail(SBR_E_ARG, "receptionModel must be 0 or 1");
     if (!(p.frequency > 0)) return fail(SBR_E_ARG, "frequency must be > 0");
     exact = exact || p.exactTraversal;
@@ -1906,7 +1964,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         // stable by tx * nrx + rx -> dWinRx (keys), dWinIdx2 (index into the key-sorted arrays)
         HIPCHK(hipcub::DeviceRadixSort::SortPairs(dScratch.p, tb2, dWinRx2.p, dWinRx.p, dWinIdx.p, dWinIdx2.p, (int)Wn,
                                                   0, txrxBits, stream));
-        FieldArgs FA{Wn, m};
+        FieldArgs FA{Wn, m, dPattern.p, patNT, patNP};
         k_field<<<blocks_for(Wn, 256), 256, 0, stream>>>(FA, dTx.p, dTree.p, dTris.p, surf, dMats.p, dRx.p, dWinRx.p,
                                                           dWinIdx2.p, dWinNode.p, dWinField.p);
         HIPCHK(hipGetLastError());

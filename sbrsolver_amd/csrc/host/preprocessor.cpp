@@ -177,6 +177,30 @@ bool SoupToGeometry(const float* coords9, const float* normals3, const int32_t* 
   return make_triangles(coords, normals, tris, mats, output);
 }
 
+bool ReadPatternFile(std::string fileName, GainMap& output) {
+  std::ifstream in(fileName);
+  if (!in) return false;
+  std::string line, tok;
+  while (std::getline(in, line)) {
+    std::istringstream ls(line);
+    std::vector<std::string> w;
+    while (ls >> tok) w.push_back(tok);
+    if (w.size() != 6) continue;
+    float v[6];
+    bool ok = true;
+    for (int i = 0; i < 6 && ok; i++) {
+      char* end = nullptr;
+      v[i] = std::strtof(w[i].c_str(), &end);
+      ok = end && *end == 0 && end != w[i].c_str();
+    }
+    if (!ok) continue;
+    GainInfo g{};
+    for (int i = 0; i < 4; i++) g.data[i] = v[i + 2];
+    output.insert({ThetaPhiAngle{v[0], v[1]}, g});   // first entry of an angle pair wins, as std::map::insert
+  }
+  return true;
+}
+
 bool ReadLocationFile(std::string fileName, VecVec3& output) {
   std::ifstream in(fileName);
   if (!in) return false;

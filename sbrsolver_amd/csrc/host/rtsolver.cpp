@@ -30,6 +30,42 @@ void RTSolver::Cleanup() {
 
 const char* RTSolver::LastError() const { return _ctx ? sbr_last_error(_ctx) : "no context"; }
 
+// The map must hold a full grid: theta 0..180 degrees and phi 0..360 - dphi degrees, each evenly
+// spaced, every (theta, phi) pair present.
+int RTSolver::SetTxPattern(const GainMap& pattern) {
+  std::vector<float> th, ph;
+  for (const auto& kv : pattern) {
+    th.push_back(kv.first.first);
+    ph.push_back(kv.first.second);
+  }
+  auto uniq = [](std::vector<float>& v) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  };
+  uniq(th);
+  uniq(ph);
+  const int nt = (int)th.size(), np = (int)ph.size();
+  if (nt < 2 || np < 1 || (size_t)nt * np != pattern.size() || th.front() != 0.0f || th.back() != 180.0f ||
+      ph.front() != 0.0f)
+    return SBR_E_ARG;
+  for (int i = 0; i < nt; i++)
+    if (std::fabs(th[i] - 180.0 * i / (nt - 1)) > 1e-3) return SBR_E_ARG;
+  for (int j = 0; j < np; j++)
+    if (std::fabs(ph[j] - 360.0 * j / np) > 1e-3) return SBR_E_ARG;
+  std::vector<float> rows(4 * (size_t)nt * np);
+  for (int i = 0; i < nt; i++)
+    for (int j = 0; j < np; j++) {
+      const GainInfo& g = pattern.at(ThetaPhiAngle{th[i], ph[j]});
+      std::copy(g.data, g.data + 4, rows.begin() + 4 * ((size_t)i * np + j));
+    }
+  _pattern.swap(rows);
+  _patternRows = nt;
+  _patternCols = np;
+  _patternVersion++;
+  options.antennaType = 3;
+  return SBR_OK;
+}
+
 int RTSolver::EnsureContext() {
   if (_ctx) return SBR_OK;
   return sbr_create(options.device, &_ctx);
@@ -66,6 +102,11 @@ int RTSolver::ExecuteRayTracing(Vec3 sourcePoint, const Mat3& txCoordSys, int ma
   }
   rc = sbr_materials(_ctx, reinterpret_cast<const sbr_material*>(_materialProperties), _materialCount);
   if (rc) return rc;
+  if (_patternRows > 0 && _patternSent != _patternVersion) {
+    rc = sbr_set_tx_pattern(_ctx, _pattern.data(), _patternRows, _patternCols);
+    if (rc) return rc;
+    _patternSent = _patternVersion;
+  }
   sbr_trace_params p{};
   for (int a = 0; a < 3; a++) p.tx[a] = sourcePoint(a);
   txCoordSys.to_row_major(p.txCoordSys);
