@@ -106,6 +106,9 @@ typedef struct {
   int32_t q1MeshIndexing;/* 1 (reference): surface data of mesh[hitSurfaceID]; 0: by id */
   int32_t exactTraversal;/* 1: visit every box the line touches (reference order); 0: clipped */
   int32_t countTraversal;/* 1: count BVH node visits and triangle tests (sbr_stats) */
+  double wallWidth;      /* extension: > 0 with useFresnel = 1: every wall is a slab of this thickness
+                            (m) with the ITU-R P.2040 slab coefficients, so transmissions carry field;
+                            0: the reference's infinite wall (reflection Gamma, transmission 0) */
 } sbr_trace_params;
 
 /* Counters and stage times of the last trace. */

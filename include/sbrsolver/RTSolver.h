@@ -38,6 +38,7 @@ struct RTSolverOptions {
   int receptionModel = 0;     // 0: reference sphere (RTSolver.cpp:838-841); 1: alpha*L/sqrt(3)
   int maxTotal = 0;           // cap on reflections + transmissions, 0 = none (reference)
   int q1MeshIndexing = 1;     // 1: surface data of mesh[hitSurfaceID] (reference); 0: by id
+  double wallWidth = 0;       // > 0: slab walls of this thickness in m (physical mode); 0: reference walls
   int device = 0;
 };
 

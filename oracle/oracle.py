@@ -21,7 +21,7 @@ class Params(C.Structure):
         ("freq", C.c_double), ("pt", C.c_double),
         ("useFresnel", C.c_int), ("tess", C.c_int), ("antennaType", C.c_int),
         ("receptionModel", C.c_int), ("launchStride", C.c_int), ("launchOffset", C.c_int),
-        ("q1MeshIndexing", C.c_int), ("countCanonical", C.c_int),
+        ("q1MeshIndexing", C.c_int), ("countCanonical", C.c_int), ("wallWidth", C.c_double),
     ]
 
 
@@ -75,7 +75,7 @@ def _p(a, t):
 
 def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None, maxTotal=0, useFresnel=True,
                 antennaType=1, receptionModel=0, launchStride=1, launchOffset=0, q1MeshIndexing=1,
-                countCanonical=0):
+                countCanonical=0, wallWidth=0.0):
     p = Params()
     for i in range(3):
         p.tx[i] = float(tx[i])
@@ -88,6 +88,7 @@ def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None
     p.receptionModel, p.launchStride, p.launchOffset = int(receptionModel), int(launchStride), int(launchOffset)
     p.q1MeshIndexing = int(q1MeshIndexing)
     p.countCanonical = int(countCanonical)
+    p.wallWidth = float(wallWidth)
     return p
 
 
@@ -251,8 +252,10 @@ def pattern_eval(theta, phi, pt=1.0):
     return complex(out[0], out[1]), complex(out[2], out[3])
 
 
-def fresnel(theta_i, rel_perm, sigma, freq, width=0.0, inf_wall=True):
+def fresnel(theta_i, rel_perm, sigma, freq, width=0.0, inf_wall=True, slab=False):
+    """Reflection / transmission coefficients: the reference's (inf_wall, or its slab branch of the
+    given width), or with slab=True the ITU-R P.2040 slab of the given width (extension)."""
     out = np.zeros(8)
-    lib().orc_fresnel(theta_i, rel_perm, sigma, freq, width, int(bool(inf_wall)), _p(out, C.c_double))
+    lib().orc_fresnel(theta_i, rel_perm, sigma, freq, width, 2 if slab else int(bool(inf_wall)), _p(out, C.c_double))
     c = out[0::2] + 1j * out[1::2]
     return dict(rTM=c[0], tTM=c[1], rTE=c[2], tTE=c[3])

@@ -734,6 +734,7 @@ struct Params {
   int launchStride, launchOffset;
   int q1MeshIndexing;     // 1: mesh[hitSurfaceID] on the BVH-sorted vector (reference), 0: by id
   int countCanonical = 0; // 1: also run BVH::CanonicalCount per bounce (work count only)
+  double wallWidth = 0;   // > 0 with useFresnel: slab walls (extension, see SlabCoeffs)
 };
 
 struct Tracer {
@@ -940,6 +941,21 @@ struct Tracer {
       tran = ((1.0 - g * g) * ph) / den;
     }
   }
+  // Slab of thickness width (extension; ITU-R P.2040 eqs. for a dielectric slab in air): the
+  // reference's slab branch (above, never reached with width > 0 by the reference) uses
+  // sqrt(n_t - sin^2) for the phase thickness and e^{-2jq} on the transmitted wave; here
+  // q = (2 pi d / lambda) sqrt(eps_t - sin^2 theta_i), R = G (1 - e^{-2jq}) / (1 - G^2 e^{-2jq}),
+  // T = (1 - G^2) e^{-jq} / (1 - G^2 e^{-2jq}), with G the half-space coefficient above.
+  static void SlabCoeffs(bool te, cd ti, cd tt, cd ei, cd et, double lamda, double width, cd& ref, cd& tran) {
+    cd g, unusedT;
+    Coeffs(te, ti, tt, ei, et, lamda, 0.0, true, g, unusedT);
+    cd si = std::sin(ti);
+    cd q = (kTWOPI * width / lamda) * std::sqrt(et - si * si);
+    cd e1 = std::exp(-kJ * q), e2 = e1 * e1;
+    cd den = 1.0 - g * g * e2;
+    ref = g * (1.0 - e2) / den;
+    tran = (1.0 - g * g) * e1 / den;
+  }
   // FieldCompute.cpp:185-235
   V3c RefTrans(bool refl, const V3c& e, double relPerm, double sigma, double freq, double thetaI, double width,
                bool infWall) const {
@@ -948,8 +964,13 @@ struct Tracer {
     cd ei(1, 0);
     cd tt = TransAngle(thetaI, ei, et);
     cd rTE, rTM, tTE, tTM;
-    Coeffs(false, cd(thetaI, 0), tt, ei, et, lamda, width, infWall, rTM, tTM);
-    Coeffs(true, cd(thetaI, 0), tt, ei, et, lamda, width, infWall, rTE, tTE);
+    if (infWall && p.wallWidth > 0) {
+      SlabCoeffs(false, cd(thetaI, 0), tt, ei, et, lamda, p.wallWidth, rTM, tTM);
+      SlabCoeffs(true, cd(thetaI, 0), tt, ei, et, lamda, p.wallWidth, rTE, tTE);
+    } else {
+      Coeffs(false, cd(thetaI, 0), tt, ei, et, lamda, width, infWall, rTM, tTM);
+      Coeffs(true, cd(thetaI, 0), tt, ei, et, lamda, width, infWall, rTE, tTE);
+    }
     V3c r;
     r.x = e.x;
     if (refl) { r.y = e.y * rTM; r.z = e.z * rTE; }
@@ -1205,6 +1226,7 @@ typedef struct {
   double freq, pt;
   int useFresnel, tess, antennaType, receptionModel, launchStride, launchOffset, q1MeshIndexing;
   int countCanonical;
+  double wallWidth;
 } orc_params;
 
 void* orc_scene_from_stl(const char* path) {
@@ -1334,6 +1356,7 @@ void* orc_trace(void* sp, const orc_params* pp, const double* rx, int nrx, const
   p.antennaType = pp->antennaType; p.receptionModel = pp->receptionModel;
   p.launchStride = pp->launchStride; p.launchOffset = pp->launchOffset; p.q1MeshIndexing = pp->q1MeshIndexing;
   p.countCanonical = pp->countCanonical;
+  p.wallWidth = pp->wallWidth;
   return Trace((Scene*)sp, p, rx, nrx, mats, nmat, nthreads);
 }
 void orc_result_free(void* r) { delete (Result*)r; }
@@ -1373,6 +1396,7 @@ void* orc_trees(void* sp, const orc_params* pp) {
   p.freq = pp->freq; p.pt = pp->pt; p.useFresnel = pp->useFresnel; p.tess = pp->tess;
   p.antennaType = pp->antennaType; p.receptionModel = pp->receptionModel;
   p.launchStride = pp->launchStride; p.launchOffset = pp->launchOffset; p.q1MeshIndexing = pp->q1MeshIndexing;
+  p.wallWidth = pp->wallWidth;
   return DumpTrees((Scene*)sp, p);
 }
 int orc_trees_n(void* d) { return (int)((TreeDump*)d)->launch.size(); }
@@ -1411,8 +1435,13 @@ void orc_fresnel(double thetaI, double relPerm, double sigma, double freq, doubl
   double lamda = kC0 / freq;
   cd tt = Tracer::TransAngle(thetaI, ei, et);
   cd rTE, rTM, tTE, tTM;
-  Tracer::Coeffs(false, cd(thetaI, 0), tt, ei, et, lamda, width, infWall != 0, rTM, tTM);
-  Tracer::Coeffs(true, cd(thetaI, 0), tt, ei, et, lamda, width, infWall != 0, rTE, tTE);
+  if (infWall == 2) {   // slab of the given width (extension)
+    Tracer::SlabCoeffs(false, cd(thetaI, 0), tt, ei, et, lamda, width, rTM, tTM);
+    Tracer::SlabCoeffs(true, cd(thetaI, 0), tt, ei, et, lamda, width, rTE, tTE);
+  } else {
+    Tracer::Coeffs(false, cd(thetaI, 0), tt, ei, et, lamda, width, infWall != 0, rTM, tTM);
+    Tracer::Coeffs(true, cd(thetaI, 0), tt, ei, et, lamda, width, infWall != 0, rTE, tTE);
+  }
   cd v[4] = {rTM, tTM, rTE, tTE};
   for (int i = 0; i < 4; i++) { out[2 * i] = v[i].real(); out[2 * i + 1] = v[i].imag(); }
 }

@@ -405,8 +405,11 @@ __device__ __forceinline__ V3C matT_vec_c(const double* m, const V3C& v) {   // 
 // TE/TM coefficients (FieldCompute.cpp:237-302) for incidence thetaI from air onto a material
 // of relative permittivity relPerm and conductivity sigma; returns the two factors applied to
 // (E_theta, E_phi): reflection (refl=true) or transmission.
+// infWall (useFresnelCoeff) with slabWidth > 0: a slab of that thickness (ITU-R P.2040: phase thickness
+// q = k d sqrt(eps - sin^2 theta), R = G (1 - e^{-2jq}) / (1 - G^2 e^{-2jq}), T = (1 - G^2) e^{-jq} /
+// (1 - G^2 e^{-2jq})), an extension; slabWidth = 0 keeps the reference's walls.
 __device__ __forceinline__ void fresnel(bool refl, double thetaI, double relPerm, double sigma, double freq,
-                                        bool infWall, cx* fTM, cx* fTE) {
+                                        bool infWall, double slabWidth, cx* fTM, cx* fTE) {
   const double U0 = 4 * SBR_PI * 1e-7;
   const double E0 = 1.0 / (U0 * SBR_C0 * SBR_C0);
   const double ETA0 = U0 * SBR_C0;
@@ -420,6 +423,21 @@ __device__ __forceinline__ void fresnel(bool refl, double thetaI, double relPerm
   cx gtm = cdiv(cadd(cmul(cneg(etat), ct), cmul(etai, ci)), cadd(cmul(etat, ct), cmul(etai, ci)));
   cx a = cdiv(etat, ct), b = cdiv(etai, ci);
   cx gte = cneg(cdiv(csub(a, b), cadd(a, b)));
+  if (infWall && slabWidth > 0.0) {
+    cx q = cmul(mkc(2 * SBR_PI * slabWidth / (SBR_C0 / freq), 0.0), csqrt_(csub(et, cmul(si, si))));
+    cx e1 = cexp_(cmul(mkc(0.0, -1.0), q)), e2 = cmul(e1, e1);
+    cx g2[2] = {gtm, gte};
+    cx outR[2], outT[2];
+    for (int k = 0; k < 2; k++) {
+      cx g = g2[k];
+      cx den = csub(mkc(1.0, 0.0), cmul(cmul(g, g), e2));
+      outR[k] = cdiv(cmul(g, csub(mkc(1.0, 0.0), e2)), den);
+      outT[k] = cdiv(cmul(csub(mkc(1.0, 0.0), cmul(g, g)), e1), den);
+    }
+    if (refl) { *fTM = outR[0]; *fTE = outR[1]; }
+    else { *fTM = outT[0]; *fTE = outT[1]; }
+    return;
+  }
   if (infWall) {
     if (refl) { *fTM = gtm; *fTE = gte; }
     else { *fTM = mkc(0, 0); *fTE = mkc(0, 0); }

@@ -100,6 +100,7 @@ struct TxDesc {
   int32_t maxR, maxT, maxTotal;
   int32_t useFresnel, antennaType, receptionModel;
   double capK;               // rcoef / (1 - rcoef) rounded up (INFINITY for rcoef >= 1): capture prefilter
+  double wallWidth;          // > 0: slab walls of this thickness (physical mode), 0: reference walls
 };
 
 // ---- interaction tree ---------------------------------------------------------------------------

@@ -1189,7 +1189,8 @@ __global__ void __launch_bounds__(256) k_field(FieldArgs F, const TxDesc* __rest
       double ang = angle_between_rad(vg, v3(cur[6], cur[7], cur[8]));
       int mat = tris[tree[chain[k - 1]].hitDfs].mat;
       cx fTM, fTE;
-      fresnel(sg.type == 1, ang, mats[6 * mat + 3], mats[6 * mat + 5], K.freq, K.useFresnel != 0, &fTM, &fTE);
+      fresnel(sg.type == 1, ang, mats[6 * mat + 3], mats[6 * mat + 5], K.freq, K.useFresnel != 0, K.wallWidth, &fTM,
+              &fTE);
       tot.x = inc.x;
       tot.y = cmul(inc.y, fTM);
       tot.z = cmul(inc.z, fTE);
@@ -1629,6 +1630,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     T.maxTotal = p.maxTotal;
     T.useFresnel = p.useFresnel;
     T.antennaType = p.antennaType;
+    if (!(p.wallWidth >= 0.0) || !std::isfinite(p.wallWidth)) return fail(SBR_E_ARG, "wallWidth must be finite and >= 0");
+    T.wallWidth = p.wallWidth;
     T.receptionModel = p.receptionModel;
   }
   if (nRoot >= (int64_t(1) << 31)) return fail(SBR_E_LIMIT, "more than 2^31 launches in one trace");

@@ -118,6 +118,7 @@ int RTSolver::ExecuteRayTracing(Vec3 sourcePoint, const Mat3& txCoordSys, int ma
   p.useFresnel = useFresnelCoeff ? 1 : 0;
   p.tess = txTesslation;
   p.antennaType = options.antennaType;
+  p.wallWidth = options.wallWidth;
   p.receptionModel = options.receptionModel;
   p.launchStride = 1;
   p.launchOffset = 0;

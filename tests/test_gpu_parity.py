@@ -106,6 +106,7 @@ CASES = [
     ("urban_L3_physical_indexing", "c2", dict(tess=3, q1MeshIndexing=0), None),
     ("urban_L3_total_cap3", "c2", dict(tess=3, maxTotal=3), None),
     ("urban_L3_isotropic_nofresnel", "c2", dict(tess=3, antennaType=0, useFresnel=False), None),
+    ("urban_L3_slab_walls_model1", "c2", dict(tess=3, receptionModel=1, wallWidth=0.2), None),
 ]
 
 
