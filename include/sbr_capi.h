@@ -109,6 +109,9 @@ typedef struct {
   double wallWidth;      /* extension: > 0 with useFresnel = 1: every wall is a slab of this thickness
                             (m) with the ITU-R P.2040 slab coefficients, so transmissions carry field;
                             0: the reference's infinite wall (reflection Gamma, transmission 0) */
+  int32_t typedDedup;    /* extension: 1 keys duplicate removal by the coplanar ids and the interaction
+                            types (reflection / transmission) of a path; 0: coplanar ids only, as the
+                            reference (RTSolver.cpp:435-459, 875-883) */
 } sbr_trace_params;
 
 /* Counters and stage times of the last trace. */

@@ -735,6 +735,7 @@ struct Params {
   int q1MeshIndexing;     // 1: mesh[hitSurfaceID] on the BVH-sorted vector (reference), 0: by id
   int countCanonical = 0; // 1: also run BVH::CanonicalCount per bounce (work count only)
   double wallWidth = 0;   // > 0 with useFresnel: slab walls (extension, see SlabCoeffs)
+  int typedDedup = 0;     // 1: duplicate removal also keyed by the segments' interaction types (extension)
 };
 
 struct Tracer {
@@ -1132,6 +1133,9 @@ static Result* Trace(Scene* sc, const Params& p, const double* rx, int nrx, cons
       for (auto& lp : surv) {
         std::vector<int> key;
         for (const Ray& r : lp.second) key.push_back(r.hitCoplanarId);
+        if (p.typedDedup)   // extension: then how each segment left its start surface
+          for (const Ray& r : lp.second)
+            key.push_back(r.reflectionMaterialId >= 0 ? 1 : (r.penetrationMaterialId >= 0 ? 2 : 0));
         if (seen.count(key)) continue;
         seen.insert(std::make_pair(key, 0));
         kept[k].push_back(lp);
@@ -1227,6 +1231,7 @@ typedef struct {
   int useFresnel, tess, antennaType, receptionModel, launchStride, launchOffset, q1MeshIndexing;
   int countCanonical;
   double wallWidth;
+  int typedDedup;
 } orc_params;
 
 void* orc_scene_from_stl(const char* path) {
@@ -1357,6 +1362,7 @@ void* orc_trace(void* sp, const orc_params* pp, const double* rx, int nrx, const
   p.launchStride = pp->launchStride; p.launchOffset = pp->launchOffset; p.q1MeshIndexing = pp->q1MeshIndexing;
   p.countCanonical = pp->countCanonical;
   p.wallWidth = pp->wallWidth;
+  p.typedDedup = pp->typedDedup;
   return Trace((Scene*)sp, p, rx, nrx, mats, nmat, nthreads);
 }
 void orc_result_free(void* r) { delete (Result*)r; }
@@ -1397,6 +1403,7 @@ void* orc_trees(void* sp, const orc_params* pp) {
   p.antennaType = pp->antennaType; p.receptionModel = pp->receptionModel;
   p.launchStride = pp->launchStride; p.launchOffset = pp->launchOffset; p.q1MeshIndexing = pp->q1MeshIndexing;
   p.wallWidth = pp->wallWidth;
+  p.typedDedup = pp->typedDedup;
   return DumpTrees((Scene*)sp, p);
 }
 int orc_trees_n(void* d) { return (int)((TreeDump*)d)->launch.size(); }

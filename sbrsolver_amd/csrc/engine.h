@@ -101,6 +101,7 @@ struct TxDesc {
   int32_t useFresnel, antennaType, receptionModel;
   double capK;               // rcoef / (1 - rcoef) rounded up (INFINITY for rcoef >= 1): capture prefilter
   double wallWidth;          // > 0: slab walls of this thickness (physical mode), 0: reference walls
+  int32_t typedDedup;        // 1: duplicate-removal groups also keyed by the interaction types
 };
 
 // ---- interaction tree ---------------------------------------------------------------------------

@@ -618,6 +618,14 @@ __device__ void node_key(const CaptureBufs& B, NodeCtx& c) {
   for (int k = c.depth - 1; k >= 0; k--) { anc[k] = B.tree[q].cop; q = B.tree[q].parent; }
   seq_seed(c.txi, c.depth, &c.seqA, &c.seqB);
   for (int k = 0; k < c.depth; k++) seq_fold(anc[k], k, &c.seqA, &c.seqB);
+  if (B.txs[c.txi].typedDedup) {   // the types of the interactions, leaf first, at positions after the ids
+    int32_t t = c.node;
+    for (int m = c.depth; m >= 1; m--) {
+      const TNode& nd = B.tree[t];
+      seq_fold((int32_t)nd.type + 0x40000000, kMaxDepth + m, &c.seqA, &c.seqB);
+      t = nd.parent;
+    }
+  }
   c.orderKey = order_key(B.tree[c.node]);
   c.keyed = true;
 }
@@ -1632,6 +1640,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     T.antennaType = p.antennaType;
     if (!(p.wallWidth >= 0.0) || !std::isfinite(p.wallWidth)) return fail(SBR_E_ARG, "wallWidth must be finite and >= 0");
     T.wallWidth = p.wallWidth;
+    if (p.typedDedup != 0 && p.typedDedup != 1) return fail(SBR_E_ARG, "typedDedup must be 0 or 1");
+    T.typedDedup = p.typedDedup;
     T.receptionModel = p.receptionModel;
   }
   if (nRoot >= (int64_t(1) << 31)) return fail(SBR_E_LIMIT, "more than 2^31 launches in one trace");

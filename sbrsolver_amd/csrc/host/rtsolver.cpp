@@ -119,6 +119,7 @@ int RTSolver::ExecuteRayTracing(Vec3 sourcePoint, const Mat3& txCoordSys, int ma
   p.tess = txTesslation;
   p.antennaType = options.antennaType;
   p.wallWidth = options.wallWidth;
+  p.typedDedup = options.typedDedup;
   p.receptionModel = options.receptionModel;
   p.launchStride = 1;
   p.launchOffset = 0;
