@@ -112,6 +112,9 @@ typedef struct {
   int32_t typedDedup;    /* extension: 1 keys duplicate removal by the coplanar ids and the interaction
                             types (reflection / transmission) of a path; 0: coplanar ids only, as the
                             reference (RTSolver.cpp:435-459, 875-883) */
+  int32_t checkOcclusion;/* extension: 1 drops a corrected multi-bounce path when a scene triangle lies
+                            on one of its segments (closest hit nearer than the segment's end); 0:
+                            no such test, as the reference (RTSolver.cpp:330-373) */
 } sbr_trace_params;
 
 /* Counters and stage times of the last trace. */

@@ -40,6 +40,7 @@ struct RTSolverOptions {
   int q1MeshIndexing = 1;     // 1: surface data of mesh[hitSurfaceID] (reference); 0: by id
   double wallWidth = 0;       // > 0: slab walls of this thickness in m (physical mode); 0: reference walls
   int typedDedup = 0;         // 1: duplicate removal keyed by coplanar ids and interaction types
+  int checkOcclusion = 0;     // 1: drop corrected paths with a scene triangle on a segment
   int device = 0;
 };
 

@@ -736,6 +736,7 @@ struct Params {
   int countCanonical = 0; // 1: also run BVH::CanonicalCount per bounce (work count only)
   double wallWidth = 0;   // > 0 with useFresnel: slab walls (extension, see SlabCoeffs)
   int typedDedup = 0;     // 1: duplicate removal also keyed by the segments' interaction types (extension)
+  int checkOcclusion = 0; // 1: corrected multi-bounce paths must be unobstructed (extension, Unobstructed)
 };
 
 struct Tracer {
@@ -885,6 +886,18 @@ struct Tracer {
     return true;
   }
   // RTSolver.cpp:375-394
+  // Extension (checkOcclusion): a corrected multi-bounce path is dropped when the closest hit along
+  // one of its segments (source point towards target point, the line of sight's test) lies nearer
+  // than the segment's end; a hit at the end is the next surface or lies beyond the receiver.
+  bool Unobstructed(BVH& bvh, const Path& mp) const {
+    for (const Ray& r : mp) {
+      V3 d = sub(r.targetPoint, r.sourcePoint);
+      HitInfo h;
+      if (bvh.RayIntersects(0, r.sourcePoint, d, h) && h.distance < norm(d) * (1.0 - 1e-7)) return false;
+    }
+    return true;
+  }
+
   bool DirectPathCorrection(BVH& bvh, Path& dp, V3 rxp) const {
     if (dp.size() != 1) return false;
     V3 txp = mk(p.tx[0], p.tx[1], p.tx[2]);
@@ -1126,6 +1139,7 @@ static Result* Trace(Scene* sc, const Params& p, const double* rx, int nrx, cons
       for (auto& lp : perRx[k]) {
         Path pth = lp.second;
         bool ok = pth.size() == 1 ? tr.DirectPathCorrection(bvh, pth, rxp) : tr.MultiPathCorrection(pth, rxp);
+        if (ok && p.checkOcclusion && pth.size() > 1) ok = tr.Unobstructed(bvh, pth);
         if (ok) surv.push_back(std::make_pair(lp.first, pth));
       }
       // RemoveDuplicatePath (RTSolver.cpp:435-459): first (launch, BFS) occurrence of a coplanar key wins
@@ -1232,6 +1246,7 @@ typedef struct {
   int countCanonical;
   double wallWidth;
   int typedDedup;
+  int checkOcclusion;
 } orc_params;
 
 void* orc_scene_from_stl(const char* path) {
@@ -1363,6 +1378,7 @@ void* orc_trace(void* sp, const orc_params* pp, const double* rx, int nrx, const
   p.countCanonical = pp->countCanonical;
   p.wallWidth = pp->wallWidth;
   p.typedDedup = pp->typedDedup;
+  p.checkOcclusion = pp->checkOcclusion;
   return Trace((Scene*)sp, p, rx, nrx, mats, nmat, nthreads);
 }
 void orc_result_free(void* r) { delete (Result*)r; }
@@ -1404,6 +1420,7 @@ void* orc_trees(void* sp, const orc_params* pp) {
   p.launchStride = pp->launchStride; p.launchOffset = pp->launchOffset; p.q1MeshIndexing = pp->q1MeshIndexing;
   p.wallWidth = pp->wallWidth;
   p.typedDedup = pp->typedDedup;
+  p.checkOcclusion = pp->checkOcclusion;
   return DumpTrees((Scene*)sp, p);
 }
 int orc_trees_n(void* d) { return (int)((TreeDump*)d)->launch.size(); }

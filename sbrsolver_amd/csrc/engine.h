@@ -102,6 +102,7 @@ struct TxDesc {
   double capK;               // rcoef / (1 - rcoef) rounded up (INFINITY for rcoef >= 1): capture prefilter
   double wallWidth;          // > 0: slab walls of this thickness (physical mode), 0: reference walls
   int32_t typedDedup;        // 1: duplicate-removal groups also keyed by the interaction types
+  int32_t checkOcclusion;    // 1: corrected multi-bounce paths must be unobstructed
 };
 
 // ---- interaction tree ---------------------------------------------------------------------------

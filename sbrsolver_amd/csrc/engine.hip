@@ -574,6 +574,15 @@ struct CaptureBufs {
   unsigned long long* keys;       // per node with records: seqA, seqB, order key (3 words)
   const int32_t* cand;            // nodes listed by the bounce kernel
   const unsigned long long* nCand;   // their number
+  // scene, for the occlusion test of corrected paths (k_dedup<true> only)
+  const DNode* nodes;
+  const DNodeF* nodesF;
+  const DTri* tris;
+  const DBox32* tbox;
+  const DSub* subs;
+  const int32_t* subTri;
+  float ext;
+  int32_t stackDepth;
 };
 
 // Returns false when no receiver can be captured by the node.
@@ -751,9 +760,36 @@ __device__ __forceinline__ void store_key(const CaptureBufs& B, NodeCtx& c) {
   k[2] = c.orderKey;
 }
 
+__device__ bool corrected_points(V3 tx, const TNode* __restrict__ tree, const DSurf* __restrict__ surf,
+                                 const int32_t* chain, int depth, V3 C, V3* pts);
+
+// Occlusion test of a corrected multi-bounce path (extension, checkOcclusion): no scene triangle
+// nearer than the end of any segment (a hit at the end itself is the next surface or, for the last
+// segment, lies beyond the receiver).  Closest hits as the line of sight takes them.
+__device__ bool unobstructed(const CaptureBufs& B, const NodeCtx& c, V3 C, int* stkN, float* stkT) {
+  int32_t chain[kMaxDepth + 1];
+  int32_t q = c.node;
+  for (int k = c.depth; k >= 0; k--) { chain[k] = q; q = B.tree[q].parent; }
+  const TxDesc& T = B.txs[c.txi];
+  V3 pts[kMaxDepth + 2];
+  if (!corrected_points(v3(T.tx[0], T.tx[1], T.tx[2]), B.tree, B.surf, chain, c.depth, C, pts)) return false;
+  for (int k = 0; k <= c.depth; k++) {
+    const V3 d = vsub(pts[k + 1], pts[k]);
+    Hit h;
+    if (closest_hit<false, true>(B.nodes, B.nodesF, B.tris, B.tbox, B.subs, B.subTri, B.ext, pts[k], d, h, stkN, stkT,
+                                 blockDim.x) &&
+        h.dist < vnorm(d) * (1.0 - 1e-7))
+      return false;
+  }
+  return true;
+}
+
 // Dedup pass, one thread per record: the group key of the node's coplanar-id sequence, the prune
-// against the key its group already holds, image correction, insertion.
+// against the key its group already holds, image correction (and, with OCCL, the occlusion test of
+// the transmitters that ask for it), insertion.
+template <bool OCCL>
 __global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int64_t nRec) {
+  extern __shared__ __align__(16) unsigned char smemD[];
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long myCor = 0, myUsed = 0;
   if (k < nRec) {
@@ -795,7 +831,13 @@ __global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int
         slot = (slot + 1) & A.tabMask;
       }
     }
-    if (keep && corrected(B, c, r, ld3(B.rx + 3 * (int64_t)r))) {
+    bool ok = keep && corrected(B, c, r, ld3(B.rx + 3 * (int64_t)r));
+    if (OCCL && ok && c.depth > 0 && B.txs[c.txi].checkOcclusion) {
+      int* stkN = reinterpret_cast<int*>(smemD) + threadIdx.x;
+      float* stkT = reinterpret_cast<float*>(smemD + (size_t)B.stackDepth * blockDim.x * 4) + threadIdx.x;
+      ok = unobstructed(B, c, ld3(B.rx + 3 * (int64_t)r), stkN, stkT);
+    }
+    if (ok) {
       myCor++;
       unsigned long long slot = hA & A.tabMask;
       bool placed = false;
@@ -1593,7 +1635,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   // ---- transmitters and their launch directions
   std::vector<TxDesc> txs(nt);
   int maxDepth = 0;
-  bool exact = false, count = false;
+  bool exact = false, count = false, anyOcclusion = false;
   int64_t nRoot = 0;
   for (int t = 0; t < nt; t++) {
     const sbr_trace_params& p = P[t];
@@ -1642,6 +1684,9 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     T.wallWidth = p.wallWidth;
     if (p.typedDedup != 0 && p.typedDedup != 1) return fail(SBR_E_ARG, "typedDedup must be 0 or 1");
     T.typedDedup = p.typedDedup;
+    if (p.checkOcclusion != 0 && p.checkOcclusion != 1) return fail(SBR_E_ARG, "checkOcclusion must be 0 or 1");
+    T.checkOcclusion = p.checkOcclusion;
+    anyOcclusion = anyOcclusion || p.checkOcclusion != 0;
     T.receptionModel = p.receptionModel;
   }
   if (nRoot >= (int64_t(1) << 31)) return fail(SBR_E_LIMIT, "more than 2^31 launches in one trace");
@@ -1919,7 +1964,19 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
                        dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, recCap / kRecParts, dRecCount.p,
                        dRecPrefix.p, 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
-        k_dedup<<<blocks_for((int64_t)nRec, 256), 256, 0, stream>>>(CA, CB, (int64_t)nRec);
+        CB.nodes = dNodes.p;
+        CB.nodesF = dNodesF.p;
+        CB.tris = dTris.p;
+        CB.tbox = dTriBox32.p;
+        CB.subs = dSub.p;
+        CB.subTri = dSubTri.p;
+        CB.ext = sceneExtent;
+        CB.stackDepth = stackDepth();
+        if (anyOcclusion)
+          k_dedup<true><<<blocks_for((int64_t)nRec, 256), 256, stack_bytes(stackDepth(), 256), stream>>>(
+              CA, CB, (int64_t)nRec);
+        else
+          k_dedup<false><<<blocks_for((int64_t)nRec, 256), 256, 0, stream>>>(CA, CB, (int64_t)nRec);
         HIPCHK(hipGetLastError());
       }
       HIPCHK(hipEventRecord(kev[1], stream));
