@@ -856,13 +856,21 @@ __global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int
       if (!placed) atomicOr(&B.counters[3], 1ull);
     }
   }
+  // one atomic per block and counter (appends to one address serialise)
   for (int off = 32; off > 0; off >>= 1) {
     myCor += __shfl_xor(myCor, off, 64);
     myUsed += __shfl_xor(myUsed, off, 64);
   }
+  __shared__ unsigned long long sCor[4], sUsed[4];
   if ((threadIdx.x & 63) == 0) {
-    if (myCor) atomicAdd(&B.counters[1], myCor);
-    if (myUsed) atomicAdd(&B.counters[2], myUsed);
+    sCor[threadIdx.x >> 6] = myCor;
+    sUsed[threadIdx.x >> 6] = myUsed;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long c = sCor[0] + sCor[1] + sCor[2] + sCor[3], u = sUsed[0] + sUsed[1] + sUsed[2] + sUsed[3];
+    if (c) atomicAdd(&B.counters[1], c);
+    if (u) atomicAdd(&B.counters[2], u);
   }
 }
 
@@ -1046,30 +1054,45 @@ __global__ void __launch_bounds__(256) k_flag_scatter(const uint8_t* __restrict_
     *nOut = (unsigned long long)tileOffset[nTiles - 1] + (unsigned long long)tileCount[nTiles - 1];
 }
 
-__global__ void k_collect(const unsigned long long* __restrict__ tabA, const unsigned long long* __restrict__ tabOrder,
-                          const int32_t* __restrict__ tabRx, unsigned long long cap, unsigned long long glLo,
-                          unsigned long long glHi, unsigned long long* __restrict__ winKey,
-                          int32_t* __restrict__ winRx, unsigned long long* __restrict__ counter) {
-  const unsigned long long s = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x;
-  bool take = false;
-  unsigned long long key = 0;
-  if (s < cap && tabA[s] != 0ull) {
-    key = tabOrder[s];
-    const unsigned long long gl = key >> 32;
-    take = gl >= glLo && gl < glHi;
+constexpr int kCollectPerThread = 16;   // table slots per thread (a block takes 4096 contiguous slots)
+
+__global__ void __launch_bounds__(256) k_collect(const unsigned long long* __restrict__ tabA,
+                                                 const unsigned long long* __restrict__ tabOrder,
+                                                 const int32_t* __restrict__ tabRx, unsigned long long cap,
+                                                 unsigned long long glLo, unsigned long long glHi,
+                                                 unsigned long long* __restrict__ winKey, int32_t* __restrict__ winRx,
+                                                 unsigned long long* __restrict__ counter) {
+  // Two passes over the block's tile: count, a block scan and one atomic per block for the base,
+  // then write (the slots are read again from the cache).  One append per block rather than per wave:
+  // appends to a single counter serialise.  The order of the winners does not matter (they are
+  // sorted by key next).
+  const unsigned long long tile = (unsigned long long)blockIdx.x * (256ull * kCollectPerThread);
+  auto takes = [&](unsigned long long s, unsigned long long* key) {
+    if (s >= cap || tabA[s] == 0ull) return false;
+    *key = tabOrder[s];
+    const unsigned long long gl = *key >> 32;
+    return gl >= glLo && gl < glHi;
+  };
+  int cnt = 0;
+  for (int i = 0; i < kCollectPerThread; i++) {
+    unsigned long long key;
+    if (takes(tile + threadIdx.x + 256ull * i, &key)) cnt++;
   }
-  // wave-aggregated append
-  const unsigned long long m = __ballot(take);
-  if (m == 0ull) return;
-  const int lane = (int)(threadIdx.x & 63);
-  const int leader = __ffsll(m) - 1;
-  unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-  base = __shfl(base, leader, 64);
-  if (take) {
-    const unsigned long long w = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
-    winKey[w] = key;
-    winRx[w] = tabRx[s];
+  int total = 0;
+  const int excl = block_scan256(cnt, &total);
+  __shared__ unsigned long long sBase;
+  if (threadIdx.x == 0) sBase = total > 0 ? atomicAdd(counter, (unsigned long long)total) : 0ull;
+  __syncthreads();
+  if (cnt == 0) return;
+  unsigned long long w = sBase + (unsigned long long)excl;
+  for (int i = 0; i < kCollectPerThread; i++) {
+    const unsigned long long sl = tile + threadIdx.x + 256ull * i;
+    unsigned long long key;
+    if (takes(sl, &key)) {
+      winKey[w] = key;
+      winRx[w] = tabRx[sl];
+      w++;
+    }
   }
 }
 
@@ -2012,7 +2035,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         ENSURE(dWinNode, Wc);
         ENSURE(dWinField, 6 * Wc);
         HIPCHK(hipMemsetAsync(dCounters.p + 4, 0, sizeof(unsigned long long), stream));
-        k_collect<<<blocks_for((int64_t)cap, 256), 256, 0, stream>>>(dTabA.p, dTabOrder.p, dTabRx.p, cap,
+        k_collect<<<blocks_for((int64_t)cap, 256 * kCollectPerThread), 256, 0, stream>>>(dTabA.p, dTabOrder.p, dTabRx.p, cap,
                                                                       (unsigned long long)ws,
                                                                       (unsigned long long)(ws + W), dWinKey.p,
                                                                       dWinRx.p, dCounters.p + 4);
