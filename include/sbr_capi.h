@@ -115,6 +115,9 @@ typedef struct {
   int32_t checkOcclusion;/* extension: 1 drops a corrected multi-bounce path when a scene triangle lies
                             on one of its segments (closest hit nearer than the segment's end); 0:
                             no such test, as the reference (RTSolver.cpp:330-373) */
+  int32_t longMiss;      /* extension: 1 extends a ray that misses the scene after a bounce to 1e10 m,
+                            as a missing launch ray; 0: a 100 m stub (RTSolver.cpp:224-256), so no
+                            receiver beyond it captures the ray */
 } sbr_trace_params;
 
 /* Counters and stage times of the last trace. */

@@ -41,6 +41,7 @@ struct RTSolverOptions {
   double wallWidth = 0;       // > 0: slab walls of this thickness in m (physical mode); 0: reference walls
   int typedDedup = 0;         // 1: duplicate removal keyed by coplanar ids and interaction types
   int checkOcclusion = 0;     // 1: drop corrected paths with a scene triangle on a segment
+  int longMiss = 0;           // 1: misses after a bounce extend 1e10 m instead of a 100 m stub
   int device = 0;
 };
 

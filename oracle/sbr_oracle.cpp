@@ -737,6 +737,7 @@ struct Params {
   double wallWidth = 0;   // > 0 with useFresnel: slab walls (extension, see SlabCoeffs)
   int typedDedup = 0;     // 1: duplicate removal also keyed by the segments' interaction types (extension)
   int checkOcclusion = 0; // 1: corrected multi-bounce paths must be unobstructed (extension, Unobstructed)
+  int longMiss = 0;       // 1: a miss after a bounce extends 1e10 m, as at the root (extension)
 };
 
 struct Tracer {
@@ -787,7 +788,7 @@ struct Tracer {
       }
     } else {
       if (rc > 0 || tc > 0) {
-        PTN* c = NewNode(src, add(src, mul(dir, 1E2)), transMat, refMat, -1, -1, kINF, lastAngle);
+        PTN* c = NewNode(src, add(src, mul(dir, p.longMiss ? 1E10 : 1E2)), transMat, refMat, -1, -1, kINF, lastAngle);
         c->ray.trueLength = trueLen;
         if (refMat >= 0) node->childReflect = c;
         else if (transMat >= 0) node->childTransmit = c;
@@ -1247,6 +1248,7 @@ typedef struct {
   double wallWidth;
   int typedDedup;
   int checkOcclusion;
+  int longMiss;
 } orc_params;
 
 void* orc_scene_from_stl(const char* path) {
@@ -1379,6 +1381,7 @@ void* orc_trace(void* sp, const orc_params* pp, const double* rx, int nrx, const
   p.wallWidth = pp->wallWidth;
   p.typedDedup = pp->typedDedup;
   p.checkOcclusion = pp->checkOcclusion;
+  p.longMiss = pp->longMiss;
   return Trace((Scene*)sp, p, rx, nrx, mats, nmat, nthreads);
 }
 void orc_result_free(void* r) { delete (Result*)r; }
@@ -1421,6 +1424,7 @@ void* orc_trees(void* sp, const orc_params* pp) {
   p.wallWidth = pp->wallWidth;
   p.typedDedup = pp->typedDedup;
   p.checkOcclusion = pp->checkOcclusion;
+  p.longMiss = pp->longMiss;
   return DumpTrees((Scene*)sp, p);
 }
 int orc_trees_n(void* d) { return (int)((TreeDump*)d)->launch.size(); }

@@ -103,6 +103,7 @@ struct TxDesc {
   double wallWidth;          // > 0: slab walls of this thickness (physical mode), 0: reference walls
   int32_t typedDedup;        // 1: duplicate-removal groups also keyed by the interaction types
   int32_t checkOcclusion;    // 1: corrected multi-bounce paths must be unobstructed
+  int32_t longMiss;          // 1: a miss after a bounce extends 1e10 m (0: a 100 m stub)
 };
 
 // ---- interaction tree ---------------------------------------------------------------------------

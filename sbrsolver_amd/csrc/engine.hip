@@ -392,8 +392,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
         nd.hitDfs = h.idx;
         mask = allow;   // transmitted child (bit 0, BFS order: first), reflected child (bit 1)
       } else {
-        // A miss leaves a 100 m stub below the root, a 1e10 m segment at the root (RTSolver.cpp:224-256).
-        const double len = A.depth == 0 ? 1E10 : 1E2;
+        // A miss leaves a 100 m stub below the root, a 1e10 m segment at the root (RTSolver.cpp:224-256);
+        // with longMiss (extension) every miss is 1e10 m.
+        const double len = A.depth == 0 || txs[nd.txi].longMiss ? 1E10 : 1E2;
         const V3 t = vadd(src, vmul(dir, len));
         nd.tgt[0] = t.x; nd.tgt[1] = t.y; nd.tgt[2] = t.z;
         nd.pl = INFINITY;
@@ -1709,6 +1710,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     T.typedDedup = p.typedDedup;
     if (p.checkOcclusion != 0 && p.checkOcclusion != 1) return fail(SBR_E_ARG, "checkOcclusion must be 0 or 1");
     T.checkOcclusion = p.checkOcclusion;
+    if (p.longMiss != 0 && p.longMiss != 1) return fail(SBR_E_ARG, "longMiss must be 0 or 1");
+    T.longMiss = p.longMiss;
     anyOcclusion = anyOcclusion || p.checkOcclusion != 0;
     T.receptionModel = p.receptionModel;
   }

@@ -121,6 +121,7 @@ int RTSolver::ExecuteRayTracing(Vec3 sourcePoint, const Mat3& txCoordSys, int ma
   p.wallWidth = options.wallWidth;
   p.typedDedup = options.typedDedup;
   p.checkOcclusion = options.checkOcclusion;
+  p.longMiss = options.longMiss;
   p.receptionModel = options.receptionModel;
   p.launchStride = 1;
   p.launchOffset = 0;
