@@ -118,6 +118,10 @@ typedef struct {
   int32_t longMiss;      /* extension: 1 extends a ray that misses the scene after a bounce to 1e10 m,
                             as a missing launch ray; 0: a 100 m stub (RTSolver.cpp:224-256), so no
                             receiver beyond it captures the ray */
+  int32_t physicalField; /* extension: 1 carries the field through each interaction in the global frame,
+                            split into the components perpendicular (s) and parallel (p) to the plane
+                            of incidence with the incidence angle in [0, 90] degrees; 0: the
+                            reference's rotations through the surface frames (FieldCompute.cpp:71-151) */
 } sbr_trace_params;
 
 /* Counters and stage times of the last trace. */

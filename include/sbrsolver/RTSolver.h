@@ -42,6 +42,7 @@ struct RTSolverOptions {
   int typedDedup = 0;         // 1: duplicate removal keyed by coplanar ids and interaction types
   int checkOcclusion = 0;     // 1: drop corrected paths with a scene triangle on a segment
   int longMiss = 0;           // 1: misses after a bounce extend 1e10 m instead of a 100 m stub
+  int physicalField = 0;      // 1: s/p field transport in the plane of incidence
   int device = 0;
 };
 

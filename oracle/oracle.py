@@ -23,6 +23,7 @@ class Params(C.Structure):
         ("receptionModel", C.c_int), ("launchStride", C.c_int), ("launchOffset", C.c_int),
         ("q1MeshIndexing", C.c_int), ("countCanonical", C.c_int), ("wallWidth", C.c_double),
         ("typedDedup", C.c_int), ("checkOcclusion", C.c_int), ("longMiss", C.c_int),
+        ("physicalField", C.c_int),
     ]
 
 
@@ -76,7 +77,8 @@ def _p(a, t):
 
 def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None, maxTotal=0, useFresnel=True,
                 antennaType=1, receptionModel=0, launchStride=1, launchOffset=0, q1MeshIndexing=1,
-                countCanonical=0, wallWidth=0.0, typedDedup=0, checkOcclusion=0, longMiss=0):
+                countCanonical=0, wallWidth=0.0, typedDedup=0, checkOcclusion=0, longMiss=0,
+                physicalField=0):
     p = Params()
     for i in range(3):
         p.tx[i] = float(tx[i])
@@ -93,6 +95,7 @@ def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None
     p.typedDedup = int(typedDedup)
     p.checkOcclusion = int(checkOcclusion)
     p.longMiss = int(longMiss)
+    p.physicalField = int(physicalField)
     return p
 
 

@@ -738,6 +738,7 @@ struct Params {
   int typedDedup = 0;     // 1: duplicate removal also keyed by the segments' interaction types (extension)
   int checkOcclusion = 0; // 1: corrected multi-bounce paths must be unobstructed (extension, Unobstructed)
   int longMiss = 0;       // 1: a miss after a bounce extends 1e10 m, as at the root (extension)
+  int physicalField = 0;  // 1: s/p field transport in the plane of incidence (extension, FieldForPath)
 };
 
 struct Tracer {
@@ -1011,6 +1012,35 @@ struct Tracer {
       cur = next;
       if (ray.hitSurfaceID == -1) next = G;
       else next = MeshTri(ray.hitSurfaceID).coordSys;
+      if (p.physicalField && i > 0) {
+        // Extension: the global field split at the interaction into its components along s (normal
+        // to the plane of incidence) and p (in it: s x direction, before and after), with the
+        // Fresnel coefficients of the incidence angle in [0, 90] degrees (the reference's TE
+        // coefficient is minus the usual s one for a reflection, its TM one the usual p one).
+        V3 din = normalized(sub(path[i - 1].targetPoint, path[i - 1].sourcePoint)), dout = normalized(vg);
+        V3 nrm = normalized(mk(cur.m[6], cur.m[7], cur.m[8]));
+        double cth = std::fabs(dot(din, nrm));
+        if (cth > 1.0) cth = 1.0;
+        V3 s = cross(din, nrm);
+        if (norm(s) < 1e-12) {
+          V3 ax = std::fabs(din.x) <= std::fabs(din.y) && std::fabs(din.x) <= std::fabs(din.z) ? mk(1, 0, 0)
+                  : (std::fabs(din.y) <= std::fabs(din.z) ? mk(0, 1, 0) : mk(0, 0, 1));
+          s = cross(din, ax);
+        }
+        s = normalized(s);
+        V3 pin = cross(s, din), pout = cross(s, dout);
+        bool refl = ray.reflectionMaterialId >= 0;
+        const double* m = &mats[6 * (refl ? ray.reflectionMaterialId : ray.penetrationMaterialId)];
+        V3c unit = {cd(0, 0), cd(1, 0), cd(1, 0)};
+        V3c c = RefTrans(refl, unit, m[3], m[5], p.freq, std::acos(cth), 0, p.useFresnel != 0);
+        cd cs = refl ? -c.z : c.z, cp = c.y;
+        cd es = tot.x * s.x + tot.y * s.y + tot.z * s.z, ep = tot.x * pin.x + tot.y * pin.y + tot.z * pin.z;
+        cd as = cs * es, ap = cp * ep;
+        tot.x = as * s.x + ap * pout.x;
+        tot.y = as * s.y + ap * pout.y;
+        tot.z = as * s.z + ap * pout.z;
+        continue;
+      }
       if (i == 0) inc = Pattern(p.antennaType, theta, phi, p.pt);
       else {
         inc = mvc(RotationMatrix(cur, G), tot);
@@ -1249,6 +1279,7 @@ typedef struct {
   int typedDedup;
   int checkOcclusion;
   int longMiss;
+  int physicalField;
 } orc_params;
 
 void* orc_scene_from_stl(const char* path) {
@@ -1382,6 +1413,7 @@ void* orc_trace(void* sp, const orc_params* pp, const double* rx, int nrx, const
   p.typedDedup = pp->typedDedup;
   p.checkOcclusion = pp->checkOcclusion;
   p.longMiss = pp->longMiss;
+  p.physicalField = pp->physicalField;
   return Trace((Scene*)sp, p, rx, nrx, mats, nmat, nthreads);
 }
 void orc_result_free(void* r) { delete (Result*)r; }
@@ -1425,6 +1457,7 @@ void* orc_trees(void* sp, const orc_params* pp) {
   p.typedDedup = pp->typedDedup;
   p.checkOcclusion = pp->checkOcclusion;
   p.longMiss = pp->longMiss;
+  p.physicalField = pp->physicalField;
   return DumpTrees((Scene*)sp, p);
 }
 int orc_trees_n(void* d) { return (int)((TreeDump*)d)->launch.size(); }

@@ -27,7 +27,7 @@ class TraceParams(C.Structure):
         ("receptionModel", C.c_int32), ("launchStride", C.c_int32), ("launchOffset", C.c_int32),
         ("q1MeshIndexing", C.c_int32), ("exactTraversal", C.c_int32), ("countTraversal", C.c_int32),
         ("wallWidth", C.c_double), ("typedDedup", C.c_int32), ("checkOcclusion", C.c_int32),
-        ("longMiss", C.c_int32),
+        ("longMiss", C.c_int32), ("physicalField", C.c_int32),
     ]
 
 
@@ -148,7 +148,7 @@ def _a(x):
 def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None, maxTotal=0, useFresnel=True,
                 antennaType=1, receptionModel=0, launchStride=1, launchOffset=0, q1MeshIndexing=1,
                 exactTraversal=0, countTraversal=0, wallWidth=0.0, typedDedup=0, checkOcclusion=0,
-                longMiss=0):
+                longMiss=0, physicalField=0):
     p = TraceParams()
     for i in range(3):
         p.tx[i] = float(tx[i])
@@ -165,6 +165,7 @@ def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None
     p.typedDedup = int(typedDedup)
     p.checkOcclusion = int(checkOcclusion)
     p.longMiss = int(longMiss)
+    p.physicalField = int(physicalField)
     return p
 
 

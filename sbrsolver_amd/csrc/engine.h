@@ -104,6 +104,7 @@ struct TxDesc {
   int32_t typedDedup;        // 1: duplicate-removal groups also keyed by the interaction types
   int32_t checkOcclusion;    // 1: corrected multi-bounce paths must be unobstructed
   int32_t longMiss;          // 1: a miss after a bounce extends 1e10 m (0: a 100 m stub)
+  int32_t physicalField;     // 1: s/p transport in the plane of incidence (0: reference rotations)
 };
 
 // ---- interaction tree ---------------------------------------------------------------------------

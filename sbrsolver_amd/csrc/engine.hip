@@ -1239,6 +1239,36 @@ __global__ void __launch_bounds__(256) k_field(FieldArgs F, const TxDesc* __rest
     cur = next;
     int hitSurf = k == depth ? -1 : sg.tri;
     next = hitSurf == -1 ? I3 : surf[hitSurf].cs;
+    if (K.physicalField && k > 0) {
+      // Extension: the global field split at the interaction into the component along s (normal to
+      // the plane of incidence) and along p (in it, s x direction before and after), with the
+      // Fresnel coefficients of the incidence angle in [0, 90] degrees; the reference's TE
+      // coefficient is minus the usual s one (reflection), its TM coefficient the usual p one.
+      const V3 din = vnormalized(vsub(pts[k], pts[k - 1])), dout = vnormalized(vg);
+      const V3 nrm = vnormalized(v3(cur[6], cur[7], cur[8]));
+      double cth = fabs(vdot(din, nrm));
+      cth = cth > 1.0 ? 1.0 : cth;
+      V3 s = vcross(din, nrm);
+      if (vnorm(s) < 1e-12) {   // normal incidence: any direction normal to din
+        const V3 ax = fabs(din.x) <= fabs(din.y) && fabs(din.x) <= fabs(din.z) ? v3(1, 0, 0)
+                      : (fabs(din.y) <= fabs(din.z) ? v3(0, 1, 0) : v3(0, 0, 1));
+        s = vcross(din, ax);
+      }
+      s = vnormalized(s);
+      const V3 pin = vcross(s, din), pout = vcross(s, dout);
+      const int mat = tris[tree[chain[k - 1]].hitDfs].mat;
+      cx fTM, fTE;
+      fresnel(sg.type == 1, acos(cth), mats[6 * mat + 3], mats[6 * mat + 5], K.freq, K.useFresnel != 0, K.wallWidth,
+              &fTM, &fTE);
+      const cx cs = sg.type == 1 ? cneg(fTE) : fTE, cp = fTM;
+      const cx es = cadd(cadd(cscale(tot.x, s.x), cscale(tot.y, s.y)), cscale(tot.z, s.z));
+      const cx ep = cadd(cadd(cscale(tot.x, pin.x), cscale(tot.y, pin.y)), cscale(tot.z, pin.z));
+      const cx as = cmul(cs, es), ap = cmul(cp, ep);
+      tot.x = cadd(cscale(as, s.x), cscale(ap, pout.x));
+      tot.y = cadd(cscale(as, s.y), cscale(ap, pout.y));
+      tot.z = cadd(cscale(as, s.z), cscale(ap, pout.z));
+      continue;
+    }
     V3C inc;
     if (k == 0) {
       inc.x = mkc(0, 0);
@@ -1712,6 +1742,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     T.checkOcclusion = p.checkOcclusion;
     if (p.longMiss != 0 && p.longMiss != 1) return fail(SBR_E_ARG, "longMiss must be 0 or 1");
     T.longMiss = p.longMiss;
+    if (p.physicalField != 0 && p.physicalField != 1) return fail(SBR_E_ARG, "physicalField must be 0 or 1");
+    T.physicalField = p.physicalField;
     anyOcclusion = anyOcclusion || p.checkOcclusion != 0;
     T.receptionModel = p.receptionModel;
   }

@@ -122,6 +122,7 @@ int RTSolver::ExecuteRayTracing(Vec3 sourcePoint, const Mat3& txCoordSys, int ma
   p.typedDedup = options.typedDedup;
   p.checkOcclusion = options.checkOcclusion;
   p.longMiss = options.longMiss;
+  p.physicalField = options.physicalField;
   p.receptionModel = options.receptionModel;
   p.launchStride = 1;
   p.launchOffset = 0;
