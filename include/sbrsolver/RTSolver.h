@@ -43,6 +43,12 @@ struct RTSolverOptions {
   int checkOcclusion = 0;     // 1: drop corrected paths with a scene triangle on a segment
   int longMiss = 0;           // 1: misses after a bounce extend 1e10 m instead of a 100 m stub
   int physicalField = 0;      // 1: s/p field transport in the plane of incidence
+  // Every correction of the reference's quirks at once (wallWidth, a thickness, is left as it is).
+  void SetPhysical() {
+    q1MeshIndexing = 0;
+    receptionModel = 1;
+    typedDedup = checkOcclusion = longMiss = physicalField = 1;
+  }
   int device = 0;
 };
 

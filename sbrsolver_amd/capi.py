@@ -148,7 +148,13 @@ def _a(x):
 def make_params(tx, tess, maxR, maxT=0, frequency=1.8e9, pt=1.0, txCoordSys=None, maxTotal=0, useFresnel=True,
                 antennaType=1, receptionModel=0, launchStride=1, launchOffset=0, q1MeshIndexing=1,
                 exactTraversal=0, countTraversal=0, wallWidth=0.0, typedDedup=0, checkOcclusion=0,
-                longMiss=0, physicalField=0):
+                longMiss=0, physicalField=0, physical=False):
+    """sbr_trace_params from keyword arguments (reference defaults).  physical=True switches on every
+    correction of the reference's quirks at once: mesh indexing by id, the alpha*L/sqrt(3) reception
+    radius, typed duplicate removal, the occlusion test, long misses and the s/p field transport
+    (wallWidth stays as given: it is a wall thickness)."""
+    if physical:
+        q1MeshIndexing, receptionModel, typedDedup, checkOcclusion, longMiss, physicalField = 0, 1, 1, 1, 1, 1
     p = TraceParams()
     for i in range(3):
         p.tx[i] = float(tx[i])

@@ -47,3 +47,18 @@ def test_physical_field_urban_matches_oracle(gpu_ctx):
     g = gpu_ctx.trace(make_params(**tr), cfg["rx"])
     o = O.trace(O.Scene.from_soup(*cfg["soup"]), O.make_params(**tr), cfg["rx"], cfg["materials"], nthreads=8)
     compare(o, g, "urban physical field")
+
+
+def test_physical_mode_urban_matches_oracle(gpu_ctx):
+    print("[case] urban L3 R6T1, every physical switch, 0.2 m slab walls", flush=True)
+    cfg = scenes.config2(tess=3)
+    gpu_ctx.scene_from_soup(*cfg["soup"])
+    gpu_ctx.materials(cfg["materials"])
+    p = make_params(**dict(cfg["trace"], physical=True, wallWidth=0.2))
+    assert (p.q1MeshIndexing, p.receptionModel, p.typedDedup, p.checkOcclusion, p.longMiss, p.physicalField) == \
+        (0, 1, 1, 1, 1, 1)
+    g = gpu_ctx.trace(p, cfg["rx"])
+    tr = dict(cfg["trace"], q1MeshIndexing=0, receptionModel=1, typedDedup=1, checkOcclusion=1, longMiss=1,
+              physicalField=1, wallWidth=0.2)
+    o = O.trace(O.Scene.from_soup(*cfg["soup"]), O.make_params(**tr), cfg["rx"], cfg["materials"], nthreads=8)
+    compare(o, g, "urban physical mode")
