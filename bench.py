@@ -328,6 +328,15 @@ def main():
     if not args.no_indoor:
         extra("indoor", lambda: indoor_run(Context, make_params, scenes))
 
+    # Batched and served throughput as a fraction of the same HBM roofline: their bounces priced at
+    # the headline workload's algorithmic bytes per bounce (same scene; the per-bounce traversal
+    # counts of their levels are not committed, so this is a model figure).
+    for name, key in (("batched", "batched_mrays"), ("serving", "capacity_mray_bounces_s"),
+                      ("serving", "mray_bounces_per_s")):
+        if name in out and key in out[name]:
+            gbs = out[name][key] * 1e6 * b_bounce / 1e9
+            out[name]["roofline_frac_" + key] = round(gbs / HBM_PEAK_GBS, 5)
+
     if args.calib:
         calib_copy(256 << 20)
     if not args.no_cpu:
