@@ -7,7 +7,8 @@ import pytest
 from oracle import oracle as O
 from sbrsolver_amd import make_params, scenes
 from tests.parity import compare
-from tests.test_gpu_slab import KW, MATS, STL, _rx
+from tests.office import KW, MATS, STL
+from tests.office import scene_and_rx as _rx
 from tests.test_long_miss import ground_soup
 from tests.test_physical_field import FREQ, H, HORZ
 

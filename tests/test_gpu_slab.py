@@ -2,28 +2,16 @@
 are kept): the same paths and fields as the oracle; slab walls change the field of every path
 through a wall (the reference's infinite walls transmit nothing of its theta and phi components);
 the paths do not change and the direct paths keep their fields."""
-import os
-
 import numpy as np
 import pytest
 
 from oracle import oracle as O
 from sbrsolver_amd import make_params
+from tests.office import KW, MATS, STL
+from tests.office import scene_and_rx as _rx
 from tests.parity import compare
 
 pytestmark = pytest.mark.gpu
-
-STL = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "stl", "office.stl")
-MATS = np.array([[1.8e9, -1, -1, 2.5, 0.3, 0.03], [1.8e9, -1, -1, 6.0, 0.05, 0.005]])
-KW = dict(tx=(0.835938, 4.53906, 2.5), tess=3, maxR=2, maxT=1, frequency=1.8e9, antennaType=2)
-
-
-def _rx():
-    sc = O.Scene.from_stl(STL)
-    v = sc.triangles()["verts"].reshape(-1, 3)
-    lo, hi = v.min(0), v.max(0)
-    g1, g2 = np.linspace(lo[0], hi[0], 9), np.linspace(lo[1], hi[1], 9)
-    return sc, np.array([[a, b, (lo[2] + hi[2]) / 2 + 0.1] for a in g1 for b in g2])
 
 
 def test_slab_walls_match_oracle(gpu_ctx):

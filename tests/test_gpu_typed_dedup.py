@@ -6,7 +6,8 @@ import pytest
 from oracle import oracle as O
 from sbrsolver_amd import make_params
 from tests.parity import compare
-from tests.test_gpu_slab import KW, MATS, STL, _rx
+from tests.office import KW, MATS, STL
+from tests.office import scene_and_rx as _rx
 
 pytestmark = pytest.mark.gpu
 

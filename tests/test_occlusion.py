@@ -10,7 +10,8 @@ import pytest
 
 from oracle import oracle as O
 from tests.parity import oracle_paths, path_table
-from tests.test_gpu_slab import KW, MATS, _rx
+from tests.office import KW, MATS
+from tests.office import scene_and_rx as _rx
 
 
 def _blocked(sc, res, i):

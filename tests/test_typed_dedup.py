@@ -9,7 +9,8 @@ import numpy as np
 
 from oracle import oracle as O
 from tests.parity import oracle_paths, path_table
-from tests.test_gpu_slab import KW, MATS, _rx
+from tests.office import KW, MATS
+from tests.office import scene_and_rx as _rx
 
 
 def _per_rx(table):
