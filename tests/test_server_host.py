@@ -3,7 +3,6 @@ release, statistics and error paths.  Queries fail with SBR_E_STATE on a host-on
 exercises the failure path of every query without a GPU."""
 import threading
 
-import numpy as np
 import pytest
 
 from sbrsolver_amd import Context, SbrError, Server, SBR_PENDING, make_params, scenes
