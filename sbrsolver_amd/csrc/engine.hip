@@ -750,6 +750,28 @@ __device__ __forceinline__ void capture_one(const CaptureBufs& B, NodeCtx& c, co
   c.emitted = true;
 }
 
+// capture_one for the wave-per-node kernel: the node's ancestors (capture halves, nearest first) are
+// read from the wave's copy in LDS (anc: 8 doubles each), not walked through the tree again for every
+// receiver hit; qRest continues the walk in the tree past the copied ones.
+__device__ __forceinline__ void capture_one_cached(const CaptureBufs& B, NodeCtx& c, const RxCell& x,
+                                                   unsigned long long& myTests, const double* anc, int nAnc,
+                                                   int32_t qRest) {
+  myTests++;
+  const V3 C = ld3(x.p);
+  if (!sphere_hit(c.S, c.Tg, C, c.base, c.model, c.rc)) return;
+  for (int j = 0; j < nAnc; j++) {
+    const double* a = anc + 8 * j;
+    if (sphere_hit(v3(a[0], a[1], a[2]), v3(a[3], a[4], a[5]), C, a[6], c.model, c.rc)) return;
+  }
+  for (int32_t q = qRest; q >= 0;) {
+    const TNode& a = B.tree[q];
+    if (sphere_hit(ld3(a.src), ld3(a.tgt), C, a.base, c.model, c.rc)) return;
+    q = a.parent;
+  }
+  append_record(B, c.node, x.r);
+  c.emitted = true;
+}
+
 // After a node's candidates: the first lane of the wave that emitted records for it stores its key
 // (all lanes of the wave-per-node kernel share the node; in the thread-per-node kernel each lane
 // has its own).
@@ -938,6 +960,9 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
 // cells / 64 steps whatever the shape of its region.
 __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBufs B) {
   extern __shared__ uint16_t sCell[];
+  // The grid is sized for the most heavy nodes (their count is read here, on the device); blocks
+  // beyond the count leave before staging the cell starts.
+  if ((int64_t)blockIdx.x * (blockDim.x >> 6) >= (int64_t)B.counters[5]) return;
   if (A.cellLds > 0) {
     for (int k = threadIdx.x; k < A.cellLds; k += blockDim.x) sCell[k] = (uint16_t)B.cellStart[k];
     __syncthreads();
@@ -948,9 +973,33 @@ __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBuf
   const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   unsigned long long myTests = 0;
+  __shared__ double sAnc[4][8 * kMaxDepth];   // per wave: the current node's ancestors
+  double* anc = sAnc[threadIdx.x >> 6];
   for (int64_t h = wave; h < nHeavy; h += nWaves) {   // wave-uniform
     NodeCtx c;
     if (!node_setup(A, B, B.heavy[h], c)) continue;    // same outcome on every lane
+    // The first lane copies the ancestor chain to the wave's LDS.  A wave's LDS accesses complete in
+    // order, and the fences and wave barriers keep the compiler from moving the other lanes' reads
+    // (after the exchange of the count) above the copy or the next node's copy above them.
+    int nAnc = 0;
+    int32_t qRest = c.parent;
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+      while (qRest >= 0 && nAnc < kMaxDepth) {
+        const TNode& a = B.tree[qRest];
+        double* o = anc + 8 * nAnc;
+        o[0] = a.src[0]; o[1] = a.src[1]; o[2] = a.src[2];
+        o[3] = a.tgt[0]; o[4] = a.tgt[1]; o[5] = a.tgt[2];
+        o[6] = a.base;
+        nAnc++;
+        qRest = a.parent;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    nAnc = __shfl(nAnc, 0, 64);
+    qRest = __shfl(qRest, 0, 64);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (int jb = c.j0; jb <= c.j1; jb += 64) {
       const int j = jb + lane;
       int i0 = 0, i1 = -1;
@@ -982,7 +1031,7 @@ __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBuf
         if (f < total) {
           const int cell = (jb + r) * A.g.nx + rowFirst + (f - rowExcl);
           const int q1 = cellStart(cell + 1);
-          for (int q = cellStart(cell); q < q1; q++) capture_one(B, c, B.cellRx[q], myTests);
+          for (int q = cellStart(cell); q < q1; q++) capture_one_cached(B, c, B.cellRx[q], myTests, anc, nAnc, qRest);
         }
       }
     }
