@@ -148,7 +148,7 @@ def indoor_run(Context, make_params, scenes, tess=10):
                 ms_capture=round(s["ms_capture"], 1), paths_kept=int(s["winners"]))
 
 
-def serve_run(ctx, make_params, scenes, Server, n=48, slots=3, load=0.7):
+def serve_run(ctx, make_params, scenes, Server, n=96, slots=3, load=0.7):
     """Serving loop on the urban scene: n coverage queries (levels 6-10, 4 reflections, 1..50k
     receivers).  Phase 1 submits all at once to measure capacity (req/s when saturated); phase 2
     submits the same queries with Poisson arrivals at `load` x capacity and reports the latency
