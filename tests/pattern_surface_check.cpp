@@ -2,7 +2,8 @@
 // TXPattern.dat reader, Preprocessor.cpp:3-35) and RTSolver::SetTxPattern.  Usage:
 //   pattern_surface_check <pattern file>
 // Prints "<read ok> <entries> <SetTxPattern status> <antennaType> <status without one entry>
-// <status of an unreadable file's map> <read of a missing file>".
+// <status of an empty map> <read of a missing file>", then the physical-mode switches of a default
+// RTSolverOptions and of one after SetPhysical().
 #include <cstdio>
 #include <iterator>
 
@@ -25,5 +26,12 @@ int main(int argc, char** argv) {
   GainMap missing;
   const bool okMissing = Preprocessor::ReadPatternFile(std::string(argv[1]) + ".does-not-exist", missing);
   std::printf("%d %zu %d %d %d %d %d\n", ok ? 1 : 0, m.size(), rc, type, rcHoled, rcEmpty, okMissing ? 1 : 0);
+  // Second line: the physical-mode switches before and after RTSolverOptions::SetPhysical().
+  RTSolverOptions o;
+  std::printf("%d %d %d %d %d %d", o.q1MeshIndexing, o.receptionModel, o.typedDedup, o.checkOcclusion, o.longMiss,
+              o.physicalField);
+  o.SetPhysical();
+  std::printf(" %d %d %d %d %d %d %g\n", o.q1MeshIndexing, o.receptionModel, o.typedDedup, o.checkOcclusion,
+              o.longMiss, o.physicalField, o.wallWidth);
   return 0;
 }

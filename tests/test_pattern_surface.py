@@ -2,7 +2,8 @@
 reads a file in TXPattern.dat's layout (a 4-token header, then theta, phi, theta gain, theta phase,
 phi gain, phi phase per line, theta fastest) into a GainMap, skipping lines that are not six
 numbers, and RTSolver::SetTxPattern accepts a full evenly spaced grid (switching antennaType to 3)
-and rejects a grid with a hole or an empty map."""
+and rejects a grid with a hole or an empty map; RTSolverOptions::SetPhysical() switches the physical
+mode on."""
 import os
 import subprocess
 
@@ -32,8 +33,11 @@ def test_pattern_file_to_solver(checker, tmp_path, step):
             lines.append(f"{i * step} {j * step} {-3.5 + 0.01 * i} {j * 0.5} {-40 + 0.02 * j} 0")
     path = tmp_path / "TXPattern.dat"
     path.write_text("\n".join(lines) + "\n")
-    out = subprocess.check_output([checker, str(path)], text=True).split()
-    ok, n, rc, typ, rc_holed, rc_empty, ok_missing = (int(x) for x in out)
+    lines = subprocess.check_output([checker, str(path)], text=True).splitlines()
+    ok, n, rc, typ, rc_holed, rc_empty, ok_missing = (int(x) for x in lines[0].split())
+    # RTSolverOptions: reference defaults, and SetPhysical() switches every correction on (not the
+    # wall thickness)
+    assert lines[1].split() == ["1", "0", "0", "0", "0", "0", "0", "1", "1", "1", "1", "1", "0"]
     assert ok == 1 and n == nt * np_
     assert rc == 0 and typ == 3
     assert rc_holed < 0 and rc_empty < 0
