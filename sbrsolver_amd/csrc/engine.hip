@@ -256,7 +256,7 @@ __device__ __forceinline__ bool may_capture(const RxGrid& g, V3 S, V3 Tg, double
   return true;
 }
 
-constexpr int kTraceOversub = 16;   // bounce-kernel blocks per resident block slot
+constexpr int kTraceOversub = 8;   // bounce-kernel blocks per resident block slot
 
 struct TraceArgs {
   int32_t depth;
