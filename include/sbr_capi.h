@@ -229,10 +229,15 @@ int sbr_result_trees(sbr_ctx* ctx, int32_t* launch, int32_t* depth, int64_t* cod
  * A server keeps the scene and material tables of a context resident on its device and runs
  * coverage queries (one transmitter, its own receivers) submitted from any host thread.  It owns
  * `slots` engines, each with its own stream and worker thread: a query waiting in the queue is
- * admitted as soon as a slot is free and the estimated device memory of the queries in flight plus
- * its own stays within the HBM budget, so queries that arrive while others run overlap with them on
- * the device (continuous batching).  A query's result does not depend on what runs beside it.
- * Threading: every sbr_server call is thread-safe. */
+ * admitted as soon as a slot is free and the device memory of all slots with it stays within the
+ * HBM budget, so queries that arrive while others run overlap with them on the device (continuous
+ * batching).  The budget covers what each slot holds between queries (per-trace buffers kept at
+ * their largest size, cached launch directions) and, while a query runs, the larger of that and the
+ * query's estimate; a slot holding more than budget / slots after a query releases its per-trace
+ * buffers.  The scene copies are outside the budget.  A query's result does not depend on what
+ * runs beside it.  Threading: every sbr_server call is thread-safe; the strings returned by
+ * sbr_server_last_error and sbr_request_error are copies owned by the calling thread, valid until
+ * its next call of either. */
 #define SBR_PENDING 1      /* sbr_poll: the query has not finished yet */
 
 typedef struct sbr_server sbr_server;
@@ -240,7 +245,7 @@ typedef struct sbr_server sbr_server;
 typedef struct {
   int32_t slots;             /* concurrent queries on the device; 0 = 3 */
   int32_t max_queued;        /* queries waiting for admission before sbr_submit refuses; 0 = 4096 */
-  int64_t hbm_budget_bytes;  /* admission budget for the queries in flight; 0 = 3/4 of the free memory */
+  int64_t hbm_budget_bytes;  /* admission budget for the slots' device memory; 0 = 3/4 of the free memory */
 } sbr_server_config;
 
 typedef struct {
@@ -258,7 +263,7 @@ typedef struct {
 typedef struct {
   int64_t submitted, completed, failed, refused, queued, running;
   int64_t bounces;           /* of completed queries */
-  int64_t budget_bytes, peak_inflight_bytes;
+  int64_t budget_bytes, peak_inflight_bytes;   /* peak of the slots' counted device memory */
   double p50_ms, p99_ms, mean_ms, max_ms;   /* submit -> result latency of completed queries */
   double window_s;           /* first submission -> last completion */
   double req_per_s;          /* completed / window_s */
@@ -272,8 +277,8 @@ int sbr_server_create(sbr_ctx* src, const sbr_server_config* cfg, sbr_server** o
 /* Drains: waits for running queries, fails the queued ones with SBR_E_STATE, frees everything. */
 int sbr_server_destroy(sbr_server* s);
 const char* sbr_server_last_error(const sbr_server* s);
-/* Queues a query; rx[n_rx][3] is copied.  SBR_E_LIMIT when the queue is full, SBR_E_NOMEM when
- * the query alone exceeds the HBM budget.  *id identifies it for the calls below. */
+/* Queues a query; rx[n_rx][3] is copied.  SBR_E_ARG for tess outside [0, 11], SBR_E_LIMIT when
+ * the queue is full, SBR_E_NOMEM when the query alone exceeds the HBM budget.  *id identifies it for the calls below. */
 int sbr_submit(sbr_server* s, const sbr_trace_params* params, const double* rx, int32_t n_rx, int64_t* id);
 /* Waits up to wait_ms (< 0: no limit) for the query.  Returns SBR_OK (done), SBR_PENDING, the
  * query's SBR_E_* status if it failed, or SBR_E_ARG for an unknown id.  info may be NULL. */

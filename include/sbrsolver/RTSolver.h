@@ -4,8 +4,9 @@
 // materials; ExecuteRayTracing(...) takes the same arguments, keeps non-owning pointers to the
 // receivers and the mesh, and returns N_rays * N_rx like the reference (RTSolver.cpp:52-126); the
 // Save* writers produce the reference's file formats (RTSolver.cpp:510-766).  The trace itself runs
-// through the C-ABI (sbr_capi.h); the scene is uploaded once per (mesh, BVH) pair and stays
-// resident on the device.  Extensions: result getters, trace options, batch tracing.
+// through the C-ABI (sbr_capi.h).  Each trace flattens the live mesh and BVH, as the reference reads
+// them on every call; the device copy is replaced whenever their content hash changes and stays
+// resident otherwise.  Extensions: result getters, trace options, batch tracing.
 #pragma once
 
 #include <string>
@@ -97,8 +98,6 @@ class RTSolver {
   sbr_ctx* _ctx = nullptr;
   std::vector<float> _pattern;   // GainInfo rows, theta-major
   int _patternRows = 0, _patternCols = 0, _patternVersion = 0, _patternSent = -1;
-  const void* _uploadedMesh = nullptr;
-  const void* _uploadedBvh = nullptr;
-  size_t _uploadedSize = 0;
+  uint64_t _uploadedFingerprint = 0;   // content hash of the scene on the device, 0: none
   int EnsureContext();
 };

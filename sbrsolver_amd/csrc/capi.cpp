@@ -37,6 +37,10 @@ static int put(DBuf<T>& b, const std::vector<T>& h, hipStream_t s) {
 // triangle is the reference's visiting order, and builds the surface tables addressed by
 // hitSurfaceID (through the mesh vector, and by id).
 int Engine::upload(const sbr_scene_desc* d) {
+  // A failed upload leaves no scene: device buffers are replaced one by one below, so after an error
+  // they may hold parts of two scenes, and no trace may run on them.
+  haveScene = false;
+  haveResult = false;
   if (!d || d->n_tri <= 0 || d->n_node <= 0 || !d->tri_v || !d->tri_n || !d->tri_cs || !d->tri_cop || !d->tri_mat ||
       !d->mesh_tri_id || !d->node_box || !d->node_child || !d->node_first || !d->node_count || !d->leaf_tri)
     return fail(SBR_E_ARG, "scene: missing arrays");
@@ -246,6 +250,8 @@ int Engine::upload(const sbr_scene_desc* d) {
 }
 
 int Engine::clone_scene(const Engine& s) {
+  haveScene = false;
+  haveResult = false;
   if (!s.haveScene) return fail(SBR_E_STATE, "source has no scene");
   if (s.mats.empty()) return fail(SBR_E_STATE, "source has no materials");
   hNodes = s.hNodes;

@@ -14,6 +14,7 @@
 
 namespace sbr {
 
+constexpr int kMaxTess = 11;        // icosahedron subdivision levels accepted (level 11: 41.9 M launches)
 constexpr int kMaxDepth = 24;       // reflections + transmissions per path (tree depth)
 constexpr int kMaxStack = 64;       // BVH traversal stack (tree height bound, checked on upload)
 constexpr int kBlock = 256;         // threads per block of the traversal kernels
@@ -313,6 +314,12 @@ struct Engine {
             double* tgt, double* pl);
   int path_tx(int32_t* out, int64_t n);
   int path_fields(double* out, int64_t n);
+  // Device bytes held between traces: the per-trace buffers (kept at their largest size so that the
+  // next trace need not reallocate) and the cached launch directions.  The scene is not counted.
+  size_t held_bytes() const;
+  // Releases the per-trace buffers and the sizing hints that would grow them back to the last
+  // trace's size (the result of the last trace is dropped; the scene and directions stay).
+  void trim();
 
   bool multiWindow = false;   // last trace ran in several launch windows (paths kept in `cache`)
   PathCache cache;

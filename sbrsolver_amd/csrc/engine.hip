@@ -1507,7 +1507,7 @@ static void ico_base(std::vector<double>& v, std::vector<int32_t>& f, std::vecto
 
 int Engine::generate_rays(int tess, DBuf<double>& out, int64_t* nOut) {
   if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
-  if (tess < 0 || tess > 11) return fail(SBR_E_ARG, "tessellation must be in [0, 11]");
+  if (tess < 0 || tess > kMaxTess) return fail(SBR_E_ARG, "tessellation must be in [0, " + std::to_string(kMaxTess) + "]");
   const int64_t nvFinal = 10 * (int64_t(1) << (2 * tess)) + 2;
   std::vector<double> v0;
   std::vector<int32_t> f0, a0;
@@ -2196,6 +2196,34 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
 }
 
 // Kept paths of the window whose winners are in dWin* (winners of them).
+template <typename F>
+static void for_each_trace_buffer(Engine& e, F&& f) {
+  f(e.dTx); f(e.dTree); f(e.dQueue[0]); f(e.dQueue[1]); f(e.dLevels); f(e.dScratch); f(e.dRx);
+  f(e.dCellStart); f(e.dCellRx); f(e.dLos); f(e.dTabA); f(e.dTabB); f(e.dTabOrder); f(e.dTabRx);
+  f(e.dHeavy); f(e.dCand); f(e.dCandFlag); f(e.dTileCount); f(e.dTileOffset); f(e.dRec); f(e.dRecCount);
+  f(e.dRecPrefix); f(e.dNodeKey); f(e.dWinKey); f(e.dWinKey2); f(e.dWinRx); f(e.dWinRx2); f(e.dWinIdx);
+  f(e.dWinIdx2); f(e.dWinNode); f(e.dWinField); f(e.dField);
+}
+
+size_t Engine::held_bytes() const {
+  size_t b = 0;
+  for_each_trace_buffer(const_cast<Engine&>(*this), [&](auto& buf) { b += buf.n * sizeof(*buf.p); });
+  for (const auto& kv : rayCache) b += kv.second->n * sizeof(double);
+  return b;
+}
+
+void Engine::trim() {
+  if (device >= 0) (void)hipSetDevice(device);
+  for_each_trace_buffer(*this, [](auto& buf) { buf.release(); });
+  nodeCap = 0;
+  recCap = 0;
+  tabCapHint = 0;
+  perRootHint = 0;
+  haveResult = false;
+  multiWindow = false;
+  cache = PathCache{};
+}
+
 int Engine::paths_window(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
                          int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
                          bool countOnly) {

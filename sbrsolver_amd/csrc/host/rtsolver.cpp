@@ -26,7 +26,44 @@ void RTSolver::Cleanup() {
   _efield.clear();
   _kept.clear();
   _isSceneInitialized = false;
+  _uploadedFingerprint = 0;   // the next trace uploads the scene again
 }
+
+namespace {
+
+// FNV-1a over the bytes of the flattened scene: the device copy is reused only while the triangles,
+// their frames, ids and materials and the BVH are unchanged (the reference reads the live objects on
+// every trace, so an edit in place, a rebuilt BVH or a new mesh at the old address must be seen).
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 0x100000001b3ull;
+  return h;
+}
+
+template <typename T>
+uint64_t fnv1a_vec(uint64_t h, const std::vector<T>& v) {
+  const uint64_t n = v.size();
+  h = fnv1a(h, &n, sizeof n);
+  return v.empty() ? h : fnv1a(h, v.data(), v.size() * sizeof(T));
+}
+
+uint64_t scene_fingerprint(const SceneArrays& sa) {
+  uint64_t h = 0xcbf29ce484222325ull;
+  h = fnv1a_vec(h, sa.tri_v);
+  h = fnv1a_vec(h, sa.tri_n);
+  h = fnv1a_vec(h, sa.tri_cs);
+  h = fnv1a_vec(h, sa.tri_cop);
+  h = fnv1a_vec(h, sa.tri_mat);
+  h = fnv1a_vec(h, sa.mesh_tri_id);
+  h = fnv1a_vec(h, sa.node_box);
+  h = fnv1a_vec(h, sa.node_child);
+  h = fnv1a_vec(h, sa.node_first);
+  h = fnv1a_vec(h, sa.node_count);
+  h = fnv1a_vec(h, sa.leaf_tri);
+  return h | 1;   // 0 means "nothing uploaded"
+}
+
+}  // namespace
 
 const char* RTSolver::LastError() const { return _ctx ? sbr_last_error(_ctx) : "no context"; }
 
@@ -90,15 +127,16 @@ int RTSolver::ExecuteRayTracing(Vec3 sourcePoint, const Mat3& txCoordSys, int ma
   if (!_isSceneInitialized || !_bvh) return SBR_E_STATE;
   int rc = EnsureContext();
   if (rc) return rc;
-  if (_uploadedMesh != (const void*)triangleMesh.data() || _uploadedBvh != (const void*)_bvh ||
-      _uploadedSize != triangleMesh.size()) {
+  {
     SceneArrays sa;
     if (!sa.build(triangleMesh, *_bvh)) return SBR_E_ARG;
-    rc = sbr_scene_upload(_ctx, &sa.desc);
-    if (rc) return rc;
-    _uploadedMesh = triangleMesh.data();
-    _uploadedBvh = _bvh;
-    _uploadedSize = triangleMesh.size();
+    const uint64_t fp = scene_fingerprint(sa);
+    if (fp != _uploadedFingerprint) {
+      _uploadedFingerprint = 0;
+      rc = sbr_scene_upload(_ctx, &sa.desc);
+      if (rc) return rc;
+      _uploadedFingerprint = fp;
+    }
   }
   rc = sbr_materials(_ctx, reinterpret_cast<const sbr_material*>(_materialProperties), _materialCount);
   if (rc) return rc;

@@ -3,8 +3,11 @@
 // A server owns `slots` engines.  Each engine holds its own device copy of the scene and materials,
 // its own stream and its own per-trace buffers, and is driven by one worker thread, so the queries
 // of different slots overlap on the device.  Admission is first come, first served: the query at
-// the head of the queue starts on the first free slot once the estimated device memory of the
-// queries in flight plus its own fits the HBM budget.  A query is traced exactly as sbr_trace
+// the head of the queue starts on the first free slot once the device memory of all slots fits the
+// HBM budget with it.  A slot's memory is what its engine holds between queries (per-trace buffers
+// kept at their largest size, cached launch directions), or while it runs a query the larger of
+// that and the query's estimate; after a query, a slot holding more than its share of the budget
+// releases its per-trace buffers.  The scene copies are outside the budget.  A query is traced exactly as sbr_trace
 // traces it (one transmitter, its own receivers), so its result does not depend on its neighbours.
 #include <hip/hip_runtime.h>
 
@@ -75,7 +78,8 @@ struct sbr_server {
   bool stop = false;
   int device = -1;
   int32_t maxQueued = 4096;
-  int64_t budget = 0, inflight = 0, peakInflight = 0;
+  int64_t budget = 0, inflight = 0, peakInflight = 0;   // inflight: the sum of slotUse
+  std::vector<int64_t> slotHeld, slotUse;   // per slot: bytes held between queries; bytes counted now
   int64_t submitted = 0, completed = 0, failed = 0, refused = 0, running = 0, bounces = 0;
   std::vector<double> latency;   // ms, completed queries
   bool anySubmit = false;
@@ -104,17 +108,20 @@ struct sbr_server {
     return (int64_t)bytes;
   }
 
-  bool head_fits() const {
+  // Whether the head of the queue may start on idle slot `slot`: always when nothing runs (so that
+  // a query that alone exceeds what the idle slots leave free still runs), otherwise when all slots'
+  // memory with the head's on this slot fits the budget.
+  bool head_fits(int slot) const {
     if (queue.empty()) return false;
     const Request& r = *reqs.at(queue.front());
-    return inflight == 0 || inflight + r.est <= budget;
+    return running == 0 || inflight - slotUse[slot] + std::max(slotHeld[slot], r.est) <= budget;
   }
 
   void worker(int slot) {
     sbr::Engine& e = *eng[slot];
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-      cvWork.wait(lk, [&] { return stop || head_fits(); });
+      cvWork.wait(lk, [&] { return stop || head_fits(slot); });
       if (stop) return;
       const int64_t id = queue.front();
       queue.pop_front();
@@ -122,7 +129,9 @@ struct sbr_server {
       r->state = kRunning;
       r->slot = slot;
       r->tStart = Clock::now();
-      inflight += r->est;
+      const int64_t use = std::max(slotHeld[slot], r->est);
+      inflight += use - slotUse[slot];
+      slotUse[slot] = use;
       peakInflight = std::max(peakInflight, inflight);
       running++;
       lk.unlock();
@@ -150,10 +159,15 @@ struct sbr_server {
         rc = SBR_E_INTERNAL;
         msg = x.what();
       }
+      // only this worker touches its engine: its buffers are measured (and trimmed) unlocked
+      const int64_t share = budget / (int64_t)eng.size();
+      if ((int64_t)e.held_bytes() > share) e.trim();
+      const int64_t held = (int64_t)e.held_bytes();
 
       lk.lock();
       r->tEnd = Clock::now();
-      inflight -= r->est;
+      inflight += held - slotUse[slot];
+      slotHeld[slot] = slotUse[slot] = held;
       running--;
       if (rc == SBR_OK) {
         r->state = kDone;
@@ -249,6 +263,8 @@ int sbr_server_create(sbr_ctx* src, const sbr_server_config* cfg, sbr_server** o
     } else {
       s->budget = int64_t(1) << 34;
     }
+    s->slotHeld.assign(slots, 0);
+    s->slotUse.assign(slots, 0);
     for (int i = 0; i < slots; i++) s->workers.emplace_back([p = s.get(), i] { p->worker(i); });
   } catch (const std::bad_alloc&) {
     return src->eng.fail(SBR_E_NOMEM, "server: host allocation failed");
@@ -282,13 +298,23 @@ int sbr_server_destroy(sbr_server* s) {
   return SBR_OK;
 }
 
-const char* sbr_server_last_error(const sbr_server* s) { return s ? s->err.c_str() : "null server"; }
+// The message is copied under the lock into storage of the calling thread, valid until that
+// thread's next call of sbr_server_last_error or sbr_request_error.
+static thread_local std::string tlMessage;
+
+const char* sbr_server_last_error(const sbr_server* s) {
+  if (!s) return "null server";
+  std::lock_guard<std::mutex> g(const_cast<sbr_server*>(s)->mu);
+  tlMessage = s->err;
+  return tlMessage.c_str();
+}
 
 int sbr_submit(sbr_server* s, const sbr_trace_params* params, const double* rx, int32_t n_rx, int64_t* id) {
   return guarded_srv(s, [&] {
     std::unique_lock<std::mutex> lk(s->mu);
     if (!params || !id || n_rx < 0 || (n_rx > 0 && !rx)) return s->fail(SBR_E_ARG, "submit: bad arguments");
-    if (params->tess < 0 || params->tess > 12) return s->fail(SBR_E_ARG, "submit: tess must be in [0, 12]");
+    if (params->tess < 0 || params->tess > sbr::kMaxTess)
+      return s->fail(SBR_E_ARG, "submit: tess must be in [0, " + std::to_string(sbr::kMaxTess) + "]");
     if (s->stop) return s->fail(SBR_E_STATE, "submit: server stopping");
     if ((int32_t)s->queue.size() >= s->maxQueued) {
       s->refused++;
@@ -346,7 +372,8 @@ const char* sbr_request_error(sbr_server* s, int64_t id) {
   if (!s) return "null server";
   std::lock_guard<std::mutex> g(s->mu);
   auto it = s->reqs.find(id);
-  return it == s->reqs.end() ? "unknown query id" : it->second->err.c_str();
+  tlMessage = it == s->reqs.end() ? std::string("unknown query id") : it->second->err;
+  return tlMessage.c_str();
 }
 
 int sbr_request_fields(sbr_server* s, int64_t id, double* out, int32_t n_rx) {

@@ -50,6 +50,21 @@ def test_host_only_context_rejects_device_work():
             c.scene_from_stl("/nonexistent.stl")
 
 
+def test_failed_upload_leaves_no_scene():
+    """A scene upload that fails leaves the context without a scene (never a mix of two scenes):
+    later calls that need one report SBR_E_STATE."""
+    import ctypes as C
+    with Context(-1) as c:
+        c.scene_from_stl(os.path.join(GOLDEN, "stl", "office.stl"))
+        assert c.scene_info()["n_tri"] > 0
+        desc = (C.c_char * 512)()   # zeroed: no triangles, no arrays
+        assert lib().sbr_scene_upload(c.h, desc) == -1   # SBR_E_ARG
+        with pytest.raises(SbrError, match="no scene"):
+            c.scene_info()
+        c.scene_from_stl(os.path.join(GOLDEN, "stl", "cube.stl"))   # a good upload restores it
+        assert c.scene_info()["n_tri"] == 12
+
+
 def _dfs_leaf_order(ob):
     """DFS (children[0] first) leaf order of the oracle's BVH = the device triangle order."""
     return ob["leaf_tri"]

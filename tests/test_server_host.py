@@ -63,8 +63,10 @@ def test_admission_refuses_query_over_budget(host_ctx):
 def test_bad_arguments(host_ctx):
     ctx, cfg = host_ctx
     with Server(ctx, slots=1) as srv:
-        with pytest.raises(SbrError, match="tess"):
-            srv.submit(make_params(**dict(cfg["trace"], tess=13)), cfg["rx"])
+        for tess in (12, 13, -1):   # the engine accepts levels 0..11: a level-12 query is refused at submit
+            with pytest.raises(SbrError, match="tess"):
+                srv.submit(make_params(**dict(cfg["trace"], tess=tess)), cfg["rx"])
+        assert srv.stats()["submitted"] == 0
         with pytest.raises(SbrError, match="unknown"):
             srv.poll(12345)
 
