@@ -201,6 +201,18 @@ int sbr_get_tuning(sbr_ctx* ctx, sbr_tuning* tuning);
 int sbr_generate_rays(sbr_ctx* ctx, int32_t tess, double* dirs_out, int64_t capacity, int64_t* n_out);
 int sbr_closest_hit(sbr_ctx* ctx, const double* orig, const double* dirs, int32_t n, int32_t exact,
                     int32_t* tri_out, double* point_out, double* dist_out);
+/* Allocation-free variant of sbr_closest_hit (no counterpart in the reference; for callers that
+ * cannot allocate device memory: code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results as sbr_closest_hit, with n at most
+ * SBR_NOALLOC_MAX_RAYS (SBR_E_LIMIT beyond, nothing run).  The rays are staged in a fixed array of
+ * SBR_NOALLOC_SCRATCH_BYTES in device memory, one per GPU, held from the library's first use on that
+ * GPU until the process ends; no device memory is allocated or freed by the call.  All calls on one
+ * GPU share that array: calls that overlap on one GPU (from several threads or contexts) race.  The
+ * call returns after its results are on the host. */
+#define SBR_NOALLOC_SCRATCH_BYTES (32u << 20)
+#define SBR_NOALLOC_MAX_RAYS 399442
+int sbr_closest_hit_noalloc(sbr_ctx* ctx, const double* orig, const double* dirs, int32_t n, int32_t exact,
+                            int32_t* tri_out, double* point_out, double* dist_out);
 
 int sbr_trace(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx, int32_t n_rx);
 /* Several transmitters over the same scene and receivers in one pass (no reference counterpart:

@@ -104,6 +104,7 @@ SIGNATURES = {
     "sbr_autotune": ([_vp, _vp, _vp, _i32, _vp], C.c_int),
     "sbr_generate_rays": ([_vp, _i32, _vp, _i64, _vp], C.c_int),
     "sbr_closest_hit": ([_vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp], C.c_int),
+    "sbr_closest_hit_noalloc": ([_vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp], C.c_int),
     "sbr_trace": ([_vp, _vp, _vp, _i32], C.c_int),
     "sbr_trace_batch": ([_vp, _vp, _i32, _vp, _i32], C.c_int),
     "sbr_result_path_tx": ([_vp, _vp, _i64], C.c_int),

@@ -526,6 +526,15 @@ int sbr_closest_hit(sbr_ctx* ctx, const double* o, const double* d, int32_t n, i
   });
 }
 
+int sbr_closest_hit_noalloc(sbr_ctx* ctx, const double* o, const double* d, int32_t n, int32_t exact, int32_t* tri,
+                            double* pt, double* dist) {
+  return guarded(ctx, [&] {
+    if (n < 0 || (n > 0 && (!o || !d || !tri || !pt || !dist)))
+      return ctx->eng.fail(SBR_E_ARG, "closest_hit_noalloc: bad args");
+    return ctx->eng.closest_hit_noalloc(o, d, n, exact, tri, pt, dist);
+  });
+}
+
 int sbr_trace(sbr_ctx* ctx, const sbr_trace_params* p, const double* rx, int32_t n_rx) {
   return guarded(ctx, [&] { return ctx->eng.trace(p, 1, rx, n_rx); });
 }

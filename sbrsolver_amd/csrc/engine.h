@@ -307,6 +307,9 @@ struct Engine {
   int rays_for(int tess, const double** out, int64_t* nOut);   // cached launch directions
   int trace(const sbr_trace_params* p, int ntx, const double* rx, int m);
   int closest_hit(const double* o, const double* d, int n, int exact, int32_t* tri, double* pt, double* dist);
+  // closest_hit with the rays staged in the library's fixed device array (no allocation).
+  int closest_hit_noalloc(const double* o, const double* d, int n, int exact, int32_t* tri, double* pt, double* dist);
+  unsigned char* scratchPtr = nullptr;   // device address of that array on this engine's device
   int paths(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
             int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
             bool countOnly);

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 
 #include "dedup_hash.h"
 #include "device_common.h"
@@ -1427,6 +1428,8 @@ Engine::~Engine() {
   if (stream) (void)hipStreamDestroy(stream);
 }
 
+static int scratch_address(int dev, unsigned char** out);   // staging array of closest_hit_noalloc
+
 int Engine::init(int dev) {
   if (dev == -1) {   // host-only context: scene preparation and queries, no device work
     device = -1;
@@ -1443,6 +1446,8 @@ int Engine::init(int dev) {
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, dev));
   numCU = prop.multiProcessorCount;
+  const int rc = scratch_address(dev, &scratchPtr);   // this device is current
+  if (rc != SBR_OK) return fail(rc, "hipGetSymbolAddress of the staging array failed");
   return SBR_OK;
 }
 
@@ -1570,6 +1575,76 @@ int Engine::rays_for(int tess, const double** out, int64_t* nOut) {
 }
 
 static size_t stack_bytes(int height, int block) { return (size_t)height * block * 8; }
+
+// Staging area of sbr_closest_hit_noalloc: a fixed array in device memory, one per GPU, held from the
+// library's first use on that GPU until the process ends.  Host code never passes the array's name
+// to a kernel or a copy (on the host it is a stand-in, not a device address): it uses the address
+// hipGetSymbolAddress returns on each device (Engine::scratchPtr, looked up once per device).
+// Layout for n rays, each part 256-byte aligned: origins (3 n doubles), directions (3 n doubles),
+// hit points (3 n doubles), distances (n doubles), triangle ids (n int32).
+static __device__ __align__(256) unsigned char scratch[SBR_NOALLOC_SCRATCH_BYTES];
+static size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+static size_t noalloc_bytes(size_t n) {
+  return 3 * align256(3 * n * sizeof(double)) + align256(n * sizeof(double)) + align256(n * sizeof(int32_t));
+}
+static_assert(SBR_NOALLOC_MAX_RAYS * 84 + 5 * 256 <= SBR_NOALLOC_SCRATCH_BYTES, "noalloc ray limit fits the array");
+
+static std::mutex gScratchMu;
+static unsigned char* gScratchAddr[64];   // per device ordinal; nullptr until looked up
+
+static int scratch_address(int dev, unsigned char** out) {
+  std::lock_guard<std::mutex> g(gScratchMu);
+  if (dev < 0 || dev >= 64) return SBR_E_ARG;
+  if (!gScratchAddr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return SBR_E_HIP;
+    }
+    gScratchAddr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = gScratchAddr[dev];
+  return SBR_OK;
+}
+
+int Engine::closest_hit_noalloc(const double* o, const double* d, int n, int exact, int32_t* tri, double* pt,
+                                double* dist) {
+  if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
+  if (!haveScene) return fail(SBR_E_STATE, "no scene uploaded");
+  if (n > SBR_NOALLOC_MAX_RAYS)
+    return fail(SBR_E_LIMIT, "closest_hit_noalloc: " + std::to_string(n) + " rays exceed the staging array's " +
+                                 std::to_string(SBR_NOALLOC_MAX_RAYS) + " (use sbr_closest_hit or split the call)");
+  if (n <= 0) return SBR_OK;
+  if (!scratchPtr) return fail(SBR_E_STATE, "closest_hit_noalloc: staging array address not known");
+  if (noalloc_bytes((size_t)n) > SBR_NOALLOC_SCRATCH_BYTES) return fail(SBR_E_INTERNAL, "noalloc layout");
+  HIPCHK(hipSetDevice(device));
+  unsigned char* base = scratchPtr;
+  double* sO = reinterpret_cast<double*>(base);
+  double* sD = reinterpret_cast<double*>(base + align256(3 * (size_t)n * sizeof(double)));
+  double* sP = reinterpret_cast<double*>(base + 2 * align256(3 * (size_t)n * sizeof(double)));
+  double* sDist = reinterpret_cast<double*>(base + 3 * align256(3 * (size_t)n * sizeof(double)));
+  int32_t* sT = reinterpret_cast<int32_t*>(base + 3 * align256(3 * (size_t)n * sizeof(double)) +
+                                           align256((size_t)n * sizeof(double)));
+  // Every output entry of the n rays is written by the kernel, so nothing needs clearing: what the
+  // previous call left in the array is never read.
+  HIPCHK(hipMemcpyAsync(sO, o, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(sD, d, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
+  const size_t sm = stack_bytes(stackDepth(), kBlock);
+  if (exact)
+    k_closest_batch<true><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
+                                                                         dSubTri.p, sceneExtent, sO, sD, n, stackDepth(),
+                                                                         sT, sP, sDist);
+  else
+    k_closest_batch<false><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dNodesF.p, dTris.p, dTriBox32.p,
+                                                                          dSub.p, dSubTri.p, sceneExtent, sO, sD, n,
+                                                                          stackDepth(), sT, sP, sDist);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(tri, sT, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(pt, sP, 3 * sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(dist, sDist, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SBR_OK;
+}
 
 int Engine::closest_hit(const double* o, const double* d, int n, int exact, int32_t* tri, double* pt, double* dist) {
   if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
