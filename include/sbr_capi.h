@@ -26,6 +26,11 @@
  *                          (RTSolver.cpp:532-538)
  *   sbr_generate_rays      GenerateRaysOnIcosahedron (Icosahedron.cpp:80-110)
  *   sbr_closest_hit        BVH<Triangle>::RayIntersects (BVH/BVH.h:276-328)
+ *   sbr_path_fields        FieldCompute::FieldForPath (FieldCompute.cpp:71-151), the per-path step
+ *                          of FieldCompute::FieldAtReceiver (FieldCompute.cpp:11-25) as the
+ *                          FieldCompute class (include/sbrsolver/FieldCompute.h) calls it
+ *   sbr_fresnel_sweep      FieldCompute::RefCoeffTest / TransCoeffTest (FieldCompute.cpp:27-69)
+ *                          with GetTransAngle / GetTMCoeff / GetTECoeff (FieldCompute.cpp:236-302)
  */
 #ifndef SBR_CAPI_H
 #define SBR_CAPI_H
@@ -215,6 +220,35 @@ int sbr_closest_hit_noalloc(sbr_ctx* ctx, const double* orig, const double* dirs
                             int32_t* tri_out, double* point_out, double* dist_out);
 
 int sbr_trace(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx, int32_t n_rx);
+
+/* Field of caller-given paths: FieldCompute::FieldForPath (FieldCompute.cpp:71-151) on the device,
+ * the same transport the trace applies to its kept paths.  Path p has seg_count[p] segments (1 ..
+ * 25) starting at seg_off[p]; segment s: seg_src[s][3] -> seg_tgt[s][3], seg_surf[s] the surface it
+ * ends on (its hitSurfaceID; -1 at the receiver), seg_type[s] how it left its start surface (0 from
+ * the transmitter, 1 reflected, 2 transmitted: reflectionMaterialId / penetrationMaterialId >= 0)
+ * and seg_mat[s] that surface's material (an index into mats, unused for type 0).  surf_cs[id][9]
+ * is the row-major coordSys of surface id (mesh[id]->coordSys).  fields_out[n_paths][3][2].  The
+ * scene of the context is not used; antennaType 3 uses its pattern table. */
+typedef struct {
+  double txCoordSys[9];  /* row-major Mat3 */
+  double frequency;
+  double pt;
+  int32_t useFresnel;
+  int32_t antennaType;   /* 0 isotropic, 1 Hertzian (FieldCompute.cpp:117), 2 half-wave, 3 table */
+  double wallWidth;      /* extension, as sbr_trace_params */
+  int32_t physicalField; /* extension, as sbr_trace_params */
+  int32_t reserved;
+} sbr_field_params;
+int sbr_path_fields(sbr_ctx* ctx, const sbr_field_params* params, const sbr_material* mats, int32_t n_mats,
+                    int64_t n_paths, const int64_t* seg_off, const int32_t* seg_count, const double* seg_src,
+                    const double* seg_tgt, const int32_t* seg_surf, const int32_t* seg_type, const int32_t* seg_mat,
+                    int32_t n_surf, const double* surf_cs, double* fields_out);
+/* Coefficient sweep of FieldCompute::RefCoeffTest / TransCoeffTest (FieldCompute.cpp:27-69): n
+ * incidence angles 0 .. pi/2 evenly spaced (theta_out[n], radians) between media of complex relative
+ * permittivity eps_i = {re, im} and eps_t, at `frequency`; inf_wall = 1 is the reference's infinite
+ * wall (no transmission), 0 its slab of `width` m.  coeff_out[n][4][2]: rTM, tTM, rTE, tTE. */
+int sbr_fresnel_sweep(sbr_ctx* ctx, int32_t n, double frequency, const double* eps_i, const double* eps_t,
+                      double width, int32_t inf_wall, double* theta_out, double* coeff_out);
 /* Several transmitters over the same scene and receivers in one pass (no reference counterpart:
  * the reference runs one transmitter per ExecuteRayTracing call).  params[n_tx]; every
  * transmitter's result equals what sbr_trace gives for it alone. */

@@ -11,7 +11,9 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <fstream>
 #include <limits>
+#include <string>
 #include <vector>
 
 #include "VecMatDef.h"
@@ -26,6 +28,12 @@ class BVHNode {
 
   static void EmptyAABB(Mat23& b) {
     for (int a = 0; a < 3; a++) { b(0, a) = FLT_MAX; b(1, a) = -FLT_MAX; }
+  }
+  // The box's eight corners, x varying fastest, then y, then z (BVHNode.h:39-51).
+  VecVec3 CornerPoints() const {
+    VecVec3 v(8);
+    for (int k = 0; k < 8; k++) v[k] = Vec3(bbox(k & 1, 0), bbox((k >> 1) & 1, 1), bbox((k >> 2) & 1, 2));
+    return v;
   }
   bool IntersectsAABB(const Vec3& o, const Vec3& d) const {
     double bmin[3] = {bbox(0, 0), bbox(0, 1), bbox(0, 2)}, bmax[3] = {bbox(1, 0), bbox(1, 1), bbox(1, 2)};
@@ -43,6 +51,38 @@ class BVH {
   std::vector<BVHNode<T>> nodes;
 
   void ConstructBVH(std::vector<T*>& objects) { Build(objects, 0); }
+
+  // Every node's box as a VTK hexahedron with its id, children and object count (BVH/BVH.h:57-133).
+  void SaveAsVtk(const std::string& fname) const {
+    std::ofstream ofs(fname);
+    const size_t n = nodes.size();
+    ofs << "# vtk DataFile Version 2.0" << std::endl;
+    ofs << "3D model of boudary volume hierarchy" << std::endl;   // the reference's title, typo included
+    ofs << "ASCII" << std::endl;
+    ofs << "DATASET UNSTRUCTURED_GRID" << std::endl;
+    ofs << std::endl;
+    ofs << "POINTS " << 8 * n << " float" << std::endl;
+    for (const BVHNode<T>& node : nodes)
+      for (const Vec3& v : node.CornerPoints()) ofs << " " << v.x() << " " << v.y() << " " << v.z() << std::endl;
+    ofs << "CELLS " << n << " " << 9 * n << std::endl;
+    for (size_t i = 0; i < n; i++) {
+      ofs << " 8";
+      for (size_t j = 0; j < 8; j++) ofs << " " << 8 * i + j;
+      ofs << std::endl;
+    }
+    ofs << "CELL_TYPES " << n << std::endl;
+    for (size_t i = 0; i < n; i++) ofs << " 11" << std::endl;
+    ofs << "CELL_DATA " << (int)n << std::endl;
+    ofs << "FIELD FieldData 4" << std::endl;
+    ofs << "BoxID 1 " << (int)n << " int" << std::endl;
+    for (size_t i = 0; i < n; i++) ofs << " " << i << std::endl;
+    ofs << "Child0 1 " << (int)n << " int" << std::endl;
+    for (size_t i = 0; i < n; i++) ofs << " " << nodes[i].children[0] << std::endl;
+    ofs << "Child1 1 " << (int)n << " int" << std::endl;
+    for (size_t i = 0; i < n; i++) ofs << " " << nodes[i].children[1] << std::endl;
+    ofs << "NumFaces 1 " << (int)n << " int" << std::endl;
+    for (size_t i = 0; i < n; i++) ofs << " " << nodes[i].objects.size() << std::endl;
+  }
 
   // Host closest hit in the reference's traversal order (DFS, children[0] first, strict '<').
   bool RayIntersects(int index, const Vec3& o, const Vec3& d, HitInfo& info) const {

@@ -26,9 +26,19 @@ namespace Preprocessor {
 bool ReadPatternFile(std::string fileName, GainMap& output);
 
 // STL (ASCII or binary) -> triangles with the file normals and coplanar ids
-// (Preprocessor.cpp:69-114).  Returns false when the file cannot be read.
+// (Preprocessor.cpp:69-114).  saveEdges / saveFaces write dataDir + "Edges.vtk" / "Faces.vtk"
+// (the dedup'd vertices with the edge map / the triangles with their ids).  Returns false when the
+// file cannot be read.
 bool StlToGeometry(std::string fileName, std::vector<Triangle*>& output, bool saveEdges = false,
                    bool saveFaces = false, std::string dataDir = "");
+
+// Edge map of the scene (key: the two vertex offsets into coords, 3 per vertex; value: the
+// triangles sharing the edge) as VTK lines with their connection counts (Preprocessor.cpp:290-351).
+bool SaveEdgesAsVtk(std::string fileName, const std::map<std::pair<int, int>, std::vector<int>>& edgeMap,
+                    const std::vector<float>& coords);
+// Triangles as VTK polygons with FaceID, CoplanarID and MaterialID (Preprocessor.cpp:354-421).
+bool SaveFacesAsVtk(std::string fileName, const std::vector<Triangle*>& triangles, const std::vector<float>& coords,
+                    const std::vector<unsigned int> tris);
 
 // Facet arrays as a binary STL holds them (9 vertex floats + 3 normal floats per facet), with
 // optional per-facet material ids; same vertex dedup, normal indexing and coplanar grouping.

@@ -15,24 +15,9 @@
 #include "../sbr_capi.h"
 #include "BVH.h"
 #include "Preprocessor.h"
+#include "Ray.h"
 #include "Triangle.h"
 #include "VecMatDef.h"
-
-// One segment of a kept path (Ray.h:15-27, the fields the outputs and the field computation use).
-struct Ray {
-  Vec3 sourcePoint;
-  Vec3 targetPoint;
-  int penetrationMaterialId = -1;  // >= 0: the segment leaves its start surface by transmission
-  int reflectionMaterialId = -1;   // >= 0: ... by reflection
-  int hitSurfaceID = -1;
-  int hitCoplanarId = -1;
-};
-
-// Kept paths of one (launch, receiver) pair (Ray.h:59-67).
-class Paths {
- public:
-  std::vector<std::vector<Ray>> rayPaths;
-};
 
 struct RTSolverOptions {
   int antennaType = 1;        // FieldCompute.cpp:117 hard-codes 1 (Hertzian dipole); 3: SetTxPattern's table
@@ -51,6 +36,14 @@ struct RTSolverOptions {
     typedDedup = checkOcclusion = longMiss = physicalField = 1;
   }
   int device = 0;
+  // _rayPaths (the reference's Paths[launches][receivers], RTSolver.h:45) is filled after a trace
+  // when launches * receivers is at most this (24 bytes per entry; the reference's array at the
+  // urban canyon's 2.6 M launches x 10 k receivers would be 629 GB); otherwise it stays null.
+  int64_t rayPathArrayLimit = int64_t(1) << 22;
+  // 1: number the kept paths' segments with the reference's Ray::id (the global creation order of
+  // tree nodes, reflected child first, RTSolver.cpp:82-84, Ray.cpp:18); this reads every tree node
+  // of the trace back from the device.  0: id stays -1.
+  int rayIds = 0;
 };
 
 class RTSolver {
@@ -73,6 +66,10 @@ class RTSolver {
   const std::vector<Vec3c>& Fields() const { return _efield; }
   // Kept paths per receiver: (launch id, segments), launch ascending then path order.
   const std::vector<std::vector<std::pair<int, std::vector<Ray>>>>& ReceiverPaths() const { return _kept; }
+  // The kept paths as the reference's array (see RTSolverOptions::rayPathArrayLimit), for
+  // FieldCompute(RayPaths(), mesh, receivers, launches, ...); null when not filled.
+  Paths** RayPaths() const { return _rayPaths; }
+  int LaunchCount() const { return _pathsCount; }
   const sbr_stats& Stats() const { return _stats; }
   const char* LastError() const;
   // Transmitter pattern from a TXPattern.dat map (Preprocessor::ReadPatternFile), used by the
@@ -88,6 +85,8 @@ class RTSolver {
   std::vector<Vec3> _shootRayList;
   std::vector<Vec3c> _efield;
   std::vector<std::vector<std::pair<int, std::vector<Ray>>>> _kept;
+  Paths** _rayPaths = nullptr;   // [_pathsCount][_receiverCount] when filled
+  int _rayPathRows = 0;
   sbr_stats _stats{};
   bool _isSceneInitialized = false;
   int _txTesslation = 0;
@@ -100,4 +99,9 @@ class RTSolver {
   int _patternRows = 0, _patternCols = 0, _patternVersion = 0, _patternSent = -1;
   uint64_t _uploadedFingerprint = 0;   // content hash of the scene on the device, 0: none
   int EnsureContext();
+  void FreeRayPaths();
+  int NumberTreeNodes(const std::vector<int32_t>& prx, const std::vector<int32_t>& pl, const std::vector<int32_t>& pn,
+                      const std::vector<int64_t>& poff, const std::vector<int32_t>& stri,
+                      const std::vector<int32_t>& stype, std::vector<int64_t>& ids);
+  static double AngleBetweenDeg(sbr::V3 a, sbr::V3 b);
 };

@@ -16,6 +16,8 @@ SOURCES = [
     os.path.join(HERE, "csrc", "serve.cpp"),
     os.path.join(HERE, "csrc", "host", "preprocessor.cpp"),
     os.path.join(HERE, "csrc", "host", "rtsolver.cpp"),
+    os.path.join(HERE, "csrc", "host", "ray.cpp"),
+    os.path.join(HERE, "csrc", "host", "fieldcompute.cpp"),
 ]
 DEMO = os.path.join(HERE, "sbr_demo")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

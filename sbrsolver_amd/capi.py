@@ -107,6 +107,8 @@ SIGNATURES = {
     "sbr_closest_hit_noalloc": ([_vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp], C.c_int),
     "sbr_trace": ([_vp, _vp, _vp, _i32], C.c_int),
     "sbr_trace_batch": ([_vp, _vp, _i32, _vp, _i32], C.c_int),
+    "sbr_path_fields": ([_vp, _vp, _vp, _i32, _i64] + [_vp] * 7 + [_i32, _vp, _vp], C.c_int),
+    "sbr_fresnel_sweep": ([_vp, _i32, C.c_double, _vp, _vp, C.c_double, _i32, _vp, _vp], C.c_int),
     "sbr_result_path_tx": ([_vp, _vp, _i64], C.c_int),
     "sbr_result_stats": ([_vp, _vp], C.c_int),
     "sbr_result_fields": ([_vp, _vp, _i32], C.c_int),

@@ -539,6 +539,23 @@ int sbr_trace(sbr_ctx* ctx, const sbr_trace_params* p, const double* rx, int32_t
   return guarded(ctx, [&] { return ctx->eng.trace(p, 1, rx, n_rx); });
 }
 
+int sbr_path_fields(sbr_ctx* ctx, const sbr_field_params* p, const sbr_material* mats, int32_t n_mats, int64_t n_paths,
+                    const int64_t* seg_off, const int32_t* seg_count, const double* seg_src, const double* seg_tgt,
+                    const int32_t* seg_surf, const int32_t* seg_type, const int32_t* seg_mat, int32_t n_surf,
+                    const double* surf_cs, double* fields_out) {
+  return guarded(ctx, [&] {
+    return ctx->eng.path_fields_given(p, mats, n_mats, n_paths, seg_off, seg_count, seg_src, seg_tgt, seg_surf,
+                                      seg_type, seg_mat, n_surf, surf_cs, fields_out);
+  });
+}
+
+int sbr_fresnel_sweep(sbr_ctx* ctx, int32_t n, double frequency, const double* eps_i, const double* eps_t, double width,
+                      int32_t inf_wall, double* theta_out, double* coeff_out) {
+  return guarded(ctx, [&] {
+    return ctx->eng.fresnel_sweep(n, frequency, eps_i, eps_t, width, inf_wall, theta_out, coeff_out);
+  });
+}
+
 int sbr_trace_batch(sbr_ctx* ctx, const sbr_trace_params* p, int32_t n_tx, const double* rx, int32_t n_rx) {
   return guarded(ctx, [&] { return ctx->eng.trace(p, n_tx, rx, n_rx); });
 }

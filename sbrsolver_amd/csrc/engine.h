@@ -317,6 +317,13 @@ struct Engine {
             double* tgt, double* pl);
   int path_tx(int32_t* out, int64_t n);
   int path_fields(double* out, int64_t n);
+  // FieldForPath on the device for caller-given paths (sbr_path_fields).
+  int path_fields_given(const sbr_field_params* p, const sbr_material* m, int nMats, int64_t nPaths, const int64_t* off,
+                        const int32_t* cnt, const double* src, const double* tgt, const int32_t* surfId,
+                        const int32_t* type, const int32_t* mat, int nSurf, const double* frames, double* out);
+  // Reflection / transmission coefficient sweep over incidence angles (sbr_fresnel_sweep).
+  int fresnel_sweep(int n, double freq, const double* epsI, const double* epsT, double width, int infWall,
+                    double* theta, double* out);
   // Device bytes held between traces: the per-trace buffers (kept at their largest size so that the
   // next trace need not reallocate) and the cached launch directions.  The scene is not counted.
   size_t held_bytes() const;

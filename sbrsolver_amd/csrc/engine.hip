@@ -1245,6 +1245,112 @@ __device__ __forceinline__ void pattern_lookup(const FieldArgs& F, double th, do
   *eph = mkc(o[2], o[3]);
 }
 
+// One segment of a path as the field transport reads it: its end points, the surface it ends on
+// (whose frame is the next one; -1: the receiver), how it left its start surface (0: from the
+// transmitter or no interaction, 1 reflected, 2 transmitted) and that surface's material.
+struct FieldSeg {
+  V3 a, b;
+  int32_t surf;
+  int32_t type;
+  int32_t mat;
+};
+
+// FieldForPath (FieldCompute.cpp:71-151) over nseg segments given by seg(k); frames(id) is the
+// row-major coordSys of surface id.  Writes the path's field (3 complex) to out.
+template <typename SegF, typename FrameF>
+__device__ __forceinline__ void path_field(const FieldArgs& F, const TxDesc& K, const double* __restrict__ mats,
+                                           int nseg, SegF seg, FrameF frames, double* __restrict__ out) {
+  const double lamda = SBR_C0 / K.freq;
+  const double kw = (2 * SBR_PI) / lamda;
+  const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  const double* next = K.txcs;
+  const double* cur = I3;
+  V3C tot = {mkc(0, 0), mkc(0, 0), mkc(0, 0)};
+  double L = 0.0;
+  V3 prevA = v3(0, 0, 0), prevB = v3(0, 0, 0);
+  for (int k = 0; k < nseg; k++) {
+    const FieldSeg sg = seg(k);
+    V3 vg = vsub(sg.b, sg.a);
+    V3 vl = mat_vec(next, vg);
+    double r, th, ph;
+    cart_to_sph(vl, &r, &th, &ph);
+    L += r;
+    cur = next;
+    next = sg.surf == -1 ? I3 : frames(sg.surf);
+    if (K.physicalField && k > 0) {
+      // Extension: the global field split at the interaction into the component along s (normal to
+      // the plane of incidence) and along p (in it, s x direction before and after), with the
+      // Fresnel coefficients of the incidence angle in [0, 90] degrees; the reference's TE
+      // coefficient is minus the usual s one (reflection), its TM coefficient the usual p one.
+      const V3 din = vnormalized(vsub(prevB, prevA)), dout = vnormalized(vg);
+      prevA = sg.a;
+      prevB = sg.b;
+      const V3 nrm = vnormalized(v3(cur[6], cur[7], cur[8]));
+      double cth = fabs(vdot(din, nrm));
+      cth = cth > 1.0 ? 1.0 : cth;
+      V3 s = vcross(din, nrm);
+      if (vnorm(s) < 1e-12) {   // normal incidence: any direction normal to din
+        const V3 ax = fabs(din.x) <= fabs(din.y) && fabs(din.x) <= fabs(din.z) ? v3(1, 0, 0)
+                      : (fabs(din.y) <= fabs(din.z) ? v3(0, 1, 0) : v3(0, 0, 1));
+        s = vcross(din, ax);
+      }
+      s = vnormalized(s);
+      const V3 pin = vcross(s, din), pout = vcross(s, dout);
+      cx fTM, fTE;
+      fresnel(sg.type == 1, acos(cth), mats[6 * sg.mat + 3], mats[6 * sg.mat + 5], K.freq, K.useFresnel != 0,
+              K.wallWidth, &fTM, &fTE);
+      const cx cs = sg.type == 1 ? cneg(fTE) : fTE, cp = fTM;
+      const cx es = cadd(cadd(cscale(tot.x, s.x), cscale(tot.y, s.y)), cscale(tot.z, s.z));
+      const cx ep = cadd(cadd(cscale(tot.x, pin.x), cscale(tot.y, pin.y)), cscale(tot.z, pin.z));
+      const cx as = cmul(cs, es), ap = cmul(cp, ep);
+      tot.x = cadd(cscale(as, s.x), cscale(ap, pout.x));
+      tot.y = cadd(cscale(as, s.y), cscale(ap, pout.y));
+      tot.z = cadd(cscale(as, s.z), cscale(ap, pout.z));
+      continue;
+    }
+    prevA = sg.a;
+    prevB = sg.b;
+    V3C inc;
+    if (k == 0) {
+      inc.x = mkc(0, 0);
+      inc.y = mkc(0, 0);
+      inc.z = mkc(0, 0);
+      if (K.antennaType == 0) inc.x = mkc(0, sqrt(60 * K.pt));
+      else if (K.antennaType == 1) inc.y = mkc(0, sqrt(60 * K.pt * 1.5) * sin(th));
+      else if (K.antennaType == 2) inc.y = mkc(0, sqrt(60 * K.pt) * (cos(SBR_PI * cos(th) / 2.0) / sin(th)));
+      else if (K.antennaType == 3) {
+        cx eth, eph;
+        pattern_lookup(F, th, ph, &eth, &eph);
+        inc.y = cscale(eth, sqrt(K.pt));
+        inc.z = cscale(eph, sqrt(K.pt));
+      }
+    } else {
+      inc = matT_vec_c(cur, tot);
+      double st = sin(th), sp = sin(ph), ct = cos(th), cp = cos(ph);
+      const double Tm[9] = {st * cp, ct * cp, -1 * sp, st * sp, ct * sp, cp, ct, -1 * st, 0.0};
+      inc = matT_vec_c(Tm, inc);
+    }
+    if (sg.type == 1 || sg.type == 2) {
+      double ang = angle_between_rad(vg, v3(cur[6], cur[7], cur[8]));
+      cx fTM, fTE;
+      fresnel(sg.type == 1, ang, mats[6 * sg.mat + 3], mats[6 * sg.mat + 5], K.freq, K.useFresnel != 0, K.wallWidth,
+              &fTM, &fTE);
+      tot.x = inc.x;
+      tot.y = cmul(inc.y, fTM);
+      tot.z = cmul(inc.z, fTE);
+    } else {
+      tot = inc;
+    }
+    double st = sin(th), sp = sin(ph), ct = cos(th), cp = cos(ph);
+    const double Tm[9] = {st * cp, ct * cp, -1 * sp, st * sp, ct * sp, cp, ct, -1 * st, 0.0};
+    tot = mat_vec_c(Tm, tot);
+    tot = matT_vec_c(cur, tot);
+  }
+  cx e = cexp_(mkc(-0.0 * kw * L, -1.0 * kw * L));
+  cx f[3] = {cdiv(cmul(tot.x, e), mkc(L, 0)), cdiv(cmul(tot.y, e), mkc(L, 0)), cdiv(cmul(tot.z, e), mkc(L, 0))};
+  for (int c = 0; c < 3; c++) { out[2 * c] = f[c].r; out[2 * c + 1] = f[c].i; }
+}
+
 // winTxRx / winPerm: kept paths sorted by (transmitter * nrx + receiver), each pointing into the
 // order-key-sorted arrays (winNode).
 __global__ void __launch_bounds__(256) k_field(FieldArgs F, const TxDesc* __restrict__ txs, const TNode* __restrict__ tree,
@@ -1272,94 +1378,94 @@ __global__ void __launch_bounds__(256) k_field(FieldArgs F, const TxDesc* __rest
   } else {
     (void)corrected_points(txp, tree, surf, chain, depth, C, pts);
   }
-  const double lamda = SBR_C0 / K.freq;
-  const double kw = (2 * SBR_PI) / lamda;
-  const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-  const double* next = K.txcs;
-  const double* cur = I3;
-  V3C tot = {mkc(0, 0), mkc(0, 0), mkc(0, 0)};
-  double L = 0.0;
-  for (int k = 0; k <= depth; k++) {
+  // segment k: corrected points k -> k + 1; it ends on the surface its tree node hit (the last one
+  // on the receiver); its interaction is its node's type, at the surface the previous node hit
+  auto seg = [&](int k) {
     const TNode& sg = tree[chain[k]];
-    V3 vg = vsub(pts[k + 1], pts[k]);
-    V3 vl = mat_vec(next, vg);
-    double r, th, ph;
-    cart_to_sph(vl, &r, &th, &ph);
-    L += r;
-    cur = next;
-    int hitSurf = k == depth ? -1 : sg.tri;
-    next = hitSurf == -1 ? I3 : surf[hitSurf].cs;
-    if (K.physicalField && k > 0) {
-      // Extension: the global field split at the interaction into the component along s (normal to
-      // the plane of incidence) and along p (in it, s x direction before and after), with the
-      // Fresnel coefficients of the incidence angle in [0, 90] degrees; the reference's TE
-      // coefficient is minus the usual s one (reflection), its TM coefficient the usual p one.
-      const V3 din = vnormalized(vsub(pts[k], pts[k - 1])), dout = vnormalized(vg);
-      const V3 nrm = vnormalized(v3(cur[6], cur[7], cur[8]));
-      double cth = fabs(vdot(din, nrm));
-      cth = cth > 1.0 ? 1.0 : cth;
-      V3 s = vcross(din, nrm);
-      if (vnorm(s) < 1e-12) {   // normal incidence: any direction normal to din
-        const V3 ax = fabs(din.x) <= fabs(din.y) && fabs(din.x) <= fabs(din.z) ? v3(1, 0, 0)
-                      : (fabs(din.y) <= fabs(din.z) ? v3(0, 1, 0) : v3(0, 0, 1));
-        s = vcross(din, ax);
-      }
-      s = vnormalized(s);
-      const V3 pin = vcross(s, din), pout = vcross(s, dout);
-      const int mat = tris[tree[chain[k - 1]].hitDfs].mat;
-      cx fTM, fTE;
-      fresnel(sg.type == 1, acos(cth), mats[6 * mat + 3], mats[6 * mat + 5], K.freq, K.useFresnel != 0, K.wallWidth,
-              &fTM, &fTE);
-      const cx cs = sg.type == 1 ? cneg(fTE) : fTE, cp = fTM;
-      const cx es = cadd(cadd(cscale(tot.x, s.x), cscale(tot.y, s.y)), cscale(tot.z, s.z));
-      const cx ep = cadd(cadd(cscale(tot.x, pin.x), cscale(tot.y, pin.y)), cscale(tot.z, pin.z));
-      const cx as = cmul(cs, es), ap = cmul(cp, ep);
-      tot.x = cadd(cscale(as, s.x), cscale(ap, pout.x));
-      tot.y = cadd(cscale(as, s.y), cscale(ap, pout.y));
-      tot.z = cadd(cscale(as, s.z), cscale(ap, pout.z));
+    FieldSeg f;
+    f.a = pts[k];
+    f.b = pts[k + 1];
+    f.surf = k == depth ? -1 : sg.tri;
+    f.type = sg.type;
+    f.mat = k > 0 ? tris[tree[chain[k - 1]].hitDfs].mat : 0;
+    return f;
+  };
+  auto frames = [&](int32_t id) { return surf[id].cs; };
+  path_field(F, K, mats, depth + 1, seg, frames, winField + 6 * w);
+}
+
+// Fields of caller-given paths (sbr_path_fields): path p has count[p] segments from off[p]; one
+// thread per path.  Segment s: src / tgt (3 doubles each), the surface it ends on, its interaction
+// type and material; frames[id * 9 ...] the row-major coordSys of surface id.
+__global__ void __launch_bounds__(256) k_field_paths(FieldArgs F, const TxDesc* __restrict__ K,
+                                                     const double* __restrict__ mats, int64_t nPaths,
+                                                     const int64_t* __restrict__ off, const int32_t* __restrict__ count,
+                                                     const double* __restrict__ src, const double* __restrict__ tgt,
+                                                     const int32_t* __restrict__ surfId, const int32_t* __restrict__ type,
+                                                     const int32_t* __restrict__ mat, const double* __restrict__ frames,
+                                                     double* __restrict__ out) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nPaths) return;
+  const int64_t o = off[p];
+  auto seg = [&](int k) {
+    const int64_t q = o + k;
+    FieldSeg f;
+    f.a = ld3(src + 3 * q);
+    f.b = ld3(tgt + 3 * q);
+    f.surf = surfId[q];
+    f.type = type[q];
+    f.mat = mat[q];
+    return f;
+  };
+  auto fr = [&](int32_t id) { return frames + 9 * (int64_t)id; };
+  path_field(F, *K, mats, count[p], seg, fr, out + 6 * p);
+}
+
+// Reflection and transmission coefficients between two media of complex relative permittivity
+// epsI (incident side) and epsT, for the sweeps of RefCoeffTest / TransCoeffTest (FieldCompute.cpp:
+// 27-69, 236-302): Snell's angle theta_t = asin(n_i / n_t sin theta_i), Gamma_TM =
+// (-eta_t cos t + eta_i cos i) / (eta_t cos t + eta_i cos i), Gamma_TE = -(eta_t / cos t - eta_i /
+// cos i) / (eta_t / cos t + eta_i / cos i); an infinite wall transmits nothing, otherwise the
+// reference's slab of the given width (phase q = 2 pi w / lambda sqrt(n_t - sin^2 theta_i), factor
+// e^{-2jq}).  out: rTM, tTM, rTE, tTE.
+__device__ void coeffs_general(cx epsI, cx epsT, double thetaI, double lamda, double width, bool infWall, cx* out) {
+  const double ETA0 = 4 * SBR_PI * 1e-7 * SBR_C0;
+  const cx ti = mkc(thetaI, 0.0);
+  const cx ni = csqrt_(epsI), nt = csqrt_(epsT);
+  const cx tt = casin_(cmul(cdiv(ni, nt), csin_(ti)));
+  const cx etai = cdiv(mkc(ETA0, 0.0), ni), etat = cdiv(mkc(ETA0, 0.0), nt);
+  const cx ci = ccos_(ti), ct = ccos_(tt), si = csin_(ti);
+  const cx gtm = cdiv(cadd(cmul(cneg(etat), ct), cmul(etai, ci)), cadd(cmul(etat, ct), cmul(etai, ci)));
+  const cx a = cdiv(etat, ct), b = cdiv(etai, ci);
+  const cx gte = cneg(cdiv(csub(a, b), cadd(a, b)));
+  const cx g[2] = {gtm, gte};
+  for (int k = 0; k < 2; k++) {
+    if (infWall) {
+      out[2 * k] = g[k];
+      out[2 * k + 1] = mkc(0, 0);
       continue;
     }
-    V3C inc;
-    if (k == 0) {
-      inc.x = mkc(0, 0);
-      inc.y = mkc(0, 0);
-      inc.z = mkc(0, 0);
-      if (K.antennaType == 0) inc.x = mkc(0, sqrt(60 * K.pt));
-      else if (K.antennaType == 1) inc.y = mkc(0, sqrt(60 * K.pt * 1.5) * sin(th));
-      else if (K.antennaType == 2) inc.y = mkc(0, sqrt(60 * K.pt) * (cos(SBR_PI * cos(th) / 2.0) / sin(th)));
-      else if (K.antennaType == 3) {
-        cx eth, eph;
-        pattern_lookup(F, th, ph, &eth, &eph);
-        inc.y = cscale(eth, sqrt(K.pt));
-        inc.z = cscale(eph, sqrt(K.pt));
-      }
-    } else {
-      inc = matT_vec_c(cur, tot);
-      double st = sin(th), sp = sin(ph), ct = cos(th), cp = cos(ph);
-      const double Tm[9] = {st * cp, ct * cp, -1 * sp, st * sp, ct * sp, cp, ct, -1 * st, 0.0};
-      inc = matT_vec_c(Tm, inc);
-    }
-    if (sg.type == 1 || sg.type == 2) {
-      double ang = angle_between_rad(vg, v3(cur[6], cur[7], cur[8]));
-      int mat = tris[tree[chain[k - 1]].hitDfs].mat;
-      cx fTM, fTE;
-      fresnel(sg.type == 1, ang, mats[6 * mat + 3], mats[6 * mat + 5], K.freq, K.useFresnel != 0, K.wallWidth, &fTM,
-              &fTE);
-      tot.x = inc.x;
-      tot.y = cmul(inc.y, fTM);
-      tot.z = cmul(inc.z, fTE);
-    } else {
-      tot = inc;
-    }
-    double st = sin(th), sp = sin(ph), ct = cos(th), cp = cos(ph);
-    const double Tm[9] = {st * cp, ct * cp, -1 * sp, st * sp, ct * sp, cp, ct, -1 * st, 0.0};
-    tot = mat_vec_c(Tm, tot);
-    tot = matT_vec_c(cur, tot);
+    const cx q = cmul(mkc(2 * SBR_PI * width / lamda, 0.0), csqrt_(csub(nt, cmul(si, si))));
+    const cx ph = cexp_(cmul(mkc(0.0, -2.0), q));
+    const cx den = csub(mkc(1.0, 0.0), cmul(cmul(g[k], g[k]), ph));
+    out[2 * k] = cdiv(cmul(g[k], csub(mkc(1.0, 0.0), ph)), den);
+    out[2 * k + 1] = cdiv(cmul(csub(mkc(1.0, 0.0), cmul(g[k], g[k])), ph), den);
   }
-  cx e = cexp_(mkc(-0.0 * kw * L, -1.0 * kw * L));
-  cx f[3] = {cdiv(cmul(tot.x, e), mkc(L, 0)), cdiv(cmul(tot.y, e), mkc(L, 0)), cdiv(cmul(tot.z, e), mkc(L, 0))};
-  double* o = winField + 6 * w;
-  for (int c = 0; c < 3; c++) { o[2 * c] = f[c].r; o[2 * c + 1] = f[c].i; }
+}
+
+// Angles 0 .. pi/2 in n even steps (Eigen's LinSpaced(n, 0, pi/2): low + i * step, the last one
+// exactly pi/2) and their coefficients: theta[n], out[n][4][2] (rTM, tTM, rTE, tTE).
+__global__ void k_coeff_sweep(int32_t n, double lamda, cx epsI, cx epsT, double width, int32_t infWall,
+                              double* __restrict__ theta, double* __restrict__ out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double hi = SBR_PI / 2;
+  const double step = n > 1 ? hi / (double)(n - 1) : 0.0;
+  const double th = (n == 1 || i == n - 1) ? hi : (double)i * step;
+  cx c[4];
+  coeffs_general(epsI, epsT, th, lamda, width, infWall != 0, c);
+  theta[i] = th;
+  for (int k = 0; k < 4; k++) { out[8 * i + 2 * k] = c[k].r; out[8 * i + 2 * k + 1] = c[k].i; }
 }
 
 // Per (transmitter, receiver): running sum of its kept paths' fields in (launch, BFS) order,
@@ -2506,6 +2612,103 @@ int Engine::trees(int32_t* launch, int32_t* depth, int64_t* code, int32_t* tri, 
     for (int a = 0; a < 3; a++) { src[3 * k + a] = t.src[a]; tgt[3 * k + a] = t.tgt[a]; }
     pl[k] = t.pl;
   }
+  return SBR_OK;
+}
+
+int Engine::path_fields_given(const sbr_field_params* P, const sbr_material* m, int nMats, int64_t nPaths,
+                              const int64_t* off, const int32_t* cnt, const double* src, const double* tgt,
+                              const int32_t* surfId, const int32_t* type, const int32_t* mat, int nSurf,
+                              const double* frames, double* out) {
+  if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
+  if (!P || nPaths < 0 || nMats < 0 || nSurf < 0) return fail(SBR_E_ARG, "path_fields: bad arguments");
+  if (nPaths == 0) return SBR_OK;
+  if (!off || !cnt || !src || !tgt || !surfId || !type || !mat || !out || (nSurf > 0 && !frames) || (nMats > 0 && !m))
+    return fail(SBR_E_ARG, "path_fields: missing arrays");
+  if (P->antennaType < 0 || P->antennaType > 3) return fail(SBR_E_ARG, "path_fields: antennaType must be 0..3");
+  if (P->antennaType == 3 && patNT == 0) return fail(SBR_E_STATE, "path_fields: antennaType 3 without a pattern");
+  if (!(P->frequency > 0)) return fail(SBR_E_ARG, "path_fields: frequency must be positive");
+  // segments are checked on the host: every index the kernel follows stays in range
+  int64_t nSeg = 0;
+  for (int64_t q = 0; q < nPaths; q++) {
+    if (cnt[q] < 1 || cnt[q] > kMaxDepth + 1) return fail(SBR_E_LIMIT, "path_fields: a path has 0 or too many segments");
+    if (off[q] < 0) return fail(SBR_E_ARG, "path_fields: negative segment offset");
+    nSeg = std::max(nSeg, off[q] + cnt[q]);
+  }
+  for (int64_t q = 0; q < nPaths; q++)
+    for (int k = 0; k < cnt[q]; k++) {
+      const int64_t g = off[q] + k;
+      if (surfId[g] < -1 || surfId[g] >= nSurf) return fail(SBR_E_ARG, "path_fields: surface id out of range");
+      if (type[g] < 0 || type[g] > 2) return fail(SBR_E_ARG, "path_fields: interaction type must be 0, 1 or 2");
+      if (type[g] != 0 && (mat[g] < 0 || mat[g] >= nMats))
+        return fail(SBR_E_ARG, "path_fields: material id out of range");
+    }
+  HIPCHK(hipSetDevice(device));
+  TxDesc K{};
+  for (int k = 0; k < 9; k++) K.txcs[k] = P->txCoordSys[k];
+  K.freq = P->frequency;
+  K.pt = P->pt;
+  K.useFresnel = P->useFresnel ? 1 : 0;
+  K.antennaType = P->antennaType;
+  K.wallWidth = P->wallWidth;
+  K.physicalField = P->physicalField ? 1 : 0;
+  std::vector<int32_t> matc(mat, mat + nSeg);
+  for (auto& v : matc) v = std::max(v, 0);   // unused for type 0; kept a valid index
+  DBuf<TxDesc> dK;
+  DBuf<double> dM, dSrc, dTgt, dFr, dOut;
+  DBuf<int64_t> dOff;
+  DBuf<int32_t> dCnt, dSurfI, dType, dMat;
+  ENSURE(dK, 1);
+  ENSURE(dM, std::max<size_t>(6 * (size_t)nMats, 6));
+  ENSURE(dSrc, 3 * (size_t)nSeg);
+  ENSURE(dTgt, 3 * (size_t)nSeg);
+  ENSURE(dFr, std::max<size_t>(9 * (size_t)nSurf, 9));
+  ENSURE(dOut, 6 * (size_t)nPaths);
+  ENSURE(dOff, nPaths);
+  ENSURE(dCnt, nPaths);
+  ENSURE(dSurfI, nSeg);
+  ENSURE(dType, nSeg);
+  ENSURE(dMat, nSeg);
+  HIPCHK(hipMemcpyAsync(dK.p, &K, sizeof K, hipMemcpyHostToDevice, stream));
+  if (nMats > 0) HIPCHK(hipMemcpyAsync(dM.p, m, 6 * sizeof(double) * nMats, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dSrc.p, src, 3 * sizeof(double) * nSeg, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dTgt.p, tgt, 3 * sizeof(double) * nSeg, hipMemcpyHostToDevice, stream));
+  if (nSurf > 0) HIPCHK(hipMemcpyAsync(dFr.p, frames, 9 * sizeof(double) * nSurf, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dOff.p, off, sizeof(int64_t) * nPaths, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dCnt.p, cnt, sizeof(int32_t) * nPaths, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dSurfI.p, surfId, sizeof(int32_t) * nSeg, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dType.p, type, sizeof(int32_t) * nSeg, hipMemcpyHostToDevice, stream));
+  HIPCHK(hipMemcpyAsync(dMat.p, matc.data(), sizeof(int32_t) * nSeg, hipMemcpyHostToDevice, stream));
+  FieldArgs F{};
+  F.winners = nPaths;
+  F.pat = dPattern.p;
+  F.patNT = patNT;
+  F.patNP = patNP;
+  k_field_paths<<<blocks_for(nPaths, 256), 256, 0, stream>>>(F, dK.p, dM.p, nPaths, dOff.p, dCnt.p, dSrc.p, dTgt.p,
+                                                             dSurfI.p, dType.p, dMat.p, dFr.p, dOut.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, dOut.p, 6 * sizeof(double) * nPaths, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return SBR_OK;
+}
+
+int Engine::fresnel_sweep(int n, double freq, const double* epsI, const double* epsT, double width, int infWall,
+                          double* theta, double* out) {
+  if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
+  if (n <= 0 || !(freq > 0) || !epsI || !epsT || !theta || !out) return fail(SBR_E_ARG, "fresnel_sweep: bad arguments");
+  HIPCHK(hipSetDevice(device));
+  DBuf<double> dT, dO;
+  ENSURE(dT, n);
+  ENSURE(dO, 8 * (size_t)n);
+  cx ei, et;
+  ei.r = epsI[0];
+  ei.i = epsI[1];
+  et.r = epsT[0];
+  et.i = epsT[1];
+  k_coeff_sweep<<<blocks_for(n, 256), 256, 0, stream>>>(n, SBR_C0 / freq, ei, et, width, infWall, dT.p, dO.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(theta, dT.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(out, dO.p, 8 * sizeof(double) * n, hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
   return SBR_OK;
 }
 

@@ -60,7 +60,8 @@ bool looks_ascii(const std::string& fn) {
 // Triangles from deduplicated geometry.  As in Preprocessor.cpp:87-101 the normal (and here the
 // material id) of triangle k is read from facet slot k of the input, i.e. after dropped facets.
 bool make_triangles(const std::vector<float>& coords, const std::vector<float>& normals,
-                    const std::vector<unsigned>& tris, const int32_t* mats, std::vector<Triangle*>& out) {
+                    const std::vector<unsigned>& tris, const int32_t* mats, std::vector<Triangle*>& out,
+                    std::map<std::pair<int, int>, std::vector<int>>* edgesOut = nullptr) {
   const size_t nt = tris.size() / 3;
   if (normals.size() < 3 * nt) return false;
   std::map<std::pair<int, int>, std::vector<int>> edges;
@@ -104,6 +105,7 @@ bool make_triangles(const std::vector<float>& coords, const std::vector<float>& 
     }
     cop++;
   }
+  if (edgesOut) edgesOut->swap(edges);
   return true;
 }
 
@@ -160,11 +162,74 @@ bool ReadStl(const std::string& fn, std::vector<float>& coords, std::vector<floa
   return true;
 }
 
-bool StlToGeometry(std::string fileName, std::vector<Triangle*>& output, bool, bool, std::string) {
+bool StlToGeometry(std::string fileName, std::vector<Triangle*>& output, bool saveEdges, bool saveFaces,
+                   std::string dataDir) {
   std::vector<float> coords, normals;
   std::vector<unsigned> tris;
   if (!ReadStl(fileName, coords, normals, tris)) return false;
-  return make_triangles(coords, normals, tris, nullptr, output);
+  const size_t first = output.size();
+  std::map<std::pair<int, int>, std::vector<int>> edges;
+  if (!make_triangles(coords, normals, tris, nullptr, output, saveEdges ? &edges : nullptr)) return false;
+  if (saveEdges) SaveEdgesAsVtk(dataDir + "Edges.vtk", edges, coords);
+  if (saveFaces) {
+    const std::vector<Triangle*> mine(output.begin() + first, output.end());
+    SaveFacesAsVtk(dataDir + "Faces.vtk", mine, coords, tris);
+  }
+  return true;
+}
+
+bool SaveEdgesAsVtk(std::string fileName, const std::map<std::pair<int, int>, std::vector<int>>& edgeMap,
+                    const std::vector<float>& coords) {
+  std::ofstream ofs(fileName);
+  ofs << "# vtk DataFile Version 2.0" << std::endl;
+  ofs << "3D model of edge geometry" << std::endl;
+  ofs << "ASCII" << std::endl;
+  ofs << "DATASET POLYDATA" << std::endl;
+  ofs << std::endl;
+  ofs << "POINTS " << int(coords.size() / 3) << " float" << std::endl;
+  // one point per line; the last line runs into the LINES header, as the reference writes it
+  for (size_t i = 0; i < coords.size(); i++) {
+    if (i > 0 && i % 3 == 0) ofs << std::endl;
+    ofs << " " << coords[i];
+  }
+  ofs << "LINES " << edgeMap.size() << " " << 3 * edgeMap.size() << std::endl;
+  for (const auto& kv : edgeMap) ofs << " 2 " << kv.first.first / 3 << " " << kv.first.second / 3 << std::endl;
+  ofs << "CELL_DATA " << edgeMap.size() << std::endl;
+  ofs << "FIELD FieldData 2" << std::endl;
+  ofs << "EdgeID 1 " << edgeMap.size() << " int" << std::endl;
+  for (size_t i = 0; i < edgeMap.size(); i++) ofs << " " << i << std::endl;
+  ofs << "Connection 1 " << edgeMap.size() << " int" << std::endl;
+  for (const auto& kv : edgeMap) ofs << " " << int(kv.second.size()) << std::endl;
+  return (bool)ofs;
+}
+
+bool SaveFacesAsVtk(std::string fileName, const std::vector<Triangle*>& triangles, const std::vector<float>& coords,
+                    const std::vector<unsigned int> tris) {
+  std::ofstream ofs(fileName);
+  ofs << "# vtk DataFile Version 2.0" << std::endl;
+  ofs << "3D model of face geometry" << std::endl;
+  ofs << "ASCII" << std::endl;
+  ofs << "DATASET POLYDATA" << std::endl;
+  ofs << std::endl;
+  ofs << "POINTS " << int(coords.size() / 3) << " float" << std::endl;
+  for (size_t i = 0; i < coords.size(); i++) {
+    if (i > 0 && i % 3 == 0) ofs << std::endl;
+    ofs << " " << coords[i];
+  }
+  ofs << std::endl;
+  const int numTris = int(tris.size() / 3);
+  ofs << "POLYGONS " << numTris << " " << 4 * numTris << std::endl;
+  for (int i = 0; i < numTris; i++)
+    ofs << " 3 " << tris[3 * i] << " " << tris[3 * i + 1] << " " << tris[3 * i + 2] << std::endl;
+  ofs << "CELL_DATA " << numTris << std::endl;
+  ofs << "FIELD FieldData 3" << std::endl;
+  ofs << "FaceID 1 " << triangles.size() << " int" << std::endl;
+  for (const Triangle* t : triangles) ofs << " " << t->triangleId << std::endl;
+  ofs << "CoplanarID 1 " << triangles.size() << " int" << std::endl;
+  for (const Triangle* t : triangles) ofs << " " << t->coplanarId << std::endl;
+  ofs << "MaterialID 1 " << triangles.size() << " int" << std::endl;
+  for (const Triangle* t : triangles) ofs << " " << t->materialId << std::endl;
+  return (bool)ofs;
 }
 
 bool SoupToGeometry(const float* coords9, const float* normals3, const int32_t* mats, int nFacets,

@@ -7,10 +7,24 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <limits>
+#include <map>
+#include <tuple>
 
 #include "sbrsolver/SceneDesc.h"
 
 RTSolver::RTSolver() {}
+
+// AngleBetween (VecMatDef.h:56-77) in degrees: both vectors normalized, the angle from the chord
+// length, on the obtuse side from the chord to the reversed vector.
+double RTSolver::AngleBetweenDeg(sbr::V3 a, sbr::V3 b) {
+  a = sbr::vnormalized(a);
+  b = sbr::vnormalized(b);
+  const double kPi = 3.141592653589793;
+  const double th = sbr::vdot(a, b) < 0 ? kPi - 2.0 * std::asin(sbr::vnorm(sbr::vsub(sbr::vneg(a), b)) / 2.0)
+                                         : 2.0 * std::asin(sbr::vnorm(sbr::vsub(a, b)) / 2.0);
+  return th * (180.0 / kPi);
+}
 
 RTSolver::~RTSolver() {
   Cleanup();
@@ -18,7 +32,15 @@ RTSolver::~RTSolver() {
   _ctx = nullptr;
 }
 
+void RTSolver::FreeRayPaths() {
+  for (int i = 0; i < _rayPathRows; i++) delete[] _rayPaths[i];
+  delete[] _rayPaths;
+  _rayPaths = nullptr;
+  _rayPathRows = 0;
+}
+
 void RTSolver::Cleanup() {
+  FreeRayPaths();
   delete[] _materialProperties;
   _materialProperties = nullptr;
   _materialCount = 0;
@@ -192,7 +214,13 @@ int RTSolver::ExecuteRayTracing(Vec3 sourcePoint, const Mat3& txCoordSys, int ma
   std::vector<int> matOfId(triangleMesh.size(), 0);
   for (const Triangle* t : triangleMesh)
     if (t->triangleId >= 0 && t->triangleId < (int)matOfId.size()) matOfId[t->triangleId] = t->materialId;
+  std::vector<int64_t> nodeIds;
+  if (options.rayIds && np > 0) {
+    rc = NumberTreeNodes(prx, pl, pn, poff, stri, stype, nodeIds);
+    if (rc) return rc;
+  }
   _kept.assign(_receiverCount, {});
+  const double inf = std::numeric_limits<double>::infinity();
   for (int64_t k = 0; k < np; k++) {
     std::vector<Ray> path(pn[k]);
     for (int s = 0; s < pn[k]; s++) {
@@ -205,11 +233,91 @@ int RTSolver::ExecuteRayTracing(Vec3 sourcePoint, const Mat3& txCoordSys, int ma
       const int leftMat = s > 0 && stri[q - 1] >= 0 ? matOfId[stri[q - 1]] : -1;
       if (stype[q] == 1) r.reflectionMaterialId = leftMat;
       if (stype[q] == 2) r.penetrationMaterialId = leftMat;
+      // Lengths and angles as the corrections leave them: a direct path gets its length
+      // (RTSolver.cpp:384); in a corrected multi-bounce path each later segment gets its length and
+      // the angle (degrees) between it and the normal of mesh[its start surface] (RTSolver.cpp:
+      // 353-357), and the first one the length from the first bounce point to itself, 0 (:348).
+      const sbr::V3 seg = sbr::vsub(r.targetPoint.v(), r.sourcePoint.v());
+      if (pn[k] == 1) {
+        r.pathLength = sbr::vnorm(seg);
+      } else if (s == 0) {
+        r.pathLength = 0.0;
+      } else {
+        r.pathLength = sbr::vnorm(seg);
+        const int surf = stri[q - 1];
+        if (surf >= 0 && surf < (int)triangleMesh.size()) r.angleFromSurfaceNormal = AngleBetweenDeg(triangleMesh[surf]->norm.v(), seg);
+      }
+      if (pn[k] == 1 || s == 0) r.angleFromSurfaceNormal = inf;
+      r.captured = s == pn[k] - 1;   // the capturing node; no ancestor captured (RTSolver.cpp:259-297)
+      if (!nodeIds.empty()) r.id = (int)nodeIds[q];
     }
     _kept[prx[k]].push_back(std::make_pair((int)pl[k], std::move(path)));
   }
+  FreeRayPaths();
+  if ((int64_t)_pathsCount * _receiverCount <= options.rayPathArrayLimit && _pathsCount > 0) {
+    _rayPaths = new Paths*[_pathsCount]();
+    _rayPathRows = _pathsCount;
+    for (int i = 0; i < _pathsCount; i++) _rayPaths[i] = new Paths[_receiverCount];
+    for (int r = 0; r < _receiverCount; r++)
+      for (auto& lp : _kept[r]) _rayPaths[lp.first][r].rayPaths.push_back(lp.second);
+  }
   _shootRayList.clear();
   return _pathsCount * _receiverCount;
+}
+
+// Reference node ids of the kept paths' segments: the id of a node is its position in the creation
+// order of all tree nodes of the trace, launch by launch, each tree depth first with the reflected
+// child before the transmitted one (RTSolver.cpp:82-84, 186-220; Ray.cpp:18).  The trees are read
+// back from the device; within a launch a node's rank in that order follows from its branch code.
+int RTSolver::NumberTreeNodes(const std::vector<int32_t>& prx, const std::vector<int32_t>& pl,
+                              const std::vector<int32_t>& pn, const std::vector<int64_t>& poff,
+                              const std::vector<int32_t>& stri, const std::vector<int32_t>& stype,
+                              std::vector<int64_t>& ids) {
+  (void)prx;
+  (void)stri;
+  int64_t n = 0;
+  int rc = sbr_result_tree_count(_ctx, &n);
+  if (rc) return rc;
+  std::vector<int32_t> launch(n), depth(n), tri(n), cop(n), type(n);
+  std::vector<int64_t> code(n);
+  std::vector<double> src(3 * n), tgt(3 * n), plen(n);
+  if (n > 0) {
+    rc = sbr_result_trees(_ctx, launch.data(), depth.data(), code.data(), tri.data(), cop.data(), type.data(),
+                          src.data(), tgt.data(), plen.data());
+    if (rc) return rc;
+  }
+  // Depth-first rank with the reflected branch first: compare branch strings bit by bit from the
+  // first level, reflect (1) before transmit (0), a prefix before its extensions.
+  auto dfs_less = [&](int64_t a, int64_t b) {
+    const int da = depth[a], db = depth[b], m = std::min(da, db);
+    for (int l = 0; l < m; l++) {
+      const int ba = (int)((code[a] >> (da - 1 - l)) & 1), bb = (int)((code[b] >> (db - 1 - l)) & 1);
+      if (ba != bb) return ba > bb;
+    }
+    return da < db;
+  };
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return launch[a] != launch[b] ? launch[a] < launch[b] : dfs_less(a, b);
+  });
+  // (launch, depth, code) -> global id
+  std::map<std::tuple<int32_t, int32_t, int64_t>, int64_t> idOf;
+  for (int64_t r = 0; r < n; r++) {
+    const int64_t i = order[r];
+    idOf.emplace(std::make_tuple(launch[i], depth[i], code[i]), r);
+  }
+  ids.assign(stype.size(), -1);
+  for (size_t k = 0; k < pn.size(); k++) {
+    int64_t c = 0;
+    for (int s = 0; s < pn[k]; s++) {
+      const int64_t q = poff[k] + s;
+      if (s > 0) c = (c << 1) | (stype[q] == 1 ? 1 : 0);
+      auto it = idOf.find(std::make_tuple(pl[k], s, c));
+      if (it != idOf.end()) ids[q] = it->second;
+    }
+  }
+  return SBR_OK;
 }
 
 namespace {
