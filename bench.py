@@ -36,14 +36,28 @@ COUNTS = os.path.join(ROOT, "tests", "golden", "traversal_counts.json")
 # HBM traffic of one k_trace launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by the
 # readings for a copy of known size (line "traffic_bytes_per_launch: N" in this summary).
 TRAFFIC = os.path.join(ROOT, "profiles", "k_trace_traffic_urban_L9_R6T1.md")
+# VALU lane activity and wait fraction of the same kernel (rocprofv3 --pmc SQ_* counters).
+VALU = os.path.join(ROOT, "profiles", "k_trace_valu_cache_urban_L9_R6T1.md")
 
 
 def host_cores():
+    """Host cores this process may run on (its affinity mask; all of them are used)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(cfg, trace_kw, stride):
@@ -58,8 +72,55 @@ def cpu_baseline(cfg, trace_kw, stride):
     dt = time.perf_counter() - t0
     return dict(value=r.bounces / dt / 1e6, unit="Mray-bounces/s", cores=cores, kind="port",
                 sample=f"every {stride}th launch ({r.bounces} bounces, {len(cfg['rx'])} receivers), "
-                       f"{dt:.1f} s wall, reference traversal + per-receiver tree capture",
+                       f"{dt:.1f} s wall, reference traversal + per-receiver tree capture, OpenMP over launches "
+                       f"on {cores} threads (nproc {os.cpu_count()}, {cpu_model()})",
                 seconds=dt, bounces=int(r.bounces))
+
+
+def cpu_single_thread_box_room():
+    """The oracle on one thread, structure-faithful (per-receiver tree clones as RTSolver.cpp:82-121
+    does), on the box room in full (12 triangles, level 5 = 10,242 launches, R 4, 100 receivers)."""
+    from oracle import oracle as O
+    from sbrsolver_amd import scenes
+    cfg = scenes.config1()
+    sc = O.Scene.from_soup(*cfg["soup"])
+    t0 = time.perf_counter()
+    r = O.trace(sc, O.make_params(**cfg["trace"]), cfg["rx"], cfg["materials"], nthreads=1)
+    dt = time.perf_counter() - t0
+    return dict(value=round(r.bounces / dt / 1e6, 5), unit="Mray-bounces/s", cores=1, kind="port",
+                workload=cfg["name"], bounces=int(r.bounces), seconds=round(dt, 3), cpu=cpu_model())
+
+
+def kernel_counters(traffic, avg_ms, valu_tflops):
+    """What bounds k_trace, from the committed rocprofv3 counter summaries (profiles/): the measured
+    HBM traffic as a fraction of peak bandwidth, the FP64 VALU model as a fraction of peak, the
+    lanes active per VALU instruction and the fraction of wave cycles spent waiting.  The algorithmic
+    bytes of the roofline line stay in LDS and the caches (the counters see a fifth of them in HBM),
+    so neither HBM nor VALU peaks bind: the kernel is bound by memory latency at its occupancy."""
+    out = {}
+    if traffic:
+        out["hbm_measured_GBs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
+        out["hbm_measured_frac"] = round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    out["valu_fp64_model_frac"] = round(valu_tflops / FP64_PEAK_TFLOPS, 4)
+    if os.path.exists(VALU):
+        import re
+        txt = open(VALU).read()
+        def num(name):
+            m = re.search(r"\| " + name + r" \| ([0-9,]+) \|", txt)
+            return float(m.group(1).replace(",", "")) if m else None
+        tc, iv, wait, wc = (num("SQ_THREAD_CYCLES_VALU"), num("SQ_INSTS_VALU"), num("SQ_WAIT_ANY"),
+                            num("SQ_WAVE_CYCLES"))
+        if tc and iv:
+            out["valu_active_lanes_frac"] = round(tc / iv / 64.0, 4)
+        if wait and wc:
+            out["wave_wait_frac"] = round(wait / wc, 4)
+    fr = [v for k, v in out.items() if k in ("hbm_measured_frac", "valu_fp64_model_frac")]
+    out["binding"] = ("memory latency (waves wait {:.0%} of their cycles; HBM {:.0%} and FP64 VALU {:.0%} of "
+                      "peak)".format(out.get("wave_wait_frac", float("nan")), out.get("hbm_measured_frac", float("nan")),
+                                     out["valu_fp64_model_frac"])
+                      if fr and max(fr) < 0.5 else ("hbm" if out.get("hbm_measured_frac", 0) >= out["valu_fp64_model_frac"]
+                                                    else "valu"))
+    return out
 
 
 def calib_copy(nbytes):
@@ -148,14 +209,24 @@ def indoor_run(Context, make_params, scenes, tess=10):
                 ms_capture=round(s["ms_capture"], 1), paths_kept=int(s["winners"]))
 
 
-def serve_run(ctx, make_params, scenes, Server, n=96, slots=3, load=0.7):
-    """Serving loop on the urban scene: n coverage queries (levels 6-10, 4 reflections, 1..50k
-    receivers).  Phase 1 submits all at once to measure capacity (req/s when saturated); phase 2
-    submits the same queries with Poisson arrivals at `load` x capacity and reports the latency
-    (submit -> fields on the host) of that phase."""
+def serve_run(Context, make_params, scenes, Server, n=96, slots=3, load=0.7):
+    """Serving stream on the seeded city (about 1M triangles, 2000 buildings): the scene is built and
+    made resident once (its load time is reported), then n coverage queries (a transmitter in a
+    street at 3-30 m, levels 6-10, 4 reflections, 1..50k receivers over the central 1.35 x 1.35 km).
+    Phase 1 submits all at once to measure capacity (req/s when saturated); phase 2 submits the same
+    queries with Poisson arrivals at `load` x capacity and reports the latency (submit -> fields on
+    the host) of that phase."""
     import random
-    _, boxes, _ = scenes.urban_scene()
-    qs = scenes.coverage_queries(n, boxes)
+    t0 = time.perf_counter()
+    name, soup, boxes, half, mats = scenes.config5_scene()
+    t_gen = time.perf_counter() - t0
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    ctx.scene_from_soup(*soup)
+    ctx.materials(mats)
+    t_load = time.perf_counter() - t0
+    print(f"[bench] serving: {name} generated in {t_gen:.1f} s, loaded in {t_load:.1f} s", file=sys.stderr, flush=True)
+    qs = scenes.coverage_queries(n, boxes, half=0.5 * half)
     params = [make_params(**kw) for kw, _ in qs]
 
     def finish(srv, ids):
@@ -188,10 +259,12 @@ def serve_run(ctx, make_params, scenes, Server, n=96, slots=3, load=0.7):
         infos = finish(srv, ids)
         t_poisson = time.perf_counter() - t0
         st = srv.stats()
+    ctx.close()
     lat = np.sort([i["ms_total"] for i in infos])
     bounces = sum(i["bounces"] for i in infos)
     nearest = lambda q: float(lat[min(len(lat), max(1, int(np.ceil(q * len(lat))))) - 1])
-    return dict(workload="urban_canyon_482tri_coverage_queries_L6-10_R4_rx1-50k", queries=n, slots=slots,
+    return dict(workload=f"{name}_coverage_queries_L6-10_R4_rx1-50k", queries=n, slots=slots,
+                triangles=int(len(soup[2])), scene_build_s=round(t_gen, 2), scene_load_s=round(t_load, 2),
                 capacity_req_s=round(capacity, 2), capacity_mray_bounces_s=round(bounces / t_sat / 1e6, 2),
                 offered_req_s=round(rate, 2), req_per_s=round(n / t_poisson, 2),
                 mray_bounces_per_s=round(bounces / t_poisson / 1e6, 2), p50_ms=round(nearest(0.5), 3),
@@ -229,6 +302,14 @@ def main():
     ctx.scene_from_soup(*cfg["soup"])
     ctx.materials(cfg["materials"])
     print(f"[bench] scene {cfg['name']} uploaded; warmup {args.warmup}", file=sys.stderr, flush=True)
+
+    # One cold step: the first trace of the context generates the level's launch directions (later
+    # steps reuse them) and sizes its buffers.
+    t0 = time.perf_counter()
+    cold = ctx.trace(make_params(**trace_kw), cfg["rx"]).stats
+    cold_step = dict(ms_wall=round((time.perf_counter() - t0) * 1e3, 3), ms_raygen=round(cold["ms_raygen"], 3),
+                     ms_total=round(cold["ms_total"], 3),
+                     note="first trace of a fresh context: ray generation and buffer allocation included")
 
     # The kernel's own traversal counts (one extra, untimed step; reported beside the canonical ones).
     cnt = ctx.trace(make_params(**trace_kw, countTraversal=1), cfg["rx"]).stats
@@ -268,8 +349,10 @@ def main():
         traffic = float(m.group(1)) if m else None
     ops = bounces * (OPS_PER_NODE * canon["nodes_per_bounce"] + OPS_PER_TRI * canon["tris_per_bounce"])
     valu_tflops = ops / (k_trace_ms * 1e-3) / 1e12
+    counters = kernel_counters(traffic, avg_ms, valu_tflops)
     roofline = dict(bound="hbm", achieved=round(achieved, 2), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 5), traffic=traffic, kernel="k_trace",
+                    binding=counters.pop("binding"), counters=counters,
                     avg_launch_ms=round(avg_ms, 4), launches_per_step=k_trace_n,
                     bytes_per_bounce=round(b_bounce, 1), algorithmic_bytes_per_launch=int(per_launch_bytes),
                     canonical_nodes_per_bounce=round(canon["nodes_per_bounce"], 4),
@@ -301,6 +384,7 @@ def main():
                    "paths_kept": int(stats[-1]["winners"]), "capture_records": int(stats[-1]["capture_records"]),
                    "stage_ms": {k: round(v, 3) for k, v in stage.items()}},
         "roofline": roofline,
+        "cold_step": cold_step,
     }
     def extra(name, fn):
         """Secondary lines: a failure is reported in the line instead of failing the headline."""
@@ -324,15 +408,17 @@ def main():
         extra("batched", lambda: batched_run(ctx, make_params, scenes))
     if not args.no_serve:
         from sbrsolver_amd import Server
-        extra("serving", lambda: serve_run(ctx, make_params, scenes, Server))
+        extra("serving", lambda: serve_run(Context, make_params, scenes, Server))
     if not args.no_indoor:
         extra("indoor", lambda: indoor_run(Context, make_params, scenes))
 
-    # Batched and served throughput as a fraction of the same HBM roofline: their bounces priced at
-    # the headline workload's algorithmic bytes per bounce (same scene; the per-bounce traversal
-    # counts of their levels are not committed, so this is a model figure).
-    for name, key in (("batched", "batched_mrays"), ("serving", "capacity_mray_bounces_s"),
-                      ("serving", "mray_bounces_per_s")):
+    # Batched throughput as a fraction of the same HBM roofline: its bounces priced at the headline
+    # workload's algorithmic bytes per bounce (same scene; the per-bounce traversal counts of its
+    # levels are not committed, so this is a model figure).  The served city is another scene.
+    if "batched" in out and "batched_mrays" in out["batched"]:
+        # the second denominator: the plain single-transmitter kernel on the headline workload
+        out["batched"]["speedup_vs_headline_single_tx"] = round(out["batched"]["batched_mrays"] / value, 3)
+    for name, key in (("batched", "batched_mrays"),):
         if name in out and key in out[name]:
             gbs = out[name][key] * 1e6 * b_bounce / 1e9
             out[name]["roofline_frac_" + key] = round(gbs / HBM_PEAK_GBS, 5)
@@ -343,6 +429,8 @@ def main():
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(cfg, trace_kw, args.cpu_stride)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        print("[bench] cpu baseline, single thread, box room", file=sys.stderr, flush=True)
+        out["cpu_baseline_single_thread"] = cpu_single_thread_box_room()
     ctx.close()
     print(json.dumps(out), flush=True)
 
