@@ -222,10 +222,10 @@ def serve_run(Context, make_params, scenes, Server, n=96, slots=3, load=0.7):
     t_gen = time.perf_counter() - t0
     ctx = Context(0)
     t0 = time.perf_counter()
-    ctx.scene_from_soup(*soup)
+    ms_dev = ctx.scene_from_soup_device(*soup)   # triangles, coplanar ids and a linear BVH built on the device
     ctx.materials(mats)
     t_load = time.perf_counter() - t0
-    print(f"[bench] serving: {name} generated in {t_gen:.1f} s, loaded in {t_load:.1f} s", file=sys.stderr, flush=True)
+    print(f"[bench] serving: {name} generated in {t_gen:.1f} s, loaded in {t_load:.2f} s", file=sys.stderr, flush=True)
     qs = scenes.coverage_queries(n, boxes, half=0.5 * half)
     params = [make_params(**kw) for kw, _ in qs]
 
@@ -264,7 +264,9 @@ def serve_run(Context, make_params, scenes, Server, n=96, slots=3, load=0.7):
     bounces = sum(i["bounces"] for i in infos)
     nearest = lambda q: float(lat[min(len(lat), max(1, int(np.ceil(q * len(lat))))) - 1])
     return dict(workload=f"{name}_coverage_queries_L6-10_R4_rx1-50k", queries=n, slots=slots,
-                triangles=int(len(soup[2])), scene_build_s=round(t_gen, 2), scene_load_s=round(t_load, 2),
+                triangles=int(len(soup[2])), scene_generate_s=round(t_gen, 2), scene_load_s=round(t_load, 3),
+                scene_device_build_ms=round(ms_dev[0], 1),
+                scene_load_note="sbr_scene_from_soup_device: vertex dedup, coplanar ids and a linear BVH on the device, then the upload",
                 capacity_req_s=round(capacity, 2), capacity_mray_bounces_s=round(bounces / t_sat / 1e6, 2),
                 offered_req_s=round(rate, 2), req_per_s=round(n / t_poisson, 2),
                 mray_bounces_per_s=round(bounces / t_poisson / 1e6, 2), p50_ms=round(nearest(0.5), 3),

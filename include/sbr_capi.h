@@ -14,6 +14,8 @@
  *   sbr_scene_from_stl     Preprocessor::StlToGeometry + BVH<Triangle>::ConstructBVH
  *                          (Preprocessor.cpp:69-114, BVH/BVH.h:51-55, SBRSolver_rev2.cpp:67-68)
  *   sbr_scene_from_soup    same, from the facet arrays stl_reader delivers (StlReader.h:637-700)
+ *   sbr_scene_from_soup_device  the same triangle set (Preprocessor.cpp:69-246, StlReader.h:448-508)
+ *                          built on the device, with a linear BVH in place of BVH/BVH.h:135-274
  *   sbr_scene_upload       the BVH<Triangle>& / std::vector<Triangle*>& pair that
  *                          RTSolver::Init (RTSolver.cpp:13-22) and ExecuteRayTracing take
  *   sbr_materials          the MaterialProperties[] of RTSolver::Init (RTSolver.cpp:13-22)
@@ -181,6 +183,15 @@ int sbr_scene_from_stl(sbr_ctx* ctx, const char* path);
 int sbr_scene_from_soup(sbr_ctx* ctx, const float* coords9, const float* normals3, const int32_t* material_ids,
                         int32_t n_facets);
 int sbr_scene_upload(sbr_ctx* ctx, const sbr_scene_desc* desc);
+/* Scene ingestion on the device for large scenes (no reference counterpart as one call): the
+ * triangle set of sbr_scene_from_soup (vertex dedup, degenerate facets dropped, file normals,
+ * coplanar ids; identical to the host's) built with sorts and a union-find on the device, and a
+ * linear BVH (Morton order of the triangle centres, ties by triangle id, leaves of at most 4
+ * triangles) instead of the reference's SAH.  The mesh vector keeps triangle-id order.  Traces run
+ * the reference's algorithm over this tree (sbr_scene_bvh_dfs exports it).  ms_out[2] (may be
+ * NULL): milliseconds until the scene description was built, and including the upload. */
+int sbr_scene_from_soup_device(sbr_ctx* ctx, const float* coords9, const float* normals3, const int32_t* material_ids,
+                               int32_t n_facets, double* ms_out);
 /* Scene as stored: per triangleId coplanar id, material id, vertices, normal; mesh order. */
 int sbr_scene_info(sbr_ctx* ctx, int32_t* n_tri, int32_t* n_node, int32_t* n_coplanar);
 int sbr_scene_triangles(sbr_ctx* ctx, int32_t* coplanar, int32_t* material, double* verts9, double* norm3,

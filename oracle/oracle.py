@@ -117,6 +117,23 @@ class Scene:
         return cls(lib().orc_scene_from_soup(_p(c, C.c_float), _p(n, C.c_float),
                                              None if m is None else _p(m, C.c_int), c.shape[0]))
 
+    @classmethod
+    def from_arrays(cls, tri, bvh):
+        """A scene over given triangles (dict of sbr_scene_triangles: verts, normals, coplanar,
+        material, mesh_tri_id) and a given DFS-ordered BVH (dict of sbr_scene_bvh_dfs: box, a, b, leaf,
+        tri_dfs_id): the reference's traversal over a tree built elsewhere."""
+        L = lib()
+        L.orc_scene_from_arrays.restype = C.c_void_p
+        L.orc_scene_from_arrays.argtypes = [C.c_int] + [C.c_void_p] * 5 + [C.c_int] + [C.c_void_p] * 5
+        arr = lambda x, t: np.ascontiguousarray(x, dtype=t)
+        keep = [arr(tri["verts"], np.float64), arr(tri["normals"], np.float64), arr(tri["coplanar"], np.int32),
+                arr(tri["material"], np.int32), arr(tri["mesh_tri_id"], np.int32), arr(bvh["box"], np.float64),
+                arr(bvh["a"], np.int32), arr(bvh["b"], np.int32), arr(bvh["leaf"], np.int32),
+                arr(bvh["tri_dfs_id"], np.int32)]
+        h = L.orc_scene_from_arrays(len(keep[2]), *[k.ctypes.data for k in keep[:5]], len(keep[6]),
+                                    *[k.ctypes.data for k in keep[5:]])
+        return cls(h)
+
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             _lib.orc_scene_free(self.h)

@@ -1309,6 +1309,45 @@ void* orc_scene_from_soup(const float* coords9, const float* normals3, const int
   return BuildScene(c, n, t, mats);
 }
 
+// A scene over given triangles and a given BVH (not built here): triangles by id (vertices,
+// file normal, coplanar and material ids), the mesh order (meshTriId[k] = id at position k), and the
+// BVH as nodes in DFS order with node 0 the root: box, and either children (a, b) or a leaf's
+// triangle range [a, a + b) of the DFS triangle list triDfs.  Used to run the reference's traversal
+// and capture over a tree the library built on the device.  Returns nullptr on malformed input.
+void* orc_scene_from_arrays(int ntri, const double* verts9, const double* norm3, const int* cop, const int* mat,
+                            const int* meshTriId, int nnode, const double* box6, const int* a, const int* b,
+                            const int* leaf, const int* triDfs) {
+  if (ntri <= 0 || nnode <= 0) return nullptr;
+  Scene* sc = new Scene();
+  for (int i = 0; i < ntri; i++) {
+    const double* v = verts9 + 9 * (size_t)i;
+    Tri* t = new Tri();
+    t->init(mk(v[0], v[1], v[2]), mk(v[3], v[4], v[5]), mk(v[6], v[7], v[8]),
+            mk(norm3[3 * i], norm3[3 * i + 1], norm3[3 * i + 2]), i);
+    t->coplanarId = cop[i];
+    t->materialId = mat[i];
+    sc->owned.push_back(t);
+  }
+  for (int k = 0; k < ntri; k++) {
+    if (meshTriId[k] < 0 || meshTriId[k] >= ntri) { delete sc; return nullptr; }
+    sc->mesh.push_back(sc->owned[meshTriId[k]]);
+  }
+  sc->bvh.nodes.assign(nnode, BNode());
+  for (int k = 0; k < nnode; k++) {
+    BNode& n = sc->bvh.nodes[k];
+    for (int q = 0; q < 3; q++) { n.bmin[q] = box6[6 * k + q]; n.bmax[q] = box6[6 * k + 3 + q]; }
+    if (leaf[k]) {
+      if (a[k] < 0 || b[k] <= 0 || a[k] + b[k] > ntri) { delete sc; return nullptr; }
+      for (int j = a[k]; j < a[k] + b[k]; j++) n.objects.push_back(sc->owned[triDfs[j]]);
+    } else {
+      if (a[k] <= k || b[k] <= k || a[k] >= nnode || b[k] >= nnode) { delete sc; return nullptr; }
+      n.child[0] = a[k];
+      n.child[1] = b[k];
+    }
+  }
+  return sc;
+}
+
 void orc_scene_free(void* s) { delete (Scene*)s; }
 int orc_scene_ntri(void* s) { return (int)((Scene*)s)->owned.size(); }
 int orc_scene_nnodes(void* s) { return (int)((Scene*)s)->bvh.nodes.size(); }

@@ -12,6 +12,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libsbr_amd.so")
 SOURCES = [
     os.path.join(HERE, "csrc", "engine.hip"),
+    os.path.join(HERE, "csrc", "ingest.hip"),
     os.path.join(HERE, "csrc", "capi.cpp"),
     os.path.join(HERE, "csrc", "serve.cpp"),
     os.path.join(HERE, "csrc", "host", "preprocessor.cpp"),

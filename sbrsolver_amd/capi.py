@@ -103,6 +103,7 @@ SIGNATURES = {
     "sbr_get_tuning": ([_vp, _vp], C.c_int),
     "sbr_autotune": ([_vp, _vp, _vp, _i32, _vp], C.c_int),
     "sbr_generate_rays": ([_vp, _i32, _vp, _i64, _vp], C.c_int),
+    "sbr_scene_from_soup_device": ([_vp, _vp, _vp, _vp, _i32, _vp], C.c_int),
     "sbr_closest_hit": ([_vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp], C.c_int),
     "sbr_closest_hit_noalloc": ([_vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp], C.c_int),
     "sbr_trace": ([_vp, _vp, _vp, _i32], C.c_int),
@@ -295,6 +296,15 @@ class Context:
         n = np.ascontiguousarray(normals3, dtype=np.float32).reshape(-1, 3)
         m = None if materials is None else np.ascontiguousarray(materials, dtype=np.int32)
         self._chk(lib().sbr_scene_from_soup(self.h, _a(c), _a(n), _a(m), c.shape[0]))
+
+    def scene_from_soup_device(self, coords9, normals3, materials=None):
+        """The scene built on the device (linear BVH); returns (ms until the description, ms total)."""
+        c = np.ascontiguousarray(coords9, dtype=np.float32).reshape(-1, 9)
+        n = np.ascontiguousarray(normals3, dtype=np.float32).reshape(-1, 3)
+        m = None if materials is None else np.ascontiguousarray(materials, dtype=np.int32)
+        ms = np.zeros(2)
+        self._chk(lib().sbr_scene_from_soup_device(self.h, _a(c), _a(n), _a(m), c.shape[0], _a(ms)))
+        return float(ms[0]), float(ms[1])
 
     def scene_info(self):
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()

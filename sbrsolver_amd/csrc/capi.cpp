@@ -232,7 +232,7 @@ int Engine::upload(const sbr_scene_desc* d) {
   hVerts.assign(d->tri_v, d->tri_v + 9 * (size_t)n);
   hNorms.assign(d->tri_n, d->tri_n + 3 * (size_t)n);
   ntri = n;
-  nnode = nn;
+  nnode = (int)hNodes.size();   // nodes reachable from the root (a description may hold others)
   treeHeight = height;
   hTriBox32.swap(tb32);
   hNodesF.swap(nf);
@@ -378,6 +378,11 @@ int sbr_scene_from_soup(sbr_ctx* ctx, const float* coords9, const float* normals
 
 int sbr_scene_upload(sbr_ctx* ctx, const sbr_scene_desc* desc) {
   return guarded(ctx, [&] { return ctx->eng.upload(desc); });
+}
+
+int sbr_scene_from_soup_device(sbr_ctx* ctx, const float* coords9, const float* normals3, const int32_t* material_ids,
+                               int32_t n_facets, double* ms_out) {
+  return guarded(ctx, [&] { return ctx->eng.ingest_soup(coords9, normals3, material_ids, n_facets, ms_out); });
 }
 
 int sbr_scene_info(sbr_ctx* ctx, int32_t* n_tri, int32_t* n_node, int32_t* n_coplanar) {

@@ -301,6 +301,10 @@ struct Engine {
   int fail(int code, const std::string& msg) { err = msg; return code; }
   int init(int dev);
   int upload(const sbr_scene_desc* d);
+  // Triangles, coplanar ids and a linear BVH built on the device from STL-style facets, then uploaded
+  // as upload() does (ingest.hip).  msOut (may be null): [0] until the description is on the host,
+  // [1] including the upload.
+  int ingest_soup(const float* coords9, const float* normals3, const int32_t* matIds, int nf, double* msOut);
   int clone_scene(const Engine& src);   // scene and materials of another engine (device copies made here)
   int set_materials(const sbr_material* m, int n);
   int set_pattern(const float* gainInfo, int nTheta, int nPhi);

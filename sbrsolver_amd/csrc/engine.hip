@@ -80,6 +80,7 @@ void HostPinned::release() {
 static_assert(sizeof(LevelRec) * (kMaxDepth + 2) <= 1024, "level records fit their staging slot");
 
 template struct DBuf<double>;
+template struct DBuf<float>;
 template struct DBuf<int32_t>;
 template struct DBuf<uint8_t>;
 template struct DBuf<unsigned long long>;

@@ -114,3 +114,21 @@ def test_config2_shape():
     assert v.shape == (482, 9) and n.shape == (482, 3) and m.shape == (482,)
     assert cfg["rx"].shape == (10000, 3)
     assert np.allclose(np.abs(n).sum(axis=1), 1.0)
+
+
+def test_oracle_over_an_exported_tree_equals_its_own_build():
+    """The oracle traced over the library's exported DFS tree (Scene.from_arrays) gives its own
+    result: the same tree in another node numbering, so every trace and path is identical."""
+    cfg = scenes.config2(tess=3)
+    with Context(-1) as c:
+        c.scene_from_soup(*cfg["soup"])
+        t = c.scene_triangles()
+        b = c.scene_bvh_dfs()
+    a = O.Scene.from_arrays(t, b)
+    own = O.Scene.from_soup(*cfg["soup"])
+    kw = dict(cfg["trace"], tess=3)
+    ra = O.trace(a, O.make_params(**kw), cfg["rx"][::7], cfg["materials"], nthreads=8)
+    ro = O.trace(own, O.make_params(**kw), cfg["rx"][::7], cfg["materials"], nthreads=8)
+    assert ra.bounces == ro.bounces and ra.np == ro.np > 0
+    assert np.array_equal(ra.stri, ro.stri) and np.array_equal(ra.ssrc, ro.ssrc)
+    assert np.array_equal(ra.fields, ro.fields, equal_nan=True)
