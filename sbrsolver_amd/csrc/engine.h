@@ -265,7 +265,7 @@ struct Engine {
   DBuf<int32_t> dCellStart;
   DBuf<RxCell> dCellRx;
   DBuf<uint8_t> dLos;
-  DBuf<unsigned long long> dTabA, dTabB, dTabOrder;
+  DBuf<unsigned long long> dTabA, dSeqTab, dTabOrder;   // group table keys, sequence trie, order keys
   DBuf<int32_t> dTabRx;
   DBuf<int32_t> dHeavy;          // nodes left to the wave-per-node capture
   DBuf<int32_t> dCand;           // nodes whose segment may reach the receivers' capture region

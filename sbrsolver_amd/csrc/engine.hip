@@ -28,7 +28,7 @@
 #include <cstring>
 #include <mutex>
 
-#include "dedup_hash.h"
+#include "dedup_keys.h"
 #include "device_common.h"
 #include "engine.h"
 
@@ -546,7 +546,8 @@ struct NodeCtx {
   int model, depth, txi;
   int32_t node, parent;
   bool all, keyed, emitted;
-  unsigned long long seqA, seqB, orderKey;
+  int64_t seqId;   // id of the path's coplanar-id sequence in the sequence trie (-1 - txi: empty)
+  unsigned long long orderKey;
   int64_t losBase;
   int j0, j1;
 };
@@ -559,12 +560,14 @@ struct CaptureBufs {
   const int32_t* cellStart;
   const RxCell* cellRx;           // receivers in cell order (cellStart: first of each cell)
   const uint8_t* los;
-  unsigned long long* tabA;
-  unsigned long long* tabB;
+  unsigned long long* tabA;         // group key (sequence id, receiver), 0: empty
+  unsigned long long* seqTab;       // sequence trie: exact (parent id, element) keys, kSeqEmpty: empty
+  unsigned long long seqMask;
   unsigned long long* tabOrder;
   int32_t* tabRx;
-  // [0] records [1] corrected [2] slots used [3] flags (bit 0 table full, bit 1 hash collision,
-  // bit 2 record buffer full) [4] collected [5] heavy nodes [6] candidate sphere tests (count mode)
+  // [0] records [1] corrected [2] slots used [3] flags (bit 0 table full, bit 2 record buffer full,
+  // bit 3 sequence table full) [4] collected [5] heavy nodes [6] candidate sphere tests (count mode)
+  // [7] sequence-table slots used
   unsigned long long* counters;
   int32_t* heavy;
   // (node, receiver) records of the candidate pass, in kRecParts partitions of recPart slots each
@@ -574,7 +577,7 @@ struct CaptureBufs {
   unsigned long long* recCount;
   const unsigned long long* recPrefix;   // dedup pass: records before each partition
   int countTests;
-  unsigned long long* keys;       // per node with records: seqA, seqB, order key (3 words)
+  unsigned long long* keys;       // per node with records: sequence id, order key (3 words, the third unused)
   const int32_t* cand;            // nodes listed by the bounce kernel
   const unsigned long long* nCand;   // their number
   // scene, for the occlusion test of corrected paths (k_dedup<true> only)
@@ -622,22 +625,49 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
   return true;
 }
 
-// Dedup key of the node's path (hash of the ancestors' coplanar ids, root first, without the
-// receiver) and its order key; computed once, at the node's first capture.
+// Id of the sequence (parent sequence id, element) in the trie: the slot holding that exact key,
+// claimed by the first thread to insert it.  kSeqNone if the table is full (flagged; the trace is
+// redone with a larger table).
+__device__ int64_t seq_intern(const CaptureBufs& B, int64_t parent, uint32_t elem) {
+  const unsigned long long key = seq_key(parent, elem);
+  unsigned long long slot = mix64(key) & B.seqMask;
+  for (unsigned long long probe = 0; probe <= B.seqMask; probe++) {
+    const unsigned long long cur =
+        __hip_atomic_load(&B.seqTab[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == key) return (int64_t)slot;
+    if (cur == kSeqEmpty) {
+      const unsigned long long prev = atomicCAS(&B.seqTab[slot], kSeqEmpty, key);
+      if (prev == kSeqEmpty) {
+        atomicAdd(&B.counters[7], 1ull);
+        return (int64_t)slot;
+      }
+      if (prev == key) return (int64_t)slot;
+    }
+    slot = (slot + 1) & B.seqMask;
+  }
+  atomicOr(&B.counters[3], 8ull);
+  return kSeqNone;
+}
+
+// Dedup key of the node's path: the id of its ancestors' coplanar-id sequence (root first, without
+// the receiver; with typedDedup each element also carries the type of the interaction that follows
+// it), and its order key; computed once, at the node's first capture.
 __device__ void node_key(const CaptureBufs& B, NodeCtx& c) {
-  int32_t anc[kMaxDepth];
+  int32_t anc[kMaxDepth], typ[kMaxDepth];
   int32_t q = c.parent;
   for (int k = c.depth - 1; k >= 0; k--) { anc[k] = B.tree[q].cop; q = B.tree[q].parent; }
-  seq_seed(c.txi, c.depth, &c.seqA, &c.seqB);
-  for (int k = 0; k < c.depth; k++) seq_fold(anc[k], k, &c.seqA, &c.seqB);
-  if (B.txs[c.txi].typedDedup) {   // the types of the interactions, leaf first, at positions after the ids
+  const bool typed = B.txs[c.txi].typedDedup != 0;
+  if (typed) {   // the interaction at depth m is node m's type
     int32_t t = c.node;
     for (int m = c.depth; m >= 1; m--) {
       const TNode& nd = B.tree[t];
-      seq_fold((int32_t)nd.type + 0x40000000, kMaxDepth + m, &c.seqA, &c.seqB);
+      typ[m - 1] = nd.type;
       t = nd.parent;
     }
   }
+  int64_t id = seq_root(c.txi);
+  for (int k = 0; k < c.depth && id != kSeqNone; k++) id = seq_intern(B, id, seq_elem(anc[k], typed ? typ[k] : 0));
+  c.seqId = id;
   c.orderKey = order_key(B.tree[c.node]);
   c.keyed = true;
 }
@@ -780,9 +810,8 @@ __device__ __forceinline__ void capture_one_cached(const CaptureBufs& B, NodeCtx
 __device__ __forceinline__ void store_key(const CaptureBufs& B, NodeCtx& c) {
   node_key(B, c);
   unsigned long long* k = B.keys + 3 * (int64_t)c.node;
-  k[0] = c.seqA;
-  k[1] = c.seqB;
-  k[2] = c.orderKey;
+  k[0] = (unsigned long long)c.seqId;
+  k[1] = c.orderKey;
 }
 
 __device__ bool corrected_points(V3 tx, const TNode* __restrict__ tree, const DSurf* __restrict__ surf,
@@ -834,22 +863,23 @@ __global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int
     c.txi = nd.txi;
     c.losBase = (int64_t)c.txi * A.nrx;
     const unsigned long long* key = B.keys + 3 * (int64_t)c.node;
-    c.seqA = key[0];
-    c.seqB = key[1];
-    c.orderKey = key[2];
+    c.seqId = (int64_t)key[0];
+    c.orderKey = key[1];
     c.keyed = true;
-    unsigned long long hA, hB;
-    group_hash(c.seqA, c.seqB, r, &hA, &hB);
-    bool keep = true;
+    // the sequence table filled up (flagged, the trace is redone): no key
+    const bool keyed = c.seqId != kSeqNone;
+    const unsigned long long gk = group_key(c.seqId, r);
+    const unsigned long long hA = gk;
+    bool keep = keyed;
     // A record that cannot beat the key already stored for its group needs no correction: the
     // stored key belongs to a corrected path and keys only decrease.  A stale read is only less
     // selective.
-    {
-      unsigned long long slot = hA & A.tabMask;
+    if (keep) {
+      unsigned long long slot = mix64(gk) & A.tabMask;
       for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
         const unsigned long long a = B.tabA[slot];
         if (a == 0ull) break;
-        if (a == hA) {
+        if (a == gk) {
           if (B.tabOrder[slot] < c.orderKey) keep = false;
           break;
         }
@@ -864,14 +894,12 @@ __global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int
     }
     if (ok) {
       myCor++;
-      unsigned long long slot = hA & A.tabMask;
+      unsigned long long slot = mix64(gk) & A.tabMask;
       bool placed = false;
       for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
         unsigned long long prev = atomicCAS(&B.tabA[slot], 0ull, hA);
         if (prev == 0ull || prev == hA) {
           if (prev == 0ull) { B.tabRx[slot] = (int32_t)(c.losBase + r); myUsed++; }
-          unsigned long long pb = atomicCAS(&B.tabB[slot], 0ull, hB);
-          if (pb != 0ull && pb != hB) atomicOr(&B.counters[3], 2ull);
           atomicMin(&B.tabOrder[slot], c.orderKey);
           placed = true;
           break;
@@ -1917,6 +1945,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 5; i++) (void)hipEventDestroy(e[i]); } } guard{ev};
   HIPCHK(hipEventRecord(ev[0], stream));
 
+  if (ncop >= (1 << 30)) return fail(SBR_E_LIMIT, "coplanar ids above 2^30 do not fit the duplicate-removal keys");
   // ---- transmitters and their launch directions
   std::vector<TxDesc> txs(nt);
   int maxDepth = 0;
@@ -2108,12 +2137,12 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     cache = PathCache{};
     if (m > 0) {
       ENSURE(dTabA, cap);
-      ENSURE(dTabB, cap);
+      ENSURE(dSeqTab, cap);
       ENSURE(dTabOrder, cap);
       ENSURE(dTabRx, cap);
       ENSURE(dField, 6 * (size_t)nOut);
       HIPCHK(hipMemsetAsync(dTabA.p, 0, cap * sizeof(unsigned long long), stream));
-      HIPCHK(hipMemsetAsync(dTabB.p, 0, cap * sizeof(unsigned long long), stream));
+      HIPCHK(hipMemsetAsync(dSeqTab.p, 0xFF, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dTabOrder.p, 0xFF, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dField.p, 0, 6 * sizeof(double) * nOut, stream));
       HIPCHK(hipMemsetAsync(dCounters.p, 0, 8 * sizeof(unsigned long long), stream));
@@ -2206,8 +2235,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         HIPCHK(hipMemsetAsync(dRecCount.p, 0, sizeof(unsigned long long) * kRecParts * kRecStride, stream));
         HIPCHK(hipMemsetAsync(dCounters.p + 3, 0, sizeof(unsigned long long), stream));
         HIPCHK(hipMemsetAsync(dCounters.p + 5, 0, 2 * sizeof(unsigned long long), stream));
-        CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
-                       dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p, nullptr,
+        CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dSeqTab.p,
+                       (unsigned long long)(cap - 1), dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p, nullptr,
                        count ? 1 : 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
         if (nodes > 0) {
           k_capture<<<(unsigned)std::min<int64_t>(blocks_for(nodes, 256), (int64_t)numCU * 16), 256,
@@ -2250,8 +2279,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       stats.capture_nodes += capNodes;
       // dedup pass over the records
       if (nRec > 0) {
-        CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dTabB.p,
-                       dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, recCap / kRecParts, dRecCount.p,
+        CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dSeqTab.p,
+                       (unsigned long long)(cap - 1), dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, recCap / kRecParts, dRecCount.p,
                        dRecPrefix.p, 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
         CB.nodes = dNodes.p;
         CB.nodesF = dNodesF.p;
@@ -2270,14 +2299,13 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       }
       HIPCHK(hipEventRecord(kev[1], stream));
       unsigned long long* ctr = pinSmall.at<unsigned long long>(kPinCapCtr);
-      HIPCHK(hipMemcpyAsync(ctr, dCounters.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(ctr, dCounters.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
       float kms = 0;
       HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
       stats.ms_k_capture += kms;
       stats.ms_capture += kms;
-      if (ctr[3] & 2ull) return fail(SBR_E_INTERNAL, "dedup hash collision detected");
-      if ((ctr[3] & 1ull) || ctr[2] * 10 > cap * 7) {
+      if ((ctr[3] & 9ull) || ctr[2] * 10 > cap * 7 || ctr[7] * 10 > cap * 7) {
         tableFull = true;
         break;
       }
@@ -2381,7 +2409,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
 template <typename F>
 static void for_each_trace_buffer(Engine& e, F&& f) {
   f(e.dTx); f(e.dTree); f(e.dQueue[0]); f(e.dQueue[1]); f(e.dLevels); f(e.dScratch); f(e.dRx);
-  f(e.dCellStart); f(e.dCellRx); f(e.dLos); f(e.dTabA); f(e.dTabB); f(e.dTabOrder); f(e.dTabRx);
+  f(e.dCellStart); f(e.dCellRx); f(e.dLos); f(e.dTabA); f(e.dSeqTab); f(e.dTabOrder); f(e.dTabRx);
   f(e.dHeavy); f(e.dCand); f(e.dCandFlag); f(e.dTileCount); f(e.dTileOffset); f(e.dRec); f(e.dRecCount);
   f(e.dRecPrefix); f(e.dNodeKey); f(e.dWinKey); f(e.dWinKey2); f(e.dWinRx); f(e.dWinRx2); f(e.dWinIdx);
   f(e.dWinIdx2); f(e.dWinNode); f(e.dWinField); f(e.dField);
