@@ -7,6 +7,7 @@
 #include <climits>
 
 #include "engine.h"
+#include "capture_prefilter.h"
 #include "prefilter.h"
 
 namespace sbr {
@@ -295,17 +296,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
 // Reception sphere: HitReceptionSphere (RTSolver.cpp:805-863), and the alpha*L/sqrt(3) model.
 // ---------------------------------------------------------------------------------------------
 // base: the totalPathLength handed to the node (model 0) or the true length to S (model 1).
-__device__ __forceinline__ bool sphere_hit(V3 S, V3 T, V3 C, double base, int model, double rcoef) {
-  V3 rayDir = vsub(T, S);
-  double t0 = vdot(rayDir, vsub(C, S)) / vsqnorm(rayDir);
-  if (t0 < 0) return false;
-  if (t0 >= 1) t0 = 1;
-  V3 cp = vadd(S, vmul(rayDir, t0));
-  double unfolded = model == 0 ? base + vnorm(cp) : base + vnorm(vmul(rayDir, t0));
-  double radius = unfolded * rcoef;
-  double distance = vnorm(vsub(C, cp));
-  return distance <= radius;
-}
+// (sphere_hit and its single-precision pre-test: capture_prefilter.h)
 
 // ImagePoint (RTSolver.cpp:910-924)
 __device__ __forceinline__ V3 image_point(const DSurf& s, V3 src) {

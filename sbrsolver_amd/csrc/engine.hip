@@ -527,6 +527,7 @@ struct CaptureArgs {
   // within the capture radius still falls in a cell of the region.
   double invCell;
   unsigned long long tabMask;
+  int32_t fuse;   // 1: the wave-per-node kernel deduplicates its nodes' captures itself
 };
 
 __device__ __forceinline__ unsigned long long order_key(const TNode& nd) {
@@ -541,11 +542,15 @@ __device__ __forceinline__ unsigned long long order_key(const TNode& nd) {
 // correction.
 struct NodeCtx {
   V3 S, Tg, P0, P1;
+  SphereF sf;      // single-precision pre-test of its reception sphere
+  SphereK sk;
   double base, rc, R;
   double invDy;   // 1 / (P1.y - P0.y), 0 when that is below 1e-300 in magnitude
   int model, depth, txi;
   int32_t node, parent;
   bool all, keyed, emitted;
+  bool spilled;    // some of its captures became records (k_dedup reads its key)
+  bool chainOk;    // its image chain is held in LDS (wave-per-node kernel, once keyed)
   int64_t seqId;   // id of the path's coplanar-id sequence in the sequence trie (-1 - txi: empty)
   unsigned long long orderKey;
   int64_t losBase;
@@ -567,7 +572,8 @@ struct CaptureBufs {
   int32_t* tabRx;
   // [0] records [1] corrected [2] slots used [3] flags (bit 0 table full, bit 2 record buffer full,
   // bit 3 sequence table full) [4] collected [5] heavy nodes [6] candidate sphere tests (count mode)
-  // [7] sequence-table slots used
+  // [7] sequence-table slots used [8] records and [9] corrections of the wave-per-node kernel's own
+  // duplicate removal (cleared with each candidate pass)
   unsigned long long* counters;
   int32_t* heavy;
   // (node, receiver) records of the candidate pass, in kRecParts partitions of recPart slots each
@@ -591,6 +597,17 @@ struct CaptureBufs {
   int32_t stackDepth;
 };
 
+// The wave-per-node kernel removes duplicates of its own nodes' captures: the receivers a node
+// captures wait in the wave's stash (kStash entries in LDS) and are deduplicated kStashFlush or more
+// at a time, against the node's image chain (kFuseDepth interactions at most) held in LDS.  A step
+// adds at most 64, so the stash does not fill.  The receivers of a chunk of the node's rows are
+// listed in LDS first (kList per chunk), so that each step's receivers are known a step ahead.
+constexpr int kStash = 256;
+constexpr int kStashFlush = 128;
+static_assert(kStashFlush + 64 <= kStash, "stash bound");
+constexpr int kFuseDepth = 8;
+constexpr int kList = 512;
+
 // Returns false when no receiver can be captured by the node.
 __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n, NodeCtx& c) {
   const TNode& nd = B.tree[n];   // capture half only
@@ -604,8 +621,11 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
   c.base = nd.base;
   c.keyed = false;
   c.emitted = false;
+  c.spilled = false;
   c.S = ld3(nd.src);
   c.Tg = ld3(nd.tgt);
+  c.sk = sphere_k(c.rc, c.model);
+  c.sf = sphere_f(c.S, c.Tg, c.base, c.sk);
   SegClip sc;
   if (!seg_clip(A.g, c.S, c.Tg, c.base, c.model, c.rc, sc)) return false;
   c.P0 = sc.P0;
@@ -771,7 +791,8 @@ __device__ __forceinline__ void capture_one(const CaptureBufs& B, NodeCtx& c, co
   myTests++;
   const V3 C = ld3(x.p);
   const int32_t r = x.r;
-  if (!sphere_hit(c.S, c.Tg, C, c.base, c.model, c.rc)) return;
+  const int pv = sphere_pre(c.sf, c.sk, c.S, C);
+  if (pv < 0 || (pv == 0 && !sphere_hit(c.S, c.Tg, C, c.base, c.model, c.rc))) return;
   int32_t q = c.parent;
   while (q >= 0) {
     const TNode& a = B.tree[q];
@@ -780,35 +801,36 @@ __device__ __forceinline__ void capture_one(const CaptureBufs& B, NodeCtx& c, co
   }
   append_record(B, c.node, r);
   c.emitted = true;
+  c.spilled = true;
 }
 
-// capture_one for the wave-per-node kernel: the node's ancestors (capture halves, nearest first) are
-// read from the wave's copy in LDS (anc: 8 doubles each), not walked through the tree again for every
+// The candidate test of the wave-per-node kernel: the node captures receiver C and none of its
+// ancestors does.  The ancestors (capture halves, nearest first) are read from the wave's copy in
+// LDS (anc: 8 doubles each, ancF: their pre-tests), not walked through the tree again for every
 // receiver hit; qRest continues the walk in the tree past the copied ones.
-__device__ __forceinline__ void capture_one_cached(const CaptureBufs& B, NodeCtx& c, const RxCell& x,
-                                                   unsigned long long& myTests, const double* anc, int nAnc,
-                                                   int32_t qRest) {
-  myTests++;
-  const V3 C = ld3(x.p);
-  if (!sphere_hit(c.S, c.Tg, C, c.base, c.model, c.rc)) return;
+__device__ __forceinline__ bool captures_cached(const CaptureBufs& B, const NodeCtx& c, V3 C, const double* anc,
+                                                const SphereF* ancF, int nAnc, int32_t qRest) {
+  const int pv = sphere_pre(c.sf, c.sk, c.S, C);
+  if (pv < 0 || (pv == 0 && !sphere_hit(c.S, c.Tg, C, c.base, c.model, c.rc))) return false;
   for (int j = 0; j < nAnc; j++) {
     const double* a = anc + 8 * j;
-    if (sphere_hit(v3(a[0], a[1], a[2]), v3(a[3], a[4], a[5]), C, a[6], c.model, c.rc)) return;
+    const V3 Sa = v3(a[0], a[1], a[2]);
+    const int pa = sphere_pre(ancF[j], c.sk, Sa, C);
+    if (pa > 0 || (pa == 0 && sphere_hit(Sa, v3(a[3], a[4], a[5]), C, a[6], c.model, c.rc))) return false;
   }
   for (int32_t q = qRest; q >= 0;) {
     const TNode& a = B.tree[q];
-    if (sphere_hit(ld3(a.src), ld3(a.tgt), C, a.base, c.model, c.rc)) return;
+    if (sphere_hit(ld3(a.src), ld3(a.tgt), C, a.base, c.model, c.rc)) return false;
     q = a.parent;
   }
-  append_record(B, c.node, x.r);
-  c.emitted = true;
+  return true;
 }
 
 // After a node's candidates: the first lane of the wave that emitted records for it stores its key
 // (all lanes of the wave-per-node kernel share the node; in the thread-per-node kernel each lane
 // has its own).
 __device__ __forceinline__ void store_key(const CaptureBufs& B, NodeCtx& c) {
-  node_key(B, c);
+  if (!c.keyed) node_key(B, c);
   unsigned long long* k = B.keys + 3 * (int64_t)c.node;
   k[0] = (unsigned long long)c.seqId;
   k[1] = c.orderKey;
@@ -836,6 +858,41 @@ __device__ bool unobstructed(const CaptureBufs& B, const NodeCtx& c, V3 C, int* 
       return false;
   }
   return true;
+}
+
+// The prune: false when the group already holds a corrected path with a smaller order key (keys only
+// decrease, so such a record cannot win and needs no correction).  A stale read is only less
+// selective.
+__device__ __forceinline__ bool group_may_win(const CaptureArgs& A, const CaptureBufs& B, unsigned long long gk,
+                                              unsigned long long orderKey) {
+  unsigned long long slot = mix64(gk) & A.tabMask;
+  for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
+    const unsigned long long a = B.tabA[slot];
+    if (a == 0ull) return true;
+    if (a == gk) return B.tabOrder[slot] >= orderKey;
+    slot = (slot + 1) & A.tabMask;
+  }
+  return true;
+}
+
+// Inserts a corrected path: the group's slot (claimed by the first insertion, which also records the
+// receiver) keeps the smallest order key.  Returns 1 if this call claimed the slot; flags the table
+// full when no slot is left.
+__device__ __forceinline__ unsigned long long group_insert(const CaptureArgs& A, const CaptureBufs& B,
+                                                           unsigned long long gk, unsigned long long orderKey,
+                                                           int64_t rxOut) {
+  unsigned long long slot = mix64(gk) & A.tabMask;
+  for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
+    const unsigned long long prev = atomicCAS(&B.tabA[slot], 0ull, gk);
+    if (prev == 0ull || prev == gk) {
+      if (prev == 0ull) B.tabRx[slot] = (int32_t)rxOut;
+      atomicMin(&B.tabOrder[slot], orderKey);
+      return prev == 0ull ? 1ull : 0ull;
+    }
+    slot = (slot + 1) & A.tabMask;
+  }
+  atomicOr(&B.counters[3], 1ull);
+  return 0ull;
 }
 
 // Dedup pass, one thread per record: the group key of the node's coplanar-id sequence, the prune
@@ -869,23 +926,7 @@ __global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int
     // the sequence table filled up (flagged, the trace is redone): no key
     const bool keyed = c.seqId != kSeqNone;
     const unsigned long long gk = group_key(c.seqId, r);
-    const unsigned long long hA = gk;
-    bool keep = keyed;
-    // A record that cannot beat the key already stored for its group needs no correction: the
-    // stored key belongs to a corrected path and keys only decrease.  A stale read is only less
-    // selective.
-    if (keep) {
-      unsigned long long slot = mix64(gk) & A.tabMask;
-      for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
-        const unsigned long long a = B.tabA[slot];
-        if (a == 0ull) break;
-        if (a == gk) {
-          if (B.tabOrder[slot] < c.orderKey) keep = false;
-          break;
-        }
-        slot = (slot + 1) & A.tabMask;
-      }
-    }
+    const bool keep = keyed && group_may_win(A, B, gk, c.orderKey);
     bool ok = keep && corrected(B, c, r, ld3(B.rx + 3 * (int64_t)r));
     if (OCCL && ok && c.depth > 0 && B.txs[c.txi].checkOcclusion) {
       int* stkN = reinterpret_cast<int*>(smemD) + threadIdx.x;
@@ -894,19 +935,7 @@ __global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int
     }
     if (ok) {
       myCor++;
-      unsigned long long slot = mix64(gk) & A.tabMask;
-      bool placed = false;
-      for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
-        unsigned long long prev = atomicCAS(&B.tabA[slot], 0ull, hA);
-        if (prev == 0ull || prev == hA) {
-          if (prev == 0ull) { B.tabRx[slot] = (int32_t)(c.losBase + r); myUsed++; }
-          atomicMin(&B.tabOrder[slot], c.orderKey);
-          placed = true;
-          break;
-        }
-        slot = (slot + 1) & A.tabMask;
-      }
-      if (!placed) atomicOr(&B.counters[3], 1ull);
+      myUsed += group_insert(A, B, gk, c.orderKey, c.losBase + r);
     }
   }
   // one atomic per block and counter (appends to one address serialise)
@@ -977,18 +1006,181 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
           const int q1 = cellStart(j * A.g.nx + i1 + 1);
           for (int q = cellStart(j * A.g.nx + i0); q < q1; q++) capture_one(B, c, B.cellRx[q], myTests);
         }
-        if (c.emitted) store_key(B, c);
+        if (c.spilled) store_key(B, c);
       }
     }
   }
   flush_tests(B, myTests);
 }
 
-// Wave per heavy node.  The node's rows are taken 64 at a time, one row's cell range per lane; a
-// wave scan of the ranges' lengths flattens their cells, which are then dealt to the lanes one
-// each (a lane finds its row by a binary search over the scan), so a node costs about
-// cells / 64 steps whatever the shape of its region.
-__global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBufs B) {
+// The wave-per-node kernel's copy of a node's ancestors: 8 doubles each, nearest first, the capture
+// half (src, tgt, base) and, in the last word, the ancestor's hit triangle (low 32 bits), coplanar id
+// (bits 32-61) and interaction type (bits 62-63).
+__device__ __forceinline__ double anc_ids(const TNode& a) {
+  const unsigned long long w = (unsigned long long)(uint32_t)a.tri |
+                               ((unsigned long long)(((uint32_t)a.cop & ((1u << 30) - 1)) | ((uint32_t)a.type << 30)) << 32);
+  return __longlong_as_double((long long)w);
+}
+__device__ __forceinline__ int32_t anc_tri(const double* a) { return (int32_t)(uint32_t)__double_as_longlong(a[7]); }
+__device__ __forceinline__ int32_t anc_cop(const double* a) {
+  return (int32_t)(((unsigned long long)__double_as_longlong(a[7]) >> 32) & ((1u << 30) - 1));
+}
+__device__ __forceinline__ int anc_type(const double* a) { return (int)((unsigned long long)__double_as_longlong(a[7]) >> 62); }
+
+// node_key's sequence id from the ancestors in LDS (every ancestor has a hit, so its coplanar id
+// fits the 30 bits kept): the same elements, interned in the same order.
+__device__ int64_t node_seq_lds(const CaptureBufs& B, const NodeCtx& c, const double* anc, int nodeType) {
+  const bool typed = B.txs[c.txi].typedDedup != 0;
+  int64_t id = seq_root(c.txi);
+  for (int k = 0; k < c.depth && id != kSeqNone; k++) {
+    // element k: the coplanar id of the ancestor at depth k, typed by the interaction at depth k + 1
+    const int t = k + 1 == c.depth ? nodeType : anc_type(anc + 8 * (c.depth - 2 - k));
+    id = seq_intern(B, id, seq_elem(anc_cop(anc + 8 * (c.depth - 1 - k)), typed ? t : 0));
+  }
+  return id;
+}
+
+// Image chain of the node for the correction, into the wave's LDS (12 doubles per interaction m =
+// 1..depth: image source m, then v1, e1, e2 of the surface of interaction m).  The image sources are
+// the prefixes of one pass over the path, the same operations image_source repeats for every k.
+// Returns false if an interaction has no surface (no corrected path then).
+__device__ bool image_chain_lds(const CaptureBufs& B, const NodeCtx& c, const double* anc, int nodeType, double* img) {
+  const TxDesc& T = B.txs[c.txi];
+  V3 cur = v3(T.tx[0], T.tx[1], T.tx[2]);
+  for (int m = 1; m <= c.depth; m++) {
+    const int32_t sid = anc_tri(anc + 8 * (c.depth - m));   // hit surface of the node at depth m - 1
+    if (sid < 0) return false;
+    const DSurf& sf = B.surf[sid];
+    const int t = m == c.depth ? nodeType : anc_type(anc + 8 * (c.depth - 1 - m));
+    if (t == 1) cur = image_point(sf, cur);
+    double* o = img + 12 * (m - 1);
+    o[0] = cur.x; o[1] = cur.y; o[2] = cur.z;
+    for (int a = 0; a < 3; a++) {
+      o[3 + a] = sf.v1[a];
+      o[6 + a] = sf.e1[a];
+      o[9 + a] = sf.e2[a];
+    }
+  }
+  return true;
+}
+
+// corrected() over an image chain in LDS: the same re-intersections, from the receiver back.
+__device__ __forceinline__ bool corrected_chain(const double* img, int depth, V3 C) {
+  V3 target = C;
+  for (int k = depth; k >= 1; k--) {
+    const double* e = img + 12 * (k - 1);
+    const V3 I = v3(e[0], e[1], e[2]);
+    V3 p;
+    double dist;
+    if (!tri_intersect(v3(e[3], e[4], e[5]), v3(e[6], e[7], e[8]), v3(e[9], e[10], e[11]), I, vsub(target, I), &p,
+                       &dist))
+      return false;
+    target = p;
+  }
+  return true;
+}
+
+// Duplicate removal of the n receivers in the wave's stash (called by the whole wave, where its
+// control flow is uniform): the node's key and image chain are taken once (first lane), and the
+// lanes share the receivers: prune, correction, insertion, as k_dedup does for a record.
+__device__ void dedup_stash(const CaptureArgs& A, const CaptureBufs& B, NodeCtx& c, int n, const int32_t* stash,
+                            const double* anc, double* img, unsigned long long& myRec, unsigned long long& myCor,
+                            unsigned long long& myUsed) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int lane = threadIdx.x & 63;
+  if (!c.keyed) {   // the same on every lane
+    int64_t seq = 0;
+    int ok = 0;
+    if (lane == 0) {
+      const int nodeType = B.tree[c.node].type;
+      seq = node_seq_lds(B, c, anc, nodeType);
+      ok = c.depth == 0 || image_chain_lds(B, c, anc, nodeType, img) ? 1 : 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    c.seqId = __shfl(seq, 0, 64);
+    c.chainOk = __shfl(ok, 0, 64) != 0;
+    c.orderKey = order_key(B.tree[c.node]);
+    c.keyed = true;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  if (lane == 0) myRec += (unsigned long long)n;
+  if (c.seqId != kSeqNone) {   // a full sequence table is flagged and the trace redone
+    for (int e = lane; e < n; e += 64) {
+      const int32_t r = stash[e];
+      const unsigned long long gk = group_key(c.seqId, r);
+      if (!group_may_win(A, B, gk, c.orderKey)) continue;
+      const V3 C = ld3(B.rx + 3 * (int64_t)r);
+      const bool ok = c.depth == 0 ? B.los[c.losBase + r] != 0 : (c.chainOk && corrected_chain(img, c.depth, C));
+      if (ok) {
+        myCor++;
+        myUsed += group_insert(A, B, gk, c.orderKey, c.losBase + r);
+      }
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Wave per heavy node.  The node's rows are taken 64 at a time, one row's cell range per lane; the
+// receivers of a row's range are contiguous in cell order, and a wave scan of their counts flattens
+// them, to be dealt to the lanes one each (a lane finds its row by a binary search over the scan;
+// neighbouring lanes read neighbouring receivers), so a node costs about receivers / 64 steps
+// whatever the shape of its region.
+// The same value on every lane, as the compiler can see (kept in scalar registers).
+__device__ __forceinline__ float wave_uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ double wave_uniform(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ V3 wave_uniform(V3 v) { return v3(wave_uniform(v.x), wave_uniform(v.y), wave_uniform(v.z)); }
+
+// A node's state is the same on every lane of the wave-per-node kernel; saying so keeps it out of
+// the vector registers.
+__device__ __forceinline__ void ctx_uniform(NodeCtx& c) {
+  c.S = wave_uniform(c.S);
+  c.Tg = wave_uniform(c.Tg);
+  c.P0 = wave_uniform(c.P0);
+  c.P1 = wave_uniform(c.P1);
+  c.base = wave_uniform(c.base);
+  c.rc = wave_uniform(c.rc);
+  c.R = wave_uniform(c.R);
+  c.invDy = wave_uniform(c.invDy);
+  c.sf.dx = wave_uniform(c.sf.dx);
+  c.sf.dy = wave_uniform(c.sf.dy);
+  c.sf.dz = wave_uniform(c.sf.dz);
+  c.sf.l2inv = wave_uniform(c.sf.l2inv);
+  c.sf.linv = wave_uniform(c.sf.linv);
+  c.sf.sx = wave_uniform(c.sf.sx);
+  c.sf.sy = wave_uniform(c.sf.sy);
+  c.sf.sz = wave_uniform(c.sf.sz);
+  c.sf.base = wave_uniform(c.sf.base);
+  c.sf.mT = wave_uniform(c.sf.mT);
+  c.sf.m0 = wave_uniform(c.sf.m0);
+  c.sk.rc = wave_uniform(c.sk.rc);
+  c.sk.mX = wave_uniform(c.sk.mX);
+  c.sk.model = wave_uniform(c.sk.model);
+  c.model = wave_uniform(c.model);
+  c.depth = wave_uniform(c.depth);
+  c.j0 = wave_uniform(c.j0);
+  c.j1 = wave_uniform(c.j1);
+}
+
+// Wave per heavy node.  The node's rows are taken 64 at a time, one row's cell range per lane; the
+// receivers of a row's range are contiguous in cell order.  A wave scan of their counts places them
+// in a list in LDS (kList at a time), from which they are dealt to the lanes one each, a step's
+// receivers loaded while the previous step's are tested (neighbouring lanes read neighbouring
+// receivers), so a node costs about receivers / 64 steps whatever the shape of its region.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_capture_heavy(CaptureArgs A,
+                                                                                   CaptureBufs B) {
   extern __shared__ uint16_t sCell[];
   // The grid is sized for the most heavy nodes (their count is read here, on the device); blocks
   // beyond the count leave before staging the cell starts.
@@ -1000,19 +1192,31 @@ __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBuf
   auto cellStart = [&](int k) { return A.cellLds > 0 ? (int)sCell[k] : B.cellStart[k]; };
   const int64_t nHeavy = (int64_t)B.counters[5];   // appended by k_capture
   const int lane = threadIdx.x & 63;
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned long long lanesBelow = (1ull << lane) - 1ull;
+  const int64_t wave = wave_uniform((int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   unsigned long long myTests = 0;
   __shared__ double sAnc[4][8 * kMaxDepth];   // per wave: the current node's ancestors
+  __shared__ SphereF sAncF[4][kMaxDepth];      // and their pre-tests
+  __shared__ double sImg[4][12 * kFuseDepth];  // per wave: its image chain
+  __shared__ int32_t sStash[4][kStash];        // per wave: receivers it captured, awaiting dedup
+  __shared__ int32_t sList[4][kList];          // per wave: receivers of the current chunk
   double* anc = sAnc[threadIdx.x >> 6];
+  SphereF* ancF = sAncF[threadIdx.x >> 6];
+  double* img = sImg[threadIdx.x >> 6];
+  int32_t* stash = sStash[threadIdx.x >> 6];
+  int32_t* list = sList[threadIdx.x >> 6];
+  unsigned long long myRec = 0, myCor = 0, myUsed = 0;
   for (int64_t h = wave; h < nHeavy; h += nWaves) {   // wave-uniform
     NodeCtx c;
     if (!node_setup(A, B, B.heavy[h], c)) continue;    // same outcome on every lane
+    ctx_uniform(c);
     // The first lane copies the ancestor chain to the wave's LDS.  A wave's LDS accesses complete in
     // order, and the fences and wave barriers keep the compiler from moving the other lanes' reads
     // (after the exchange of the count) above the copy or the next node's copy above them.
     int nAnc = 0;
     int32_t qRest = c.parent;
+    const bool fuse = A.fuse != 0 && c.depth <= kFuseDepth;   // the same on every lane
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) {
       while (qRest >= 0 && nAnc < kMaxDepth) {
@@ -1021,6 +1225,8 @@ __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBuf
         o[0] = a.src[0]; o[1] = a.src[1]; o[2] = a.src[2];
         o[3] = a.tgt[0]; o[4] = a.tgt[1]; o[5] = a.tgt[2];
         o[6] = a.base;
+        o[7] = anc_ids(a);
+        ancF[nAnc] = sphere_f(ld3(a.src), ld3(a.tgt), a.base, c.sk);
         nAnc++;
         qRest = a.parent;
       }
@@ -1030,45 +1236,91 @@ __global__ void __launch_bounds__(256) k_capture_heavy(CaptureArgs A, CaptureBuf
     nAnc = __shfl(nAnc, 0, 64);
     qRest = __shfl(qRest, 0, 64);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    nAnc = wave_uniform(nAnc);
+    qRest = wave_uniform(qRest);
+    int nSt = 0;   // receivers in the stash (the same on every lane)
     for (int jb = c.j0; jb <= c.j1; jb += 64) {
+      // lane: row jb + lane, whose cells i0..i1 hold the receivers qa..qa+cnt-1 (contiguous)
       const int j = jb + lane;
-      int i0 = 0, i1 = -1;
-      if (j > c.j1 || !row_range(A, c, j, &i0, &i1)) {
-        i0 = 0;
-        i1 = -1;
+      int i0 = 0, i1 = -1, qa = 0, cnt = 0;
+      if (j <= c.j1 && row_range(A, c, j, &i0, &i1)) {
+        qa = cellStart(j * A.g.nx + i0);
+        cnt = cellStart(j * A.g.nx + i1 + 1) - qa;
       }
-      const int cnt = i1 - i0 + 1;
-      int incl = cnt;   // inclusive scan of the row lengths over the lanes (all lanes active)
+      int incl = cnt;   // inclusive scan of the rows' receiver counts over the lanes (all lanes active)
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
         const int v = __shfl_up(incl, off, 64);
         if (lane >= off) incl += v;
       }
       const int excl = incl - cnt;
-      const int total = __shfl(incl, 63, 64);
-      for (int base = 0; base < total; base += 64) {   // wave-uniform trip count
-        const int f = base + lane;
-        // the row holding flattened cell f: the last lane whose exclusive prefix is <= f (rows
-        // with no cells share their prefix with the next row, so they are never the last)
-        int r = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1) {
-          const int cand = r + step;
-          const int e = __shfl(excl, cand & 63, 64);
-          if (cand < 64 && e <= f) r = cand;
+      const int total = wave_uniform(__shfl(incl, 63, 64));
+      for (int cb = 0; cb < total; cb += kList) {   // wave-uniform
+        const int nCh = total - cb < kList ? total - cb : kList;
+        // the chunk's list: row j's receivers k with excl + k in [cb, cb + nCh)
+        {
+          const int k0 = cb - excl > 0 ? cb - excl : 0;
+          const int k1 = cb + nCh - excl < cnt ? cb + nCh - excl : cnt;
+          for (int k = k0; k < k1; k++) list[excl + k - cb] = qa + k;
         }
-        const int rowFirst = __shfl(i0, r, 64), rowExcl = __shfl(excl, r, 64);
-        if (f < total) {
-          const int cell = (jb + r) * A.g.nx + rowFirst + (f - rowExcl);
-          const int q1 = cellStart(cell + 1);
-          for (int q = cellStart(cell); q < q1; q++) capture_one_cached(B, c, B.cellRx[q], myTests, anc, nAnc, qRest);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        V3 C = v3(0.0, 0.0, 0.0);
+        int32_t r = 0;
+        if (lane < nCh) {
+          const RxCell& x = B.cellRx[list[lane]];
+          C = ld3(x.p);
+          r = x.r;
         }
+        for (int base = 0; base < nCh; base += 64) {   // wave-uniform trip count
+          // next step's receiver, loaded while this step's is tested
+          const int fN = base + 64 + lane;
+          V3 CN = v3(0.0, 0.0, 0.0);
+          int32_t rN = 0;
+          if (fN < nCh) {
+            const RxCell& x = B.cellRx[list[fN]];
+            CN = ld3(x.p);
+            rN = x.r;
+          }
+          const bool have = base + lane < nCh;
+          myTests += have ? 1 : 0;
+          const bool hit = have && captures_cached(B, c, C, anc, ancF, nAnc, qRest);
+          if (fuse) {
+            const unsigned long long m = __ballot(hit);
+            if (hit) stash[nSt + __popcll(m & lanesBelow)] = r;
+            nSt += __popcll(m);
+            if (nSt >= kStashFlush) {
+              dedup_stash(A, B, c, nSt, stash, anc, img, myRec, myCor, myUsed);
+              nSt = 0;
+            }
+          } else if (hit) {
+            append_record(B, c.node, r);
+            c.spilled = true;
+          }
+          C = CN;
+          r = rN;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
-    const unsigned long long em = __ballot(c.emitted);
-    if (em && lane == __ffsll(em) - 1) store_key(B, c);
+    if (fuse && nSt > 0) dedup_stash(A, B, c, nSt, stash, anc, img, myRec, myCor, myUsed);
+    const unsigned long long sp = __ballot(c.spilled);
+    if (sp && lane == __ffsll(sp) - 1) store_key(B, c);
   }
   flush_tests(B, myTests);
+  for (int off = 32; off > 0; off >>= 1) {
+    myRec += __shfl_xor(myRec, off, 64);
+    myCor += __shfl_xor(myCor, off, 64);
+    myUsed += __shfl_xor(myUsed, off, 64);
+  }
+  if (lane == 0) {
+    if (myRec) atomicAdd(&B.counters[8], myRec);
+    if (myCor) atomicAdd(&B.counters[9], myCor);
+    if (myUsed) atomicAdd(&B.counters[2], myUsed);
+  }
 }
 
 // Groups whose smallest order key has its root in [glLo, glHi) (those first seen in the window).
@@ -2103,7 +2355,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   size_t cap = 1;
   while (cap < (size_t)std::max<int64_t>(std::max<int64_t>(1 << 16, 4 * nOut), (int64_t)(tabCapHint / 4))) cap <<= 1;
   ENSURE(dLevels, kMaxDepth + 2);
-  ENSURE(dCounters, 8);
+  ENSURE(dCounters, 10);
   ENSURE(dTraceCounters, 8);
   const size_t budget = std::max<size_t>(tune.windowNodes > 0 ? (size_t)tune.windowNodes : kWindowNodes, 4096);
   // Bound of the tree size per root item: one node per interaction sequence, i.e. sum over the
@@ -2145,7 +2397,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       HIPCHK(hipMemsetAsync(dSeqTab.p, 0xFF, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dTabOrder.p, 0xFF, cap * sizeof(unsigned long long), stream));
       HIPCHK(hipMemsetAsync(dField.p, 0, 6 * sizeof(double) * nOut, stream));
-      HIPCHK(hipMemsetAsync(dCounters.p, 0, 8 * sizeof(unsigned long long), stream));
+      HIPCHK(hipMemsetAsync(dCounters.p, 0, 10 * sizeof(unsigned long long), stream));
     }
     unsigned long long usedBefore = 0, corBefore = 0;
     // Nodes per root item for sizing windows: the last trace's densest window, then the densest
@@ -2215,6 +2467,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       CA.g = g;
       CA.invCell = 1.0 / g.cell;
       CA.tabMask = cap - 1;
+      CA.fuse = anyOcclusion ? 0 : 1;   // the occlusion test stays in k_dedup
       if (losWait) {
         HIPCHK(hipStreamWaitEvent(stream, evSide[1], 0));
         losWait = false;
@@ -2235,6 +2488,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         HIPCHK(hipMemsetAsync(dRecCount.p, 0, sizeof(unsigned long long) * kRecParts * kRecStride, stream));
         HIPCHK(hipMemsetAsync(dCounters.p + 3, 0, sizeof(unsigned long long), stream));
         HIPCHK(hipMemsetAsync(dCounters.p + 5, 0, 2 * sizeof(unsigned long long), stream));
+        HIPCHK(hipMemsetAsync(dCounters.p + 8, 0, 2 * sizeof(unsigned long long), stream));
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dSeqTab.p,
                        (unsigned long long)(cap - 1), dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p, nullptr,
                        count ? 1 : 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
@@ -2299,7 +2553,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       }
       HIPCHK(hipEventRecord(kev[1], stream));
       unsigned long long* ctr = pinSmall.at<unsigned long long>(kPinCapCtr);
-      HIPCHK(hipMemcpyAsync(ctr, dCounters.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(ctr, dCounters.p, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
       float kms = 0;
       HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
@@ -2309,8 +2563,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         tableFull = true;
         break;
       }
-      stats.capture_records += (int64_t)nRec;
-      stats.corrected += (int64_t)(ctr[1] - corBefore);
+      stats.capture_records += (int64_t)(nRec + ctr[8]);
+      stats.corrected += (int64_t)(ctr[1] - corBefore + ctr[9]);
       corBefore = ctr[1];
       const int64_t Wn = (int64_t)(ctr[2] - usedBefore);   // groups first seen in this window
       usedBefore = ctr[2];

@@ -18,3 +18,18 @@ def test_prefilter_decisions_sound(tmp_path):
     assert bad == 0, r.stdout
     # not vacuous: every kind of decision occurs many times
     assert cases > 500_000 and left > 100_000 and rpass > 10_000 and rmiss > 10_000 and tri_hits > 10_000, r.stdout
+
+
+def test_sphere_pretest_decisions_sound(tmp_path):
+    """Single-precision reception-sphere pre-test (sbrsolver_amd/csrc/capture_prefilter.h, host
+    build, one-ulp reciprocals and square roots) against the FP64 test on boundary-hugging,
+    end-of-segment, long and tiny cases (tests/sphere_prefilter_check.cpp): every decided case
+    agrees with the FP64 result, and both decisions are frequent."""
+    exe = str(tmp_path / "sphere_prefilter_check")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+                           "-I", os.path.join(ROOT, "sbrsolver_amd", "csrc"),
+                           os.path.join(ROOT, "tests", "sphere_prefilter_check.cpp"), "-o", exe])
+    r = subprocess.run([exe, "4000000"], capture_output=True, text=True, timeout=300)
+    cases, miss, hit, undecided, bad = map(int, r.stdout.split()[-5:])
+    assert bad == 0, r.stdout + r.stderr
+    assert cases == 4_000_000 and miss > 1_000_000 and hit > 500_000 and undecided < cases // 2, r.stdout
