@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "capture_prefilter.h"
 #include "sbr_capi.h"
 #include "sbrsolver/sbr_math.h"
 
@@ -138,6 +139,18 @@ struct alignas(128) TNode {
   int32_t childR;    // node index of the reflected child, -1 if none
 };
 static_assert(sizeof(TNode) == 128, "TNode layout");
+
+// What the capture's ancestor rule reads of a tree node, 160 B, held in LDS by the wave-per-node
+// kernel for the ancestors of its node: the capture half (src, tgt, base, and the hit ids packed in
+// one word), the single-precision pre-test of its reception sphere, and the box its sphere can
+// reach.
+struct alignas(16) AncRec {
+  double cap[8];
+  SphereF f;
+  float pad;
+  double box[6];
+};
+static_assert(sizeof(AncRec) == 160, "AncRec layout");
 
 // Per-depth work bookkeeping on the device.
 struct LevelRec {

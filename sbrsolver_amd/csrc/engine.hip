@@ -606,7 +606,8 @@ constexpr int kStash = 256;
 constexpr int kStashFlush = 128;
 static_assert(kStashFlush + 64 <= kStash, "stash bound");
 constexpr int kFuseDepth = 8;
-constexpr int kList = 512;
+constexpr int kList = 256;
+constexpr int kPend = 128;   // per wave: ring of receivers its node captures, before the ancestor rule
 
 // Returns false when no receiver can be captured by the node.
 __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n, NodeCtx& c) {
@@ -804,19 +805,25 @@ __device__ __forceinline__ void capture_one(const CaptureBufs& B, NodeCtx& c, co
   c.spilled = true;
 }
 
-// The candidate test of the wave-per-node kernel: the node captures receiver C and none of its
-// ancestors does.  The ancestors (capture halves, nearest first) are read from the wave's copy in
-// LDS (anc: 8 doubles each, ancF: their pre-tests), not walked through the tree again for every
-// receiver hit; qRest continues the walk in the tree past the copied ones.
-__device__ __forceinline__ bool captures_cached(const CaptureBufs& B, const NodeCtx& c, V3 C, const double* anc,
-                                                const SphereF* ancF, int nAnc, int32_t qRest) {
+// Candidate tests of the wave-per-node kernel.  The node's own reception sphere first; the receivers
+// it captures are then tested against the ancestors (the ancestor rule), read from the wave's copy
+// of their AncRecs in LDS (the box first: a receiver outside it is not captured by that ancestor).
+// qRest continues the walk in the tree past the copied ones.
+__device__ __forceinline__ bool own_captures(const NodeCtx& c, V3 C) {
   const int pv = sphere_pre(c.sf, c.sk, c.S, C);
-  if (pv < 0 || (pv == 0 && !sphere_hit(c.S, c.Tg, C, c.base, c.model, c.rc))) return false;
+  return pv > 0 || (pv == 0 && sphere_hit(c.S, c.Tg, C, c.base, c.model, c.rc));
+}
+
+__device__ __forceinline__ bool no_ancestor_captures(const CaptureBufs& B, const NodeCtx& c, V3 C, const AncRec* anc,
+                                                     int nAnc, int32_t qRest) {
   for (int j = 0; j < nAnc; j++) {
-    const double* a = anc + 8 * j;
-    const V3 Sa = v3(a[0], a[1], a[2]);
-    const int pa = sphere_pre(ancF[j], c.sk, Sa, C);
-    if (pa > 0 || (pa == 0 && sphere_hit(Sa, v3(a[3], a[4], a[5]), C, a[6], c.model, c.rc))) return false;
+    const AncRec& a = anc[j];
+    if (C.x < a.box[0] || C.x > a.box[3] || C.y < a.box[1] || C.y > a.box[4] || C.z < a.box[2] || C.z > a.box[5])
+      continue;
+    const V3 Sa = v3(a.cap[0], a.cap[1], a.cap[2]);
+    const int pa = sphere_pre(a.f, c.sk, Sa, C);
+    if (pa > 0 || (pa == 0 && sphere_hit(Sa, v3(a.cap[3], a.cap[4], a.cap[5]), C, a.cap[6], c.model, c.rc)))
+      return false;
   }
   for (int32_t q = qRest; q >= 0;) {
     const TNode& a = B.tree[q];
@@ -824,6 +831,26 @@ __device__ __forceinline__ bool captures_cached(const CaptureBufs& B, const Node
     q = a.parent;
   }
   return true;
+}
+
+// Box of the points a segment's reception sphere can reach: the segment's box grown by
+// (base + max(|S|, |T|)) rc (model 0: |cp| <= max(|S|, |T|) on the segment) or (base + |T - S|) rc,
+// widened by 1e-6 of itself and 1e-9 of the magnitudes involved (far above the FP64 test's rounding).
+__device__ __forceinline__ void capture_box(V3 S, V3 T, double base, int model, double rc, double* bx) {
+  const double L = vnorm(vsub(T, S));
+  const double m = fmax(vnorm(S), vnorm(T));
+  const double R = (base + (model == 0 ? m : L)) * rc;
+  const double g = R * (1.0 + 1e-6) + 1e-9 * (m + L + base) + 1e-30;
+  bx[0] = fmin(S.x, T.x) - g;
+  bx[1] = fmin(S.y, T.y) - g;
+  bx[2] = fmin(S.z, T.z) - g;
+  bx[3] = fmax(S.x, T.x) + g;
+  bx[4] = fmax(S.y, T.y) + g;
+  bx[5] = fmax(S.z, T.z) + g;
+  if (!(g < 1e300)) {   // no bound (infinite or NaN radius): every receiver is in the box
+    bx[0] = bx[1] = bx[2] = -INFINITY;
+    bx[3] = bx[4] = bx[5] = INFINITY;
+  }
 }
 
 // After a node's candidates: the first lane of the wave that emitted records for it stores its key
@@ -1013,29 +1040,30 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
   flush_tests(B, myTests);
 }
 
-// The wave-per-node kernel's copy of a node's ancestors: 8 doubles each, nearest first, the capture
-// half (src, tgt, base) and, in the last word, the ancestor's hit triangle (low 32 bits), coplanar id
-// (bits 32-61) and interaction type (bits 62-63).
+// Hit triangle (low 32 bits), coplanar id (bits 32-61) and interaction type (bits 62-63) of a tree
+// node, packed in one word of its AncRec.
 __device__ __forceinline__ double anc_ids(const TNode& a) {
   const unsigned long long w = (unsigned long long)(uint32_t)a.tri |
                                ((unsigned long long)(((uint32_t)a.cop & ((1u << 30) - 1)) | ((uint32_t)a.type << 30)) << 32);
   return __longlong_as_double((long long)w);
 }
-__device__ __forceinline__ int32_t anc_tri(const double* a) { return (int32_t)(uint32_t)__double_as_longlong(a[7]); }
-__device__ __forceinline__ int32_t anc_cop(const double* a) {
-  return (int32_t)(((unsigned long long)__double_as_longlong(a[7]) >> 32) & ((1u << 30) - 1));
+__device__ __forceinline__ int32_t anc_tri(const AncRec& a) { return (int32_t)(uint32_t)__double_as_longlong(a.cap[7]); }
+__device__ __forceinline__ int32_t anc_cop(const AncRec& a) {
+  return (int32_t)(((unsigned long long)__double_as_longlong(a.cap[7]) >> 32) & ((1u << 30) - 1));
 }
-__device__ __forceinline__ int anc_type(const double* a) { return (int)((unsigned long long)__double_as_longlong(a[7]) >> 62); }
+__device__ __forceinline__ int anc_type(const AncRec& a) {
+  return (int)((unsigned long long)__double_as_longlong(a.cap[7]) >> 62);
+}
 
 // node_key's sequence id from the ancestors in LDS (every ancestor has a hit, so its coplanar id
 // fits the 30 bits kept): the same elements, interned in the same order.
-__device__ int64_t node_seq_lds(const CaptureBufs& B, const NodeCtx& c, const double* anc, int nodeType) {
+__device__ int64_t node_seq_lds(const CaptureBufs& B, const NodeCtx& c, const AncRec* anc, int nodeType) {
   const bool typed = B.txs[c.txi].typedDedup != 0;
   int64_t id = seq_root(c.txi);
   for (int k = 0; k < c.depth && id != kSeqNone; k++) {
     // element k: the coplanar id of the ancestor at depth k, typed by the interaction at depth k + 1
-    const int t = k + 1 == c.depth ? nodeType : anc_type(anc + 8 * (c.depth - 2 - k));
-    id = seq_intern(B, id, seq_elem(anc_cop(anc + 8 * (c.depth - 1 - k)), typed ? t : 0));
+    const int t = k + 1 == c.depth ? nodeType : anc_type(anc[c.depth - 2 - k]);
+    id = seq_intern(B, id, seq_elem(anc_cop(anc[c.depth - 1 - k]), typed ? t : 0));
   }
   return id;
 }
@@ -1044,14 +1072,14 @@ __device__ int64_t node_seq_lds(const CaptureBufs& B, const NodeCtx& c, const do
 // 1..depth: image source m, then v1, e1, e2 of the surface of interaction m).  The image sources are
 // the prefixes of one pass over the path, the same operations image_source repeats for every k.
 // Returns false if an interaction has no surface (no corrected path then).
-__device__ bool image_chain_lds(const CaptureBufs& B, const NodeCtx& c, const double* anc, int nodeType, double* img) {
+__device__ bool image_chain_lds(const CaptureBufs& B, const NodeCtx& c, const AncRec* anc, int nodeType, double* img) {
   const TxDesc& T = B.txs[c.txi];
   V3 cur = v3(T.tx[0], T.tx[1], T.tx[2]);
   for (int m = 1; m <= c.depth; m++) {
-    const int32_t sid = anc_tri(anc + 8 * (c.depth - m));   // hit surface of the node at depth m - 1
+    const int32_t sid = anc_tri(anc[c.depth - m]);   // hit surface of the node at depth m - 1
     if (sid < 0) return false;
     const DSurf& sf = B.surf[sid];
-    const int t = m == c.depth ? nodeType : anc_type(anc + 8 * (c.depth - 1 - m));
+    const int t = m == c.depth ? nodeType : anc_type(anc[c.depth - 1 - m]);
     if (t == 1) cur = image_point(sf, cur);
     double* o = img + 12 * (m - 1);
     o[0] = cur.x; o[1] = cur.y; o[2] = cur.z;
@@ -1084,7 +1112,7 @@ __device__ __forceinline__ bool corrected_chain(const double* img, int depth, V3
 // control flow is uniform): the node's key and image chain are taken once (first lane), and the
 // lanes share the receivers: prune, correction, insertion, as k_dedup does for a record.
 __device__ void dedup_stash(const CaptureArgs& A, const CaptureBufs& B, NodeCtx& c, int n, const int32_t* stash,
-                            const double* anc, double* img, unsigned long long& myRec, unsigned long long& myCor,
+                            const AncRec* anc, double* img, unsigned long long& myRec, unsigned long long& myCor,
                             unsigned long long& myUsed) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1125,11 +1153,17 @@ __device__ void dedup_stash(const CaptureArgs& A, const CaptureBufs& B, NodeCtx&
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Wave per heavy node.  The node's rows are taken 64 at a time, one row's cell range per lane; the
-// receivers of a row's range are contiguous in cell order, and a wave scan of their counts flattens
-// them, to be dealt to the lanes one each (a lane finds its row by a binary search over the scan;
-// neighbouring lanes read neighbouring receivers), so a node costs about receivers / 64 steps
-// whatever the shape of its region.
+// AncRec of tree node a (transmitter T).
+__device__ __forceinline__ void anc_rec(const TNode& a, const TxDesc& T, AncRec& r) {
+  r.cap[0] = a.src[0]; r.cap[1] = a.src[1]; r.cap[2] = a.src[2];
+  r.cap[3] = a.tgt[0]; r.cap[4] = a.tgt[1]; r.cap[5] = a.tgt[2];
+  r.cap[6] = a.base;
+  r.cap[7] = anc_ids(a);
+  r.f = sphere_f(ld3(a.src), ld3(a.tgt), a.base, sphere_k(T.rcoef, T.receptionModel));
+  r.pad = 0.0f;
+  capture_box(ld3(a.src), ld3(a.tgt), a.base, T.receptionModel, T.rcoef, r.box);
+}
+
 // The same value on every lane, as the compiler can see (kept in scalar registers).
 __device__ __forceinline__ float wave_uniform(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
@@ -1178,7 +1212,9 @@ __device__ __forceinline__ void ctx_uniform(NodeCtx& c) {
 // receivers of a row's range are contiguous in cell order.  A wave scan of their counts places them
 // in a list in LDS (kList at a time), from which they are dealt to the lanes one each, a step's
 // receivers loaded while the previous step's are tested (neighbouring lanes read neighbouring
-// receivers), so a node costs about receivers / 64 steps whatever the shape of its region.
+// receivers), so a node costs about receivers / 64 steps whatever the shape of its region.  The
+// receivers the node's own sphere captures gather in a ring in LDS and meet the ancestor rule 64 at
+// a time; those left gather in the stash for the duplicate removal (or become records).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_capture_heavy(CaptureArgs A,
                                                                                    CaptureBufs B) {
   extern __shared__ uint16_t sCell[];
@@ -1196,14 +1232,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   const int64_t wave = wave_uniform((int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   unsigned long long myTests = 0;
-  __shared__ double sAnc[4][8 * kMaxDepth];   // per wave: the current node's ancestors
-  __shared__ SphereF sAncF[4][kMaxDepth];      // and their pre-tests
-  __shared__ double sImg[4][12 * kFuseDepth];  // per wave: its image chain
-  __shared__ int32_t sStash[4][kStash];        // per wave: receivers it captured, awaiting dedup
-  __shared__ int32_t sList[4][kList];          // per wave: receivers of the current chunk
-  double* anc = sAnc[threadIdx.x >> 6];
-  SphereF* ancF = sAncF[threadIdx.x >> 6];
+  // per wave: the current node's nearest ancestors (kFuseDepth at most) with their pre-tests and
+  // boxes, its image chain, the ring, the stash and the list
+  __shared__ AncRec sAnc[4][kFuseDepth];
+  __shared__ int32_t sAncIdx[4][kFuseDepth];
+  __shared__ double sImg[4][12 * kFuseDepth];
+  __shared__ int32_t sPend[4][kPend];
+  __shared__ int32_t sStash[4][kStash];
+  __shared__ int32_t sList[4][kList];
+  AncRec* anc = sAnc[threadIdx.x >> 6];
+  int32_t* ancIdx = sAncIdx[threadIdx.x >> 6];
   double* img = sImg[threadIdx.x >> 6];
+  int32_t* pend = sPend[threadIdx.x >> 6];
   int32_t* stash = sStash[threadIdx.x >> 6];
   int32_t* list = sList[threadIdx.x >> 6];
   unsigned long long myRec = 0, myCor = 0, myUsed = 0;
@@ -1211,34 +1251,67 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     NodeCtx c;
     if (!node_setup(A, B, B.heavy[h], c)) continue;    // same outcome on every lane
     ctx_uniform(c);
-    // The first lane copies the ancestor chain to the wave's LDS.  A wave's LDS accesses complete in
-    // order, and the fences and wave barriers keep the compiler from moving the other lanes' reads
-    // (after the exchange of the count) above the copy or the next node's copy above them.
+    // The first lane walks the ancestor chain (nearest first, kFuseDepth at most) and lists it in the
+    // wave's LDS; lane k then writes the AncRec of ancestor k there.  A wave's LDS accesses complete
+    // in order, and the fences and wave barriers keep the compiler from moving the other lanes' reads
+    // above the writes they follow, or the next node's writes above them.
     int nAnc = 0;
     int32_t qRest = c.parent;
     const bool fuse = A.fuse != 0 && c.depth <= kFuseDepth;   // the same on every lane
     __builtin_amdgcn_wave_barrier();
     if (lane == 0) {
-      while (qRest >= 0 && nAnc < kMaxDepth) {
-        const TNode& a = B.tree[qRest];
-        double* o = anc + 8 * nAnc;
-        o[0] = a.src[0]; o[1] = a.src[1]; o[2] = a.src[2];
-        o[3] = a.tgt[0]; o[4] = a.tgt[1]; o[5] = a.tgt[2];
-        o[6] = a.base;
-        o[7] = anc_ids(a);
-        ancF[nAnc] = sphere_f(ld3(a.src), ld3(a.tgt), a.base, c.sk);
-        nAnc++;
-        qRest = a.parent;
+      while (qRest >= 0 && nAnc < kFuseDepth) {
+        ancIdx[nAnc++] = qRest;
+        qRest = B.tree[qRest].parent;
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    nAnc = __shfl(nAnc, 0, 64);
+    nAnc = wave_uniform(__shfl(nAnc, 0, 64));
     qRest = __shfl(qRest, 0, 64);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane < nAnc) anc_rec(B.tree[ancIdx[lane]], B.txs[c.txi], anc[lane]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     nAnc = wave_uniform(nAnc);
     qRest = wave_uniform(qRest);
-    int nSt = 0;   // receivers in the stash (the same on every lane)
+    const bool anyAnc = nAnc > 0 || qRest >= 0;
+    int nSt = 0;              // receivers in the stash
+    int pHead = 0, pTail = 0;  // ring: receivers waiting for the ancestor rule (all the same on every lane)
+    // receivers that passed the ancestor rule (keep): to the stash, or records
+    auto emit = [&](bool keep, int32_t r) {
+      if (fuse) {
+        const unsigned long long m = __ballot(keep);
+        if (keep) stash[nSt + __popcll(m & lanesBelow)] = r;
+        nSt += __popcll(m);
+        if (nSt >= kStashFlush) {
+          dedup_stash(A, B, c, nSt, stash, anc, img, myRec, myCor, myUsed);
+          nSt = 0;
+        }
+      } else if (keep) {
+        append_record(B, c.node, r);
+        c.spilled = true;
+      }
+    };
+    // the ancestor rule for the n (<= 64) receivers at the head of the ring
+    auto ancestor_rule = [&](int n) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      bool keep = false;
+      int32_t r = 0;
+      if (lane < n) {
+        const RxCell& x = B.cellRx[pend[(pHead + lane) & (kPend - 1)]];
+        r = x.r;
+        keep = no_ancestor_captures(B, c, ld3(x.p), anc, nAnc, qRest);
+      }
+      pHead += n;
+      emit(keep, r);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     for (int jb = c.j0; jb <= c.j1; jb += 64) {
       // lane: row jb + lane, whose cells i0..i1 hold the receivers qa..qa+cnt-1 (contiguous)
       const int j = jb + lane;
@@ -1267,9 +1340,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         V3 C = v3(0.0, 0.0, 0.0);
-        int32_t r = 0;
+        int32_t r = 0, q = 0;
         if (lane < nCh) {
-          const RxCell& x = B.cellRx[list[lane]];
+          q = list[lane];
+          const RxCell& x = B.cellRx[q];
           C = ld3(x.p);
           r = x.r;
         }
@@ -1277,35 +1351,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           // next step's receiver, loaded while this step's is tested
           const int fN = base + 64 + lane;
           V3 CN = v3(0.0, 0.0, 0.0);
-          int32_t rN = 0;
+          int32_t rN = 0, qN = 0;
           if (fN < nCh) {
-            const RxCell& x = B.cellRx[list[fN]];
+            qN = list[fN];
+            const RxCell& x = B.cellRx[qN];
             CN = ld3(x.p);
             rN = x.r;
           }
           const bool have = base + lane < nCh;
           myTests += have ? 1 : 0;
-          const bool hit = have && captures_cached(B, c, C, anc, ancF, nAnc, qRest);
-          if (fuse) {
+          const bool hit = have && own_captures(c, C);
+          if (anyAnc) {
             const unsigned long long m = __ballot(hit);
-            if (hit) stash[nSt + __popcll(m & lanesBelow)] = r;
-            nSt += __popcll(m);
-            if (nSt >= kStashFlush) {
-              dedup_stash(A, B, c, nSt, stash, anc, img, myRec, myCor, myUsed);
-              nSt = 0;
-            }
-          } else if (hit) {
-            append_record(B, c.node, r);
-            c.spilled = true;
+            if (hit) pend[(pTail + __popcll(m & lanesBelow)) & (kPend - 1)] = q;
+            pTail += __popcll(m);
+            if (pTail - pHead >= 64) ancestor_rule(64);
+          } else {
+            emit(hit, r);
           }
           C = CN;
           r = rN;
+          q = qN;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
+    if (pTail > pHead) ancestor_rule(pTail - pHead);
     if (fuse && nSt > 0) dedup_stash(A, B, c, nSt, stash, anc, img, myRec, myCor, myUsed);
     const unsigned long long sp = __ballot(c.spilled);
     if (sp && lane == __ffsll(sp) - 1) store_key(B, c);
