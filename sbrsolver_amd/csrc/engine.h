@@ -152,6 +152,19 @@ struct alignas(16) AncRec {
 };
 static_assert(sizeof(AncRec) == 160, "AncRec layout");
 
+// A node left to the wave-per-node capture, 128 B: what the thread-per-node pass computed for it
+// (the segment clipped to the receivers' region, the search radius and rows, the sphere pre-test).
+struct alignas(16) HeavyJob {
+  double P0[3], P1[3];
+  double R, invDy;
+  SphereF sf;
+  int32_t node;
+  int32_t j0, j1;
+  int32_t all;
+  int32_t pad;
+};
+static_assert(sizeof(HeavyJob) == 128, "HeavyJob layout");
+
 // Per-depth work bookkeeping on the device.
 struct LevelRec {
   unsigned long long base;   // node index of the level's first node
@@ -280,7 +293,7 @@ struct Engine {
   DBuf<uint8_t> dLos;
   DBuf<unsigned long long> dTabA, dSeqTab, dTabOrder;   // group table keys, sequence trie, order keys
   DBuf<int32_t> dTabRx;
-  DBuf<int32_t> dHeavy;          // nodes left to the wave-per-node capture
+  DBuf<HeavyJob> dHeavy;         // nodes left to the wave-per-node capture
   DBuf<int32_t> dCand;           // nodes whose segment may reach the receivers' capture region
   DBuf<uint8_t> dCandFlag;       // per node: 1 if listed (written by the bounce kernel)
   DBuf<int32_t> dTileCount, dTileOffset;   // compaction of the flags: per-tile counts and offsets

@@ -89,6 +89,7 @@ template struct DBuf<DNode>;
 template struct DBuf<DTri>;
 template struct DBuf<DBox32>;
 template struct DBuf<DSub>;
+template struct DBuf<HeavyJob>;
 template struct DBuf<DNodeF>;
 template struct DBuf<int2>;
 template struct DBuf<DSurf>;
@@ -575,7 +576,7 @@ struct CaptureBufs {
   // [7] sequence-table slots used [8] records and [9] corrections of the wave-per-node kernel's own
   // duplicate removal (cleared with each candidate pass)
   unsigned long long* counters;
-  int32_t* heavy;
+  HeavyJob* heavy;                // nodes left to the wave-per-node kernel
   // (node, receiver) records of the candidate pass, in kRecParts partitions of recPart slots each
   // (appends spread over that many counters, kRecStride words apart, instead of contending on one)
   int2* rec;
@@ -597,17 +598,14 @@ struct CaptureBufs {
   int32_t stackDepth;
 };
 
-// The wave-per-node kernel removes duplicates of its own nodes' captures: the receivers a node
-// captures wait in the wave's stash (kStash entries in LDS) and are deduplicated kStashFlush or more
-// at a time, against the node's image chain (kFuseDepth interactions at most) held in LDS.  A step
-// adds at most 64, so the stash does not fill.  The receivers of a chunk of the node's rows are
-// listed in LDS first (kList per chunk), so that each step's receivers are known a step ahead.
-constexpr int kStash = 256;
-constexpr int kStashFlush = 128;
-static_assert(kStashFlush + 64 <= kStash, "stash bound");
+// The wave-per-node kernel removes duplicates of its own nodes' records, against the node's image
+// chain (kFuseDepth interactions at most) held in LDS.  The receivers of a chunk of the node's rows
+// are listed in LDS first (kList per chunk), so that each step's receivers are known a step ahead;
+// those the node's own sphere captures wait in a ring (kPend) for the ancestor rule, 64 at a time.
 constexpr int kFuseDepth = 8;
 constexpr int kList = 256;
-constexpr int kPend = 128;   // per wave: ring of receivers its node captures, before the ancestor rule
+constexpr int kPend = 128;
+static_assert(kPend >= 127, "ring bound: under 64 waiting plus a step's 64");
 
 // Returns false when no receiver can be captured by the node.
 __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n, NodeCtx& c) {
@@ -644,6 +642,35 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
   }
   c.losBase = (int64_t)c.txi * A.nrx;
   return true;
+}
+
+// node_setup for a node that the thread-per-node pass found heavy: its results from the job.
+__device__ __forceinline__ void node_from_job(const CaptureArgs& A, const CaptureBufs& B, const HeavyJob& J,
+                                              NodeCtx& c) {
+  c.node = J.node;
+  const TNode& nd = B.tree[c.node];
+  const TxDesc& T = B.txs[nd.txi];
+  c.parent = nd.parent;
+  c.depth = nd.depth;
+  c.txi = nd.txi;
+  c.model = T.receptionModel;
+  c.rc = T.rcoef;
+  c.base = nd.base;
+  c.keyed = false;
+  c.emitted = false;
+  c.spilled = false;
+  c.S = ld3(nd.src);
+  c.Tg = ld3(nd.tgt);
+  c.sk = sphere_k(c.rc, c.model);
+  c.sf = J.sf;
+  c.P0 = ld3(J.P0);
+  c.P1 = ld3(J.P1);
+  c.R = J.R;
+  c.invDy = J.invDy;
+  c.all = J.all != 0;
+  c.j0 = J.j0;
+  c.j1 = J.j1;
+  c.losBase = (int64_t)c.txi * A.nrx;
 }
 
 // Id of the sequence (parent sequence id, element) in the trie: the slot holding that exact key,
@@ -1025,7 +1052,19 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
       }
       if (heavy) {
         const unsigned long long h = atomicAdd(&B.counters[5], 1ull);
-        B.heavy[h] = (int32_t)n;
+        HeavyJob& J = B.heavy[h];
+        for (int a = 0; a < 3; a++) {
+          J.P0[a] = vget(c.P0, a);
+          J.P1[a] = vget(c.P1, a);
+        }
+        J.R = c.R;
+        J.invDy = c.invDy;
+        J.sf = c.sf;
+        J.node = (int32_t)n;
+        J.j0 = c.j0;
+        J.j1 = c.j1;
+        J.all = c.all ? 1 : 0;
+        J.pad = 0;
       } else {
         for (int j = c.j0; j <= c.j1; j++) {   // receivers of a row's cells i0..i1 are contiguous
           int i0, i1;
@@ -1108,49 +1147,41 @@ __device__ __forceinline__ bool corrected_chain(const double* img, int depth, V3
   return true;
 }
 
-// Duplicate removal of the n receivers in the wave's stash (called by the whole wave, where its
-// control flow is uniform): the node's key and image chain are taken once (first lane), and the
-// lanes share the receivers: prune, correction, insertion, as k_dedup does for a record.
-__device__ void dedup_stash(const CaptureArgs& A, const CaptureBufs& B, NodeCtx& c, int n, const int32_t* stash,
-                            const AncRec* anc, double* img, unsigned long long& myRec, unsigned long long& myCor,
-                            unsigned long long& myUsed) {
+// The node's dedup key and image chain, taken once, by the first lane, when the wave first holds a
+// record of the node (called by the whole wave, where its control flow is uniform).
+__device__ void node_key_wave(const CaptureBufs& B, NodeCtx& c, const AncRec* anc, double* img) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int lane = threadIdx.x & 63;
-  if (!c.keyed) {   // the same on every lane
-    int64_t seq = 0;
-    int ok = 0;
-    if (lane == 0) {
-      const int nodeType = B.tree[c.node].type;
-      seq = node_seq_lds(B, c, anc, nodeType);
-      ok = c.depth == 0 || image_chain_lds(B, c, anc, nodeType, img) ? 1 : 0;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    c.seqId = __shfl(seq, 0, 64);
-    c.chainOk = __shfl(ok, 0, 64) != 0;
-    c.orderKey = order_key(B.tree[c.node]);
-    c.keyed = true;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  }
-  if (lane == 0) myRec += (unsigned long long)n;
-  if (c.seqId != kSeqNone) {   // a full sequence table is flagged and the trace redone
-    for (int e = lane; e < n; e += 64) {
-      const int32_t r = stash[e];
-      const unsigned long long gk = group_key(c.seqId, r);
-      if (!group_may_win(A, B, gk, c.orderKey)) continue;
-      const V3 C = ld3(B.rx + 3 * (int64_t)r);
-      const bool ok = c.depth == 0 ? B.los[c.losBase + r] != 0 : (c.chainOk && corrected_chain(img, c.depth, C));
-      if (ok) {
-        myCor++;
-        myUsed += group_insert(A, B, gk, c.orderKey, c.losBase + r);
-      }
-    }
+  int64_t seq = 0;
+  int ok = 0;
+  if (lane == 0) {
+    const int nodeType = B.tree[c.node].type;
+    seq = node_seq_lds(B, c, anc, nodeType);
+    ok = c.depth == 0 || image_chain_lds(B, c, anc, nodeType, img) ? 1 : 0;
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
+  c.seqId = __shfl(seq, 0, 64);
+  c.chainOk = __shfl(ok, 0, 64) != 0;
+  c.orderKey = order_key(B.tree[c.node]);
+  c.keyed = true;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Duplicate removal of a record (node c, receiver r at C) in the wave-per-node kernel: the correction
+// against the image chain in LDS, then the prune and the insertion (k_dedup's steps in the other
+// order: here the correction rejects nearly every record, and it needs no memory access).
+__device__ __forceinline__ void dedup_one(const CaptureArgs& A, const CaptureBufs& B, const NodeCtx& c, V3 C,
+                                          int32_t r, const double* img, unsigned long long& myCor,
+                                          unsigned long long& myUsed) {
+  const bool ok = c.depth == 0 ? B.los[c.losBase + r] != 0 : (c.chainOk && corrected_chain(img, c.depth, C));
+  if (!ok || c.seqId == kSeqNone) return;   // a full sequence table is flagged and the trace redone
+  const unsigned long long gk = group_key(c.seqId, r);
+  if (!group_may_win(A, B, gk, c.orderKey)) return;
+  myCor++;
+  myUsed += group_insert(A, B, gk, c.orderKey, c.losBase + r);
 }
 
 // AncRec of tree node a (transmitter T).
@@ -1214,42 +1245,33 @@ __device__ __forceinline__ void ctx_uniform(NodeCtx& c) {
 // receivers loaded while the previous step's are tested (neighbouring lanes read neighbouring
 // receivers), so a node costs about receivers / 64 steps whatever the shape of its region.  The
 // receivers the node's own sphere captures gather in a ring in LDS and meet the ancestor rule 64 at
-// a time; those left gather in the stash for the duplicate removal (or become records).
+// a time; the records left are deduplicated at once (or appended, for k_dedup).
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_capture_heavy(CaptureArgs A,
                                                                                    CaptureBufs B) {
-  extern __shared__ uint16_t sCell[];
-  // The grid is sized for the most heavy nodes (their count is read here, on the device); blocks
-  // beyond the count leave before staging the cell starts.
-  if ((int64_t)blockIdx.x * (blockDim.x >> 6) >= (int64_t)B.counters[5]) return;
-  if (A.cellLds > 0) {
-    for (int k = threadIdx.x; k < A.cellLds; k += blockDim.x) sCell[k] = (uint16_t)B.cellStart[k];
-    __syncthreads();
-  }
-  auto cellStart = [&](int k) { return A.cellLds > 0 ? (int)sCell[k] : B.cellStart[k]; };
   const int64_t nHeavy = (int64_t)B.counters[5];   // appended by k_capture
   const int lane = threadIdx.x & 63;
   const unsigned long long lanesBelow = (1ull << lane) - 1ull;
   const int64_t wave = wave_uniform((int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   unsigned long long myTests = 0;
-  // per wave: the current node's nearest ancestors (kFuseDepth at most) with their pre-tests and
-  // boxes, its image chain, the ring, the stash and the list
+  // per wave: the current node's nearest ancestors (kFuseDepth at most) and their node indices, its
+  // image chain, the ring (positions and receiver indices) and the list
   __shared__ AncRec sAnc[4][kFuseDepth];
   __shared__ int32_t sAncIdx[4][kFuseDepth];
   __shared__ double sImg[4][12 * kFuseDepth];
-  __shared__ int32_t sPend[4][kPend];
-  __shared__ int32_t sStash[4][kStash];
+  __shared__ double sRingC[4][3 * kPend];
+  __shared__ int32_t sRingR[4][kPend];
   __shared__ int32_t sList[4][kList];
   AncRec* anc = sAnc[threadIdx.x >> 6];
   int32_t* ancIdx = sAncIdx[threadIdx.x >> 6];
   double* img = sImg[threadIdx.x >> 6];
-  int32_t* pend = sPend[threadIdx.x >> 6];
-  int32_t* stash = sStash[threadIdx.x >> 6];
+  double* ringC = sRingC[threadIdx.x >> 6];
+  int32_t* ringR = sRingR[threadIdx.x >> 6];
   int32_t* list = sList[threadIdx.x >> 6];
   unsigned long long myRec = 0, myCor = 0, myUsed = 0;
   for (int64_t h = wave; h < nHeavy; h += nWaves) {   // wave-uniform
     NodeCtx c;
-    if (!node_setup(A, B, B.heavy[h], c)) continue;    // same outcome on every lane
+    node_from_job(A, B, B.heavy[h], c);
     ctx_uniform(c);
     // The first lane walks the ancestor chain (nearest first, kFuseDepth at most) and lists it in the
     // wave's LDS; lane k then writes the AncRec of ancestor k there.  A wave's LDS accesses complete
@@ -1274,20 +1296,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    nAnc = wave_uniform(nAnc);
     qRest = wave_uniform(qRest);
     const bool anyAnc = nAnc > 0 || qRest >= 0;
-    int nSt = 0;              // receivers in the stash
-    int pHead = 0, pTail = 0;  // ring: receivers waiting for the ancestor rule (all the same on every lane)
-    // receivers that passed the ancestor rule (keep): to the stash, or records
-    auto emit = [&](bool keep, int32_t r) {
+    int pHead = 0, pTail = 0;  // ring: receivers waiting for the ancestor rule (the same on every lane)
+    // records (keep: node c captures receiver r at C, and no ancestor does)
+    auto record = [&](bool keep, V3 C, int32_t r) {
       if (fuse) {
         const unsigned long long m = __ballot(keep);
-        if (keep) stash[nSt + __popcll(m & lanesBelow)] = r;
-        nSt += __popcll(m);
-        if (nSt >= kStashFlush) {
-          dedup_stash(A, B, c, nSt, stash, anc, img, myRec, myCor, myUsed);
-          nSt = 0;
+        if (m != 0ull) {
+          if (!c.keyed) node_key_wave(B, c, anc, img);
+          if (lane == 0) myRec += (unsigned long long)__popcll(m);
+          if (keep) dedup_one(A, B, c, C, r, img, myCor, myUsed);
         }
       } else if (keep) {
         append_record(B, c.node, r);
@@ -1300,25 +1319,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       bool keep = false;
+      V3 C = v3(0.0, 0.0, 0.0);
       int32_t r = 0;
       if (lane < n) {
-        const RxCell& x = B.cellRx[pend[(pHead + lane) & (kPend - 1)]];
-        r = x.r;
-        keep = no_ancestor_captures(B, c, ld3(x.p), anc, nAnc, qRest);
+        const int e = (pHead + lane) & (kPend - 1);
+        C = v3(ringC[3 * e], ringC[3 * e + 1], ringC[3 * e + 2]);
+        r = ringR[e];
+        keep = no_ancestor_captures(B, c, C, anc, nAnc, qRest);
       }
       pHead += n;
-      emit(keep, r);
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      record(keep, C, r);
     };
     for (int jb = c.j0; jb <= c.j1; jb += 64) {
       // lane: row jb + lane, whose cells i0..i1 hold the receivers qa..qa+cnt-1 (contiguous)
       const int j = jb + lane;
       int i0 = 0, i1 = -1, qa = 0, cnt = 0;
       if (j <= c.j1 && row_range(A, c, j, &i0, &i1)) {
-        qa = cellStart(j * A.g.nx + i0);
-        cnt = cellStart(j * A.g.nx + i1 + 1) - qa;
+        qa = B.cellStart[j * A.g.nx + i0];
+        cnt = B.cellStart[j * A.g.nx + i1 + 1] - qa;
       }
       int incl = cnt;   // inclusive scan of the rows' receiver counts over the lanes (all lanes active)
 #pragma unroll
@@ -1340,10 +1361,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         V3 C = v3(0.0, 0.0, 0.0);
-        int32_t r = 0, q = 0;
+        int32_t r = 0;
         if (lane < nCh) {
-          q = list[lane];
-          const RxCell& x = B.cellRx[q];
+          const RxCell& x = B.cellRx[list[lane]];
           C = ld3(x.p);
           r = x.r;
         }
@@ -1351,10 +1371,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           // next step's receiver, loaded while this step's is tested
           const int fN = base + 64 + lane;
           V3 CN = v3(0.0, 0.0, 0.0);
-          int32_t rN = 0, qN = 0;
+          int32_t rN = 0;
           if (fN < nCh) {
-            qN = list[fN];
-            const RxCell& x = B.cellRx[qN];
+            const RxCell& x = B.cellRx[list[fN]];
             CN = ld3(x.p);
             rN = x.r;
           }
@@ -1363,15 +1382,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           const bool hit = have && own_captures(c, C);
           if (anyAnc) {
             const unsigned long long m = __ballot(hit);
-            if (hit) pend[(pTail + __popcll(m & lanesBelow)) & (kPend - 1)] = q;
+            if (hit) {
+              const int e = (pTail + __popcll(m & lanesBelow)) & (kPend - 1);
+              ringC[3 * e] = C.x;
+              ringC[3 * e + 1] = C.y;
+              ringC[3 * e + 2] = C.z;
+              ringR[e] = r;
+            }
             pTail += __popcll(m);
             if (pTail - pHead >= 64) ancestor_rule(64);
           } else {
-            emit(hit, r);
+            record(hit, C, r);
           }
           C = CN;
           r = rN;
-          q = qN;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -1379,7 +1403,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       }
     }
     if (pTail > pHead) ancestor_rule(pTail - pHead);
-    if (fuse && nSt > 0) dedup_stash(A, B, c, nSt, stash, anc, img, myRec, myCor, myUsed);
     const unsigned long long sp = __ballot(c.spilled);
     if (sp && lane == __ffsll(sp) - 1) store_key(B, c);
   }
@@ -2572,8 +2595,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         }
         // wave per heavy node; the count is read on the device (no host round trip)
         if (nodes > 0) {
-          k_capture_heavy<<<blocks_for((int64_t)numCU * 32 * 64, 256), 256, sizeof(uint16_t) * (size_t)CA.cellLds,
-                            stream>>>(CA, CB);
+          k_capture_heavy<<<blocks_for((int64_t)numCU * 32 * 64, 256), 256, 0, stream>>>(CA, CB);
           HIPCHK(hipGetLastError());
         }
         HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinCapCtr), dCounters.p, 7 * sizeof(unsigned long long),
