@@ -135,7 +135,10 @@ typedef struct {
 typedef struct {
   int64_t launches;
   int64_t bounces;          /* RayLaunch calls = interaction-tree nodes */
-  int64_t capture_records;  /* captured (launch, node, receiver) with no capturing ancestor */
+  int64_t capture_records;  /* (node, receiver) candidates handed to duplicate removal: captured by the
+                               node and by no ancestor; in the wave-per-node capture, which corrects
+                               before it applies the ancestor rule, only those whose image correction
+                               also succeeds */
   int64_t corrected;        /* records that survived image correction */
   int64_t winners;          /* paths kept after duplicate removal */
   int64_t max_depth;

@@ -551,6 +551,7 @@ struct NodeCtx {
   int32_t node, parent;
   bool all, keyed, emitted;
   bool spilled;    // some of its captures became records (k_dedup reads its key)
+  bool imgReady;   // its image chain is in LDS (wave-per-node kernel)
   bool chainOk;    // its image chain is held in LDS (wave-per-node kernel, once keyed)
   int64_t seqId;   // id of the path's coplanar-id sequence in the sequence trie (-1 - txi: empty)
   unsigned long long orderKey;
@@ -621,6 +622,7 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
   c.keyed = false;
   c.emitted = false;
   c.spilled = false;
+  c.imgReady = false;
   c.S = ld3(nd.src);
   c.Tg = ld3(nd.tgt);
   c.sk = sphere_k(c.rc, c.model);
@@ -659,6 +661,7 @@ __device__ __forceinline__ void node_from_job(const CaptureArgs& A, const Captur
   c.keyed = false;
   c.emitted = false;
   c.spilled = false;
+  c.imgReady = false;
   c.S = ld3(nd.src);
   c.Tg = ld3(nd.tgt);
   c.sk = sphere_k(c.rc, c.model);
@@ -1147,37 +1150,40 @@ __device__ __forceinline__ bool corrected_chain(const double* img, int depth, V3
   return true;
 }
 
-// The node's dedup key and image chain, taken once, by the first lane, when the wave first holds a
-// record of the node (called by the whole wave, where its control flow is uniform).
-__device__ void node_key_wave(const CaptureBufs& B, NodeCtx& c, const AncRec* anc, double* img) {
+// The node's image chain, and its dedup key, each taken once, by the first lane, when the wave first
+// needs it (called by the whole wave, where its control flow is uniform).
+__device__ void node_chain_wave(const CaptureBufs& B, NodeCtx& c, const AncRec* anc, double* img) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int lane = threadIdx.x & 63;
-  int64_t seq = 0;
   int ok = 0;
-  if (lane == 0) {
-    const int nodeType = B.tree[c.node].type;
-    seq = node_seq_lds(B, c, anc, nodeType);
-    ok = c.depth == 0 || image_chain_lds(B, c, anc, nodeType, img) ? 1 : 0;
-  }
+  if ((threadIdx.x & 63) == 0) ok = c.depth == 0 || image_chain_lds(B, c, anc, B.tree[c.node].type, img) ? 1 : 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
-  c.seqId = __shfl(seq, 0, 64);
   c.chainOk = __shfl(ok, 0, 64) != 0;
-  c.orderKey = order_key(B.tree[c.node]);
-  c.keyed = true;
+  c.imgReady = true;
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Duplicate removal of a record (node c, receiver r at C) in the wave-per-node kernel: the correction
-// against the image chain in LDS, then the prune and the insertion (k_dedup's steps in the other
-// order: here the correction rejects nearly every record, and it needs no memory access).
-__device__ __forceinline__ void dedup_one(const CaptureArgs& A, const CaptureBufs& B, const NodeCtx& c, V3 C,
-                                          int32_t r, const double* img, unsigned long long& myCor,
-                                          unsigned long long& myUsed) {
-  const bool ok = c.depth == 0 ? B.los[c.losBase + r] != 0 : (c.chainOk && corrected_chain(img, c.depth, C));
-  if (!ok || c.seqId == kSeqNone) return;   // a full sequence table is flagged and the trace redone
+__device__ void node_seq_wave(const CaptureBufs& B, NodeCtx& c, const AncRec* anc) {
+  int64_t seq = 0;
+  if ((threadIdx.x & 63) == 0) seq = node_seq_lds(B, c, anc, B.tree[c.node].type);
+  c.seqId = __shfl(seq, 0, 64);
+  c.orderKey = order_key(B.tree[c.node]);
+  c.keyed = true;
+}
+
+// The image correction of a candidate (node c, receiver r at C) in the wave-per-node kernel, against
+// the image chain in LDS (the line of sight for a node at depth 0).
+__device__ __forceinline__ bool corrected_fast(const CaptureBufs& B, const NodeCtx& c, V3 C, int32_t r,
+                                               const double* img) {
+  return c.depth == 0 ? B.los[c.losBase + r] != 0 : (c.chainOk && corrected_chain(img, c.depth, C));
+}
+
+// Insertion of a corrected record that passed the ancestor rule: the prune, then the group table.
+__device__ __forceinline__ void dedup_insert(const CaptureArgs& A, const CaptureBufs& B, const NodeCtx& c, int32_t r,
+                                             unsigned long long& myCor, unsigned long long& myUsed) {
+  if (c.seqId == kSeqNone) return;   // a full sequence table is flagged and the trace redone
   const unsigned long long gk = group_key(c.seqId, r);
   if (!group_may_win(A, B, gk, c.orderKey)) return;
   myCor++;
@@ -1244,8 +1250,12 @@ __device__ __forceinline__ void ctx_uniform(NodeCtx& c) {
 // in a list in LDS (kList at a time), from which they are dealt to the lanes one each, a step's
 // receivers loaded while the previous step's are tested (neighbouring lanes read neighbouring
 // receivers), so a node costs about receivers / 64 steps whatever the shape of its region.  The
-// receivers the node's own sphere captures gather in a ring in LDS and meet the ancestor rule 64 at
-// a time; the records left are deduplicated at once (or appended, for k_dedup).
+// receivers the node's own sphere captures gather in a ring in LDS and are taken 64 at a time.  A
+// path is kept for a receiver iff the node captures it, no ancestor does, its image correction
+// succeeds and it has the smallest order key of its group; the first three are independent tests of
+// the pair, so here the correction (against the image chain in LDS, no memory traffic) comes before
+// the ancestor rule, which then runs only for the few pairs it passes.  With records for k_dedup
+// (occlusion tests, deep nodes) the ancestor rule comes first and the records are appended.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_capture_heavy(CaptureArgs A,
                                                                                    CaptureBufs B) {
   const int64_t nHeavy = (int64_t)B.counters[5];   // appended by k_capture
@@ -1297,41 +1307,39 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     qRest = wave_uniform(qRest);
-    const bool anyAnc = nAnc > 0 || qRest >= 0;
-    int pHead = 0, pTail = 0;  // ring: receivers waiting for the ancestor rule (the same on every lane)
-    // records (keep: node c captures receiver r at C, and no ancestor does)
-    auto record = [&](bool keep, V3 C, int32_t r) {
-      if (fuse) {
-        const unsigned long long m = __ballot(keep);
-        if (m != 0ull) {
-          if (!c.keyed) node_key_wave(B, c, anc, img);
-          if (lane == 0) myRec += (unsigned long long)__popcll(m);
-          if (keep) dedup_one(A, B, c, C, r, img, myCor, myUsed);
-        }
-      } else if (keep) {
-        append_record(B, c.node, r);
-        c.spilled = true;
-      }
-    };
-    // the ancestor rule for the n (<= 64) receivers at the head of the ring
-    auto ancestor_rule = [&](int n) {
+    int pHead = 0, pTail = 0;  // ring: receivers the node captures, waiting (the same on every lane)
+    // the n (<= 64) receivers at the head of the ring
+    auto take = [&](int n) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      bool keep = false;
       V3 C = v3(0.0, 0.0, 0.0);
       int32_t r = 0;
       if (lane < n) {
         const int e = (pHead + lane) & (kPend - 1);
         C = v3(ringC[3 * e], ringC[3 * e + 1], ringC[3 * e + 2]);
         r = ringR[e];
-        keep = no_ancestor_captures(B, c, C, anc, nAnc, qRest);
       }
       pHead += n;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      record(keep, C, r);
+      if (fuse) {
+        if (!c.imgReady) node_chain_wave(B, c, anc, img);
+        bool ok = lane < n && corrected_fast(B, c, C, r, img);
+        if (__ballot(ok) != 0ull) {
+          ok = ok && no_ancestor_captures(B, c, C, anc, nAnc, qRest);
+          const unsigned long long m = __ballot(ok);
+          if (m != 0ull) {
+            if (!c.keyed) node_seq_wave(B, c, anc);
+            if (lane == 0) myRec += (unsigned long long)__popcll(m);
+            if (ok) dedup_insert(A, B, c, r, myCor, myUsed);
+          }
+        }
+      } else if (lane < n && no_ancestor_captures(B, c, C, anc, nAnc, qRest)) {
+        append_record(B, c.node, r);
+        c.spilled = true;
+      }
     };
     for (int jb = c.j0; jb <= c.j1; jb += 64) {
       // lane: row jb + lane, whose cells i0..i1 hold the receivers qa..qa+cnt-1 (contiguous)
@@ -1380,20 +1388,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           const bool have = base + lane < nCh;
           myTests += have ? 1 : 0;
           const bool hit = have && own_captures(c, C);
-          if (anyAnc) {
-            const unsigned long long m = __ballot(hit);
-            if (hit) {
-              const int e = (pTail + __popcll(m & lanesBelow)) & (kPend - 1);
-              ringC[3 * e] = C.x;
-              ringC[3 * e + 1] = C.y;
-              ringC[3 * e + 2] = C.z;
-              ringR[e] = r;
-            }
-            pTail += __popcll(m);
-            if (pTail - pHead >= 64) ancestor_rule(64);
-          } else {
-            record(hit, C, r);
+          const unsigned long long m = __ballot(hit);
+          if (hit) {
+            const int e = (pTail + __popcll(m & lanesBelow)) & (kPend - 1);
+            ringC[3 * e] = C.x;
+            ringC[3 * e + 1] = C.y;
+            ringC[3 * e + 2] = C.z;
+            ringR[e] = r;
           }
+          pTail += __popcll(m);
+          if (pTail - pHead >= 64) take(64);
           C = CN;
           r = rN;
         }
@@ -1402,7 +1406,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       }
     }
-    if (pTail > pHead) ancestor_rule(pTail - pHead);
+    if (pTail > pHead) take(pTail - pHead);
     const unsigned long long sp = __ballot(c.spilled);
     if (sp && lane == __ffsll(sp) - 1) store_key(B, c);
   }
