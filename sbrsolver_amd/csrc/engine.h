@@ -152,18 +152,21 @@ struct alignas(16) AncRec {
 };
 static_assert(sizeof(AncRec) == 160, "AncRec layout");
 
-// A node left to the wave-per-node capture, 128 B: what the thread-per-node pass computed for it
-// (the segment clipped to the receivers' region, the search radius and rows, the sphere pre-test).
+// A node left to the wave-per-node capture, 160 B: what the thread-per-node pass computed for it
+// (the segment clipped to the receivers' region, the search radius and rows, the sphere pre-test)
+// and its nearest ancestors (node indices, nearest first; nAnc of them, the walk continuing at qRest).
+constexpr int kJobAnc = 8;
 struct alignas(16) HeavyJob {
   double P0[3], P1[3];
   double R, invDy;
   SphereF sf;
   int32_t node;
   int32_t j0, j1;
-  int32_t all;
-  int32_t pad;
+  int32_t flags;   // bit 0: every cell is searched (no radius bound); bits 1..: nAnc
+  int32_t qRest;
+  int32_t anc[kJobAnc];
 };
-static_assert(sizeof(HeavyJob) == 128, "HeavyJob layout");
+static_assert(sizeof(HeavyJob) == 160, "HeavyJob layout");
 
 // Per-depth work bookkeeping on the device.
 struct LevelRec {

@@ -670,7 +670,7 @@ __device__ __forceinline__ void node_from_job(const CaptureArgs& A, const Captur
   c.P1 = ld3(J.P1);
   c.R = J.R;
   c.invDy = J.invDy;
-  c.all = J.all != 0;
+  c.all = (J.flags & 1) != 0;
   c.j0 = J.j0;
   c.j1 = J.j1;
   c.losBase = (int64_t)c.txi * A.nrx;
@@ -1066,8 +1066,14 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
         J.node = (int32_t)n;
         J.j0 = c.j0;
         J.j1 = c.j1;
-        J.all = c.all ? 1 : 0;
-        J.pad = 0;
+        int nA = 0;
+        int32_t q = c.parent;
+        while (q >= 0 && nA < kJobAnc) {
+          J.anc[nA++] = q;
+          q = B.tree[q].parent;
+        }
+        J.flags = (c.all ? 1 : 0) | (nA << 1);
+        J.qRest = q;
       } else {
         for (int j = c.j0; j <= c.j1; j++) {   // receivers of a row's cells i0..i1 are contiguous
           int i0, i1;
@@ -1267,13 +1273,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   // per wave: the current node's nearest ancestors (kFuseDepth at most) and their node indices, its
   // image chain, the ring (positions and receiver indices) and the list
   __shared__ AncRec sAnc[4][kFuseDepth];
-  __shared__ int32_t sAncIdx[4][kFuseDepth];
   __shared__ double sImg[4][12 * kFuseDepth];
   __shared__ double sRingC[4][3 * kPend];
   __shared__ int32_t sRingR[4][kPend];
   __shared__ int32_t sList[4][kList];
   AncRec* anc = sAnc[threadIdx.x >> 6];
-  int32_t* ancIdx = sAncIdx[threadIdx.x >> 6];
   double* img = sImg[threadIdx.x >> 6];
   double* ringC = sRingC[threadIdx.x >> 6];
   int32_t* ringR = sRingR[threadIdx.x >> 6];
@@ -1283,26 +1287,19 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     NodeCtx c;
     node_from_job(A, B, B.heavy[h], c);
     ctx_uniform(c);
-    // The first lane walks the ancestor chain (nearest first, kFuseDepth at most) and lists it in the
-    // wave's LDS; lane k then writes the AncRec of ancestor k there.  A wave's LDS accesses complete
-    // in order, and the fences and wave barriers keep the compiler from moving the other lanes' reads
-    // above the writes they follow, or the next node's writes above them.
-    int nAnc = 0;
-    int32_t qRest = c.parent;
+    // The nearest ancestors (kFuseDepth at most, listed in the job): lane k writes the AncRec of
+    // ancestor k to the wave's LDS.  A wave's LDS accesses complete in order, and the fences and wave
+    // barriers keep the compiler from moving the other lanes' reads above the writes they follow, or
+    // the next node's writes above them.
+    static_assert(kFuseDepth == kJobAnc, "the job lists the ancestors held in LDS");
     const bool fuse = A.fuse != 0 && c.depth <= kFuseDepth;   // the same on every lane
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) {
-      while (qRest >= 0 && nAnc < kFuseDepth) {
-        ancIdx[nAnc++] = qRest;
-        qRest = B.tree[qRest].parent;
-      }
-    }
+    const HeavyJob& J = B.heavy[h];
+    const int nAnc = wave_uniform(J.flags >> 1);
+    int32_t qRest = J.qRest;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    nAnc = wave_uniform(__shfl(nAnc, 0, 64));
-    qRest = __shfl(qRest, 0, 64);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (lane < nAnc) anc_rec(B.tree[ancIdx[lane]], B.txs[c.txi], anc[lane]);
+    if (lane < nAnc) anc_rec(B.tree[J.anc[lane]], B.txs[c.txi], anc[lane]);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
