@@ -8,6 +8,7 @@
 
 #include "engine.h"
 #include "capture_prefilter.h"
+#include "correction_prefilter.h"
 #include "prefilter.h"
 
 namespace sbr {

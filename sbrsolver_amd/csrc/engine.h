@@ -303,6 +303,7 @@ struct Engine {
   DBuf<int2> dRec;               // (node, receiver) records of the capture's candidate pass
   DBuf<unsigned long long> dRecCount, dRecPrefix;   // per-partition record counts / prefix
   DBuf<unsigned long long> dNodeKey;   // dedup and order keys of the nodes with records
+  DBuf<double> dImg;             // image source of each tree node (reference-sphere traces)
   size_t recCap = 0;
   DBuf<unsigned long long> dCounters;        // capture: records, corrected, groups, flags, collected, heavy
   DBuf<unsigned long long> dTraceCounters;   // bounce loop: overflow flags, node visits, triangle tests

@@ -557,6 +557,8 @@ struct NodeCtx {
   unsigned long long orderKey;
   int64_t losBase;
   int j0, j1;
+  ConeF cone;      // the correction's first-step pre-test (state 0: off)
+  V3 I;            // the node's image source
 };
 
 struct CaptureBufs {
@@ -597,6 +599,9 @@ struct CaptureBufs {
   const int32_t* subTri;
   float ext;
   int32_t stackDepth;
+  // image source of each tree node (FP64, the operations of GetImageSources), or null: then the
+  // correction's first-step pre-test is off
+  const double* img;
 };
 
 // The wave-per-node kernel removes duplicates of its own nodes' records, against the node's image
@@ -607,6 +612,60 @@ constexpr int kFuseDepth = 8;
 constexpr int kList = 256;
 constexpr int kPend = 128;
 static_assert(kPend >= 127, "ring bound: under 64 waiting plus a step's 64");
+
+// The correction's first-step pre-test of a node at depth >= 1, from its image source and the
+// surface its segment starts on (that of its parent's hit); off at depth 0 (the line of sight decides
+// there) or without image sources.  False when no receiver can pass the correction.
+__device__ __forceinline__ bool node_cone(const CaptureArgs& A, const CaptureBufs& B, NodeCtx& c) {
+  c.cone.state = 0;
+  c.I = v3(0.0, 0.0, 0.0);
+  if (B.img == nullptr || c.depth == 0) return true;
+  c.I = ld3(B.img + 3 * (int64_t)c.node);
+  const DSurf& s = B.surf[B.tree[c.parent].tri];
+  c.cone = cone_setup(c.I, ld3(s.v1), ld3(s.e1), ld3(s.e2), cone_reach(c.I, A.g.bmin, A.g.bmax));
+  return c.cone.state >= 0;
+}
+
+// Rows [j0, j1] narrowed to those that can hold a receiver passing the node's correction pre-test.
+// Receivers are binned by floor((y - gy0) / cell); the bounds are padded well above the rounding of
+// that division and of the reciprocal used here.
+__device__ __forceinline__ void cone_rows(const CaptureArgs& A, NodeCtx& c) {
+  if (c.cone.state <= 0) return;
+  const double band[6] = {A.g.bmin[0], A.g.bmin[1], A.g.bmin[2], A.g.bmax[0], A.g.bmax[1], A.g.bmax[2]};
+  double lo, hi;
+  if (!cone_interval(c.cone, c.I, band, 1, &lo, &hi)) {
+    c.j1 = c.j0 - 1;
+    return;
+  }
+  const double pad = 1e-6 * A.g.cell + 1e-9 * (fabs(A.g.gy0) + fabs(A.g.bmax[1]));
+  c.j0 = max(c.j0, (int)fmax(0.0, floor((lo - pad - A.g.gy0) * A.invCell)));
+  c.j1 = min(c.j1, (int)fmin((double)(A.g.ny - 1), floor((hi + pad - A.g.gy0) * A.invCell)));
+}
+
+// Image source of every tree node of one depth (GetImageSources, RTSolver.cpp:396-433, one
+// interaction at a time): the transmitter at depth 0; below it the parent's image source, mirrored
+// across the parent's hit surface when the node is a reflection.  These are the operations
+// image_source repeats per path, in the same order, so the values are the same.
+__global__ void __launch_bounds__(256) k_images(const TNode* __restrict__ tree, const TxDesc* __restrict__ txs,
+                                                const DSurf* __restrict__ surf, int64_t base, int64_t count,
+                                                int depth, double* __restrict__ img) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t n = base + i;
+    const TNode& nd = tree[n];
+    V3 I;
+    if (depth == 0) {
+      const TxDesc& T = txs[nd.txi];
+      I = v3(T.tx[0], T.tx[1], T.tx[2]);
+    } else {
+      const int32_t p = nd.parent;
+      const V3 Ip = ld3(img + 3 * (int64_t)p);
+      I = nd.type == 1 ? image_point(surf[tree[p].tri], Ip) : Ip;
+    }
+    img[3 * n] = I.x;
+    img[3 * n + 1] = I.y;
+    img[3 * n + 2] = I.z;
+  }
+}
 
 // Returns false when no receiver can be captured by the node.
 __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n, NodeCtx& c) {
@@ -643,7 +702,9 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
     c.j1 = (int)fmin((double)(A.g.ny - 1), floor((yhi - A.g.gy0) * A.invCell));
   }
   c.losBase = (int64_t)c.txi * A.nrx;
-  return true;
+  if (!node_cone(A, B, c)) return false;
+  cone_rows(A, c);
+  return c.j0 <= c.j1;
 }
 
 // node_setup for a node that the thread-per-node pass found heavy: its results from the job.
@@ -674,6 +735,7 @@ __device__ __forceinline__ void node_from_job(const CaptureArgs& A, const Captur
   c.j0 = J.j0;
   c.j1 = J.j1;
   c.losBase = (int64_t)c.txi * A.nrx;
+  (void)node_cone(A, B, c);   // the pass listed the node, so some receiver can pass
 }
 
 // Id of the sequence (parent sequence id, element) in the trie: the slot holding that exact key,
@@ -724,7 +786,7 @@ __device__ void node_key(const CaptureBufs& B, NodeCtx& c) {
 }
 
 // Grid cells [i0, i1] of row j that can hold a receiver within R of the clipped segment (xy).
-__device__ __forceinline__ bool row_range(const CaptureArgs& A, const NodeCtx& c, int j, int* i0, int* i1) {
+__device__ __forceinline__ bool row_range_capsule(const CaptureArgs& A, const NodeCtx& c, int j, int* i0, int* i1) {
   if (c.all) {
     *i0 = 0;
     *i1 = A.g.nx - 1;
@@ -747,6 +809,22 @@ __device__ __forceinline__ bool row_range(const CaptureArgs& A, const NodeCtx& c
   }
   *i0 = (int)fmax(0.0, floor((xa - c.R - A.g.gx0) * A.invCell));
   *i1 = (int)fmin((double)(A.g.nx - 1), floor((xb + c.R - A.g.gx0) * A.invCell));
+  return *i0 <= *i1;
+}
+
+// ... narrowed to the cells that can hold a receiver passing the node's correction pre-test (the
+// row's y band padded as in cone_rows).
+__device__ __forceinline__ bool row_range(const CaptureArgs& A, const NodeCtx& c, int j, int* i0, int* i1) {
+  if (!row_range_capsule(A, c, j, i0, i1)) return false;
+  if (c.cone.state <= 0) return true;
+  const double padY = 1e-6 * A.g.cell + 1e-9 * (fabs(A.g.gy0) + fabs(A.g.bmax[1]));
+  const double band[6] = {A.g.bmin[0], A.g.gy0 + j * A.g.cell - padY, A.g.bmin[2],
+                          A.g.bmax[0], A.g.gy0 + (j + 1) * A.g.cell + padY, A.g.bmax[2]};
+  double lo, hi;
+  if (!cone_interval(c.cone, c.I, band, 0, &lo, &hi)) return false;
+  const double pad = 1e-6 * A.g.cell + 1e-9 * (fabs(A.g.gx0) + fabs(A.g.bmax[0]));
+  *i0 = max(*i0, (int)fmax(0.0, floor((lo - pad - A.g.gx0) * A.invCell)));
+  *i1 = min(*i1, (int)fmin((double)(A.g.nx - 1), floor((hi + pad - A.g.gx0) * A.invCell)));
   return *i0 <= *i1;
 }
 
@@ -822,6 +900,7 @@ __device__ __forceinline__ void capture_one(const CaptureBufs& B, NodeCtx& c, co
   myTests++;
   const V3 C = ld3(x.p);
   const int32_t r = x.r;
+  if (!cone_may_pass(c.cone, c.I, C)) return;   // the correction would fail
   const int pv = sphere_pre(c.sf, c.sk, c.S, C);
   if (pv < 0 || (pv == 0 && !sphere_hit(c.S, c.Tg, C, c.base, c.model, c.rc))) return;
   int32_t q = c.parent;
@@ -1249,6 +1328,12 @@ __device__ __forceinline__ void ctx_uniform(NodeCtx& c) {
   c.depth = wave_uniform(c.depth);
   c.j0 = wave_uniform(c.j0);
   c.j1 = wave_uniform(c.j1);
+  for (int k = 0; k < 4; k++) {
+    for (int a = 0; a < 3; a++) c.cone.n[k][a] = wave_uniform(c.cone.n[k][a]);
+    c.cone.m[k] = wave_uniform(c.cone.m[k]);
+  }
+  c.cone.state = wave_uniform(c.cone.state);
+  c.I = wave_uniform(c.I);
 }
 
 // Wave per heavy node.  The node's rows are taken 64 at a time, one row's cell range per lane; the
@@ -1256,7 +1341,9 @@ __device__ __forceinline__ void ctx_uniform(NodeCtx& c) {
 // in a list in LDS (kList at a time), from which they are dealt to the lanes one each, a step's
 // receivers loaded while the previous step's are tested (neighbouring lanes read neighbouring
 // receivers), so a node costs about receivers / 64 steps whatever the shape of its region.  The
-// receivers the node's own sphere captures gather in a ring in LDS and are taken 64 at a time.  A
+// rows and cells are those that can hold a receiver passing the correction's first-step pre-test
+// (node_cone), and each receiver takes that pre-test before its sphere test.  The receivers the
+// node's own sphere captures gather in a ring in LDS and are taken 64 at a time.  A
 // path is kept for a receiver iff the node captures it, no ancestor does, its image correction
 // succeeds and it has the smallest order key of its group; the first three are independent tests of
 // the pair, so here the correction (against the image chain in LDS, no memory traffic) comes before
@@ -1384,7 +1471,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
           }
           const bool have = base + lane < nCh;
           myTests += have ? 1 : 0;
-          const bool hit = have && own_captures(c, C);
+          const bool hit = have && cone_may_pass(c.cone, c.I, C) && own_captures(c, C);
           const unsigned long long m = __ballot(hit);
           if (hit) {
             const int e = (pTail + __popcll(m & lanesBelow)) & (kPend - 1);
@@ -2298,7 +2385,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   // ---- transmitters and their launch directions
   std::vector<TxDesc> txs(nt);
   int maxDepth = 0;
-  bool exact = false, count = false, anyOcclusion = false;
+  bool exact = false, count = false, anyOcclusion = false, anyModel0 = false;
   int64_t nRoot = 0;
   for (int t = 0; t < nt; t++) {
     const sbr_trace_params& p = P[t];
@@ -2354,6 +2441,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     if (p.physicalField != 0 && p.physicalField != 1) return fail(SBR_E_ARG, "physicalField must be 0 or 1");
     T.physicalField = p.physicalField;
     anyOcclusion = anyOcclusion || p.checkOcclusion != 0;
+    anyModel0 = anyModel0 || p.receptionModel == 0;
     T.receptionModel = p.receptionModel;
   }
   if (nRoot >= (int64_t(1) << 31)) return fail(SBR_E_LIMIT, "more than 2^31 launches in one trace");
@@ -2549,6 +2637,19 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       }
 
       // capture + correction + duplicate removal for this window
+      // Reference-sphere transmitters capture over large regions, where the correction's first-step
+      // pre-test (from each node's image source) rejects most candidates before their sphere tests.
+      const bool cone = anyModel0 && nodes > 0;
+      if (cone) {
+        ENSURE(dImg, 3 * (size_t)nodes);
+        const LevelRec* lvw = pinSmall.at<LevelRec>(kPinLevels);
+        for (int d = 0; d <= maxDepth; d++) {
+          if (lvw[d].count == 0) break;
+          k_images<<<(unsigned)std::min<int64_t>(blocks_for((int64_t)lvw[d].count, 256), (int64_t)numCU * 8), 256, 0,
+                     stream>>>(dTree.p, dTx.p, surf, (int64_t)lvw[d].base, (int64_t)lvw[d].count, d, dImg.p);
+          HIPCHK(hipGetLastError());
+        }
+      }
       hipEvent_t kev[4];
       for (auto& e : kev) HIPCHK(hipEventCreate(&e));
       struct KevGuard { hipEvent_t* e; ~KevGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } kguard{kev};
@@ -2589,6 +2690,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dSeqTab.p,
                        (unsigned long long)(cap - 1), dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p, nullptr,
                        count ? 1 : 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
+        CB.img = cone ? dImg.p : nullptr;
         if (nodes > 0) {
           k_capture<<<(unsigned)std::min<int64_t>(blocks_for(nodes, 256), (int64_t)numCU * 16), 256,
                       sizeof(uint16_t) * (size_t)CA.cellLds, stream>>>(CA, CB);
@@ -2762,7 +2864,7 @@ static void for_each_trace_buffer(Engine& e, F&& f) {
   f(e.dCellStart); f(e.dCellRx); f(e.dLos); f(e.dTabA); f(e.dSeqTab); f(e.dTabOrder); f(e.dTabRx);
   f(e.dHeavy); f(e.dCand); f(e.dCandFlag); f(e.dTileCount); f(e.dTileOffset); f(e.dRec); f(e.dRecCount);
   f(e.dRecPrefix); f(e.dNodeKey); f(e.dWinKey); f(e.dWinKey2); f(e.dWinRx); f(e.dWinRx2); f(e.dWinIdx);
-  f(e.dWinIdx2); f(e.dWinNode); f(e.dWinField); f(e.dField);
+  f(e.dWinIdx2); f(e.dWinNode); f(e.dWinField); f(e.dField); f(e.dImg);
 }
 
 size_t Engine::held_bytes() const {

@@ -92,8 +92,8 @@ struct sbr_server {
   }
 
   // Device bytes a query needs while it runs: interaction trees, level queues, the capture's node
-  // list, per-node keys and heavy list, the launch directions, receivers and their grid, line of
-  // sight, fields and the duplicate-removal table.
+  // list, per-node keys and heavy list (and image sources), the launch directions, receivers and
+  // their grid, line of sight, fields and the duplicate-removal table.
   int64_t estimate(const sbr_trace_params& p, int32_t nrx) const {
     const int64_t launches = launches_of(p);
     double perLaunch = std::min(p.maxR + p.maxT + 1, 8);
@@ -103,7 +103,10 @@ struct sbr_server {
     const double nodes = (double)launches * perLaunch + 1024.0;
     int64_t tab = 1;
     while (tab < std::max<int64_t>(1 << 16, 4 * (int64_t)nrx)) tab <<= 1;
-    const double bytes = nodes * (double)(sizeof(sbr::TNode) + 4 * sizeof(int32_t) + 3 * sizeof(uint64_t)) + 24.0 * (double)launches +
+    // reference-sphere queries also hold an image source per node (the correction's pre-test)
+    const double perNode = (double)(sizeof(sbr::TNode) + 4 * sizeof(int32_t) + 3 * sizeof(uint64_t)) +
+                           (p.receptionModel == 0 ? 3.0 * sizeof(double) : 0.0);
+    const double bytes = nodes * perNode + 24.0 * (double)launches +
                          (double)nrx * (24.0 + 8.0 + 1.0 + 48.0) + (double)tab * 28.0;
     return (int64_t)bytes;
   }

@@ -33,3 +33,20 @@ def test_sphere_pretest_decisions_sound(tmp_path):
     cases, miss, hit, undecided, bad = map(int, r.stdout.split()[-5:])
     assert bad == 0, r.stdout + r.stderr
     assert cases == 4_000_000 and miss > 1_000_000 and hit > 500_000 and undecided < cases // 2, r.stdout
+
+
+def test_correction_cone_pretest_sound(tmp_path):
+    """Single-precision pre-test of the image correction's first step (sbrsolver_amd/csrc/
+    correction_prefilter.h, host build) against the FP64 Moller-Trumbore test it stands in front of,
+    on receivers hugging the cone through the triangle's edges and vertices, slivers, image sources in
+    or near the triangle's plane (tests/cone_prefilter_check.cpp): no receiver that the FP64 test
+    passes is rejected, every accepted receiver lies inside the row / column intervals derived from
+    the same planes, and the pre-test rejects most receivers that fail."""
+    exe = str(tmp_path / "cone_prefilter_check")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "include"),
+                           "-I", os.path.join(ROOT, "sbrsolver_amd", "csrc"),
+                           os.path.join(ROOT, "tests", "cone_prefilter_check.cpp"), "-o", exe])
+    r = subprocess.run([exe, "200000"], capture_output=True, text=True, timeout=300)
+    cases, receivers, rejected, passes, bad_reject, bad_interval = map(int, r.stdout.split()[-6:])
+    assert bad_reject == 0 and bad_interval == 0, r.stdout
+    assert receivers == 64 * cases and passes > 100_000 and rejected > 0.8 * (receivers - passes), r.stdout
