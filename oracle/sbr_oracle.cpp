@@ -1266,6 +1266,12 @@ static TreeDump* DumpTrees(Scene* sc, const Params& p) {
 // C interface (ctypes)
 // =============================================================================================
 using namespace orc;
+// memcpy for output arrays that may be empty (a null pointer with a zero size is undefined
+// behaviour for memcpy itself).
+static void copy_out(void* dst, const void* src, size_t n) {
+  if (n > 0) memcpy(dst, src, n);
+}
+
 extern "C" {
 
 typedef struct {
@@ -1363,7 +1369,7 @@ void orc_scene_triangles(void* sp, int* coplanar, int* material, double* verts9,
     coplanar[i] = t->coplanarId;
     material[i] = t->materialId;
     double v[9] = {t->v1.x, t->v1.y, t->v1.z, t->v2.x, t->v2.y, t->v2.z, t->v3.x, t->v3.y, t->v3.z};
-    memcpy(verts9 + 9 * i, v, sizeof v);
+    copy_out(verts9 + 9 * i, v, sizeof v);
     norm3[3 * i] = t->norm.x; norm3[3 * i + 1] = t->norm.y; norm3[3 * i + 2] = t->norm.z;
     meshTriId[i] = s->mesh[i]->triangleId;
   }
@@ -1371,8 +1377,8 @@ void orc_scene_triangles(void* sp, int* coplanar, int* material, double* verts9,
 // Unique vertex coordinates and triangle indices after RemoveDoubles (Faces.vtk content).
 void orc_scene_geometry(void* sp, float* coords, unsigned* tris) {
   Scene* s = (Scene*)sp;
-  memcpy(coords, s->coords.data(), s->coords.size() * sizeof(float));
-  memcpy(tris, s->tris.data(), s->tris.size() * sizeof(unsigned));
+  copy_out(coords, s->coords.data(), s->coords.size() * sizeof(float));
+  copy_out(tris, s->tris.data(), s->tris.size() * sizeof(unsigned));
 }
 // Edge map in key order: (vertex index pair, connection count) (Edges.vtk content).
 void orc_scene_edges(void* sp, int* pairs, int* counts) {
@@ -1441,8 +1447,8 @@ long long orc_icosahedron(int tess, double* out, long long cap) {
 
 void* orc_trace(void* sp, const orc_params* pp, const double* rx, int nrx, const double* mats, int nmat, int nthreads) {
   Params p;
-  memcpy(p.tx, pp->tx, sizeof p.tx);
-  memcpy(p.txCoordSys, pp->txCoordSys, sizeof p.txCoordSys);
+  copy_out(p.tx, pp->tx, sizeof p.tx);
+  copy_out(p.txCoordSys, pp->txCoordSys, sizeof p.txCoordSys);
   p.maxR = pp->maxR; p.maxT = pp->maxT; p.maxTotal = pp->maxTotal;
   p.freq = pp->freq; p.pt = pp->pt; p.useFresnel = pp->useFresnel; p.tess = pp->tess;
   p.antennaType = pp->antennaType; p.receptionModel = pp->receptionModel;
@@ -1464,7 +1470,7 @@ long long orc_result_capture_records(void* r) { return ((Result*)r)->captureReco
 double orc_result_seconds(void* r) { return ((Result*)r)->seconds; }
 void orc_result_fields(void* r, double* out) {
   Result* x = (Result*)r;
-  memcpy(out, x->fields.data(), x->fields.size() * sizeof(double));
+  copy_out(out, x->fields.data(), x->fields.size() * sizeof(double));
 }
 int orc_result_npaths(void* r) { return (int)((Result*)r)->prx.size(); }
 int orc_result_nsegs(void* r) { return (int)((Result*)r)->stri.size(); }
@@ -1472,22 +1478,22 @@ void orc_result_paths(void* r, int* prx, int* plaunch, int* pnseg, int* poff, in
                       double* ssrc, double* stgt) {
   Result* x = (Result*)r;
   size_t np = x->prx.size(), ns = x->stri.size();
-  memcpy(prx, x->prx.data(), np * sizeof(int)); memcpy(plaunch, x->plaunch.data(), np * sizeof(int));
-  memcpy(pnseg, x->pnseg.data(), np * sizeof(int)); memcpy(poff, x->poff.data(), np * sizeof(int));
-  memcpy(stri, x->stri.data(), ns * sizeof(int)); memcpy(scop, x->scop.data(), ns * sizeof(int));
-  memcpy(stype, x->stype.data(), ns * sizeof(int));
-  memcpy(ssrc, x->ssrc.data(), 3 * ns * sizeof(double)); memcpy(stgt, x->stgt.data(), 3 * ns * sizeof(double));
+  copy_out(prx, x->prx.data(), np * sizeof(int)); copy_out(plaunch, x->plaunch.data(), np * sizeof(int));
+  copy_out(pnseg, x->pnseg.data(), np * sizeof(int)); copy_out(poff, x->poff.data(), np * sizeof(int));
+  copy_out(stri, x->stri.data(), ns * sizeof(int)); copy_out(scop, x->scop.data(), ns * sizeof(int));
+  copy_out(stype, x->stype.data(), ns * sizeof(int));
+  copy_out(ssrc, x->ssrc.data(), 3 * ns * sizeof(double)); copy_out(stgt, x->stgt.data(), 3 * ns * sizeof(double));
 }
 
 void orc_result_path_fields(void* r, double* out) {
   Result* x = (Result*)r;
-  memcpy(out, x->pfield.data(), x->pfield.size() * sizeof(double));
+  copy_out(out, x->pfield.data(), x->pfield.size() * sizeof(double));
 }
 
 void* orc_trees(void* sp, const orc_params* pp) {
   Params p;
-  memcpy(p.tx, pp->tx, sizeof p.tx);
-  memcpy(p.txCoordSys, pp->txCoordSys, sizeof p.txCoordSys);
+  copy_out(p.tx, pp->tx, sizeof p.tx);
+  copy_out(p.txCoordSys, pp->txCoordSys, sizeof p.txCoordSys);
   p.maxR = pp->maxR; p.maxT = pp->maxT; p.maxTotal = pp->maxTotal;
   p.freq = pp->freq; p.pt = pp->pt; p.useFresnel = pp->useFresnel; p.tess = pp->tess;
   p.antennaType = pp->antennaType; p.receptionModel = pp->receptionModel;
@@ -1504,11 +1510,11 @@ void orc_trees_get(void* dp, int* launch, int* depth, long long* code, int* tri,
                    double* tgt, double* pl) {
   TreeDump* d = (TreeDump*)dp;
   size_t n = d->launch.size();
-  memcpy(launch, d->launch.data(), n * sizeof(int)); memcpy(depth, d->depth.data(), n * sizeof(int));
-  memcpy(code, d->code.data(), n * sizeof(long long)); memcpy(tri, d->tri.data(), n * sizeof(int));
-  memcpy(cop, d->cop.data(), n * sizeof(int)); memcpy(type, d->type.data(), n * sizeof(int));
-  memcpy(src, d->src.data(), 3 * n * sizeof(double)); memcpy(tgt, d->tgt.data(), 3 * n * sizeof(double));
-  memcpy(pl, d->pl.data(), n * sizeof(double));
+  copy_out(launch, d->launch.data(), n * sizeof(int)); copy_out(depth, d->depth.data(), n * sizeof(int));
+  copy_out(code, d->code.data(), n * sizeof(long long)); copy_out(tri, d->tri.data(), n * sizeof(int));
+  copy_out(cop, d->cop.data(), n * sizeof(int)); copy_out(type, d->type.data(), n * sizeof(int));
+  copy_out(src, d->src.data(), 3 * n * sizeof(double)); copy_out(tgt, d->tgt.data(), 3 * n * sizeof(double));
+  copy_out(pl, d->pl.data(), n * sizeof(double));
 }
 void orc_trees_free(void* d) { delete (TreeDump*)d; }
 
