@@ -114,12 +114,14 @@ __device__ __forceinline__ bool child_exact(const DChildF& c, const DNode* __res
 // inner node n, leaf n | kLeafBit, leaf-BVH node ~s -- with a lower bound of their entry parameter.
 //
 // SUB: the scene has leaf BVHs (otherwise that code is left out of the kernel).
+// COUNT: h.nodes / h.tests / h.exact64 are counted (diagnostic builds of the kernels only; the
+// counters otherwise stay zero and take no registers in the traversal loop).
 // A leaf with a leaf BVH is searched through it (EXACT=false, pre-test usable for the ray): the
 // leaf-BVH boxes are unions of the triangles' padded pre-test boxes, so a subtree is skipped only
 // when the ray provably misses all its triangles or every hit in it lies beyond the best hit,
 // i.e. exactly when the per-triangle pre-test would reject each of its triangles.  The leaf's
 // result, the minimum (distance, DFS position) over its hit triangles, is unchanged.
-template <bool EXACT, bool SUB>
+template <bool EXACT, bool SUB, bool COUNT = false>
 __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, const DNodeF* __restrict__ nodesF,
                                             const DTri* __restrict__ tris, const DBox32* __restrict__ tbox,
                                             const DSub* __restrict__ subs, const int32_t* __restrict__ subTri,
@@ -142,7 +144,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
   int cur = nodes[0].count > 0 ? kLeafBit : 0;
   // Exact test of DFS triangle k, keeping the minimum (distance, position).
   auto test_exact = [&](int k) {
-    h.tests++;
+    if (COUNT) h.tests++;
     const DTri& tr = tris[k];
     V3 p;
     double dist, t;
@@ -178,7 +180,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
   // visits the same items in the same order as a one-item-per-step loop would.
   while (true) {
     while (live && cur >= 0 && !(cur & kLeafBit)) {
-      h.nodes++;
+      if (COUNT) h.nodes++;
       const DNodeF& nf = nodesF[cur];
       float ta, tb;
       bool ina, inb;
@@ -197,7 +199,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
         bool und1 = inb && !(nf.ch[1].inner && box32_crossed(nf.box[1] + 6, nf.box[1] + 9, f));
         while (und0 || und1) {
           const bool first = und0;
-          h.exact64++;
+          if (COUNT) h.exact64++;
           double lo, hi;
           const DNode& n = nodes[(first ? nf.ch[0].item : nf.ch[1].item) & ~kLeafBit];
           const bool in = box_test(n.bmin, n.bmax, r, &lo, &hi) && keep_box<false>(lo, hi, h);
@@ -226,7 +228,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
       }
     }
     if (!live) break;
-    h.nodes++;
+    if (COUNT) h.nodes++;
     if (SUB && cur < 0) {
       const DSub sn = subs[~cur];
       if (sn.count > 0) {

@@ -274,8 +274,8 @@ struct TraceArgs {
 };
 
 // Flags: [0] bit 0 node overflow, bit 1 queue overflow; [5] FP64 box tests, [6] node visits, [7]
-// triangle tests (count mode).  [1] receives the number of listed nodes after the trace.
-template <bool EXACT, bool LDS, bool SUB>
+// triangle tests (COUNT variant).  [1] receives the number of listed nodes after the trace.
+template <bool EXACT, bool LDS, bool SUB, bool COUNT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_trace(TraceArgs A, const TxDesc* __restrict__ txs,
                                                   const DNode* __restrict__ nodes, const DNodeF* __restrict__ gNodesF,
                                                   const DTri* __restrict__ gTris,
@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
                                                   const double* __restrict__ triN, const int32_t* __restrict__ qIn,
                                                   int32_t* __restrict__ qOut, TNode* __restrict__ tree,
                                                   LevelRec* __restrict__ lv, unsigned long long* __restrict__ flags,
-                                                  uint8_t* __restrict__ candFlag, int countTraversal) {
+                                                  uint8_t* __restrict__ candFlag) {
   extern __shared__ __align__(16) unsigned char smem[];
   // The depth's item count is final (written by the previous launch); blocks beyond it leave at
   // once (the grid is sized for the largest depth and oversubscribes the CUs, so that the
@@ -384,8 +384,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       nd.pad0 = 0;
       const V3 src = ld3(nd.src), dir = ld3(nd.dir);
       Hit h;
-      if (closest_hit<EXACT, SUB>(nodes, nodesF, tris, tbox, subs, subTri, A.ext, src, dir, h, stkN, stkT,
-                                  blockDim.x)) {
+      if (closest_hit<EXACT, SUB, COUNT>(nodes, nodesF, tris, tbox, subs, subTri, A.ext, src, dir, h, stkN, stkT,
+                                         blockDim.x)) {
         const DTri& t = tris[h.idx];
         const V3 hp = hit_point(src, dir, h);
         nd.tgt[0] = hp.x; nd.tgt[1] = hp.y; nd.tgt[2] = hp.z;
@@ -406,9 +406,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
         nd.hitDfs = -1;
       }
       nd.mask = (uint8_t)mask;
-      cntNodes += h.nodes;
-      cntTests += h.tests;
-      cntExact += h.exact64;
+      if (COUNT) {
+        cntNodes += h.nodes;
+        cntTests += h.tests;
+        cntExact += h.exact64;
+      }
     }
     // Wave-aggregated append of the children to the next depth's queue; child q of that queue
     // becomes node L.base + n + q.  The wave's transmitted children go first and its reflected ones
@@ -460,7 +462,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
           may_capture(A.g, ld3(nd.src), ld3(nd.tgt), nd.base, T.receptionModel, T.rcoef, T.capK) ? 1 : 0;
     }
   }
-  if (countTraversal) {
+  if (COUNT) {
     atomicAdd(&flags[6], cntNodes);
     atomicAdd(&flags[7], cntTests);
     atomicAdd(&flags[5], cntExact);
@@ -2266,9 +2268,14 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   const bool ldsFits = sceneBytes + stk <= kSceneLdsMax;
   const bool sub = !hSub.empty();
   auto variant = [&](bool l) {
-    if (exact) return l ? k_trace<true, true, false> : k_trace<true, false, false>;
-    if (sub) return l ? k_trace<false, true, true> : k_trace<false, false, true>;
-    return l ? k_trace<false, true, false> : k_trace<false, false, false>;
+    if (count) {
+      if (exact) return l ? k_trace<true, true, false, true> : k_trace<true, false, false, true>;
+      if (sub) return l ? k_trace<false, true, true, true> : k_trace<false, false, true, true>;
+      return l ? k_trace<false, true, false, true> : k_trace<false, false, false, true>;
+    }
+    if (exact) return l ? k_trace<true, true, false, false> : k_trace<true, false, false, false>;
+    if (sub) return l ? k_trace<false, true, true, false> : k_trace<false, false, true, false>;
+    return l ? k_trace<false, true, false, false> : k_trace<false, false, false, false>;
   };
   int occGlobal = 1, occLds = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occGlobal, variant(false), kBlock, stk));
@@ -2309,7 +2316,7 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
     const int32_t* qin = dQueue[d & 1].p;
     int32_t* qout = dQueue[(d + 1) & 1].p;
     kern<<<grid, kBlock, smem, stream>>>(A, dTx.p, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p, dSubTri.p, dTriN.p, qin,
-                                         qout, dTree.p, dLevels.p, dTraceCounters.p, dCandFlag.p, count ? 1 : 0);
+                                         qout, dTree.p, dLevels.p, dTraceCounters.p, dCandFlag.p);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipEventRecord(kev[1], stream));
