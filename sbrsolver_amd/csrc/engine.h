@@ -287,6 +287,7 @@ struct Engine {
   DBuf<TxDesc> dTx;
   DBuf<TNode> dTree;
   DBuf<int32_t> dQueue[2];       // (parent << 1) | reflect
+  DBuf<unsigned long long> dQCnt;   // per depth: the queue partitions' append counters
   DBuf<LevelRec> dLevels;
   DBuf<uint8_t> dScratch;        // library temp storage (scan / sort)
   DBuf<double> dRx;

@@ -260,6 +260,11 @@ __device__ __forceinline__ bool may_capture(const RxGrid& g, V3 S, V3 Tg, double
 }
 
 constexpr int kTraceOversub = 8;   // bounce-kernel blocks per resident block slot
+// Each depth's queue is split into kQParts partitions with an append counter each (kQStride words
+// apart, one per 128-byte line): a wave appends its children to partition (wave index mod kQParts).
+// A single counter for the whole queue serialises every wave's append on one address.
+constexpr int kQParts = 64;
+constexpr int kQStride = 16;
 
 struct TraceArgs {
   int32_t depth;
@@ -271,10 +276,14 @@ struct TraceArgs {
   int32_t capture;               // 1: list the nodes whose segment can capture (receivers' box in g)
   unsigned long long nodeCap, queueCap;
   RxGrid g;
+  unsigned long long* qCnt;      // append counters: depth d, partition p at (d * kQParts + p) * kQStride
+  unsigned long long partCap;    // entries per partition (queueCap / kQParts)
 };
 
 // Flags: [0] bit 0 node overflow, bit 1 queue overflow; [5] FP64 box tests, [6] node visits, [7]
 // triangle tests (COUNT variant).  [1] receives the number of listed nodes after the trace.
+// Items of depth d > 0 are the entries of the kQParts partitions of qIn, in partition order; item i
+// is node L.base + i, and writes that index into its parent's child link.
 template <bool EXACT, bool LDS, bool SUB, bool COUNT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_trace(TraceArgs A, const TxDesc* __restrict__ txs,
                                                   const DNode* __restrict__ nodes, const DNodeF* __restrict__ gNodesF,
@@ -286,16 +295,35 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
                                                   LevelRec* __restrict__ lv, unsigned long long* __restrict__ flags,
                                                   uint8_t* __restrict__ candFlag) {
   extern __shared__ __align__(16) unsigned char smem[];
-  // The depth's item count is final (written by the previous launch); blocks beyond it leave at
-  // once (the grid is sized for the largest depth and oversubscribes the CUs, so that the
+  __shared__ unsigned long long sPre[kQParts + 1];   // items before each partition (depth > 0)
+  // The depth's item count is final (the roots, or the previous launch's appends); blocks beyond it
+  // leave at once (the grid is sized for the largest depth and oversubscribes the CUs, so that the
   // dispatcher balances the launch's end).
   const LevelRec L = lv[A.depth];
+  const int lane = threadIdx.x & 63;
   unsigned long long n = L.count;
-  if (n > A.queueCap) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&flags[0], 2ull);
-    n = A.queueCap;
+  if (A.depth > 0) {
+    static_assert(kQParts <= 64, "one partition count per lane of the first wave");
+    if (threadIdx.x < 64) {
+      unsigned long long c = 0;
+      if (lane < kQParts) c = A.qCnt[((size_t)A.depth * kQParts + lane) * kQStride];
+      if (c > A.partCap) c = A.partCap;   // an overflowing append was flagged by the appender
+      unsigned long long incl = c;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+      }
+      if (lane < kQParts) sPre[lane + 1] = incl;
+      if (lane == 0) sPre[0] = 0;
+    }
+    __syncthreads();
+    n = sPre[kQParts];
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) lv[A.depth + 1].base = L.base + n;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    lv[A.depth + 1].base = L.base + n;
+    if (A.depth > 0) lv[A.depth].count = n;
+  }
   if ((unsigned long long)blockIdx.x * blockDim.x >= n) return;
   const DNodeF* nodesF = gNodesF;
   const DTri* tris = gTris;
@@ -316,16 +344,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
   // depth's queue may hold unwritten entries, so nothing is read from it.
   if (A.depth > 0 && flags[0] != 0ull) return;
   unsigned long long cntNodes = 0, cntTests = 0, cntExact = 0;
-  const int lane = threadIdx.x & 63;
+  // this wave's partition of the next depth's queue
+  const int part = (int)(((unsigned long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kQParts);
+  unsigned long long* partCnt = A.qCnt + ((size_t)(A.depth + 1) * kQParts + part) * kQStride;
+  int32_t* partQ = qOut + (size_t)part * A.partCap;
   for (unsigned long long b0 = (unsigned long long)blockIdx.x * blockDim.x; b0 < n;
        b0 += (unsigned long long)gridDim.x * blockDim.x) {
     const unsigned long long i = b0 + threadIdx.x;
     const unsigned long long nodeIdx = L.base + i;
     int mask = 0;
     if (i < n && nodeIdx >= A.nodeCap) atomicOr(&flags[0], 1ull);
-    // The record is assembled in registers and written with one 128-byte store once the children's
-    // queue positions are known (partial-line stores before and after the traversal would cost a
-    // second write-back of the line).
+    // The record is assembled in registers and written with one 128-byte store after the traversal
+    // (partial-line stores before and after it would cost a second write-back of the line); the
+    // children fill in its child links later.
     TNode nd;
     const bool live = i < n && nodeIdx < A.nodeCap;
     if (live) {
@@ -355,9 +386,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
           nd.base = 0.0;
           totalPL = 0.0;
         } else {
-          const int32_t q = qIn[i];
+          // item i: partition p with sPre[p] <= i < sPre[p + 1]
+          int lo = 0, hi = kQParts;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (sPre[mid] <= i) lo = mid;
+            else hi = mid;
+          }
+          const int32_t q = qIn[(size_t)lo * A.partCap + (i - sPre[lo])];
           const int32_t pi = q >> 1;
           const bool refl = q & 1;
+          // the parent's link to this node (its index was not known when the parent appended it)
+          if (refl) tree[pi].childR = (int32_t)nodeIdx;
+          else tree[pi].childT = (int32_t)nodeIdx;
           const TNode& P = tree[pi];
           T = &txs[P.txi];
           const V3 pd = ld3(P.dir);
@@ -412,10 +453,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
         cntExact += h.exact64;
       }
     }
-    // Wave-aggregated append of the children to the next depth's queue; child q of that queue
-    // becomes node L.base + n + q.  The wave's transmitted children go first and its reflected ones
-    // after them, so that the next depth's waves hold rays of one kind (transmitted rays continue
-    // into the wall, reflected ones leave it).
+    // Wave-aggregated append of the children to the wave's partition of the next depth's queue.
+    // The wave's transmitted children go first and its reflected ones after them, so that the next
+    // depth's waves hold rays of one kind (transmitted rays continue into the wall, reflected ones
+    // leave it).  The children's node indices follow from the partition counts once this depth is
+    // done; each child then sets its parent's link.
     const int cT = mask & 1, cR = (mask >> 1) & 1;
     int incl = cT | (cR << 16);   // two 16-bit inclusive counts
 #pragma unroll
@@ -426,29 +468,20 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     const int tot = __shfl(incl, 63, 64);
     const int totT = tot & 0xffff, totR = tot >> 16;
     unsigned long long wbase = 0;
-    if (lane == 63 && tot > 0) wbase = atomicAdd(&lv[A.depth + 1].count, (unsigned long long)(totT + totR));
+    if (lane == 63 && tot > 0) wbase = atomicAdd(partCnt, (unsigned long long)(totT + totR));
     wbase = __shfl(wbase, 63, 64);
-    int32_t childT = -1, childR = -1;
     if (cT + cR > 0) {
-      if (wbase + (unsigned long long)(totT + totR) > A.queueCap) {
+      if (wbase + (unsigned long long)(totT + totR) > A.partCap) {
         atomicOr(&flags[0], 2ull);
       } else {
         const int32_t nodeI = (int32_t)nodeIdx;
-        if (cT) {
-          const unsigned long long pos = wbase + (unsigned long long)((incl & 0xffff) - 1);
-          childT = (int32_t)(L.base + n + pos);
-          qOut[pos] = nodeI << 1;
-        }
-        if (cR) {
-          const unsigned long long pos = wbase + (unsigned long long)(totT + (incl >> 16) - 1);
-          childR = (int32_t)(L.base + n + pos);
-          qOut[pos] = (nodeI << 1) | 1;
-        }
+        if (cT) partQ[wbase + (unsigned long long)((incl & 0xffff) - 1)] = nodeI << 1;
+        if (cR) partQ[wbase + (unsigned long long)(totT + (incl >> 16) - 1)] = (nodeI << 1) | 1;
       }
     }
     if (live) {
-      nd.childT = childT;
-      nd.childR = childR;
+      nd.childT = -1;   // set by the children
+      nd.childR = -1;
       const int4* srcw = reinterpret_cast<const int4*>(&nd);
       int4* dstw = reinterpret_cast<int4*>(&tree[nodeIdx]);
 #pragma unroll
@@ -2290,6 +2323,9 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   A.rootBase = rootBase;
   A.nodeCap = nodeCap;
   A.queueCap = nodeCap;
+  ENSURE(dQCnt, (size_t)(kMaxDepth + 2) * kQParts * kQStride);
+  A.qCnt = dQCnt.p;
+  A.partCap = nodeCap / kQParts;
   A.capture = capG ? 1 : 0;
   if (capG) A.g = *capG;
   auto kern = variant(lds);
@@ -2306,6 +2342,7 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   lv0[0].count = (unsigned long long)nRoot;
   HIPCHK(hipMemcpyAsync(dLevels.p, lv0, sizeof(LevelRec) * (kMaxDepth + 2), hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemsetAsync(dTraceCounters.p, 0, 8 * sizeof(unsigned long long), stream));
+  HIPCHK(hipMemsetAsync(dQCnt.p, 0, sizeof(unsigned long long) * (kMaxDepth + 2) * kQParts * kQStride, stream));
   hipEvent_t kev[2];
   HIPCHK(hipEventCreate(&kev[0]));
   HIPCHK(hipEventCreate(&kev[1]));
@@ -2549,7 +2586,9 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   ENSURE(dLevels, kMaxDepth + 2);
   ENSURE(dCounters, 10);
   ENSURE(dTraceCounters, 8);
-  const size_t budget = std::max<size_t>(tune.windowNodes > 0 ? (size_t)tune.windowNodes : kWindowNodes, 4096);
+  // at least 256 entries per queue partition
+  const size_t budget =
+      std::max<size_t>(tune.windowNodes > 0 ? (size_t)tune.windowNodes : kWindowNodes, (size_t)kQParts * 256);
   // Bound of the tree size per root item: one node per interaction sequence, i.e. sum over the
   // allowed (reflections r, transmissions t) of C(r + t, r), for the largest transmitter.
   double perRootBound = 1.0;
@@ -2604,7 +2643,9 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       if (nRoot == 0) W = 0;
       float msT = 0;
       for (int attempt = 0;; attempt++) {
-        const size_t want = std::min(budget, (size_t)((double)W * perRoot * 1.25) + 1024);
+        // at least 256 queue entries per partition (a wave appends up to 128 children)
+        const size_t want = std::min(budget, std::max((size_t)((double)W * perRoot * 1.25) + 1024,
+                                                      (size_t)kQParts * 256));
         if (nodeCap < want) nodeCap = want;
         if (nodeCap > budget) nodeCap = budget;
         ENSURE(dTree, nodeCap);
@@ -2871,7 +2912,7 @@ static void for_each_trace_buffer(Engine& e, F&& f) {
   f(e.dCellStart); f(e.dCellRx); f(e.dLos); f(e.dTabA); f(e.dSeqTab); f(e.dTabOrder); f(e.dTabRx);
   f(e.dHeavy); f(e.dCand); f(e.dCandFlag); f(e.dTileCount); f(e.dTileOffset); f(e.dRec); f(e.dRecCount);
   f(e.dRecPrefix); f(e.dNodeKey); f(e.dWinKey); f(e.dWinKey2); f(e.dWinRx); f(e.dWinRx2); f(e.dWinIdx);
-  f(e.dWinIdx2); f(e.dWinNode); f(e.dWinField); f(e.dField); f(e.dImg);
+  f(e.dWinIdx2); f(e.dWinNode); f(e.dWinField); f(e.dField); f(e.dImg); f(e.dQCnt);
 }
 
 size_t Engine::held_bytes() const {
