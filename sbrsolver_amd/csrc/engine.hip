@@ -702,7 +702,9 @@ __global__ void __launch_bounds__(256) k_images(const TNode* __restrict__ tree, 
   }
 }
 
-// Returns false when no receiver can be captured by the node.
+// Returns false when no receiver can be captured by the node.  CONE: the correction pre-test is
+// built for the node (B.img is set); otherwise it is off and its code left out.
+template <bool CONE>
 __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n, NodeCtx& c) {
   const TNode& nd = B.tree[n];   // capture half only
   const TxDesc& T = B.txs[nd.txi];
@@ -737,8 +739,12 @@ __device__ bool node_setup(const CaptureArgs& A, const CaptureBufs& B, int64_t n
     c.j1 = (int)fmin((double)(A.g.ny - 1), floor((yhi - A.g.gy0) * A.invCell));
   }
   c.losBase = (int64_t)c.txi * A.nrx;
-  if (!node_cone(A, B, c)) return false;
-  cone_rows(A, c);
+  if (CONE) {
+    if (!node_cone(A, B, c)) return false;
+    cone_rows(A, c);
+  } else {
+    c.cone.state = 0;
+  }
   return c.j0 <= c.j1;
 }
 
@@ -1134,11 +1140,15 @@ __device__ __forceinline__ void flush_tests(const CaptureBufs& B, unsigned long 
 }
 
 // Thread per listed node (those whose segment reaches the receivers' capture region).  Nodes whose
-// candidate region spans more than kHeavyCells grid cells are left to k_capture_heavy, which spreads
-// one node over a wave.
-constexpr int kHeavyCells = 12;
+// candidate region spans more than heavy_cells(model) grid cells are left to k_capture_heavy, which
+// spreads one node over a wave.  The split was measured on the urban workload: 12 cells for
+// alpha * L / sqrt(3) (6-24 within noise), 48 for the reference sphere, whose nodes' cells mostly
+// fail the correction pre-test per receiver (capture 23.0 ms at 12, 17.3 at 48-64, 20.3 with no
+// wave-per-node kernel at all).
+__device__ __forceinline__ int heavy_cells(int model) { return model == 0 ? 48 : 12; }
 
-__global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
+template <bool CONE>
+__device__ __forceinline__ void capture_pass(const CaptureArgs& A, const CaptureBufs& B) {
   extern __shared__ uint16_t sCell[];
   if (A.cellLds > 0) {   // cell starts in LDS: one global round trip less per row of cells
     for (int k = threadIdx.x; k < A.cellLds; k += blockDim.x) sCell[k] = (uint16_t)B.cellStart[k];
@@ -1150,19 +1160,19 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nc; k += (int64_t)gridDim.x * blockDim.x) {
     const int32_t n = B.cand[k];
     NodeCtx c;
-    if (node_setup(A, B, n, c)) {
-      // The decision is the loop's own exit flag: a `cells > kHeavyCells` test after the loop was
-      // merged by the compiler with the identical in-loop test, whose lane mask only holds the lanes
-      // still looping in the last iteration (lanes that left earlier were then taken for light ones).
-      // Rows are counted only when the bounding box of the region could hold more than
-      // kHeavyCells cells.
+    if (node_setup<CONE>(A, B, n, c)) {
+      // The decision is the loop's own exit flag: a `cells > limit` test after the loop was merged by
+      // the compiler with the identical in-loop test, whose lane mask only holds the lanes still
+      // looping in the last iteration (lanes that left earlier were then taken for light ones).
+      // Rows are counted only when the bounding box of the region could hold more than limit cells.
       int cells = 0;
       bool heavy = false;
-      const int jEnd = box_cells(A, c) > kHeavyCells ? c.j1 : c.j0 - 1;
+      const int limit = heavy_cells(c.model);
+      const int jEnd = box_cells(A, c) > limit ? c.j1 : c.j0 - 1;
       for (int j = c.j0; j <= jEnd; j++) {
         int i0, i1;
         if (row_range(A, c, j, &i0, &i1)) cells += i1 - i0 + 1;
-        if (cells > kHeavyCells) {
+        if (cells > limit) {
           heavy = true;
           break;
         }
@@ -1201,6 +1211,14 @@ __global__ void __launch_bounds__(256) k_capture(CaptureArgs A, CaptureBufs B) {
   }
   flush_tests(B, myTests);
 }
+
+// Without the correction pre-test the pass fits 3 waves per SIMD with no spills (166 VGPRs, 172
+// unbounded); with it, the bound would spill, so that variant keeps 2.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_capture(CaptureArgs A,
+                                                                                   CaptureBufs B) {
+  capture_pass<false>(A, B);
+}
+__global__ void __launch_bounds__(256) k_capture_cone(CaptureArgs A, CaptureBufs B) { capture_pass<true>(A, B); }
 
 // Hit triangle (low 32 bits), coplanar id (bits 32-61) and interaction type (bits 62-63) of a tree
 // node, packed in one word of its AncRec.
@@ -2740,8 +2758,9 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
                        count ? 1 : 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
         CB.img = cone ? dImg.p : nullptr;
         if (nodes > 0) {
-          k_capture<<<(unsigned)std::min<int64_t>(blocks_for(nodes, 256), (int64_t)numCU * 16), 256,
-                      sizeof(uint16_t) * (size_t)CA.cellLds, stream>>>(CA, CB);
+          auto kc = cone ? k_capture_cone : k_capture;
+          kc<<<(unsigned)std::min<int64_t>(blocks_for(nodes, 256), (int64_t)numCU * 16), 256,
+               sizeof(uint16_t) * (size_t)CA.cellLds, stream>>>(CA, CB);
           HIPCHK(hipGetLastError());
         }
         // wave per heavy node; the count is read on the device (no host round trip)

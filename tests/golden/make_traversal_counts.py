@@ -1,13 +1,18 @@
-"""Writes tests/golden/traversal_counts.json: the work of one bounce of the urban-canyon workload
-(482 triangles, icosahedron level 9, 6 reflections + 1 transmission) as counted by the oracle's
+"""Writes tests/golden/traversal_counts.json: the work of one bounce, as counted by the oracle's
 canonical closest-first, t-clipped BVH2 traversal (BVH::CanonicalCount in oracle/sbr_oracle.cpp):
-boxes entered and triangles tested per bounce, over every STRIDE-th launch.
+boxes entered and triangles tested per bounce, over every STRIDE-th launch of
+
+* the urban-canyon workload (482 triangles, icosahedron level 9, 6 reflections + 1 transmission),
+  every launch;
+* the dense indoor workload (206,660 triangles, level 10, reflections and transmissions up to 8
+  each and 8 in total), every 64th launch (the per-bounce figures of a sample).
 
 The counts define the algorithmic bytes of a bounce that bench.py reports:
   B = 32 (ray in) + 32 (segment out) + 32 * nodes + 48 * triangles.
 They depend on the scene and rays only, not on the receivers or on any GPU kernel.
 
-Run:  python tests/golden/make_traversal_counts.py
+Run:  python tests/golden/make_traversal_counts.py [urban] [indoor]   (default: both; an entry
+already in the file is kept unless named)
 """
 import json
 import os
@@ -15,14 +20,11 @@ import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
+PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "traversal_counts.json")
 
-STRIDE = 1
 
-
-def counts(stride=STRIDE, tess=9, nthreads=8):
+def counts(cfg, stride, nthreads=8):
     from oracle import oracle as O
-    from sbrsolver_amd import scenes
-    cfg = scenes.config2(tess=tess)
     sc = O.Scene.from_soup(*cfg["soup"])
     p = O.make_params(**dict(cfg["trace"], receptionModel=1, launchStride=stride, countCanonical=1))
     r = O.trace(sc, p, cfg["rx"][:1], cfg["materials"], nthreads=nthreads)
@@ -31,14 +33,22 @@ def counts(stride=STRIDE, tess=9, nthreads=8):
                 nodes_per_bounce=r.canon_nodes / r.bounces, tris_per_bounce=r.canon_tris / r.bounces)
 
 
-def main():
-    out = counts()
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "traversal_counts.json")
-    with open(path, "w") as f:
-        json.dump({out["workload"]: out}, f, indent=1)
+def main(which):
+    from sbrsolver_amd import scenes
+    out = {}
+    if os.path.exists(PATH):
+        with open(PATH) as f:
+            out = json.load(f)
+    jobs = {"urban": (lambda: scenes.config2(tess=9), 1), "indoor": (lambda: scenes.config3(tess=10), 64)}
+    for name in which:
+        make, stride = jobs[name]
+        e = counts(make(), stride)
+        out[e["workload"]] = e
+        print(json.dumps(e), flush=True)
+    with open(PATH, "w") as f:
+        json.dump(out, f, indent=1)
         f.write("\n")
-    print(json.dumps(out))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or ["urban", "indoor"])

@@ -243,3 +243,22 @@ def test_canonical_traversal_counts_fixture():
     assert abs(a.canon_tris / a.bounces - fx["tris_per_bounce"]) < 0.03 * fx["tris_per_bounce"]
     assert a.canon_nodes <= a.node_visits
     assert fx["canon_nodes"] / fx["bounces"] == fx["nodes_per_bounce"]
+
+
+def test_canonical_traversal_counts_indoor_sample():
+    """The dense indoor entry of tests/golden/traversal_counts.json (every 64th launch): a sparser
+    sample (every 8191st launch, about 1,300 launches) gives per-bounce figures within 15 % of it.
+    The reference's SAH leaves on the tessellated floor, ceiling and walls hold thousands of
+    triangles (a box with no surface area is never split), so its traversal tests about 1,300
+    triangles per bounce there."""
+    import json
+    with open(os.path.join(GOLDEN, "traversal_counts.json")) as f:
+        fx = json.load(f)["indoor_206660tri_L10_R8T8max8"]
+    assert fx["launch_stride"] == 64 and fx["canon_nodes"] / fx["bounces"] == fx["nodes_per_bounce"]
+    cfg = scenes.config3(tess=10)
+    sc = O.Scene.from_soup(*cfg["soup"])
+    kw = dict(cfg["trace"], receptionModel=1, launchStride=8191, countCanonical=1)
+    a = O.trace(sc, O.make_params(**kw), cfg["rx"][:1], cfg["materials"], nthreads=8)
+    assert abs(a.canon_nodes / a.bounces - fx["nodes_per_bounce"]) < 0.15 * fx["nodes_per_bounce"]
+    assert abs(a.canon_tris / a.bounces - fx["tris_per_bounce"]) < 0.15 * fx["tris_per_bounce"]
+    assert fx["tris_per_bounce"] > 1000
