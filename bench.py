@@ -38,6 +38,8 @@ COUNTS = os.path.join(ROOT, "tests", "golden", "traversal_counts.json")
 TRAFFIC = os.path.join(ROOT, "profiles", "k_trace_traffic_urban_L9_R6T1.md")
 # VALU lane activity and wait fraction of the same kernel (rocprofv3 --pmc SQ_* counters).
 VALU = os.path.join(ROOT, "profiles", "k_trace_valu_cache_urban_L9_R6T1.md")
+# HBM traffic of the bounce kernel on the dense indoor workload (line "traffic_bytes_per_bounce: N").
+TRAFFIC_INDOOR = os.path.join(ROOT, "profiles", "k_trace_traffic_indoor_L10.md")
 
 
 def host_cores():
@@ -193,7 +195,9 @@ def batched_run(ctx, make_params, scenes, root_budget=1 << 25):
 def indoor_run(Context, make_params, scenes, tess=10):
     """Dense indoor workload (about 207k triangles, icosahedron level 10, reflections and
     transmissions up to 8 each and 8 in total, 20k receivers) on a context of its own: one warm-up
-    trace, then one timed."""
+    trace, one timed, then one counting pass (the kernel's own traversal counts).  Beside them: the
+    reference traversal's canonical counts (committed fixture, every 64th launch) and the bounce
+    kernel's measured HBM traffic (committed rocprofv3 summary) as a fraction of the peak."""
     cfg = scenes.config3(tess=tess)
     with Context(0) as ictx:
         ictx.scene_from_soup(*cfg["soup"])
@@ -203,10 +207,32 @@ def indoor_run(Context, make_params, scenes, tess=10):
         t0 = time.perf_counter()
         s = ictx.trace(p, cfg["rx"]).stats
         dt = time.perf_counter() - t0
-    return dict(workload=cfg["name"], triangles=int(len(cfg["soup"][2])), launches=int(s["launches"]),
-                bounces=int(s["bounces"]), windows=int(s["windows"]), value=round(s["bounces"] / dt / 1e6, 3),
-                unit="Mray-bounces/s", s_per_trace=round(dt, 3), ms_trace=round(s["ms_trace"], 1),
-                ms_capture=round(s["ms_capture"], 1), paths_kept=int(s["winners"]))
+        c = ictx.trace(make_params(**dict(cfg["trace"], countTraversal=1)), cfg["rx"]).stats
+    b = s["bounces"]
+    out = dict(workload=cfg["name"], triangles=int(len(cfg["soup"][2])), launches=int(s["launches"]),
+               bounces=int(b), windows=int(s["windows"]), value=round(b / dt / 1e6, 3),
+               unit="Mray-bounces/s", s_per_trace=round(dt, 3), ms_trace=round(s["ms_trace"], 1),
+               ms_k_trace=round(s["ms_k_trace"], 1), k_trace_launches=int(s["n_k_trace"]),
+               ms_capture=round(s["ms_capture"], 1), paths_kept=int(s["winners"]),
+               kernel_nodes_per_bounce=round(c["node_visits"] / b, 3),
+               kernel_exact_tri_tests_per_bounce=round(c["tri_tests"] / b, 3))
+    with open(COUNTS) as f:
+        canon = json.load(f).get(cfg["name"])
+    if canon:
+        out["reference_nodes_per_bounce"] = round(canon["nodes_per_bounce"], 3)
+        out["reference_tris_per_bounce"] = round(canon["tris_per_bounce"], 3)
+        out["reference_counts_sample"] = f"every {canon['launch_stride']}th launch ({canon['bounces']} bounces)"
+    if os.path.exists(TRAFFIC_INDOOR):
+        import re
+        m = re.search(r"traffic_bytes_per_bounce:\s*([0-9.eE+]+)", open(TRAFFIC_INDOOR).read())
+        if m:
+            tb = float(m.group(1))
+            gbs = tb * b / (s["ms_k_trace"] * 1e-3) / 1e9
+            out["hbm"] = dict(bytes_per_bounce=round(tb, 1), achieved_GBs=round(gbs, 1), peak_GBs=HBM_PEAK_GBS,
+                              frac=round(gbs / HBM_PEAK_GBS, 4),
+                              note="bounce kernel's HBM traffic per bounce (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                   "scaled by a copy of known size) over its event-timed duration")
+    return out
 
 
 def serve_run(Context, make_params, scenes, Server, n=96, slots=3, load=0.7):
