@@ -109,9 +109,45 @@ __device__ __forceinline__ bool child_exact(const DChildF& c, const DNode* __res
   return true;
 }
 
-// Traversal stack: entry e of the calling thread is at stkN[e * stride] / stkT[e * stride]
-// (stride = threads per block, so a wave's entries are contiguous).  Entries are work items --
-// inner node n, leaf n | kLeafBit, leaf-BVH node ~s -- with a lower bound of their entry parameter.
+// Traversal stack of one thread.  Entries are work items -- inner node n, leaf n | kLeafBit,
+// leaf-BVH node ~s -- with a lower bound of their entry parameter.  Entries [0, lcap) are in LDS,
+// entry e at ln[e * ls] / lt[e * ls] (ls = threads per block, so a wave's entries are contiguous);
+// deeper entries, when the tree needs more than lcap, are in device memory, entry e at
+// g[(e - lcap) * gs] (gs = threads of the grid).  A short LDS part keeps the blocks per CU that the
+// registers allow on scenes with tall trees.
+struct TravStack {
+  int* ln;
+  float* lt;
+  int ls;
+  int lcap;
+  int2* g;
+  int64_t gs;
+  __device__ __forceinline__ void put(int e, int item, float t) const {
+    if (e < lcap) {
+      ln[e * ls] = item;
+      lt[e * ls] = t;
+    } else {
+      g[(int64_t)(e - lcap) * gs] = make_int2(item, __float_as_int(t));
+    }
+  }
+  // (item, entry parameter's bits)
+  __device__ __forceinline__ int2 get(int e) const {
+    if (e < lcap) return make_int2(ln[e * ls], __float_as_int(lt[e * ls]));
+    return g[(int64_t)(e - lcap) * gs];
+  }
+};
+
+// A stack held wholly in LDS (stkN / stkT per thread, stride = threads per block, depth entries).
+__device__ __forceinline__ TravStack lds_stack(int* stkN, float* stkT, int stride, int depth) {
+  TravStack st;
+  st.ln = stkN;
+  st.lt = stkT;
+  st.ls = stride;
+  st.lcap = depth;
+  st.g = nullptr;
+  st.gs = 0;
+  return st;
+}
 //
 // SUB: the scene has leaf BVHs (otherwise that code is left out of the kernel).
 // COUNT: h.nodes / h.tests / h.exact64 are counted (diagnostic builds of the kernels only; the
@@ -125,7 +161,7 @@ template <bool EXACT, bool SUB, bool COUNT = false>
 __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, const DNodeF* __restrict__ nodesF,
                                             const DTri* __restrict__ tris, const DBox32* __restrict__ tbox,
                                             const DSub* __restrict__ subs, const int32_t* __restrict__ subTri,
-                                            float ext, V3 o, V3 d, Hit& h, int* stkN, float* stkT, int stride) {
+                                            float ext, V3 o, V3 d, Hit& h, const TravStack& st) {
   h.idx = INT_MAX;
   h.dist = INFINITY;
   h.t = INFINITY;
@@ -168,8 +204,9 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
   auto pop = [&]() {
     while (sp > 0) {
       --sp;
-      if (!EXACT && h.idx != INT_MAX && (double)stkT[sp * stride] > h.t * (1.0 + 1e-7) + 1e-7) continue;
-      cur = stkN[sp * stride];
+      const int2 e = st.get(sp);
+      if (!EXACT && h.idx != INT_MAX && (double)__int_as_float(e.y) > h.t * (1.0 + 1e-7) + 1e-7) continue;
+      cur = e.x;
       return true;
     }
     return false;
@@ -217,8 +254,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
       }
       if (ina && inb) {
         const bool bFirst = !EXACT && tb < ta;   // EXACT: the reference's DFS order
-        stkN[sp * stride] = bFirst ? nf.ch[0].item : nf.ch[1].item;
-        stkT[sp * stride] = bFirst ? ta : tb;
+        st.put(sp, bFirst ? nf.ch[0].item : nf.ch[1].item, bFirst ? ta : tb);
         sp++;
         cur = bFirst ? nf.ch[1].item : nf.ch[0].item;
       } else if (ina || inb) {
@@ -239,8 +275,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
         const bool inb = box32_may_hit(subs[sn.first + 1].lo, subs[sn.first + 1].hi, f, clip, &tb);
         if (ina && inb) {
           const bool aFirst = ta <= tb;
-          stkN[sp * stride] = ~(aFirst ? sn.first + 1 : sn.first);
-          stkT[sp * stride] = aFirst ? tb : ta;
+          st.put(sp, ~(aFirst ? sn.first + 1 : sn.first), aFirst ? tb : ta);
           sp++;
           cur = ~(aFirst ? sn.first : sn.first + 1);
           continue;

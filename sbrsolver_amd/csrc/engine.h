@@ -20,6 +20,8 @@ constexpr int kMaxDepth = 24;       // reflections + transmissions per path (tre
 constexpr int kMaxStack = 64;       // BVH traversal stack (tree height bound, checked on upload)
 constexpr int kBlock = 256;         // threads per block of the traversal kernels
 constexpr size_t kSceneLdsMax = 96 * 1024;   // scenes up to this size are staged in LDS
+constexpr size_t kLdsPerCU = 160 * 1024;     // LDS of a gfx950 compute unit
+constexpr int kTraceBlocksPerCU = 4;         // bounce-kernel blocks per CU its registers allow (127 VGPRs)
 constexpr size_t kWindowNodes = size_t(1) << 27;   // tree nodes per launch window (17 GB of records)
 
 // ---- scene, device layout ---------------------------------------------------------------------
@@ -288,6 +290,7 @@ struct Engine {
   DBuf<TNode> dTree;
   DBuf<int32_t> dQueue[2];       // (parent << 1) | reflect
   DBuf<unsigned long long> dQCnt;   // per depth: the queue partitions' append counters
+  DBuf<int2> dStackG;            // traversal stack entries beyond the LDS part (tall trees)
   DBuf<LevelRec> dLevels;
   DBuf<uint8_t> dScratch;        // library temp storage (scan / sort)
   DBuf<double> dRx;

@@ -278,6 +278,8 @@ struct TraceArgs {
   RxGrid g;
   unsigned long long* qCnt;      // append counters: depth d, partition p at (d * kQParts + p) * kQStride
   unsigned long long partCap;    // entries per partition (queueCap / kQParts)
+  int32_t stackLds;              // traversal stack entries per thread in LDS (the rest in gStack)
+  int2* gStack;                  // deeper entries: (stackDepth - stackLds) per thread of the grid
 };
 
 // Flags: [0] bit 0 node overflow, bit 1 queue overflow; [5] FP64 box tests, [6] node visits, [7]
@@ -337,8 +339,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     nodesF = reinterpret_cast<const DNodeF*>(smem);
     __syncthreads();
   }
-  int* stkN = reinterpret_cast<int*>(smem + sceneB) + threadIdx.x;
-  float* stkT = reinterpret_cast<float*>(smem + sceneB + (size_t)A.stackDepth * blockDim.x * 4) + threadIdx.x;
+  TravStack st;
+  st.ln = reinterpret_cast<int*>(smem + sceneB) + threadIdx.x;
+  st.lt = reinterpret_cast<float*>(smem + sceneB + (size_t)A.stackLds * blockDim.x * 4) + threadIdx.x;
+  st.ls = blockDim.x;
+  st.lcap = A.stackLds;
+  st.gs = (int64_t)gridDim.x * blockDim.x;
+  st.g = A.gStack ? A.gStack + (int64_t)blockIdx.x * blockDim.x + threadIdx.x : nullptr;
 
   // An overflow at an earlier depth (flags[0] != 0) means the trace is redone with more room: this
   // depth's queue may hold unwritten entries, so nothing is read from it.
@@ -425,8 +432,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       nd.pad0 = 0;
       const V3 src = ld3(nd.src), dir = ld3(nd.dir);
       Hit h;
-      if (closest_hit<EXACT, SUB, COUNT>(nodes, nodesF, tris, tbox, subs, subTri, A.ext, src, dir, h, stkN, stkT,
-                                         blockDim.x)) {
+      if (closest_hit<EXACT, SUB, COUNT>(nodes, nodesF, tris, tbox, subs, subTri, A.ext, src, dir, h, st)) {
         const DTri& t = tris[h.idx];
         const V3 hp = hit_point(src, dir, h);
         nd.tgt[0] = hp.x; nd.tgt[1] = hp.y; nd.tgt[2] = hp.z;
@@ -522,7 +528,8 @@ __global__ void __launch_bounds__(kBlock) k_los(const TxDesc* __restrict__ txs, 
   V3 tx = v3(T.tx[0], T.tx[1], T.tx[2]);
   V3 d = vsub(ld3(rx + 3 * (k % m)), tx);
   Hit h;
-  bool hh = closest_hit<EXACT, true>(nodes, nodesF, tris, tbox, subs, subTri, ext, tx, d, h, stkN, stkT, blockDim.x);
+  bool hh = closest_hit<EXACT, true>(nodes, nodesF, tris, tbox, subs, subTri, ext, tx, d, h,
+                                     lds_stack(stkN, stkT, blockDim.x, stackDepth));
   los[k] = (!hh || vnorm(d) < h.dist) ? 1 : 0;
 }
 
@@ -543,7 +550,7 @@ __global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restric
   if (i >= n) return;
   Hit h;
   bool hh = closest_hit<EXACT, true>(nodes, nodesF, tris, tbox, subs, subTri, ext, ld3(org + 3 * (int64_t)i),
-                               ld3(dirs + 3 * (int64_t)i), h, stkN, stkT, blockDim.x);
+                               ld3(dirs + 3 * (int64_t)i), h, lds_stack(stkN, stkT, blockDim.x, stackDepth));
   triOut[i] = hh ? tris[h.idx].id : -1;
   const V3 hp = hit_point(ld3(org + 3 * (int64_t)i), ld3(dirs + 3 * (int64_t)i), h);
   ptOut[3 * i] = hp.x; ptOut[3 * i + 1] = hp.y; ptOut[3 * i + 2] = hp.z;
@@ -1029,8 +1036,8 @@ __device__ bool unobstructed(const CaptureBufs& B, const NodeCtx& c, V3 C, int* 
   for (int k = 0; k <= c.depth; k++) {
     const V3 d = vsub(pts[k + 1], pts[k]);
     Hit h;
-    if (closest_hit<false, true>(B.nodes, B.nodesF, B.tris, B.tbox, B.subs, B.subTri, B.ext, pts[k], d, h, stkN, stkT,
-                                 blockDim.x) &&
+    if (closest_hit<false, true>(B.nodes, B.nodesF, B.tris, B.tbox, B.subs, B.subTri, B.ext, pts[k], d, h,
+                                 lds_stack(stkN, stkT, blockDim.x, B.stackDepth)) &&
         h.dist < vnorm(d) * (1.0 - 1e-7))
       return false;
   }
@@ -2313,10 +2320,20 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
     nodes = capNodes = 0;
     return SBR_OK;
   }
+  // Traversal stack: the entries that leave LDS room for kTraceBlocksPerCU blocks per CU (the bound
+  // the kernel's registers set) next to the scene, if staged, and the kernel's own LDS; deeper entries
+  // of tall trees go to device memory.
+  const int depthAll = stackDepth();
+  auto ldsEntries = [&](size_t scene) {
+    const size_t perBlock = kLdsPerCU / kTraceBlocksPerCU - 1024;
+    const int64_t e = perBlock > scene ? (int64_t)((perBlock - scene) / ((size_t)kBlock * 8)) : 0;
+    return (int)std::max<int64_t>(4, std::min<int64_t>(depthAll, e));
+  };
+  const int kGlob = ldsEntries(0), kLds = ldsEntries(sceneBytes);
+  const size_t stk = stack_bytes(kGlob, kBlock), stkL = stack_bytes(kLds, kBlock);
   // Scene in LDS or read through the caches: automatic choice takes LDS staging unless it lowers the
   // number of resident blocks per CU.
-  const size_t stk = stack_bytes(stackDepth(), kBlock);
-  const bool ldsFits = sceneBytes + stk <= kSceneLdsMax;
+  const bool ldsFits = sceneBytes + stkL <= kSceneLdsMax;
   const bool sub = !hSub.empty();
   auto variant = [&](bool l) {
     if (count) {
@@ -2330,9 +2347,10 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   };
   int occGlobal = 1, occLds = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occGlobal, variant(false), kBlock, stk));
-  if (ldsFits) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occLds, variant(true), kBlock, sceneBytes + stk));
+  if (ldsFits) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occLds, variant(true), kBlock, sceneBytes + stkL));
   bool lds = ldsFits && (tune.ldsScene == 1 || (tune.ldsScene < 0 && occLds >= occGlobal));
-  const size_t smem = (lds ? sceneBytes : 0) + stk;
+  const int kStack = lds ? kLds : kGlob;
+  const size_t smem = (lds ? sceneBytes : 0) + stack_bytes(kStack, kBlock);
   TraceArgs A{};
   A.nTx = (int32_t)txs.size();
   A.stackDepth = stackDepth();
@@ -2355,6 +2373,12 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
       1, std::min<int64_t>((int64_t)numCU * perCU * kTraceOversub, blocks_for(std::max<int64_t>(2 * nRoot, 1), kBlock)));
   stats.trace_lds = lds ? 1 : 0;
   stats.trace_blocks_per_cu = perCU;
+  A.stackLds = kStack;
+  A.gStack = nullptr;
+  if (kStack < depthAll) {
+    ENSURE(dStackG, (size_t)(depthAll - kStack) * grid * kBlock);
+    A.gStack = dStackG.p;
+  }
   LevelRec* lv0 = pinSmall.at<LevelRec>(kPinLevels0);
   for (int d = 0; d < kMaxDepth + 2; d++) lv0[d] = LevelRec{0, 0};
   lv0[0].count = (unsigned long long)nRoot;
@@ -2931,7 +2955,7 @@ static void for_each_trace_buffer(Engine& e, F&& f) {
   f(e.dCellStart); f(e.dCellRx); f(e.dLos); f(e.dTabA); f(e.dSeqTab); f(e.dTabOrder); f(e.dTabRx);
   f(e.dHeavy); f(e.dCand); f(e.dCandFlag); f(e.dTileCount); f(e.dTileOffset); f(e.dRec); f(e.dRecCount);
   f(e.dRecPrefix); f(e.dNodeKey); f(e.dWinKey); f(e.dWinKey2); f(e.dWinRx); f(e.dWinRx2); f(e.dWinIdx);
-  f(e.dWinIdx2); f(e.dWinNode); f(e.dWinField); f(e.dField); f(e.dImg); f(e.dQCnt);
+  f(e.dWinIdx2); f(e.dWinNode); f(e.dWinField); f(e.dField); f(e.dImg); f(e.dQCnt); f(e.dStackG);
 }
 
 size_t Engine::held_bytes() const {
