@@ -216,10 +216,8 @@ struct HostPinned {
 constexpr size_t kPinLevels = 0;         // LevelRec[kMaxDepth + 2] read back
 constexpr size_t kPinLevels0 = 1024;     // LevelRec[kMaxDepth + 2] sent
 constexpr size_t kPinTraceCtr = 2048;    // 8 counters of the bounce kernel
-constexpr size_t kPinCapCtr = 2176;      // capture counters
-constexpr size_t kPinParts = 4096;       // record partition counters (kRecParts * kRecStride words)
-constexpr size_t kPinPrefix = kPinParts + 8 * kRecParts * kRecStride;   // kRecParts + 1 prefix words
-constexpr size_t kPinSmall = kPinPrefix + 8 * (kRecParts + 1) + 64;
+constexpr size_t kPinCapCtr = 2176;      // capture counters (10), record total and largest partition, listed nodes
+constexpr size_t kPinSmall = 4096;
 
 // Launch configuration of the traversal kernel (sbr_set_tuning, or chosen by sbr_autotune).
 struct Tuning {

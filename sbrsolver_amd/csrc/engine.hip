@@ -1079,15 +1079,51 @@ __device__ __forceinline__ unsigned long long group_insert(const CaptureArgs& A,
   return 0ull;
 }
 
+// Records before each partition of the candidate pass's record buffer (one block of kRecParts
+// threads): prefix[p] for p < kRecParts, the total at prefix[kRecParts] and the largest partition's
+// count, overflowing or not, at prefix[kRecParts + 1] (for resizing).
+__global__ void __launch_bounds__(kRecParts) k_rec_prefix(const unsigned long long* __restrict__ count,
+                                                          unsigned long long recPart,
+                                                          unsigned long long* __restrict__ prefix) {
+  __shared__ unsigned long long sw[kRecParts / 64], sm[kRecParts / 64];
+  const int p = threadIdx.x, lane = p & 63, w = p >> 6;
+  const unsigned long long c = count[(size_t)p * kRecStride];
+  const unsigned long long v = c < recPart ? c : recPart;
+  unsigned long long incl = v, mx = c;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned long long u = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += u;
+    const unsigned long long m2 = __shfl_xor(mx, off, 64);
+    mx = m2 > mx ? m2 : mx;
+  }
+  if (lane == 63) {
+    sw[w] = incl;
+    sm[w] = mx;
+  }
+  __syncthreads();
+  unsigned long long before = 0, total = 0, big = 0;
+  for (int k = 0; k < kRecParts / 64; k++) {
+    if (k < w) before += sw[k];
+    total += sw[k];
+    big = sm[k] > big ? sm[k] : big;
+  }
+  prefix[p] = before + incl - v;
+  if (p == 0) {
+    prefix[kRecParts] = total;
+    prefix[kRecParts + 1] = big;
+  }
+}
+
 // Dedup pass, one thread per record: the group key of the node's coplanar-id sequence, the prune
 // against the key its group already holds, image correction (and, with OCCL, the occlusion test of
 // the transmitters that ask for it), insertion.
 template <bool OCCL>
-__global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B, int64_t nRec) {
+__global__ void __launch_bounds__(256) k_dedup(CaptureArgs A, CaptureBufs B) {
   extern __shared__ __align__(16) unsigned char smemD[];
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nRec = (int64_t)B.recPrefix[kRecParts];   // records of the candidate pass
   unsigned long long myCor = 0, myUsed = 0;
-  if (k < nRec) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nRec; k += (int64_t)gridDim.x * blockDim.x) {
     int lo = 0, hi = kRecParts - 1;   // partition holding record k
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -2761,13 +2797,14 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         losWait = false;
       }
       HIPCHK(hipEventRecord(kev[0], stream));
-      // candidate pass: (node, receiver) records; redone with a larger record buffer if it fills
+      // candidate pass: (node, receiver) records, then the dedup pass over them, with no host round
+      // trip in between (the records' partition prefix is taken on the device); redone with a larger
+      // record buffer if it filled (the duplicate-removal tables only ever take smaller keys, so the
+      // redo over all records gives what one pass would)
       unsigned long long nRec = 0, nHeavy = 0;
-      const unsigned long long* partCount = pinSmall.at<unsigned long long>(kPinParts);
-      unsigned long long* prefix = pinSmall.at<unsigned long long>(kPinPrefix);
-      const unsigned long long* c4 = pinSmall.at<unsigned long long>(kPinCapCtr);
+      unsigned long long* ctr = pinSmall.at<unsigned long long>(kPinCapCtr);
       ENSURE(dRecCount, kRecParts * kRecStride);
-      ENSURE(dRecPrefix, kRecParts + 1);
+      ENSURE(dRecPrefix, kRecParts + 2);
       for (int recAttempt = 0;; recAttempt++) {
         if (recCap < (size_t)kRecParts * 1024) recCap = std::max<size_t>((size_t)kRecParts * 2048, (size_t)nodes);
         ENSURE(dRec, recCap);
@@ -2778,53 +2815,9 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         HIPCHK(hipMemsetAsync(dCounters.p + 5, 0, 2 * sizeof(unsigned long long), stream));
         HIPCHK(hipMemsetAsync(dCounters.p + 8, 0, 2 * sizeof(unsigned long long), stream));
         CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dSeqTab.p,
-                       (unsigned long long)(cap - 1), dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p, nullptr,
-                       count ? 1 : 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
+                       (unsigned long long)(cap - 1), dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p,
+                       dRecPrefix.p, count ? 1 : 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
         CB.img = cone ? dImg.p : nullptr;
-        if (nodes > 0) {
-          auto kc = cone ? k_capture_cone : k_capture;
-          kc<<<(unsigned)std::min<int64_t>(blocks_for(nodes, 256), (int64_t)numCU * 16), 256,
-               sizeof(uint16_t) * (size_t)CA.cellLds, stream>>>(CA, CB);
-          HIPCHK(hipGetLastError());
-        }
-        // wave per heavy node; the count is read on the device (no host round trip)
-        if (nodes > 0) {
-          k_capture_heavy<<<blocks_for((int64_t)numCU * 32 * 64, 256), 256, 0, stream>>>(CA, CB);
-          HIPCHK(hipGetLastError());
-        }
-        HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinCapCtr), dCounters.p, 7 * sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinTraceCtr) + 1, dTraceCounters.p + 1,
-                              sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipMemcpyAsync(pinSmall.at<unsigned long long>(kPinParts), dRecCount.p,
-                              sizeof(unsigned long long) * kRecParts * kRecStride, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        nHeavy = c4[5];
-        capNodes = (int64_t)pinSmall.at<unsigned long long>(kPinTraceCtr)[1];
-        prefix[0] = 0;
-        unsigned long long maxPart = 0;
-        for (int p = 0; p < kRecParts; p++) {
-          const unsigned long long cnt = partCount[(size_t)p * kRecStride];
-          prefix[p + 1] = prefix[p] + std::min(cnt, part);
-          maxPart = std::max(maxPart, cnt);
-        }
-        nRec = prefix[kRecParts];
-        if (!(c4[3] & 4ull)) {
-          stats.capture_tests += (int64_t)c4[6];
-          break;
-        }
-        if (recAttempt > 4) return fail(SBR_E_LIMIT, "capture record buffer could not be sized");
-        recCap = std::max<size_t>(recCap * 2, (size_t)(maxPart + maxPart / 4 + 1024) * kRecParts);
-      }
-      HIPCHK(hipMemcpyAsync(dRecPrefix.p, prefix, sizeof(unsigned long long) * (kRecParts + 1),
-                            hipMemcpyHostToDevice, stream));
-      stats.heavy_nodes += (int64_t)nHeavy;
-      stats.capture_nodes += capNodes;
-      // dedup pass over the records
-      if (nRec > 0) {
-        CaptureBufs CB{dTx.p, dTree.p, surf, dRx.p, dCellStart.p, dCellRx.p, dLos.p, dTabA.p, dSeqTab.p,
-                       (unsigned long long)(cap - 1), dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, recCap / kRecParts, dRecCount.p,
-                       dRecPrefix.p, 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
         CB.nodes = dNodes.p;
         CB.nodesF = dNodesF.p;
         CB.tris = dTris.p;
@@ -2833,17 +2826,44 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         CB.subTri = dSubTri.p;
         CB.ext = sceneExtent;
         CB.stackDepth = stackDepth();
-        if (anyOcclusion)
-          k_dedup<true><<<blocks_for((int64_t)nRec, 256), 256, stack_bytes(stackDepth(), 256), stream>>>(
-              CA, CB, (int64_t)nRec);
-        else
-          k_dedup<false><<<blocks_for((int64_t)nRec, 256), 256, 0, stream>>>(CA, CB, (int64_t)nRec);
-        HIPCHK(hipGetLastError());
+        if (nodes > 0) {
+          auto kc = cone ? k_capture_cone : k_capture;
+          kc<<<(unsigned)std::min<int64_t>(blocks_for(nodes, 256), (int64_t)numCU * 16), 256,
+               sizeof(uint16_t) * (size_t)CA.cellLds, stream>>>(CA, CB);
+          HIPCHK(hipGetLastError());
+          // wave per heavy node; the count is read on the device
+          k_capture_heavy<<<blocks_for((int64_t)numCU * 32 * 64, 256), 256, 0, stream>>>(CA, CB);
+          HIPCHK(hipGetLastError());
+          k_rec_prefix<<<1, kRecParts, 0, stream>>>(dRecCount.p, part, dRecPrefix.p);
+          HIPCHK(hipGetLastError());
+          // dedup pass: a grid-stride loop over the records, their count read on the device
+          const unsigned dg = (unsigned)std::min<int64_t>(blocks_for((int64_t)recCap, 256), (int64_t)numCU * 16);
+          if (anyOcclusion)
+            k_dedup<true><<<dg, 256, stack_bytes(stackDepth(), 256), stream>>>(CA, CB);
+          else
+            k_dedup<false><<<dg, 256, 0, stream>>>(CA, CB);
+          HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(kev[1], stream));
+        HIPCHK(hipMemcpyAsync(ctr, dCounters.p, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(ctr + 10, dRecPrefix.p + kRecParts, 2 * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipMemcpyAsync(ctr + 12, dTraceCounters.p + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        nHeavy = ctr[5];
+        capNodes = (int64_t)ctr[12];
+        nRec = nodes > 0 ? ctr[10] : 0;
+        if (!(ctr[3] & 4ull)) {
+          stats.capture_tests += (int64_t)ctr[6];
+          break;
+        }
+        if (recAttempt > 4) return fail(SBR_E_LIMIT, "capture record buffer could not be sized");
+        const unsigned long long maxPart = ctr[11];
+        recCap = std::max<size_t>(recCap * 2, (size_t)(maxPart + maxPart / 4 + 1024) * kRecParts);
       }
-      HIPCHK(hipEventRecord(kev[1], stream));
-      unsigned long long* ctr = pinSmall.at<unsigned long long>(kPinCapCtr);
-      HIPCHK(hipMemcpyAsync(ctr, dCounters.p, 10 * sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-      HIPCHK(hipStreamSynchronize(stream));
+      stats.heavy_nodes += (int64_t)nHeavy;
+      stats.capture_nodes += capNodes;
       float kms = 0;
       HIPCHK(hipEventElapsedTime(&kms, kev[0], kev[1]));
       stats.ms_k_capture += kms;
