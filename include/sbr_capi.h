@@ -111,7 +111,13 @@ typedef struct {
   int32_t launchStride;  /* trace launches i with i % stride == offset (1 = all) */
   int32_t launchOffset;
   int32_t q1MeshIndexing;/* 1 (reference): surface data of mesh[hitSurfaceID]; 0: by id */
-  int32_t exactTraversal;/* 1: visit every box the line touches (reference order); 0: clipped */
+  int32_t exactTraversal;/* closest-hit traversal, all with the reference's result: 0 the wide tree
+                            (four children per node, single-precision boxes), its winner's leaf
+                            confirmed by the reference tree's own FP64 box tests (the ray is redone on
+                            the reference tree otherwise); 1 the reference tree, every box the line
+                            touches (reference order); 2 the reference tree with the clipping cuts.
+                            One traversal per call: 1 if any transmitter asks for it, else 2 if any
+                            does, else 0 */
   int32_t countTraversal;/* 1: count BVH node visits and triangle tests (sbr_stats) */
   double wallWidth;      /* extension: > 0 with useFresnel = 1: every wall is a slab of this thickness
                             (m) with the ITU-R P.2040 slab coefficients, so transmissions carry field;
@@ -155,6 +161,8 @@ typedef struct {
   int64_t capture_tests;    /* reception-sphere tests of the capture's candidate pass (countTraversal) */
   int64_t box_tests_fp64;   /* child boxes the traversal decided with the FP64 slab test (countTraversal) */
   int64_t capture_nodes;    /* tree nodes whose segment reaches the receivers' capture region */
+  int64_t wide_redone;      /* bounces whose wide-tree winner the reference tree's box tests did not
+                               confirm, redone on the reference tree (countTraversal) */
 } sbr_stats;
 
 /* Launch configuration of the traversal kernel (no reference counterpart).  lds_scene: -1 stage
@@ -219,6 +227,7 @@ int sbr_set_tuning(sbr_ctx* ctx, const sbr_tuning* tuning);
 int sbr_get_tuning(sbr_ctx* ctx, sbr_tuning* tuning);
 
 int sbr_generate_rays(sbr_ctx* ctx, int32_t tess, double* dirs_out, int64_t capacity, int64_t* n_out);
+/* exact: the traversal, as sbr_trace_params.exactTraversal (0, 1 or 2; SBR_E_ARG otherwise). */
 int sbr_closest_hit(sbr_ctx* ctx, const double* orig, const double* dirs, int32_t n, int32_t exact,
                     int32_t* tri_out, double* point_out, double* dist_out);
 /* Allocation-free variant of sbr_closest_hit (no counterpart in the reference; for callers that

@@ -41,7 +41,7 @@ class Stats(C.Structure):
         ("node_visits", C.c_int64), ("tri_tests", C.c_int64),
         ("trace_lds", C.c_int32), ("trace_blocks_per_cu", C.c_int32), ("heavy_nodes", C.c_int64),
         ("windows", C.c_int64), ("capture_tests", C.c_int64), ("box_tests_fp64", C.c_int64),
-        ("capture_nodes", C.c_int64),
+        ("capture_nodes", C.c_int64), ("wide_redone", C.c_int64),
     ]
 
     def as_dict(self):
@@ -366,14 +366,16 @@ class Context:
         assert got.value == n
         return out
 
-    def closest_hit(self, org, dirs, exact=False):
+    def closest_hit(self, org, dirs, exact=0):
+        """exact: 0 wide tree (winner confirmed on the reference tree), 1 reference tree in the
+        reference's order, 2 reference tree with the clipping cuts; all give the reference's result."""
         o = np.ascontiguousarray(org, np.float64).reshape(-1, 3)
         d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
         if o.shape[0] == 1 and d.shape[0] > 1:
             o = np.ascontiguousarray(np.repeat(o, d.shape[0], axis=0))
         n = d.shape[0]
         tri = np.zeros(n, np.int32); pt = np.zeros((n, 3)); dist = np.zeros(n)
-        self._chk(lib().sbr_closest_hit(self.h, _a(o), _a(d), n, int(bool(exact)), _a(tri), _a(pt), _a(dist)))
+        self._chk(lib().sbr_closest_hit(self.h, _a(o), _a(d), n, int(exact), _a(tri), _a(pt), _a(dist)))
         return tri, pt, dist
 
     def trace(self, params, rx):

@@ -195,6 +195,90 @@ int Engine::upload(const sbr_scene_desc* d) {
       nf[k].ch[c].inner = inner ? 1 : 0;
     }
   }
+  // Wide traversal tree over the reference tree (DNode4), its leaves, and per reference-tree node the
+  // data of the winner check (DVer).
+  std::vector<DNode4> n4;
+  std::vector<DLeaf> leaves;
+  std::vector<DVer> ver(nodesOut.size());
+  int wideStack = 0;
+  {
+    std::vector<int32_t> leafSlot(nodesOut.size(), -1);
+    for (size_t k = 0; k < nodesOut.size(); k++) {
+      ver[k].parent = -1;
+      if (nodesOut[k].count > 0) {
+        leafSlot[k] = (int32_t)leaves.size();
+        leaves.push_back(DLeaf{nodesOut[k].a, nodesOut[k].count, nodesOut[k].sub, (int32_t)k});
+        for (int i = 0; i < nodesOut[k].count; i++) trisOut[nodesOut[k].a + i].leaf = (int32_t)k;
+      }
+    }
+    // The check of a winner walks up from its leaf and stops at the first node whose shrunk box the
+    // line crosses; that this implies the tests of all ancestors needs every box to contain its
+    // children's boxes, which a description normally has.  Without it every node on the way is tested.
+    bool nested = true;
+    for (size_t k = 0; k < nodesOut.size(); k++) {
+      const DNode& nd = nodesOut[k];
+      if (nd.count > 0) continue;
+      for (int c : {nd.a, nd.b}) {
+        ver[c].parent = (int32_t)k;
+        for (int a = 0; a < 3; a++)
+          if (!(nodesOut[c].bmin[a] >= nd.bmin[a] && nodesOut[c].bmax[a] <= nd.bmax[a])) nested = false;
+      }
+    }
+    for (size_t k = 0; k < nodesOut.size(); k++)
+      ver[k].shrunk = (box_shrunk(nodesOut[k].bmin, nodesOut[k].bmax, delta, ver[k].lo, ver[k].hi) && nested) ? 1 : 0;
+    auto area = [&](int k) {
+      const DNode& x = nodesOut[k];
+      const double e0 = x.bmax[0] - x.bmin[0], e1 = x.bmax[1] - x.bmin[1], e2 = x.bmax[2] - x.bmin[2];
+      return e0 * e1 + e1 * e2 + e2 * e0;
+    };
+    // build(k, deferred): wide node over reference node k (a leaf only at the root); deferred = the
+    // siblings the traversal may hold on its stack above this node.
+    std::function<int(int, int)> build = [&](int k, int deferred) {
+      int c[4] = {k, -1, -1, -1};
+      int m = 1;
+      if (nodesOut[k].count == 0) {
+        c[0] = nodesOut[k].a;
+        c[1] = nodesOut[k].b;
+        m = 2;
+      }
+      while (m < 4) {
+        int best = -1;
+        double ba = -1.0;
+        for (int i = 0; i < m; i++)
+          if (nodesOut[c[i]].count == 0 && area(c[i]) > ba) {
+            best = i;
+            ba = area(c[i]);
+          }
+        if (best < 0) break;
+        const int x = c[best];
+        for (int j = m; j > best + 1; j--) c[j] = c[j - 1];
+        c[best] = nodesOut[x].a;
+        c[best + 1] = nodesOut[x].b;
+        m++;
+      }
+      const int me = (int)n4.size();
+      n4.emplace_back();
+      wideStack = std::max(wideStack, deferred + m - 1);
+      for (int i = 0; i < 4; i++) {
+        float lo[3], hi[3];
+        int32_t item = -1;
+        if (i < m) {
+          const DNode& cn = nodesOut[c[i]];
+          box_grown(cn.bmin, cn.bmax, 2.0 * delta, lo, hi);
+          item = cn.count > 0 ? (kLeafBit | leafSlot[c[i]]) : build(c[i], deferred + m - 1);
+        } else {
+          for (int a = 0; a < 3; a++) lo[a] = hi[a] = std::nanf("");
+        }
+        for (int a = 0; a < 3; a++) {
+          n4[me].lo[a][i] = lo[a];
+          n4[me].hi[a][i] = hi[a];
+        }
+        n4[me].item[i] = item;
+      }
+      return me;
+    };
+    build(0, 0);
+  }
   auto surf_of = [&](int id) {
     DSurf s{};
     const double* v = d->tri_v + 9 * (size_t)id;
@@ -217,7 +301,7 @@ int Engine::upload(const sbr_scene_desc* d) {
     if (put(dNodes, nodesOut, stream) || put(dTris, trisOut, stream) || put(dTriN, triN, stream) ||
         put(dSurfQ1, sq, stream) || put(dSurfId, si, stream) ||
         put(dTriBox32, tb32, stream) || put(dSub, subs, stream) || put(dSubTri, subTri, stream) ||
-        put(dNodesF, nf, stream))
+        put(dNodesF, nf, stream) || put(dNodes4, n4, stream) || put(dLeaves, leaves, stream) || put(dVer, ver, stream))
       return fail(SBR_E_NOMEM, "scene upload");
     HIPCHK(hipStreamSynchronize(stream));
   }
@@ -241,7 +325,12 @@ int Engine::upload(const sbr_scene_desc* d) {
   subHeight = subInternal;
   sceneExtent = (float)ext;
   boxDelta = (float)delta;
-  sceneBytes = hNodesF.size() * sizeof(DNodeF);   // LDS-staged part
+  sceneBytes = hNodesF.size() * sizeof(DNodeF);   // LDS-staged part (reference-tree traversal)
+  hNodes4.swap(n4);
+  hLeaves.swap(leaves);
+  hVer.swap(ver);
+  this->wideStack = wideStack;
+  wideBytes = hNodes4.size() * sizeof(DNode4) + hLeaves.size() * sizeof(DLeaf);   // LDS-staged part (wide)
   ncop = 0;
   for (int t = 0; t < n; t++) ncop = std::max(ncop, hCop[t] + 1);
   haveScene = true;
@@ -274,13 +363,19 @@ int Engine::clone_scene(const Engine& s) {
   ncop = s.ncop;
   treeHeight = s.treeHeight;
   sceneBytes = s.sceneBytes;
+  hNodes4 = s.hNodes4;
+  hLeaves = s.hLeaves;
+  hVer = s.hVer;
+  wideStack = s.wideStack;
+  wideBytes = s.wideBytes;
   sceneExtent = s.sceneExtent;
   boxDelta = s.boxDelta;
   if (device >= 0) {
     HIPCHK(hipSetDevice(device));
     if (put(dNodes, hNodes, stream) || put(dTris, hTris, stream) || put(dTriN, hTriN, stream) ||
         put(dSurfQ1, hSurfQ1, stream) || put(dSurfId, hSurfId, stream) || put(dTriBox32, hTriBox32, stream) ||
-        put(dSub, hSub, stream) || put(dSubTri, hSubTri, stream) || put(dNodesF, hNodesF, stream))
+        put(dSub, hSub, stream) || put(dSubTri, hSubTri, stream) || put(dNodesF, hNodesF, stream) ||
+        put(dNodes4, hNodes4, stream) || put(dLeaves, hLeaves, stream) || put(dVer, hVer, stream))
       return fail(SBR_E_NOMEM, "scene upload");
     HIPCHK(hipStreamSynchronize(stream));
   }
@@ -526,7 +621,8 @@ int sbr_generate_rays(sbr_ctx* ctx, int32_t tess, double* out, int64_t cap, int6
 int sbr_closest_hit(sbr_ctx* ctx, const double* o, const double* d, int32_t n, int32_t exact, int32_t* tri, double* pt,
                     double* dist) {
   return guarded(ctx, [&] {
-    if (n < 0 || (n > 0 && (!o || !d || !tri || !pt || !dist))) return ctx->eng.fail(SBR_E_ARG, "closest_hit: bad args");
+    if (n < 0 || (n > 0 && (!o || !d || !tri || !pt || !dist)) || exact < 0 || exact > 2)
+      return ctx->eng.fail(SBR_E_ARG, "closest_hit: bad args");
     return ctx->eng.closest_hit(o, d, n, exact, tri, pt, dist);
   });
 }
@@ -534,7 +630,7 @@ int sbr_closest_hit(sbr_ctx* ctx, const double* o, const double* d, int32_t n, i
 int sbr_closest_hit_noalloc(sbr_ctx* ctx, const double* o, const double* d, int32_t n, int32_t exact, int32_t* tri,
                             double* pt, double* dist) {
   return guarded(ctx, [&] {
-    if (n < 0 || (n > 0 && (!o || !d || !tri || !pt || !dist)))
+    if (n < 0 || (n > 0 && (!o || !d || !tri || !pt || !dist)) || exact < 0 || exact > 2)
       return ctx->eng.fail(SBR_E_ARG, "closest_hit_noalloc: bad args");
     return ctx->eng.closest_hit_noalloc(o, d, n, exact, tri, pt, dist);
   });

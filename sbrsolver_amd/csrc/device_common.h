@@ -42,6 +42,7 @@ struct Hit {
   int nodes;     // BVH nodes entered
   int tests;     // ray-triangle tests
   int exact64;   // child boxes decided by the FP64 slab test (grazing or no single-precision path)
+  int redone;    // 1: the wide-tree winner was not confirmed and the ray was redone on the reference tree
 };
 
 // Hit point of a closest hit: the same operations tri_intersect applies, so the value is
@@ -148,6 +149,62 @@ __device__ __forceinline__ TravStack lds_stack(int* stkN, float* stkT, int strid
   st.gs = 0;
   return st;
 }
+
+// Exact test of DFS triangle k (Triangle::RayIntersects, Triangle.cpp), keeping the minimum
+// (distance, DFS position); clip follows the best hit's parameter for the pre-tests.
+template <bool COUNT>
+__device__ __forceinline__ void tri_exact(const DTri* __restrict__ tris, int k, V3 o, V3 d, Hit& h, float& clip) {
+  if (COUNT) h.tests++;
+  const DTri& tr = tris[k];
+  V3 p;
+  double dist, t;
+  if (tri_intersect(ld3(tr.v1), ld3(tr.e1), ld3(tr.e2), o, d, &p, &dist, &t)) {
+    if (dist < h.dist || (dist == h.dist && k < h.idx)) {
+      h.idx = k;
+      h.dist = dist;
+      h.t = t;
+      clip = __double2float_ru(t * (1.0 + 1e-6));
+    }
+  }
+}
+
+// The triangles [first, first + count) of a leaf.  With the pre-test usable for the ray: the leaf's
+// boxes are pre-tested, two loads in flight at a time, into a mask per 64 triangles; then the
+// exact tests of the surviving triangles, so that the wave runs as many exact tests as its busiest
+// lane needs rather than one wherever any lane's box passed.
+template <bool COUNT>
+__device__ __forceinline__ void leaf_tests(const DTri* __restrict__ tris, const DBox32* __restrict__ tbox, int first,
+                                           int count, const RayF& f, V3 o, V3 d, Hit& h, float& clip) {
+  if (f.on) {
+    const int end = first + count;
+    for (int k0 = first; k0 < end; k0 += 64) {
+      const int m = min(64, end - k0);
+      unsigned long long pass = 0;
+      for (int j = 0; j < m; j += 4) {
+        // Four boxes loaded and tested unconditionally (indices clamped), so that their loads
+        // are in flight together.
+        const int j1 = min(j + 1, m - 1), j2 = min(j + 2, m - 1), j3 = min(j + 3, m - 1);
+        const DBox32 b0 = tbox[k0 + j];
+        const DBox32 b1 = tbox[k0 + j1];
+        const DBox32 b2 = tbox[k0 + j2];
+        const DBox32 b3 = tbox[k0 + j3];
+        float e;
+        pass |= (unsigned long long)(box32_may_hit(b0.lo, b0.hi, f, clip, &e) ? 1u : 0u) << j;
+        pass |= (unsigned long long)(box32_may_hit(b1.lo, b1.hi, f, clip, &e) ? 1u : 0u) << j1;
+        pass |= (unsigned long long)(box32_may_hit(b2.lo, b2.hi, f, clip, &e) ? 1u : 0u) << j2;
+        pass |= (unsigned long long)(box32_may_hit(b3.lo, b3.hi, f, clip, &e) ? 1u : 0u) << j3;
+      }
+      while (pass) {
+        const int j = __builtin_ctzll(pass);
+        pass &= pass - 1;
+        tri_exact<COUNT>(tris, k0 + j, o, d, h, clip);
+      }
+    }
+  } else {
+    for (int k = first; k < first + count; k++) tri_exact<COUNT>(tris, k, o, d, h, clip);
+  }
+}
+
 //
 // SUB: the scene has leaf BVHs (otherwise that code is left out of the kernel).
 // COUNT: h.nodes / h.tests / h.exact64 are counted (diagnostic builds of the kernels only; the
@@ -168,6 +225,7 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
   h.nodes = 0;
   h.tests = 0;
   h.exact64 = 0;
+  h.redone = 0;
   const RayPre r = ray_pre(o, d);
   const RayF f = rayf_pre(o, d, ext);
   const bool fast = !EXACT && f.on;
@@ -178,26 +236,11 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
   }
   int sp = 0;
   int cur = nodes[0].count > 0 ? kLeafBit : 0;
-  // Exact test of DFS triangle k, keeping the minimum (distance, position).
-  auto test_exact = [&](int k) {
-    if (COUNT) h.tests++;
-    const DTri& tr = tris[k];
-    V3 p;
-    double dist, t;
-    if (tri_intersect(ld3(tr.v1), ld3(tr.e1), ld3(tr.e2), o, d, &p, &dist, &t)) {
-      if (dist < h.dist || (dist == h.dist && k < h.idx)) {
-        h.idx = k;
-        h.dist = dist;
-        h.t = t;
-        clip = __double2float_ru(t * (1.0 + 1e-6));
-      }
-    }
-  };
-  // ... after its pre-test.
+  // Exact test of DFS triangle k after its pre-test.
   auto test_tri = [&](int k) {
     float e;
     if (f.on && !box32_may_hit(tbox[k].lo, tbox[k].hi, f, clip, &e)) return;
-    test_exact(k);
+    tri_exact<COUNT>(tris, k, o, d, h, clip);
   };
   // Next work item from the stack, skipping entries that cannot beat the best hit; false when the
   // stack is empty.
@@ -292,40 +335,178 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
         cur = ~li.w;   // search the leaf through its leaf BVH
         continue;
       }
-      if (f.on) {
-        // Pre-test the leaf's boxes, two loads in flight at a time, into a mask per 64 triangles;
-        // then the exact tests of the surviving triangles, so that the wave runs as many exact
-        // tests as its busiest lane needs rather than one wherever any lane's box passed.
-        const int end = li.x + li.z;
-        for (int k0 = li.x; k0 < end; k0 += 64) {
-          const int m = min(64, end - k0);
-          unsigned long long pass = 0;
-          for (int j = 0; j < m; j += 4) {
-            // Four boxes loaded and tested unconditionally (indices clamped), so that their loads
-            // are in flight together.
-            const int j1 = min(j + 1, m - 1), j2 = min(j + 2, m - 1), j3 = min(j + 3, m - 1);
-            const DBox32 b0 = tbox[k0 + j];
-            const DBox32 b1 = tbox[k0 + j1];
-            const DBox32 b2 = tbox[k0 + j2];
-            const DBox32 b3 = tbox[k0 + j3];
-            float e;
-            pass |= (unsigned long long)(box32_may_hit(b0.lo, b0.hi, f, clip, &e) ? 1u : 0u) << j;
-            pass |= (unsigned long long)(box32_may_hit(b1.lo, b1.hi, f, clip, &e) ? 1u : 0u) << j1;
-            pass |= (unsigned long long)(box32_may_hit(b2.lo, b2.hi, f, clip, &e) ? 1u : 0u) << j2;
-            pass |= (unsigned long long)(box32_may_hit(b3.lo, b3.hi, f, clip, &e) ? 1u : 0u) << j3;
-          }
-          while (pass) {
-            const int j = __builtin_ctzll(pass);
-            pass &= pass - 1;
-            test_exact(k0 + j);
-          }
-        }
-      } else {
-        for (int k = li.x; k < li.x + li.z; k++) test_exact(k);
-      }
+      leaf_tests<COUNT>(tris, tbox, li.x, li.z, f, o, d, h, clip);
     }
     live = pop();
     if (!live) break;
+  }
+  return h.idx != INT_MAX;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Closest hit through the wide tree (DNode4), for rays the single-precision pre-test covers.
+//
+// The wide tree's boxes are the reference tree's boxes grown by 2 delta, so every triangle that
+// tri_intersect hits lies in boxes whose computed intervals are non-empty: the traversal finds the
+// minimum (distance, DFS position) over all hit triangles, with the same cuts as the reference-tree
+// traversal (boxes wholly behind the origin or beyond the best hit; children visited nearest entry
+// first).  The reference only tests the triangles of the leaves whose boxes, and all their
+// ancestors' boxes, pass its FP64 slab test (BVHNode.h:64-86).  If the winner's leaf is reached that
+// way (wide_reaches), it is also the minimum over the triangles the reference tests, i.e. the
+// reference's result; otherwise (a line grazing a box of the winner's path) the caller redoes the
+// ray on the reference tree.  No hit at all means the reference finds none either.
+// ---------------------------------------------------------------------------------------------
+template <bool SUB, bool COUNT>
+__device__ __forceinline__ void wide_hit(const DNode4* __restrict__ n4, const DLeaf* __restrict__ leaves,
+                                         const DTri* __restrict__ tris, const DBox32* __restrict__ tbox,
+                                         const DSub* __restrict__ subs, const int32_t* __restrict__ subTri,
+                                         const RayF& f, V3 o, V3 d, Hit& h, const TravStack& st) {
+  h.idx = INT_MAX;
+  h.dist = INFINITY;
+  h.t = INFINITY;
+  float clip = INFINITY;
+  int sp = 0;
+  int cur = 0;
+  auto pop = [&]() {
+    while (sp > 0) {
+      --sp;
+      const int2 e = st.get(sp);
+      if (h.idx != INT_MAX && (double)__int_as_float(e.y) > h.t * (1.0 + 1e-7) + 1e-7) continue;
+      cur = e.x;
+      return true;
+    }
+    return false;
+  };
+  auto push = [&](int item, float t) {
+    st.put(sp, item, t);
+    sp++;
+  };
+  bool live = true;
+  // While-while traversal as in closest_hit: lanes descend to a leaf, then the wave tests its
+  // lanes' leaves together.
+  while (true) {
+    while (live && cur >= 0 && !(cur & kLeafBit)) {
+      if (COUNT) h.nodes++;
+      const DNode4& n = n4[cur];
+      float t[4];
+      int it[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        float e;
+        const bool in = box32_may_hit(n.lo[0][k], n.lo[1][k], n.lo[2][k], n.hi[0][k], n.hi[1][k], n.hi[2][k], f, clip, &e);
+        it[k] = n.item[k];
+        t[k] = (in && it[k] != -1) ? e : INFINITY;
+      }
+      // nearest entry first: a sorting network over the four (entry, item) pairs
+      auto cswap = [&](int a, int b) {
+        if (t[b] < t[a]) {
+          const float tt = t[a];
+          t[a] = t[b];
+          t[b] = tt;
+          const int ii = it[a];
+          it[a] = it[b];
+          it[b] = ii;
+        }
+      };
+      cswap(0, 1);
+      cswap(2, 3);
+      cswap(0, 2);
+      cswap(1, 3);
+      cswap(1, 2);
+      if (t[0] == INFINITY) {
+        live = pop();
+      } else {
+        if (t[3] != INFINITY) push(it[3], t[3]);
+        if (t[2] != INFINITY) push(it[2], t[2]);
+        if (t[1] != INFINITY) push(it[1], t[1]);
+        cur = it[0];
+      }
+    }
+    if (!live) break;
+    if (COUNT) h.nodes++;
+    if (SUB && cur < 0) {
+      const DSub sn = subs[~cur];
+      if (sn.count > 0) {
+        for (int j = sn.first; j < sn.first + sn.count; j++) {
+          const int k = subTri[j];
+          float e;
+          if (box32_may_hit(tbox[k].lo, tbox[k].hi, f, clip, &e)) tri_exact<COUNT>(tris, k, o, d, h, clip);
+        }
+      } else {
+        float ta, tb;
+        const bool ina = box32_may_hit(subs[sn.first].lo, subs[sn.first].hi, f, clip, &ta);
+        const bool inb = box32_may_hit(subs[sn.first + 1].lo, subs[sn.first + 1].hi, f, clip, &tb);
+        if (ina && inb) {
+          const bool aFirst = ta <= tb;
+          push(~(aFirst ? sn.first + 1 : sn.first), aFirst ? tb : ta);
+          cur = ~(aFirst ? sn.first : sn.first + 1);
+          continue;
+        }
+        if (ina || inb) {
+          cur = ~(ina ? sn.first : sn.first + 1);
+          continue;
+        }
+      }
+    } else {
+      const DLeaf li = leaves[cur & ~kLeafBit];
+      if (SUB && li.sub >= 0) {
+        cur = ~li.sub;   // search the leaf through its leaf BVH
+        continue;
+      }
+      leaf_tests<COUNT>(tris, tbox, li.first, li.count, f, o, d, h, clip);
+    }
+    live = pop();
+    if (!live) break;
+  }
+}
+
+// Whether the reference's traversal reaches the reference-tree leaf `node`: its box and those of
+// all its ancestors pass the FP64 slab test.  Walking up, a node whose shrunk box the line crosses
+// (single precision) settles it and all its ancestors; other nodes take the FP64 test.
+template <bool COUNT>
+__device__ __forceinline__ bool wide_reaches(int node, const DVer* __restrict__ ver, const DNode* __restrict__ nodes,
+                                             const RayF& f, V3 o, V3 d, Hit& h) {
+  const RayPre r = ray_pre(o, d);
+  while (node >= 0) {
+    const DVer v = ver[node];
+    if (v.shrunk && box32_crossed(v.lo, v.hi, f)) return true;
+    if (COUNT) h.exact64++;
+    double lo, hi;
+    if (!box_test(nodes[node].bmin, nodes[node].bmax, r, &lo, &hi)) return false;
+    node = v.parent;
+  }
+  return true;
+}
+
+// Closest hit with the reference's result (BVH<Triangle>::RayIntersects): the wide tree where it
+// applies, the reference tree (closest_hit, clipped) for rays the single-precision pre-test does
+// not cover and for winners whose leaf the check does not confirm.  nodesF: the reference tree's
+// single-precision records (device memory).
+template <bool SUB, bool COUNT = false>
+__device__ __forceinline__ bool closest_hit_wide(const WideTree& w, const DNode* __restrict__ nodes,
+                                                 const DNodeF* __restrict__ nodesF, const DTri* __restrict__ tris,
+                                                 const DBox32* __restrict__ tbox, const DSub* __restrict__ subs,
+                                                 const int32_t* __restrict__ subTri, float ext, V3 o, V3 d, Hit& h,
+                                                 const TravStack& st) {
+  h.nodes = 0;
+  h.tests = 0;
+  h.exact64 = 0;
+  h.redone = 0;
+  const RayF f = rayf_pre(o, d, ext);
+  bool done = false;
+  if (f.on) {
+    wide_hit<SUB, COUNT>(w.n4, w.leaves, tris, tbox, subs, subTri, f, o, d, h, st);
+    done = h.idx == INT_MAX || wide_reaches<COUNT>(tris[h.idx].leaf, w.ver, nodes, f, o, d, h);
+  }
+  if (!done) {
+    const int n0 = h.nodes, t0 = h.tests, x0 = h.exact64;
+    closest_hit<false, SUB, COUNT>(nodes, nodesF, tris, tbox, subs, subTri, ext, o, d, h, st);
+    if (COUNT) {
+      h.nodes += n0;
+      h.tests += t0;
+      h.exact64 += x0;
+      h.redone = 1;
+    }
   }
   return h.idx != INT_MAX;
 }

@@ -60,6 +60,51 @@ struct alignas(128) DNodeF {
   int32_t pad[4];
 };
 static_assert(sizeof(DNodeF) == 128, "DNodeF layout");
+// Closest-hit traversal (sbr_trace_params.exactTraversal, sbr_closest_hit's `exact`): the wide tree
+// with the winner confirmed on the reference tree, the reference tree in the reference's order
+// (every crossed box), or the reference tree with the clipping cuts.  All give the reference's result.
+constexpr int kTravWide = 0;
+constexpr int kTravExact = 1;
+constexpr int kTravRef = 2;
+
+// Node of the wide traversal tree, 128 B.  It holds up to four nodes of the reference tree from at
+// most two levels below its own: starting from the node's two children, the inner child with the
+// largest box surface is replaced by its two children until four are held.  Each child's box is the
+// exact box grown by 2 delta and rounded outward (a ray whose computed interval of it is empty
+// misses the exact box by more than delta), as single-precision bounds per axis; an empty slot
+// has NaN bounds, which no interval test passes.  The wide tree only finds the triangles a ray may
+// hit; whether the reference's traversal reaches the winner's leaf is checked on the reference
+// tree itself (DVer).
+struct alignas(128) DNode4 {
+  float lo[3][4];   // lo[axis][child]
+  float hi[3][4];
+  int32_t item[4];  // inner: DNode4 index; leaf: kLeafBit | DLeaf index; empty slot: -1
+  int32_t pad[4];
+};
+static_assert(sizeof(DNode4) == 128, "DNode4 layout");
+// Leaf of the reference tree as the wide traversal reads it.
+struct alignas(16) DLeaf {
+  int32_t first;   // first triangle (DFS index)
+  int32_t count;
+  int32_t sub;     // leaf BVH root (DSub index) or -1
+  int32_t node;    // reference-tree node (DFS index)
+};
+// Reference-tree node for the check of a winner's leaf, 32 B: the exact box shrunk by delta and
+// rounded inward (a non-empty computed interval puts the line at depth delta / 2 inside the exact
+// box and so inside the box of every ancestor, whose FP64 slab tests then cannot fail), and the
+// parent.
+struct alignas(32) DVer {
+  float lo[3];
+  float hi[3];
+  int32_t parent;   // -1 at the root
+  int32_t shrunk;   // 1: the shrunk box is non-empty and implies the ancestors' tests
+};
+// The wide tree as the kernels take it.
+struct WideTree {
+  const DNode4* n4;
+  const DLeaf* leaves;
+  const DVer* ver;
+};
 constexpr int kSubLeaf = 64;      // leaves above this size get a leaf BVH
 constexpr int kRecParts = 1024;   // partitions of the capture record buffer (one append counter each)
 constexpr int kRecStride = 16;    // counters 128 B apart
@@ -72,7 +117,7 @@ struct alignas(32) DTri {
   int32_t id;     // triangleId
   int32_t cop;    // coplanarId
   int32_t mat;    // materialId
-  int32_t pad;
+  int32_t leaf;   // the reference-tree leaf holding the triangle (DFS node index)
   double pad2;
 };
 // Conservative single-precision box (rounded outward and padded), 32 B: a ray that provably misses
@@ -246,8 +291,10 @@ struct Engine {
   int ntri = 0, nnode = 0, ncop = 0, treeHeight = 0;
   // Traversal stack entries per ray: one deferred sibling per internal node of the current path.
   // A leaf BVH continues the path below its leaf, adding its own internal levels.
-  int stackDepth() const { return std::max(1, treeHeight - 1 + subHeight); }
-  size_t sceneBytes = 0;
+  // The wide traversal defers up to three siblings per wide node of its path (wideStack).
+  int stackDepth() const { return std::max(1, std::max(treeHeight - 1, wideStack) + subHeight); }
+  int wideStack = 0;
+  size_t sceneBytes = 0, wideBytes = 0;
   std::vector<DNode> hNodes;
   std::vector<DTri> hTris;
   std::vector<double> hTriN;                // normals in DFS order
@@ -258,6 +305,12 @@ struct Engine {
   std::vector<DNodeF> hNodesF;
   DBuf<DNodeF> dNodesF;
   DBuf<DTri> dTris;
+  std::vector<DNode4> hNodes4;     // wide traversal tree (root at 0)
+  std::vector<DLeaf> hLeaves;      // its leaves
+  std::vector<DVer> hVer;          // per reference-tree node: the winner check
+  DBuf<DNode4> dNodes4;
+  DBuf<DLeaf> dLeaves;
+  DBuf<DVer> dVer;
   std::vector<DBox32> hTriBox32;   // single-precision pre-test box per DFS triangle
   DBuf<DBox32> dTriBox32;
   std::vector<DSub> hSub;          // leaf BVHs of large leaves
@@ -372,8 +425,10 @@ struct Engine {
 
  private:
   int generate_rays(int tess, DBuf<double>& out, int64_t* nOut);
-  int trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_t nRoot, int maxDepth, bool exact,
+  int trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_t nRoot, int maxDepth, int mode,
                   bool count, const RxGrid* capG, int64_t* need, float* ms);
+  WideTree wide_tree() const;
+  hipError_t launch_closest(int mode, const double* o, const double* d, int n, int32_t* tri, double* pt, double* dist);
   int paths_window(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
                    int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
                    bool countOnly);

@@ -89,6 +89,9 @@ template struct DBuf<DNode>;
 template struct DBuf<DTri>;
 template struct DBuf<DBox32>;
 template struct DBuf<DSub>;
+template struct DBuf<DNode4>;
+template struct DBuf<DLeaf>;
+template struct DBuf<DVer>;
 template struct DBuf<HeavyJob>;
 template struct DBuf<DNodeF>;
 template struct DBuf<int2>;
@@ -280,13 +283,17 @@ struct TraceArgs {
   unsigned long long partCap;    // entries per partition (queueCap / kQParts)
   int32_t stackLds;              // traversal stack entries per thread in LDS (the rest in gStack)
   int2* gStack;                  // deeper entries: (stackDepth - stackLds) per thread of the grid
+  WideTree wide;                 // wide traversal tree (kTravWide)
+  int32_t wideNodes, wideLeaves; // its sizes (LDS staging)
 };
 
-// Flags: [0] bit 0 node overflow, bit 1 queue overflow; [5] FP64 box tests, [6] node visits, [7]
-// triangle tests (COUNT variant).  [1] receives the number of listed nodes after the trace.
+// Flags: [0] bit 0 node overflow, bit 1 queue overflow; [4] rays redone on the reference tree, [5]
+// FP64 box tests, [6] node visits, [7] triangle tests (COUNT variant).  [1] receives the number of
+// listed nodes after the trace.
 // Items of depth d > 0 are the entries of the kQParts partitions of qIn, in partition order; item i
 // is node L.base + i, and writes that index into its parent's child link.
-template <bool EXACT, bool LDS, bool SUB, bool COUNT>
+// MODE: kTravWide, kTravExact or kTravRef (closest-hit traversal, sbr_trace_params.exactTraversal).
+template <int MODE, bool LDS, bool SUB, bool COUNT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) k_trace(TraceArgs A, const TxDesc* __restrict__ txs,
                                                   const DNode* __restrict__ nodes, const DNodeF* __restrict__ gNodesF,
                                                   const DTri* __restrict__ gTris,
@@ -330,13 +337,23 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
   const DNodeF* nodesF = gNodesF;
   const DTri* tris = gTris;
   const DBox32* tbox = gBox;
+  WideTree wide = A.wide;
   size_t sceneB = 0;
   if (LDS) {   // single-precision node records in LDS; FP64 boxes, triangles and pre-test boxes via the caches
-    const int4* s0 = reinterpret_cast<const int4*>(gNodesF);
     int4* d0 = reinterpret_cast<int4*>(smem);
-    for (int i = threadIdx.x; i < A.nodeBytes / 16; i += blockDim.x) d0[i] = s0[i];
+    if (MODE == kTravWide) {   // wide nodes, then their leaves
+      const int4* s0 = reinterpret_cast<const int4*>(A.wide.n4);
+      const int4* s1 = reinterpret_cast<const int4*>(A.wide.leaves);
+      const int n0 = A.wideNodes * (int)(sizeof(DNode4) / 16), n1 = A.wideLeaves * (int)(sizeof(DLeaf) / 16);
+      for (int i = threadIdx.x; i < n0 + n1; i += blockDim.x) d0[i] = i < n0 ? s0[i] : s1[i - n0];
+      wide.n4 = reinterpret_cast<const DNode4*>(smem);
+      wide.leaves = reinterpret_cast<const DLeaf*>(smem + (size_t)n0 * 16);
+    } else {
+      const int4* s0 = reinterpret_cast<const int4*>(gNodesF);
+      for (int i = threadIdx.x; i < A.nodeBytes / 16; i += blockDim.x) d0[i] = s0[i];
+      nodesF = reinterpret_cast<const DNodeF*>(smem);
+    }
     sceneB = (size_t)A.nodeBytes;
-    nodesF = reinterpret_cast<const DNodeF*>(smem);
     __syncthreads();
   }
   TravStack st;
@@ -350,7 +367,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
   // An overflow at an earlier depth (flags[0] != 0) means the trace is redone with more room: this
   // depth's queue may hold unwritten entries, so nothing is read from it.
   if (A.depth > 0 && flags[0] != 0ull) return;
-  unsigned long long cntNodes = 0, cntTests = 0, cntExact = 0;
+  unsigned long long cntNodes = 0, cntTests = 0, cntExact = 0, cntRedo = 0;
   // this wave's partition of the next depth's queue
   const int part = (int)(((unsigned long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kQParts);
   unsigned long long* partCnt = A.qCnt + ((size_t)(A.depth + 1) * kQParts + part) * kQStride;
@@ -432,7 +449,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
       nd.pad0 = 0;
       const V3 src = ld3(nd.src), dir = ld3(nd.dir);
       Hit h;
-      if (closest_hit<EXACT, SUB, COUNT>(nodes, nodesF, tris, tbox, subs, subTri, A.ext, src, dir, h, st)) {
+      bool hit;
+      if (MODE == kTravWide)
+        hit = closest_hit_wide<SUB, COUNT>(wide, nodes, gNodesF, tris, tbox, subs, subTri, A.ext, src, dir, h, st);
+      else
+        hit = closest_hit<MODE == kTravExact, SUB, COUNT>(nodes, nodesF, tris, tbox, subs, subTri, A.ext, src, dir, h, st);
+      if (hit) {
         const DTri& t = tris[h.idx];
         const V3 hp = hit_point(src, dir, h);
         nd.tgt[0] = hp.x; nd.tgt[1] = hp.y; nd.tgt[2] = hp.z;
@@ -457,6 +479,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
         cntNodes += h.nodes;
         cntTests += h.tests;
         cntExact += h.exact64;
+        cntRedo += h.redone;
       }
     }
     // Wave-aggregated append of the children to the wave's partition of the next depth's queue.
@@ -505,14 +528,26 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))
     atomicAdd(&flags[6], cntNodes);
     atomicAdd(&flags[7], cntTests);
     atomicAdd(&flags[5], cntExact);
+    atomicAdd(&flags[4], cntRedo);
   }
 }
 
 // =============================================================================================
 // 3. Direct-path line of sight (DirectPathCorrection, RTSolver.cpp:375-394)
 // =============================================================================================
-template <bool EXACT>
-__global__ void __launch_bounds__(kBlock) k_los(const TxDesc* __restrict__ txs, int32_t nTx,
+// Closest hit in traversal mode MODE (kTravWide, kTravExact, kTravRef).
+template <int MODE>
+__device__ __forceinline__ bool closest_hit_mode(const WideTree& w, const DNode* __restrict__ nodes,
+                                                 const DNodeF* __restrict__ nodesF, const DTri* __restrict__ tris,
+                                                 const DBox32* __restrict__ tbox, const DSub* __restrict__ subs,
+                                                 const int32_t* __restrict__ subTri, float ext, V3 o, V3 d, Hit& h,
+                                                 const TravStack& st) {
+  if (MODE == kTravWide) return closest_hit_wide<true>(w, nodes, nodesF, tris, tbox, subs, subTri, ext, o, d, h, st);
+  return closest_hit<MODE == kTravExact, true>(nodes, nodesF, tris, tbox, subs, subTri, ext, o, d, h, st);
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_los(const TxDesc* __restrict__ txs, int32_t nTx, WideTree w,
                                                 const DNode* __restrict__ nodes, const DNodeF* __restrict__ nodesF,
                                                 const DTri* __restrict__ tris,
                                                 const DBox32* __restrict__ tbox, const DSub* __restrict__ subs,
@@ -528,13 +563,13 @@ __global__ void __launch_bounds__(kBlock) k_los(const TxDesc* __restrict__ txs, 
   V3 tx = v3(T.tx[0], T.tx[1], T.tx[2]);
   V3 d = vsub(ld3(rx + 3 * (k % m)), tx);
   Hit h;
-  bool hh = closest_hit<EXACT, true>(nodes, nodesF, tris, tbox, subs, subTri, ext, tx, d, h,
-                                     lds_stack(stkN, stkT, blockDim.x, stackDepth));
+  bool hh = closest_hit_mode<MODE>(w, nodes, nodesF, tris, tbox, subs, subTri, ext, tx, d, h,
+                                   lds_stack(stkN, stkT, blockDim.x, stackDepth));
   los[k] = (!hh || vnorm(d) < h.dist) ? 1 : 0;
 }
 
-template <bool EXACT>
-__global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restrict__ nodes,
+template <int MODE>
+__global__ void __launch_bounds__(kBlock) k_closest_batch(WideTree w, const DNode* __restrict__ nodes,
                                                           const DNodeF* __restrict__ nodesF,
                                                           const DTri* __restrict__ tris,
                                                           const DBox32* __restrict__ tbox,
@@ -549,8 +584,8 @@ __global__ void __launch_bounds__(kBlock) k_closest_batch(const DNode* __restric
   int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   Hit h;
-  bool hh = closest_hit<EXACT, true>(nodes, nodesF, tris, tbox, subs, subTri, ext, ld3(org + 3 * (int64_t)i),
-                               ld3(dirs + 3 * (int64_t)i), h, lds_stack(stkN, stkT, blockDim.x, stackDepth));
+  bool hh = closest_hit_mode<MODE>(w, nodes, nodesF, tris, tbox, subs, subTri, ext, ld3(org + 3 * (int64_t)i),
+                                   ld3(dirs + 3 * (int64_t)i), h, lds_stack(stkN, stkT, blockDim.x, stackDepth));
   triOut[i] = hh ? tris[h.idx].id : -1;
   const V3 hp = hit_point(ld3(org + 3 * (int64_t)i), ld3(dirs + 3 * (int64_t)i), h);
   ptOut[3 * i] = hp.x; ptOut[3 * i + 1] = hp.y; ptOut[3 * i + 2] = hp.z;
@@ -2274,6 +2309,19 @@ static int scratch_address(int dev, unsigned char** out) {
   return SBR_OK;
 }
 
+WideTree Engine::wide_tree() const { return WideTree{dNodes4.p, dLeaves.p, dVer.p}; }
+
+// One closest-hit query per ray on the device (mode: kTravWide, kTravExact, kTravRef).
+hipError_t Engine::launch_closest(int mode, const double* o, const double* d, int n, int32_t* tri, double* pt,
+                                  double* dist) {
+  const size_t sm = stack_bytes(stackDepth(), kBlock);
+  auto k = mode == kTravExact ? k_closest_batch<kTravExact>
+                              : mode == kTravRef ? k_closest_batch<kTravRef> : k_closest_batch<kTravWide>;
+  k<<<blocks_for(n, kBlock), kBlock, sm, stream>>>(wide_tree(), dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
+                                                   dSubTri.p, sceneExtent, o, d, n, stackDepth(), tri, pt, dist);
+  return hipGetLastError();
+}
+
 int Engine::closest_hit_noalloc(const double* o, const double* d, int n, int exact, int32_t* tri, double* pt,
                                 double* dist) {
   if (device < 0) return fail(SBR_E_STATE, "host-only context has no device");
@@ -2296,16 +2344,7 @@ int Engine::closest_hit_noalloc(const double* o, const double* d, int n, int exa
   // previous call left in the array is never read.
   HIPCHK(hipMemcpyAsync(sO, o, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(sD, d, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
-  const size_t sm = stack_bytes(stackDepth(), kBlock);
-  if (exact)
-    k_closest_batch<true><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
-                                                                         dSubTri.p, sceneExtent, sO, sD, n, stackDepth(),
-                                                                         sT, sP, sDist);
-  else
-    k_closest_batch<false><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dNodesF.p, dTris.p, dTriBox32.p,
-                                                                          dSub.p, dSubTri.p, sceneExtent, sO, sD, n,
-                                                                          stackDepth(), sT, sP, sDist);
-  HIPCHK(hipGetLastError());
+  HIPCHK(launch_closest(exact, sO, sD, n, sT, sP, sDist));
   HIPCHK(hipMemcpyAsync(tri, sT, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(pt, sP, 3 * sizeof(double) * n, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(dist, sDist, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
@@ -2324,16 +2363,7 @@ int Engine::closest_hit(const double* o, const double* d, int n, int exact, int3
   if (!dO.p || !dD.p || !dP.p || !dDist.p || !dT.p) return fail(SBR_E_NOMEM, "closest_hit buffers");
   HIPCHK(hipMemcpyAsync(dO.p, o, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
   HIPCHK(hipMemcpyAsync(dD.p, d, 3 * sizeof(double) * n, hipMemcpyHostToDevice, stream));
-  const size_t sm = stack_bytes(stackDepth(), kBlock);
-  if (exact)
-    k_closest_batch<true><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
-                                                                         dSubTri.p, sceneExtent, dO.p, dD.p, n,
-                                                                         stackDepth(), dT.p, dP.p, dDist.p);
-  else
-    k_closest_batch<false><<<blocks_for(n, kBlock), kBlock, sm, stream>>>(dNodes.p, dNodesF.p, dTris.p, dTriBox32.p, dSub.p,
-                                                                          dSubTri.p, sceneExtent, dO.p, dD.p, n,
-                                                                          stackDepth(), dT.p, dP.p, dDist.p);
-  HIPCHK(hipGetLastError());
+  HIPCHK(launch_closest(exact, dO.p, dD.p, n, dT.p, dP.p, dDist.p));
   HIPCHK(hipMemcpyAsync(tri, dT.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(pt, dP.p, 3 * sizeof(double) * n, hipMemcpyDeviceToHost, stream));
   HIPCHK(hipMemcpyAsync(dist, dDist.p, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
@@ -2348,7 +2378,7 @@ static double ms_since(std::chrono::steady_clock::time_point t) {
 // Bounce loop of the root items [rootBase, rootBase + nRoot): one persistent k_trace launch per
 // depth, no host synchronisation in between.  Returns SBR_E_LIMIT when the node or queue buffers
 // overflowed, with *need a lower bound of the nodes the window needs; *ms is the kernels' time.
-int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_t nRoot, int maxDepth, bool exact,
+int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_t nRoot, int maxDepth, int mode,
                         bool count, const RxGrid* capG, int64_t* need, float* ms) {
   *need = 0;
   *ms = 0;
@@ -2365,32 +2395,47 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
     const int64_t e = perBlock > scene ? (int64_t)((perBlock - scene) / ((size_t)kBlock * 8)) : 0;
     return (int)std::max<int64_t>(4, std::min<int64_t>(depthAll, e));
   };
-  const int kGlob = ldsEntries(0), kLds = ldsEntries(sceneBytes);
+  // LDS-staged part of the scene: the wide nodes and leaves, or the reference tree's records.
+  const size_t stageBytes = mode == kTravWide ? wideBytes : sceneBytes;
+  const int kGlob = ldsEntries(0), kLds = ldsEntries(stageBytes);
   const size_t stk = stack_bytes(kGlob, kBlock), stkL = stack_bytes(kLds, kBlock);
   // Scene in LDS or read through the caches: automatic choice takes LDS staging unless it lowers the
   // number of resident blocks per CU.
-  const bool ldsFits = sceneBytes + stkL <= kSceneLdsMax;
+  const bool ldsFits = stageBytes + stkL <= kSceneLdsMax;
   const bool sub = !hSub.empty();
-  auto variant = [&](bool l) {
+  using KT = decltype(&k_trace<kTravWide, false, false, false>);
+  auto pick = [](bool l, bool sb, auto k00, auto k01, auto k10, auto k11) -> KT {
+    return l ? (sb ? k11 : k10) : (sb ? k01 : k00);
+  };
+  auto variant = [&](bool l) -> KT {
     if (count) {
-      if (exact) return l ? k_trace<true, true, false, true> : k_trace<true, false, false, true>;
-      if (sub) return l ? k_trace<false, true, true, true> : k_trace<false, false, true, true>;
-      return l ? k_trace<false, true, false, true> : k_trace<false, false, false, true>;
+      if (mode == kTravExact) return l ? k_trace<kTravExact, true, false, true> : k_trace<kTravExact, false, false, true>;
+      if (mode == kTravRef)
+        return pick(l, sub, k_trace<kTravRef, false, false, true>, k_trace<kTravRef, false, true, true>,
+                    k_trace<kTravRef, true, false, true>, k_trace<kTravRef, true, true, true>);
+      return pick(l, sub, k_trace<kTravWide, false, false, true>, k_trace<kTravWide, false, true, true>,
+                  k_trace<kTravWide, true, false, true>, k_trace<kTravWide, true, true, true>);
     }
-    if (exact) return l ? k_trace<true, true, false, false> : k_trace<true, false, false, false>;
-    if (sub) return l ? k_trace<false, true, true, false> : k_trace<false, false, true, false>;
-    return l ? k_trace<false, true, false, false> : k_trace<false, false, false, false>;
+    if (mode == kTravExact) return l ? k_trace<kTravExact, true, false, false> : k_trace<kTravExact, false, false, false>;
+    if (mode == kTravRef)
+      return pick(l, sub, k_trace<kTravRef, false, false, false>, k_trace<kTravRef, false, true, false>,
+                  k_trace<kTravRef, true, false, false>, k_trace<kTravRef, true, true, false>);
+    return pick(l, sub, k_trace<kTravWide, false, false, false>, k_trace<kTravWide, false, true, false>,
+                k_trace<kTravWide, true, false, false>, k_trace<kTravWide, true, true, false>);
   };
   int occGlobal = 1, occLds = 0;
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occGlobal, variant(false), kBlock, stk));
-  if (ldsFits) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occLds, variant(true), kBlock, sceneBytes + stkL));
+  if (ldsFits) HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occLds, variant(true), kBlock, stageBytes + stkL));
   bool lds = ldsFits && (tune.ldsScene == 1 || (tune.ldsScene < 0 && occLds >= occGlobal));
   const int kStack = lds ? kLds : kGlob;
-  const size_t smem = (lds ? sceneBytes : 0) + stack_bytes(kStack, kBlock);
+  const size_t smem = (lds ? stageBytes : 0) + stack_bytes(kStack, kBlock);
   TraceArgs A{};
   A.nTx = (int32_t)txs.size();
   A.stackDepth = stackDepth();
-  A.nodeBytes = (int32_t)(hNodesF.size() * sizeof(DNodeF));
+  A.nodeBytes = (int32_t)stageBytes;
+  A.wide = wide_tree();
+  A.wideNodes = (int32_t)hNodes4.size();
+  A.wideLeaves = (int32_t)hLeaves.size();
   A.ext = sceneExtent;
   A.rootBase = rootBase;
   A.nodeCap = nodeCap;
@@ -2478,6 +2523,7 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   stats.node_visits += count ? (int64_t)ctr[6] : 0;
   stats.tri_tests += count ? (int64_t)ctr[7] : 0;
   stats.box_tests_fp64 += count ? (int64_t)ctr[5] : 0;
+  stats.wide_redone += count ? (int64_t)ctr[4] : 0;
   return SBR_OK;
 }
 
@@ -2507,7 +2553,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   // ---- transmitters and their launch directions
   std::vector<TxDesc> txs(nt);
   int maxDepth = 0;
-  bool exact = false, count = false, anyOcclusion = false, anyModel0 = false;
+  bool anyExact = false, anyRef = false, count = false, anyOcclusion = false, anyModel0 = false;
+  int mode = kTravWide;   // one traversal per call: the reference order if any transmitter asks for it
   int64_t nRoot = 0;
   for (int t = 0; t < nt; t++) {
     const sbr_trace_params& p = P[t];
@@ -2522,8 +2569,11 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     if (p.antennaType == 3 && patNT == 0) return fail(SBR_E_STATE, "antennaType 3 needs a pattern (sbr_set_tx_pattern)");
     if (p.receptionModel < 0 || p.receptionModel > 1) return fail(SBR_E_ARG, "receptionModel must be 0 or 1");
     if (!(p.frequency > 0)) return fail(SBR_E_ARG, "frequency must be > 0");
-    exact = exact || p.exactTraversal;
+    if (p.exactTraversal < 0 || p.exactTraversal > 2) return fail(SBR_E_ARG, "exactTraversal must be 0, 1 or 2");
+    anyExact = anyExact || p.exactTraversal == kTravExact;
+    anyRef = anyRef || p.exactTraversal == kTravRef;
     count = count || p.countTraversal;
+    mode = anyExact ? kTravExact : anyRef ? kTravRef : kTravWide;
     TxDesc& T = txs[t];
     memset(&T, 0, sizeof T);
     memcpy(T.tx, p.tx, sizeof T.tx);
@@ -2638,14 +2688,9 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       HIPCHK(hipEventRecord(evSide[0], stream));
       HIPCHK(hipStreamWaitEvent(sideStream, evSide[0], 0));
     }
-    if (exact)
-      k_los<true><<<blocks_for(nOut, kBlock), kBlock, sm, ls>>>(dTx.p, nt, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p,
-                                                                 dSub.p, dSubTri.p, sceneExtent, dRx.p, m, stackDepth(),
-                                                                 dLos.p);
-    else
-      k_los<false><<<blocks_for(nOut, kBlock), kBlock, sm, ls>>>(dTx.p, nt, dNodes.p, dNodesF.p, dTris.p, dTriBox32.p,
-                                                                  dSub.p, dSubTri.p, sceneExtent, dRx.p, m, stackDepth(),
-                                                                  dLos.p);
+    auto kl = mode == kTravExact ? k_los<kTravExact> : mode == kTravRef ? k_los<kTravRef> : k_los<kTravWide>;
+    kl<<<blocks_for(nOut, kBlock), kBlock, sm, ls>>>(dTx.p, nt, wide_tree(), dNodes.p, dNodesF.p, dTris.p, dTriBox32.p,
+                                                     dSub.p, dSubTri.p, sceneExtent, dRx.p, m, stackDepth(), dLos.p);
     HIPCHK(hipGetLastError());
     if (side) {
       HIPCHK(hipEventRecord(evSide[1], sideStream));
@@ -2736,7 +2781,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         nodeCap = std::min(dTree.n, std::min(dQueue[0].n, dQueue[1].n));
         if (m > 0) nodeCap = std::min(nodeCap, std::min(dCand.n, dCandFlag.n));
         int64_t need = 0;
-        int rc = trace_stage(txs, ws, W, maxDepth, exact, count, m > 0 ? &g : nullptr, &need, &msT);
+        int rc = trace_stage(txs, ws, W, maxDepth, mode, count, m > 0 ? &g : nullptr, &need, &msT);
         if (rc == SBR_OK) break;
         if (rc != SBR_E_LIMIT || attempt > 40) return rc == SBR_E_LIMIT ? fail(rc, "tree buffers could not be sized") : rc;
         // Overflowed (need is only a lower bound): first use the whole window budget, then halve

@@ -44,15 +44,20 @@ SBR_HD RayF rayf_pre(V3 o, V3 d, float ext) {
 }
 
 // True if what lies behind the padded box [lo, hi] may still hold a hit at t <= clip; *entry is
-// then a lower bound of the t of every such hit.
-SBR_HD bool box32_may_hit(const float* lo, const float* hi, const RayF& f, float clip, float* entry) {
-  const float ax = fmaf(lo[0], f.ix, f.nx), bx = fmaf(hi[0], f.ix, f.nx);
-  const float ay = fmaf(lo[1], f.iy, f.ny), by = fmaf(hi[1], f.iy, f.ny);
-  const float az = fmaf(lo[2], f.iz, f.nz), bz = fmaf(hi[2], f.iz, f.nz);
+// then a lower bound of the t of every such hit.  (Bounds given per coordinate.)
+SBR_HD bool box32_may_hit(float lx, float ly, float lz, float hx, float hy, float hz, const RayF& f, float clip,
+                          float* entry) {
+  const float ax = fmaf(lx, f.ix, f.nx), bx = fmaf(hx, f.ix, f.nx);
+  const float ay = fmaf(ly, f.iy, f.ny), by = fmaf(hy, f.iy, f.ny);
+  const float az = fmaf(lz, f.iz, f.nz), bz = fmaf(hz, f.iz, f.nz);
   const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
   const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
   *entry = tn;
   return tn <= tf && tf >= 0.0f && tn <= clip;
+}
+
+SBR_HD bool box32_may_hit(const float* lo, const float* hi, const RayF& f, float clip, float* entry) {
+  return box32_may_hit(lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], f, clip, entry);
 }
 
 // Non-empty computed interval of the box [lo, hi] (no clipping).

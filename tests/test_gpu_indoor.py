@@ -1,6 +1,6 @@
 """Dense indoor scene (about 207k triangles, tessellated floor and ceiling that the reference SAH
-leaves as leaves of thousands of triangles): closest hits of the leaf-BVH search equal the exact
-traversal and the oracle, including rays aimed at shared vertices and edges of the tessellation
+leaves as leaves of thousands of triangles): closest hits of the leaf-BVH search (through the wide
+tree, and through the reference tree with its clipping cuts) equal the exact traversal and the oracle, including rays aimed at shared vertices and edges of the tessellation
 (ties broken by DFS position); a small trace equals the oracle."""
 import numpy as np
 import pytest
@@ -44,9 +44,11 @@ def test_indoor_closest_hit_leaf_bvh_exact(indoor):
     cfg, ctx, sc = indoor
     org, d = _rays(3000, 11)
     print("[case] indoor closest hit", flush=True)
-    t1, p1, d1 = ctx.closest_hit(org, d, exact=False)
-    t0, p0, d0 = ctx.closest_hit(org, d, exact=True)
+    t1, p1, d1 = ctx.closest_hit(org, d, exact=0)
+    t0, p0, d0 = ctx.closest_hit(org, d, exact=1)
+    t2, p2, d2 = ctx.closest_hit(org, d, exact=2)
     assert np.array_equal(t1, t0) and np.array_equal(p1, p0) and np.array_equal(d1, d0)
+    assert np.array_equal(t2, t0) and np.array_equal(p2, p0) and np.array_equal(d2, d0)
     for i in range(0, len(d), 3):
         ot, op, od = sc.closest_hit(org[i], d[i])
         assert t1[i] == ot, (i, t1[i], ot)

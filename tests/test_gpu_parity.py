@@ -54,7 +54,7 @@ def _rand_rays(n, seed, center, spread):
 
 @pytest.mark.parametrize("name", ["ground.stl", "cube.stl", "office.stl", "bahen.stl", "menger_sponge.stl",
                                   "two_icospheres.stl"])
-@pytest.mark.parametrize("exact", [0, 1])
+@pytest.mark.parametrize("exact", [0, 1, 2])
 def test_closest_hit_matches_oracle(gpu_ctx, name, exact):
     say("closest_hit", name, exact)
     gpu_ctx.scene_from_stl(_stl(name))
@@ -62,7 +62,7 @@ def test_closest_hit_matches_oracle(gpu_ctx, name, exact):
     tr = sc.triangles()
     vmin, vmax = tr["verts"].reshape(-1, 3).min(0), tr["verts"].reshape(-1, 3).max(0)
     org, d = _rand_rays(2000, 7, (vmin + vmax) / 2, (vmax - vmin) * 1.2 + 1e-3)
-    tri, pt, dist = gpu_ctx.closest_hit(org, d, exact=bool(exact))
+    tri, pt, dist = gpu_ctx.closest_hit(org, d, exact=exact)
     for i in range(len(d)):
         ot, op, od = sc.closest_hit(org[i], d[i])
         assert tri[i] == ot, (i, tri[i], ot)
@@ -103,6 +103,7 @@ CASES = [
     ("urban_L3_R6T1_model0", "c2", dict(tess=3), None),
     ("urban_L4_R6T1_model1", "c2", dict(tess=4, receptionModel=1), None),
     ("urban_L3_exact_traversal", "c2", dict(tess=3, exactTraversal=1), None),
+    ("urban_L3_reference_tree_clipped", "c2", dict(tess=3, exactTraversal=2), None),
     ("urban_L3_physical_indexing", "c2", dict(tess=3, q1MeshIndexing=0), None),
     ("urban_L3_total_cap3", "c2", dict(tess=3, maxTotal=3), None),
     ("urban_L3_isotropic_nofresnel", "c2", dict(tess=3, antennaType=0, useFresnel=False), None),
