@@ -462,17 +462,17 @@ __device__ __forceinline__ void wide_hit(const DNode4* __restrict__ n4, const DL
 
 // Whether the reference's traversal reaches the reference-tree leaf `node`: its box and those of
 // all its ancestors pass the FP64 slab test.  Walking up, a node whose shrunk box the line crosses
-// (single precision) settles it and all its ancestors; other nodes take the FP64 test.
+// (single precision) settles it and all its ancestors; other nodes (flat leaves, mostly) take the
+// reference's own slab test with its divisions (box_line), too rare to warrant the reciprocals.
 template <bool COUNT>
 __device__ __forceinline__ bool wide_reaches(int node, const DVer* __restrict__ ver, const DNode* __restrict__ nodes,
                                              const RayF& f, V3 o, V3 d, Hit& h) {
-  const RayPre r = ray_pre(o, d);
   while (node >= 0) {
     const DVer v = ver[node];
     if (v.shrunk && box32_crossed(v.lo, v.hi, f)) return true;
     if (COUNT) h.exact64++;
     double lo, hi;
-    if (!box_test(nodes[node].bmin, nodes[node].bmax, r, &lo, &hi)) return false;
+    if (!box_line(nodes[node].bmin, nodes[node].bmax, o, d, &lo, &hi)) return false;
     node = v.parent;
   }
   return true;

@@ -262,6 +262,7 @@ constexpr size_t kPinLevels = 0;         // LevelRec[kMaxDepth + 2] read back
 constexpr size_t kPinLevels0 = 1024;     // LevelRec[kMaxDepth + 2] sent
 constexpr size_t kPinTraceCtr = 2048;    // 8 counters of the bounce kernel
 constexpr size_t kPinCapCtr = 2176;      // capture counters (10), record total and largest partition, listed nodes
+constexpr size_t kPinSegTotal = 2304;    // segments of a window's kept paths (multi-window traces)
 constexpr size_t kPinSmall = 4096;
 
 // Launch configuration of the traversal kernel (sbr_set_tuning, or chosen by sbr_autotune).
@@ -276,6 +277,15 @@ struct PathCache {
   std::vector<int32_t> ptx, prx, plaunch, pnseg, stri, scop, stype;
   std::vector<int64_t> poff;
   std::vector<double> ssrc, stgt, pfield;
+};
+// Kept paths of a multi-window trace, appended window by window on the device (a window's paths in
+// (tx, rx), then launch order; segments at off).  Copied to the host cache, in the order of a
+// single-window trace, when first asked for.
+struct PathStore {
+  DBuf<int32_t> txrx, launch, nseg, tri, cop, type;
+  DBuf<int64_t> off;
+  DBuf<double> field, src, tgt;
+  int64_t np = 0, ns = 0;
 };
 
 struct Engine {
@@ -420,8 +430,11 @@ struct Engine {
   // trace's size (the result of the last trace is dropped; the scene and directions stay).
   void trim();
 
-  bool multiWindow = false;   // last trace ran in several launch windows (paths kept in `cache`)
-  PathCache cache;
+  bool multiWindow = false;   // last trace ran in several launch windows (paths kept in `store`)
+  PathStore store;
+  PathCache cache;            // host copy of `store`, made on the first request
+  bool cacheReady = false;
+  DBuf<int64_t> dSegCnt, dSegOff;
 
  private:
   int generate_rays(int tess, DBuf<double>& out, int64_t* nOut);
@@ -432,7 +445,8 @@ struct Engine {
   int paths_window(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, int64_t* poff, int32_t* stri,
                    int32_t* scop, int32_t* stype, double* ssrc, double* stgt, int64_t* nPaths, int64_t* nSegs,
                    bool countOnly);
-  int cache_window_paths(int64_t w);
+  int store_window_paths(int64_t w, int maxDepth);
+  int ensure_path_cache();
   int finish_path_cache();
 };
 

@@ -2119,6 +2119,37 @@ __global__ void k_paths(int64_t W, int32_t nrx, const TxDesc* __restrict__ txs, 
   }
 }
 
+// Per kept path of a window (sorted position w): the row of the path store (tx * nrx + rx, launch,
+// segment count, field) and the segment count for the offsets' scan.
+__global__ void k_path_meta(int64_t W, const TxDesc* __restrict__ txs, int32_t nrx,
+                            const unsigned long long* __restrict__ winKey, const int32_t* __restrict__ winPerm,
+                            const int32_t* __restrict__ winTxRx, const double* __restrict__ winField,
+                            int64_t* __restrict__ segCnt, int32_t* __restrict__ oTxRx, int32_t* __restrict__ oLaunch,
+                            int32_t* __restrict__ oNseg, double* __restrict__ oField) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const unsigned long long k = winKey[winPerm[w]];
+  const int32_t nseg = (int32_t)((k >> 27) & 31ull) + 1;
+  const int64_t gl = (int64_t)(k >> 32);
+  const int32_t tr = winTxRx[w];
+  const TxDesc& T = txs[tr / nrx];
+  oTxRx[w] = tr;
+  oLaunch[w] = (int32_t)(T.launchOffset + (gl - T.launchStart) * T.launchStride);
+  oNseg[w] = nseg;
+  segCnt[w] = nseg;
+  for (int c = 0; c < 6; c++) oField[6 * w + c] = winField[6 * w + c];
+}
+
+// Store offsets of a window's paths (segBase + the window's exclusive scan) and, in segOff[W], the
+// window's segment total.
+__global__ void k_path_off(int64_t W, int64_t segBase, const int64_t* __restrict__ segCnt, int64_t* __restrict__ segOff,
+                           int64_t* __restrict__ oOff) {
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  oOff[w] = segBase + segOff[w];
+  if (w == W - 1) segOff[W] = segOff[w] + segCnt[w];
+}
+
 // =============================================================================================
 // Host orchestration
 // =============================================================================================
@@ -2741,6 +2772,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     winners = 0;
     multiWindow = false;
     cache = PathCache{};
+    cacheReady = false;
+    store.np = store.ns = 0;
     if (m > 0) {
       ENSURE(dTabA, cap);
       ENSURE(dSeqTab, cap);
@@ -2965,6 +2998,10 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         HIPCHK(hipGetLastError());
         k_rx_sum<<<blocks_for(nOut, 256), 256, 0, stream>>>(dWinRx.p, Wn, dWinField.p, nOut, dField.p);
         HIPCHK(hipGetLastError());
+        if (multiWindow) {   // this window's paths to the device store while its trees are resident
+          int rc = store_window_paths(Wn, maxDepth);
+          if (rc) return rc;
+        }
       }
       HIPCHK(hipEventRecord(kev[3], stream));
       HIPCHK(hipStreamSynchronize(stream));
@@ -2972,8 +3009,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       stats.ms_field += kms;
       winners = Wn;
       if (multiWindow && Wn > 0) {
-        int rc = cache_window_paths(Wn);
-        if (rc) return rc;
+        store.np += Wn;
+        store.ns += *pinSmall.at<int64_t>(kPinSegTotal);
       }
       ws += std::max<int64_t>(W, 1);
     }
@@ -2986,11 +3023,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   if (!done) return fail(SBR_E_LIMIT, "dedup table could not be sized");
   if (nRoot > 0 && perRootMax > 0) perRootHint = perRootMax;
   if (m > 0) tabCapHint = cap;
-  if (multiWindow) {
-    int rc = finish_path_cache();
-    if (rc) return rc;
-    winners = (int64_t)cache.prx.size();
-  }
+  if (multiWindow) winners = store.np;
   HIPCHK(hipEventRecord(ev[4], stream));
   HIPCHK(hipStreamSynchronize(stream));
   stats.winners = winners;
@@ -3101,6 +3134,13 @@ int Engine::paths(int32_t* ptx, int32_t* prx, int32_t* plaunch, int32_t* pnseg, 
   if (!haveResult) return fail(SBR_E_STATE, "no trace result");
   if (!multiWindow)
     return paths_window(ptx, prx, plaunch, pnseg, poff, stri, scop, stype, ssrc, stgt, nPaths, nSegs, countOnly);
+  if (countOnly) {
+    *nPaths = store.np;
+    *nSegs = store.ns;
+    return SBR_OK;
+  }
+  int rc = ensure_path_cache();
+  if (rc) return rc;
   const int64_t np = (int64_t)cache.prx.size(), ns = (int64_t)cache.stri.size();
   *nPaths = np;
   *nSegs = ns;
@@ -3123,6 +3163,8 @@ int Engine::path_tx(int32_t* out, int64_t n) {
   if (n != winners || (!out && n > 0)) return fail(SBR_E_ARG, "path tx: count mismatch");
   if (n == 0) return SBR_OK;
   if (multiWindow) {
+    int rc = ensure_path_cache();
+    if (rc) return rc;
     std::copy(cache.ptx.begin(), cache.ptx.end(), out);
     return SBR_OK;
   }
@@ -3138,6 +3180,8 @@ int Engine::path_fields(double* out, int64_t n) {
   if (n != winners || (!out && n > 0)) return fail(SBR_E_ARG, "path fields: count mismatch");
   if (n == 0) return SBR_OK;
   if (multiWindow) {
+    int rc = ensure_path_cache();
+    if (rc) return rc;
     std::copy(cache.pfield.begin(), cache.pfield.end(), out);
     return SBR_OK;
   }
@@ -3146,34 +3190,97 @@ int Engine::path_fields(double* out, int64_t n) {
   return SBR_OK;
 }
 
-// Appends the current window's kept paths (winners w in dWin*) to the host cache.
-int Engine::cache_window_paths(int64_t w) {
-  int64_t np = 0, ns = 0;
-  int rc = paths_window(nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &np,
-                        &ns, true);
+// Grows a device buffer to hold `want` elements, keeping the first `used` (the stream waits for the
+// copy before the old buffer is freed).
+template <typename T>
+static bool grow_keep(DBuf<T>& b, size_t used, size_t want, hipStream_t s) {
+  if (want <= b.n && b.p) return true;
+  DBuf<T> nb;
+  nb.ensure(std::max(want, 2 * b.n));
+  if (!nb.p) return false;
+  if (used > 0 && (hipMemcpyAsync(nb.p, b.p, used * sizeof(T), hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                   hipStreamSynchronize(s) != hipSuccess))
+    return false;
+  std::swap(b.p, nb.p);
+  std::swap(b.n, nb.n);
+  return true;   // nb now holds the old buffer and frees it
+}
+
+// Appends the current window's kept paths (winners w in dWin*, Wn of them) to the device store:
+// rows, segment offsets (scan of the counts) and segments (k_paths).  Enqueued only; the window's
+// segment total lands in pinned memory for the host to add after the window's synchronisation.
+int Engine::store_window_paths(int64_t Wn, int maxDepth) {
+  const int64_t np = store.np, ns = store.ns;
+  const size_t segMax = (size_t)(ns + Wn * (int64_t)(maxDepth + 1));
+  if (!grow_keep(store.txrx, np, np + Wn, stream) || !grow_keep(store.launch, np, np + Wn, stream) ||
+      !grow_keep(store.nseg, np, np + Wn, stream) || !grow_keep(store.off, np, np + Wn, stream) ||
+      !grow_keep(store.field, 6 * np, 6 * (np + Wn), stream) || !grow_keep(store.tri, ns, segMax, stream) ||
+      !grow_keep(store.cop, ns, segMax, stream) || !grow_keep(store.type, ns, segMax, stream) ||
+      !grow_keep(store.src, 3 * ns, 3 * segMax, stream) || !grow_keep(store.tgt, 3 * ns, 3 * segMax, stream))
+    return fail(SBR_E_NOMEM, "path store");
+  ENSURE(dSegCnt, Wn);
+  ENSURE(dSegOff, Wn + 1);
+  k_path_meta<<<blocks_for(Wn, 256), 256, 0, stream>>>(Wn, dTx.p, nrx, dWinKey2.p, dWinIdx2.p, dWinRx.p, dWinField.p,
+                                                        dSegCnt.p, store.txrx.p + np, store.launch.p + np,
+                                                        store.nseg.p + np, store.field.p + 6 * np);
+  HIPCHK(hipGetLastError());
+  size_t tb = 0;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, dSegCnt.p, dSegOff.p, (int)Wn, stream));
+  ENSURE(dScratch, tb);
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(dScratch.p, tb, dSegCnt.p, dSegOff.p, (int)Wn, stream));
+  k_path_off<<<blocks_for(Wn, 256), 256, 0, stream>>>(Wn, ns, dSegCnt.p, dSegOff.p, store.off.p + np);
+  HIPCHK(hipGetLastError());
+  const DSurf* surf = lastParams[0].q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
+  k_paths<<<blocks_for(Wn, 128), 128, 0, stream>>>(Wn, nrx, dTx.p, dTree.p, surf, dRx.p, dWinRx.p, dWinIdx2.p, dWinNode.p,
+                                                    dSegOff.p, store.tri.p + ns, store.cop.p + ns, store.type.p + ns,
+                                                    store.src.p + 3 * ns, store.tgt.p + 3 * ns);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(pinSmall.at<int64_t>(kPinSegTotal), dSegOff.p + Wn, sizeof(int64_t), hipMemcpyDeviceToHost,
+                        stream));
+  return SBR_OK;
+}
+
+// The host copy of the path store, in the order of a single-window trace.
+int Engine::ensure_path_cache() {
+  if (!multiWindow || cacheReady) return SBR_OK;
+  HIPCHK(hipSetDevice(device));
+  const int64_t np = store.np, ns = store.ns;
+  PathCache c;
+  std::vector<int32_t> txrx(np);
+  c.ptx.resize(np);
+  c.prx.resize(np);
+  c.plaunch.resize(np);
+  c.pnseg.resize(np);
+  c.poff.resize(np);
+  c.pfield.resize(6 * np);
+  c.stri.resize(ns);
+  c.scop.resize(ns);
+  c.stype.resize(ns);
+  c.ssrc.resize(3 * ns);
+  c.stgt.resize(3 * ns);
+  if (np > 0) {
+    HIPCHK(hipMemcpyAsync(txrx.data(), store.txrx.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(c.plaunch.data(), store.launch.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(c.pnseg.data(), store.nseg.p, sizeof(int32_t) * np, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(c.poff.data(), store.off.p, sizeof(int64_t) * np, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(c.pfield.data(), store.field.p, sizeof(double) * 6 * np, hipMemcpyDeviceToHost, stream));
+  }
+  if (ns > 0) {
+    HIPCHK(hipMemcpyAsync(c.stri.data(), store.tri.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(c.scop.data(), store.cop.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(c.stype.data(), store.type.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(c.ssrc.data(), store.src.p, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(c.stgt.data(), store.tgt.p, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, stream));
+  }
+  HIPCHK(hipStreamSynchronize(stream));
+  for (int64_t k = 0; k < np; k++) {
+    c.ptx[k] = txrx[k] / nrx;
+    c.prx[k] = txrx[k] % nrx;
+  }
+  cache = std::move(c);
+  int rc = finish_path_cache();
   if (rc) return rc;
-  if (np != w) return fail(SBR_E_INTERNAL, "window path count mismatch");
-  std::vector<int32_t> ptx(np), prx(np), pl(np), pn(np), st(ns), sc(ns), sy(ns);
-  std::vector<int64_t> po(np);
-  std::vector<double> s0(3 * ns), s1(3 * ns), pf(6 * np);
-  rc = paths_window(ptx.data(), prx.data(), pl.data(), pn.data(), po.data(), st.data(), sc.data(), sy.data(), s0.data(),
-                    s1.data(), &np, &ns, false);
-  if (rc) return rc;
-  HIPCHK(hipMemcpy(pf.data(), dWinField.p, 6 * sizeof(double) * np, hipMemcpyDeviceToHost));
-  const int64_t segBase = (int64_t)cache.stri.size();
-  for (int64_t k = 0; k < np; k++) po[k] += segBase;
-  auto app = [](auto& dst, const auto& src) { dst.insert(dst.end(), src.begin(), src.end()); };
-  app(cache.ptx, ptx);
-  app(cache.prx, prx);
-  app(cache.plaunch, pl);
-  app(cache.pnseg, pn);
-  app(cache.poff, po);
-  app(cache.stri, st);
-  app(cache.scop, sc);
-  app(cache.stype, sy);
-  app(cache.ssrc, s0);
-  app(cache.stgt, s1);
-  app(cache.pfield, pf);
+  cacheReady = true;
   return SBR_OK;
 }
 
