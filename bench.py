@@ -96,9 +96,12 @@ def cpu_single_thread_box_room():
 def kernel_counters(traffic, avg_ms, valu_tflops):
     """What bounds k_trace, from the committed rocprofv3 counter summaries (profiles/): the measured
     HBM traffic as a fraction of peak bandwidth, the FP64 VALU model as a fraction of peak, the
-    lanes active per VALU instruction and the fraction of wave cycles spent waiting.  The algorithmic
-    bytes of the roofline line stay in LDS and the caches (the counters see a fifth of them in HBM),
-    so neither HBM nor VALU peaks bind: the kernel is bound by memory latency at its occupancy."""
+    lanes active per VALU instruction, the fractions of wave cycles spent waiting on results and
+    waiting to issue, and the share of the SIMDs' VALU issue slots the kernel's instructions take.
+    The algorithmic bytes of the roofline line stay in LDS and the caches (the counters see a fifth
+    of them in HBM), so neither HBM nor the FP64 peak binds: the kernel is bound by the latency of
+    its dependent traversal steps and by instruction issue (about half of the VALU slots, a quarter
+    of the wave cycles waiting for issue) at its occupancy."""
     out = {}
     if traffic:
         out["hbm_measured_GBs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 1)
@@ -110,16 +113,24 @@ def kernel_counters(traffic, avg_ms, valu_tflops):
         def num(name):
             m = re.search(r"\| " + name + r" \| ([0-9,]+) \|", txt)
             return float(m.group(1).replace(",", "")) if m else None
-        tc, iv, wait, wc = (num("SQ_THREAD_CYCLES_VALU"), num("SQ_INSTS_VALU"), num("SQ_WAIT_ANY"),
-                            num("SQ_WAVE_CYCLES"))
+        tc, iv, wait, wc, wi = (num("SQ_THREAD_CYCLES_VALU"), num("SQ_INSTS_VALU"), num("SQ_WAIT_ANY"),
+                                num("SQ_WAVE_CYCLES"), num("SQ_WAIT_INST_ANY"))
         if tc and iv:
             out["valu_active_lanes_frac"] = round(tc / iv / 64.0, 4)
         if wait and wc:
             out["wave_wait_frac"] = round(wait / wc, 4)
+        if wi and wc:
+            out["wave_wait_issue_frac"] = round(wi / wc, 4)
+        m = re.search(r"valu_issue_frac:\s*([0-9.]+)", txt)
+        if m:
+            out["valu_issue_frac"] = float(m.group(1))
     fr = [v for k, v in out.items() if k in ("hbm_measured_frac", "valu_fp64_model_frac")]
-    out["binding"] = ("memory latency (waves wait {:.0%} of their cycles; HBM {:.0%} and FP64 VALU {:.0%} of "
-                      "peak)".format(out.get("wave_wait_frac", float("nan")), out.get("hbm_measured_frac", float("nan")),
-                                     out["valu_fp64_model_frac"])
+    nan = float("nan")
+    out["binding"] = ("latency and issue (waves wait {:.0%} of their cycles on results and {:.0%} to issue; VALU "
+                      "slots {:.0%} busy; HBM {:.0%} and FP64 VALU {:.0%} of peak)".format(
+                          out.get("wave_wait_frac", nan), out.get("wave_wait_issue_frac", nan),
+                          out.get("valu_issue_frac", nan), out.get("hbm_measured_frac", nan),
+                          out["valu_fp64_model_frac"])
                       if fr and max(fr) < 0.5 else ("hbm" if out.get("hbm_measured_frac", 0) >= out["valu_fp64_model_frac"]
                                                     else "valu"))
     return out
@@ -141,34 +152,19 @@ def calib_copy(nbytes):
     print(f"[bench] calibration copy of {nbytes} bytes done", file=sys.stderr, flush=True)
 
 
-def batched_run(ctx, make_params, scenes, root_budget=1 << 25):
+def batched_run(ctx, make_params, scenes):
     """Batched multi-transmitter workload (256 transmitters on the urban scene, levels 6-9, 4-6
-    reflections, 10k shared receivers): sbr_trace_batch over groups of transmitters holding at most
-    root_budget launches each (the library splits a call further into launch windows if needed),
-    against one sbr_trace per transmitter on the same inputs."""
+    reflections, 10k shared receivers): one sbr_trace_batch call over all transmitters (the library
+    runs it in launch windows of its tree-node budget), against one sbr_trace per transmitter on the
+    same inputs."""
     cfg = scenes.config4()
     plist = [make_params(**t) for t in cfg["txs"]]
-    groups, cur, roots = [], [], 0
-    for p in plist:
-        n = 10 * 4 ** p.tess + 2
-        if cur and roots + n > root_budget:
-            groups.append(cur)
-            cur, roots = [], 0
-        cur.append(p)
-        roots += n
-    if cur:
-        groups.append(cur)
-
     windows = []
 
     def run_batched():
-        b = 0
-        windows.clear()
-        for g in groups:
-            st = ctx.trace_batch(g, cfg["rx"]).stats
-            b += st["bounces"]
-            windows.append(st["windows"])
-        return b
+        st = ctx.trace_batch(plist, cfg["rx"]).stats
+        windows[:] = [st["windows"]]
+        return st["bounces"]
 
     def run_single():
         b = 0
@@ -185,11 +181,10 @@ def batched_run(ctx, make_params, scenes, root_budget=1 << 25):
     bs = run_single()
     ts = time.perf_counter() - t0
     assert bb == bs
-    return dict(workload=cfg["name"], transmitters=len(plist), batches=len(groups), windows=int(sum(windows)),
-                launches=int(
-        sum(10 * 4 ** p.tess + 2 for p in plist)), bounces=int(bb), batched_mrays=round(bb / tb / 1e6, 3),
-        batched_s=round(tb, 4), single_tx_mrays=round(bs / ts / 1e6, 3), single_tx_s=round(ts, 4),
-        speedup=round(ts / tb, 3))
+    return dict(workload=cfg["name"], transmitters=len(plist), batches=1, windows=int(windows[0]),
+                launches=int(sum(10 * 4 ** p.tess + 2 for p in plist)), bounces=int(bb),
+                batched_mrays=round(bb / tb / 1e6, 3), batched_s=round(tb, 4), single_tx_mrays=round(bs / ts / 1e6, 3),
+                single_tx_s=round(ts, 4), speedup=round(ts / tb, 3))
 
 
 def indoor_run(Context, make_params, scenes, tess=10):
@@ -390,6 +385,7 @@ def main():
                     capture_sphere_tests_per_bounce=round(cnt["capture_tests"] / bounces, 4),
                     capture_nodes_per_bounce=round(cnt["capture_nodes"] / bounces, 4),
                     fp64_box_tests_per_bounce=round(cnt["box_tests_fp64"] / bounces, 4),
+                    wide_redone_per_bounce=round(cnt["wide_redone"] / bounces, 7),
                     valu_model=dict(achieved=round(valu_tflops, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s (FP64)",
                                     frac=round(valu_tflops / FP64_PEAK_TFLOPS, 5)),
                     k_capture_ms_per_step=round(k_cap_ms, 3))

@@ -6,8 +6,14 @@
 //   grown box (2 delta) missed          => the FP64 slab test fails
 //   shrunk box (delta) crossed          => the FP64 slab test passes
 //   FP64 triangle hit at t              => pre-test box (delta) kept at clip t (1 + 1e-6)
-// Prints "<cases> <left to FP64> <robust passes> <robust misses> <triangle hits> <violations>" and
-// exits 1 on a violation.
+// and, for a box A enclosing the box (faces moved outward by random amounts, some by none), as an
+// ancestor's box in the reference tree encloses its descendants' boxes:
+//   shrunk box (delta) crossed          => the FP64 slab test of A passes (the wide traversal's
+//                                          check of a winner's leaf stops at such a node)
+//   FP64 triangle hit at t              => grown box (2 delta) of the box and of A kept at clip
+//                                          t (1 + 1e-6) (the wide traversal reaches every hit)
+// Prints "<cases> <left to FP64> <robust passes> <robust misses> <triangle hits> <violations>
+// <ancestor implications checked>" and exits 1 on a violation.
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -20,7 +26,7 @@ int main(int argc, char** argv) {
   const long N = argc > 1 ? atol(argv[1]) : 400000;
   std::mt19937_64 g(20261203);
   std::uniform_real_distribution<double> U(-1.0, 1.0);
-  long cases = 0, borderline = 0, bad = 0, robustPass = 0, robustMiss = 0, triHits = 0;
+  long cases = 0, borderline = 0, bad = 0, robustPass = 0, robustMiss = 0, triHits = 0, ancestors = 0;
   const double scales[4] = {1.0, 50.0, 500.0, 5000.0};
   for (long it = 0; it < N; it++) {
     const double E0 = scales[it % 4], ext = E0 + 1.0, delta = prefilter_delta(ext);
@@ -62,6 +68,21 @@ int main(int argc, char** argv) {
     if (grownCrossed && !shrunkCrossed) borderline++;
     robustPass += shrunkCrossed;
     robustMiss += !grownCrossed;
+    // enclosing box
+    double alo[3], ahi[3];
+    for (int a = 0; a < 3; a++) {
+      const double w = (hi[a] - lo[a]) + delta;
+      alo[a] = (it + a) % 4 == 0 ? lo[a] : lo[a] - std::fabs(U(g)) * w * std::pow(10.0, -4.0 * std::fabs(U(g)));
+      ahi[a] = (it + a) % 4 == 1 ? hi[a] : hi[a] + std::fabs(U(g)) * w * std::pow(10.0, -4.0 * std::fabs(U(g)));
+    }
+    if (shrunkCrossed) {
+      ancestors++;
+      double alo2, ahi2;
+      if (!box_line(alo, ahi, o, d, &alo2, &ahi2)) {
+        bad++;
+        if (bad < 10) printf("shrunk hit but enclosing box's FP64 miss: it %ld E %g\n", it, E0);
+      }
+    }
     // a triangle on the box: corner p, plus two points on the box
     V3 a0 = v3(p[0], p[1], p[2]);
     V3 a1 = v3(lo[0] + (hi[0] - lo[0]) * std::fabs(U(g)), lo[1], hi[2]);
@@ -79,8 +100,14 @@ int main(int argc, char** argv) {
       box_grown(tl, th, delta, blo, bhi);
       const float clip = std::nextafter((float)(t * (1.0 + 1e-6)), INFINITY);
       if (!box32_may_hit(blo, bhi, f, clip, &e)) { bad++; if (bad < 10) printf("triangle hit pre-tested away: it %ld\n", it); }
+      float nlo[3], nhi[3];
+      box_grown(lo, hi, 2.0 * delta, nlo, nhi);
+      const bool inBox = box32_may_hit(nlo, nhi, f, clip, &e);
+      box_grown(alo, ahi, 2.0 * delta, nlo, nhi);
+      const bool inAnc = box32_may_hit(nlo, nhi, f, clip, &e);
+      if (!inBox || !inAnc) { bad++; if (bad < 10) printf("triangle hit but a wide node box missed: it %ld\n", it); }
     }
   }
-  printf("%ld %ld %ld %ld %ld %ld\n", cases, borderline, robustPass, robustMiss, triHits, bad);
+  printf("%ld %ld %ld %ld %ld %ld %ld\n", cases, borderline, robustPass, robustMiss, triHits, bad, ancestors);
   return bad ? 1 : 0;
 }

@@ -1,7 +1,8 @@
 """Single-precision traversal decisions (sbrsolver_amd/csrc/prefilter.h, host build) against the
 reference's FP64 slab and ray-triangle tests on grazing rays: a grown-box miss never hides an FP64
-pass, a shrunk-box crossing never passes an FP64 miss, and the triangle pre-test never drops an
-FP64 hit (tests/prefilter_check.cpp)."""
+pass, a shrunk-box crossing never passes an FP64 miss (of the box, or of any box enclosing it: the
+wide traversal's check of a winner's leaf), and neither the triangle pre-test nor the grown boxes of
+the wide tree drop an FP64 hit (tests/prefilter_check.cpp)."""
 import os
 import subprocess
 
@@ -14,10 +15,11 @@ def test_prefilter_decisions_sound(tmp_path):
                            "-I", os.path.join(ROOT, "sbrsolver_amd", "csrc"),
                            os.path.join(ROOT, "tests", "prefilter_check.cpp"), "-o", exe])
     r = subprocess.run([exe, "1000000"], capture_output=True, text=True, timeout=300)
-    cases, left, rpass, rmiss, tri_hits, bad = map(int, r.stdout.split()[-6:])
+    cases, left, rpass, rmiss, tri_hits, bad, anc = map(int, r.stdout.split()[-7:])
     assert bad == 0, r.stdout
     # not vacuous: every kind of decision occurs many times
     assert cases > 500_000 and left > 100_000 and rpass > 10_000 and rmiss > 10_000 and tri_hits > 10_000, r.stdout
+    assert anc == rpass, r.stdout
 
 
 def test_sphere_pretest_decisions_sound(tmp_path):
