@@ -169,19 +169,26 @@ typedef struct {
  * the scene in LDS when it fits and does not lower the resident blocks per CU, 1 whenever it
  * fits, 0 never; blocks_per_cu: persistent blocks per compute unit, 0 = the occupancy maximum;
  * window_nodes: interaction-tree nodes held on the device at once (128 B each), 0 = 2^27; smaller
- * values than 16384 are raised to 16384.  A trace whose trees need more runs in several launch windows.  Results do not depend on
- * any of them. */
+ * values than 16384 are raised to 16384.  A trace whose trees need more runs in several launch
+ * windows.  stack_lds: traversal stack entries per ray held in LDS (1..64), the deeper ones in
+ * device memory; 0 = as many as leave LDS room for the blocks per CU the kernel's registers allow.
+ * oversub: blocks of the bounce kernel's grid per resident block (1..64; the chunks of 256 items
+ * that the hardware dispatcher hands to freed block slots), 0 = 8.  Results do not depend on any of
+ * them. */
 typedef struct {
   int32_t lds_scene;
   int32_t blocks_per_cu;
   int64_t window_nodes;
+  int32_t stack_lds;
+  int32_t oversub;
 } sbr_tuning;
 
 /* Autotuner (no reference counterpart): times the traversal kernel on a sample of the given trace
  * (every launch_stride-th launch, chosen so that about 2^18 launches run) for each launch
- * configuration -- scene staged in LDS or not, persistent blocks per CU from the occupancy maximum
- * down -- keeps the fastest, sets it on the context and returns it in *chosen.  The last trace
- * result of the context is replaced by the last trial. */
+ * configuration -- scene staged in LDS or not and persistent blocks per CU from the occupancy
+ * maximum down, then for the best of those the LDS stack entries and the grid oversubscription --
+ * keeps the fastest, sets it on the context and returns it in *chosen.  The last trace result of
+ * the context is replaced by the last trial. */
 int sbr_autotune(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx, int32_t n_rx, sbr_tuning* chosen);
 
 /* device >= 0: HIP device ordinal.  device == -1: host-only context (scene preparation and

@@ -49,7 +49,11 @@ class Stats(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("lds_scene", C.c_int32), ("blocks_per_cu", C.c_int32), ("window_nodes", C.c_int64)]
+    _fields_ = [("lds_scene", C.c_int32), ("blocks_per_cu", C.c_int32), ("window_nodes", C.c_int64),
+                ("stack_lds", C.c_int32), ("oversub", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Material(C.Structure):
@@ -339,8 +343,8 @@ class Context:
             raise ValueError("gain_info must have shape (n_theta, n_phi, 4)")
         self._chk(lib().sbr_set_tx_pattern(self.h, _a(g), g.shape[0], g.shape[1]))
 
-    def set_tuning(self, lds_scene=-1, blocks_per_cu=0, window_nodes=0):
-        t = Tuning(int(lds_scene), int(blocks_per_cu), int(window_nodes))
+    def set_tuning(self, lds_scene=-1, blocks_per_cu=0, window_nodes=0, stack_lds=0, oversub=0):
+        t = Tuning(int(lds_scene), int(blocks_per_cu), int(window_nodes), int(stack_lds), int(oversub))
         self._chk(lib().sbr_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, params, rx):
@@ -350,12 +354,12 @@ class Context:
         t = Tuning()
         self._gen += 1   # the trials replace the context's last result
         self._chk(lib().sbr_autotune(self.h, C.byref(params), _a(r), r.shape[0], C.byref(t)))
-        return dict(lds_scene=t.lds_scene, blocks_per_cu=t.blocks_per_cu, window_nodes=t.window_nodes)
+        return t.as_dict()
 
     def get_tuning(self):
         t = Tuning()
         self._chk(lib().sbr_get_tuning(self.h, C.byref(t)))
-        return dict(lds_scene=t.lds_scene, blocks_per_cu=t.blocks_per_cu, window_nodes=t.window_nodes)
+        return t.as_dict()
 
     # ---- kernels
     def generate_rays(self, tess):

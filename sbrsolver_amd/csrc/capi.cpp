@@ -531,11 +531,14 @@ int sbr_set_tx_pattern(sbr_ctx* ctx, const float* gain_info, int32_t n_theta, in
 int sbr_set_tuning(sbr_ctx* ctx, const sbr_tuning* t) {
   return guarded(ctx, [&] {
     if (!t || t->lds_scene < -1 || t->lds_scene > 1 || t->blocks_per_cu < 0 || t->blocks_per_cu > 64 ||
-        t->window_nodes < 0 || t->window_nodes > (int64_t(1) << 30))
+        t->window_nodes < 0 || t->window_nodes > (int64_t(1) << 30) || t->stack_lds < 0 || t->stack_lds > 64 ||
+        t->oversub < 0 || t->oversub > 64)
       return ctx->eng.fail(SBR_E_ARG, "tuning out of range");
     ctx->eng.tune.ldsScene = t->lds_scene;
     ctx->eng.tune.blocksPerCU = t->blocks_per_cu;
     ctx->eng.tune.windowNodes = t->window_nodes;
+    ctx->eng.tune.stackLds = t->stack_lds;
+    ctx->eng.tune.oversub = t->oversub;
     return SBR_OK;
   });
 }
@@ -553,22 +556,30 @@ int sbr_autotune(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx,
     const sbr::Tuning saved = e.tune;
     sbr::Tuning best = saved;
     double bestMs = 1e300;
-    // first trial: automatic choice, which also reports the occupancy maximum
-    std::vector<sbr::Tuning> trials{{-1, 0, saved.windowNodes}};
+    // kernel time of the sample under tuning t (first run warms buffers and caches); <0: error
+    auto timed = [&](const sbr::Tuning& t, double* ms) {
+      e.tune = t;
+      *ms = 0;
+      for (int rep = 0; rep < 3; rep++) {
+        int rc = e.trace(&p, 1, rx, n_rx);
+        if (rc) return rc;
+        if (rep > 0) *ms += e.stats.ms_k_trace;
+      }
+      return (int)SBR_OK;
+    };
+    // 1. scene staging x blocks per CU; the first trial is the automatic choice, which also
+    //    reports the occupancy maximum
+    std::vector<sbr::Tuning> trials{{-1, 0, saved.windowNodes, 0, 0}};
     for (int lds = 0; lds <= 1; lds++)
-      for (int bpc = 1; bpc <= 8; bpc++) trials.push_back({lds, bpc, saved.windowNodes});
+      for (int bpc = 1; bpc <= 8; bpc++) trials.push_back({lds, bpc, saved.windowNodes, 0, 0});
     int maxBpc = 8;
     for (size_t k = 0; k < trials.size(); k++) {
       if (k > 0 && trials[k].blocksPerCU > maxBpc) continue;
-      e.tune = trials[k];
       double ms = 0;
-      for (int rep = 0; rep < 3; rep++) {   // first run warms buffers and caches
-        int rc = e.trace(&p, 1, rx, n_rx);
-        if (rc) {
-          e.tune = saved;
-          return rc;
-        }
-        if (rep > 0) ms += e.stats.ms_k_trace;
+      int rc = timed(trials[k], &ms);
+      if (rc) {
+        e.tune = saved;
+        return rc;
       }
       if (k == 0) {   // the automatic choice, recorded as the configuration it resolved to
         maxBpc = std::max(1, (int)e.stats.trace_blocks_per_cu);
@@ -581,10 +592,43 @@ int sbr_autotune(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx,
         best = trials[k];
       }
     }
+    // 2. for that configuration: LDS stack entries (the rest in device memory), then the grid's
+    //    blocks per resident block
+    for (int sl : {2, 4, 8, 16}) {
+      sbr::Tuning t = best;
+      t.stackLds = sl;
+      double ms = 0;
+      int rc = timed(t, &ms);
+      if (rc) {
+        e.tune = saved;
+        return rc;
+      }
+      if (e.stats.trace_lds != t.ldsScene) continue;
+      if (ms < bestMs) {
+        bestMs = ms;
+        best = t;
+      }
+    }
+    for (int os : {1, 2, 4, 16}) {
+      sbr::Tuning t = best;
+      t.oversub = os;
+      double ms = 0;
+      int rc = timed(t, &ms);
+      if (rc) {
+        e.tune = saved;
+        return rc;
+      }
+      if (ms < bestMs) {
+        bestMs = ms;
+        best = t;
+      }
+    }
     e.tune = best;
     chosen->lds_scene = best.ldsScene;
     chosen->blocks_per_cu = best.blocksPerCU;
     chosen->window_nodes = best.windowNodes;
+    chosen->stack_lds = best.stackLds;
+    chosen->oversub = best.oversub;
     return SBR_OK;
   });
 }
@@ -595,6 +639,8 @@ int sbr_get_tuning(sbr_ctx* ctx, sbr_tuning* t) {
     t->lds_scene = ctx->eng.tune.ldsScene;
     t->blocks_per_cu = ctx->eng.tune.blocksPerCU;
     t->window_nodes = ctx->eng.tune.windowNodes;
+    t->stack_lds = ctx->eng.tune.stackLds;
+    t->oversub = ctx->eng.tune.oversub;
     return SBR_OK;
   });
 }

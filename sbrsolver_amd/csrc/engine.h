@@ -270,6 +270,8 @@ struct Tuning {
   int ldsScene = -1;      // -1: LDS staging when it fits and keeps occupancy, 1: whenever it fits, 0: never
   int blocksPerCU = 0;    // persistent blocks per CU; 0: occupancy maximum
   int64_t windowNodes = 0;   // tree nodes per launch window; 0: kWindowNodes
+  int stackLds = 0;       // traversal stack entries per ray in LDS; 0: what leaves room for the blocks per CU
+  int oversub = 0;        // bounce-kernel blocks per resident block; 0: kTraceOversub
 };
 
 // Kept paths gathered window by window when a trace needs several launch windows.

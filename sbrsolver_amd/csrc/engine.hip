@@ -2428,7 +2428,8 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   };
   // LDS-staged part of the scene: the wide nodes and leaves, or the reference tree's records.
   const size_t stageBytes = mode == kTravWide ? wideBytes : sceneBytes;
-  const int kGlob = ldsEntries(0), kLds = ldsEntries(stageBytes);
+  int kGlob = ldsEntries(0), kLds = ldsEntries(stageBytes);
+  if (tune.stackLds > 0) kGlob = kLds = std::min(tune.stackLds, depthAll);
   const size_t stk = stack_bytes(kGlob, kBlock), stkL = stack_bytes(kLds, kBlock);
   // Scene in LDS or read through the caches: automatic choice takes LDS staging unless it lowers the
   // number of resident blocks per CU.
@@ -2481,8 +2482,9 @@ int Engine::trace_stage(const std::vector<TxDesc>& txs, int64_t rootBase, int64_
   if (tune.blocksPerCU > 0) perCU = std::min(perCU, tune.blocksPerCU);
   // Resident blocks times kTraceOversub (capped by twice the roots' blocks): blocks that finish early
   // are replaced by waiting ones, so the launch ends evenly.
+  const int oversub = tune.oversub > 0 ? tune.oversub : kTraceOversub;
   const unsigned grid = (unsigned)std::max<int64_t>(
-      1, std::min<int64_t>((int64_t)numCU * perCU * kTraceOversub, blocks_for(std::max<int64_t>(2 * nRoot, 1), kBlock)));
+      1, std::min<int64_t>((int64_t)numCU * perCU * oversub, blocks_for(std::max<int64_t>(2 * nRoot, 1), kBlock)));
   stats.trace_lds = lds ? 1 : 0;
   stats.trace_blocks_per_cu = perCU;
   A.stackLds = kStack;
