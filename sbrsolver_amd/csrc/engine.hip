@@ -676,6 +676,8 @@ struct CaptureBufs {
   const int32_t* subTri;
   float ext;
   int32_t stackDepth;
+  WideTree wide;                  // and the trace's traversal (kTravWide: the wide tree)
+  int32_t mode;
   // image source of each tree node (FP64, the operations of GetImageSources), or null: then the
   // correction's first-step pre-test is off
   const double* img;
@@ -1060,7 +1062,8 @@ __device__ bool corrected_points(V3 tx, const TNode* __restrict__ tree, const DS
 
 // Occlusion test of a corrected multi-bounce path (extension, checkOcclusion): no scene triangle
 // nearer than the end of any segment (a hit at the end itself is the next surface or, for the last
-// segment, lies beyond the receiver).  Closest hits as the line of sight takes them.
+// segment, lies beyond the receiver).  Closest hits through the wide tree (traversal mode 0) or the
+// clipped reference tree (the other modes; all give the reference's closest hit).
 __device__ bool unobstructed(const CaptureBufs& B, const NodeCtx& c, V3 C, int* stkN, float* stkT) {
   int32_t chain[kMaxDepth + 1];
   int32_t q = c.node;
@@ -1071,10 +1074,12 @@ __device__ bool unobstructed(const CaptureBufs& B, const NodeCtx& c, V3 C, int* 
   for (int k = 0; k <= c.depth; k++) {
     const V3 d = vsub(pts[k + 1], pts[k]);
     Hit h;
-    if (closest_hit<false, true>(B.nodes, B.nodesF, B.tris, B.tbox, B.subs, B.subTri, B.ext, pts[k], d, h,
-                                 lds_stack(stkN, stkT, blockDim.x, B.stackDepth)) &&
-        h.dist < vnorm(d) * (1.0 - 1e-7))
-      return false;
+    const TravStack st = lds_stack(stkN, stkT, blockDim.x, B.stackDepth);
+    const bool hit =
+        B.mode == kTravWide
+            ? closest_hit_wide<true>(B.wide, B.nodes, B.nodesF, B.tris, B.tbox, B.subs, B.subTri, B.ext, pts[k], d, h, st)
+            : closest_hit<false, true>(B.nodes, B.nodesF, B.tris, B.tbox, B.subs, B.subTri, B.ext, pts[k], d, h, st);
+    if (hit && h.dist < vnorm(d) * (1.0 - 1e-7)) return false;
   }
   return true;
 }
@@ -2906,6 +2911,8 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         CB.subTri = dSubTri.p;
         CB.ext = sceneExtent;
         CB.stackDepth = stackDepth();
+        CB.wide = wide_tree();
+        CB.mode = mode;
         if (nodes > 0) {
           auto kc = cone ? k_capture_cone : k_capture;
           kc<<<(unsigned)std::min<int64_t>(blocks_for(nodes, 256), (int64_t)numCU * 16), 256,
