@@ -438,7 +438,8 @@ def main():
 
     # Batched throughput as a fraction of the same HBM roofline: its bounces priced at the headline
     # workload's algorithmic bytes per bounce (same scene; the per-bounce traversal counts of its
-    # levels are not committed, so this is a model figure).  The served city is another scene.
+    # levels are not committed, so this is a model figure).  The served city is priced with its own
+    # committed canonical counts (the reference traversal over the query stream, sampled).
     if "batched" in out and "batched_mrays" in out["batched"]:
         # the second denominator: the plain single-transmitter kernel on the headline workload
         out["batched"]["speedup_vs_headline_single_tx"] = round(out["batched"]["batched_mrays"] / value, 3)
@@ -446,6 +447,15 @@ def main():
         if name in out and key in out[name]:
             gbs = out[name][key] * 1e6 * b_bounce / 1e9
             out[name]["roofline_frac_" + key] = round(gbs / HBM_PEAK_GBS, 5)
+    if "serving" in out and "capacity_mray_bounces_s" in out["serving"]:
+        with open(COUNTS) as f:
+            city = json.load(f).get("city")
+        if city:
+            bc = B_RAY_IN + B_SEG_OUT + B_PER_NODE * city["nodes_per_bounce"] + B_PER_TRI * city["tris_per_bounce"]
+            sv = out["serving"]
+            sv["bytes_per_bounce"] = round(bc, 1)
+            for key in ("capacity_mray_bounces_s", "mray_bounces_per_s"):
+                sv["roofline_frac_" + key] = round(sv[key] * 1e6 * bc / 1e9 / HBM_PEAK_GBS, 5)
 
     if args.calib:
         calib_copy(256 << 20)

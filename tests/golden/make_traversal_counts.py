@@ -5,14 +5,18 @@ boxes entered and triangles tested per bounce, over every STRIDE-th launch of
 * the urban-canyon workload (482 triangles, icosahedron level 9, 6 reflections + 1 transmission),
   every launch;
 * the dense indoor workload (206,660 triangles, level 10, reflections and transmissions up to 8
-  each and 8 in total), every 64th launch (the per-bounce figures of a sample).
+  each and 8 in total), every 64th launch (the per-bounce figures of a sample);
+* the serving stream on the seeded city (1,000,370 triangles; the 96 coverage queries bench.py
+  serves, levels 6-10, 4 reflections), about 2048 launches of each query (every
+  (launches // 2048)-th), each query's counts weighted by its launch stride, i.e. estimating the
+  whole stream's bounces; counted on the oracle's SAH tree of the city (the reference's build).
 
 The counts define the algorithmic bytes of a bounce that bench.py reports:
   B = 32 (ray in) + 32 (segment out) + 32 * nodes + 48 * triangles.
 They depend on the scene and rays only, not on the receivers or on any GPU kernel.
 
-Run:  python tests/golden/make_traversal_counts.py [urban] [indoor]   (default: both; an entry
-already in the file is kept unless named)
+Run:  python tests/golden/make_traversal_counts.py [urban] [indoor] [city]   (default: all three; an
+entry already in the file is kept unless named)
 """
 import json
 import os
@@ -33,6 +37,26 @@ def counts(cfg, stride, nthreads=8):
                 nodes_per_bounce=r.canon_nodes / r.bounces, tris_per_bounce=r.canon_tris / r.bounces)
 
 
+def city_counts(n=96, sample=2048, nthreads=8):
+    from oracle import oracle as O
+    from sbrsolver_amd import scenes
+    name, soup, boxes, half, mats = scenes.config5_scene()
+    sc = O.Scene.from_soup(*soup)
+    qs = scenes.coverage_queries(n, boxes, half=0.5 * half)
+    nodes = tris = bounces = 0
+    for kw, rx in qs:
+        launches = 10 * 4 ** kw["tess"] + 2
+        stride = max(1, launches // sample)
+        p = O.make_params(**dict(kw, launchStride=stride, countCanonical=1))
+        r = O.trace(sc, p, rx[:1], mats, nthreads=nthreads)
+        nodes += r.canon_nodes * stride
+        tris += r.canon_tris * stride
+        bounces += r.bounces * stride
+    wl = f"{name}_coverage_queries_L6-10_R4"
+    return dict(workload=wl, queries=n, launches_per_query_sample=sample, bounces_estimate=int(bounces),
+                nodes_per_bounce=nodes / bounces, tris_per_bounce=tris / bounces)
+
+
 def main(which):
     from sbrsolver_amd import scenes
     out = {}
@@ -41,6 +65,11 @@ def main(which):
             out = json.load(f)
     jobs = {"urban": (lambda: scenes.config2(tess=9), 1), "indoor": (lambda: scenes.config3(tess=10), 64)}
     for name in which:
+        if name == "city":
+            e = city_counts()
+            out["city"] = e
+            print(json.dumps(e), flush=True)
+            continue
         make, stride = jobs[name]
         e = counts(make(), stride)
         out[e["workload"]] = e
@@ -51,4 +80,4 @@ def main(which):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["urban", "indoor"])
+    main(sys.argv[1:] or ["urban", "indoor", "city"])
