@@ -206,8 +206,10 @@ int Engine::upload(const sbr_scene_desc* d) {
     for (size_t k = 0; k < nodesOut.size(); k++) {
       ver[k].parent = -1;
       if (nodesOut[k].count > 0) {
-        leafSlot[k] = (int32_t)leaves.size();
-        leaves.push_back(DLeaf{nodesOut[k].a, nodesOut[k].count, nodesOut[k].sub, (int32_t)k});
+        if (nodesOut[k].sub < 0) {   // a leaf with a leaf BVH becomes a wide subtree (buildSub)
+          leafSlot[k] = (int32_t)leaves.size();
+          leaves.push_back(DLeaf{nodesOut[k].a, nodesOut[k].count, 0, 0});
+        }
         for (int i = 0; i < nodesOut[k].count; i++) trisOut[nodesOut[k].a + i].leaf = (int32_t)k;
       }
     }
@@ -230,6 +232,62 @@ int Engine::upload(const sbr_scene_desc* d) {
       const DNode& x = nodesOut[k];
       const double e0 = x.bmax[0] - x.bmin[0], e1 = x.bmax[1] - x.bmin[1], e2 = x.bmax[2] - x.bmin[2];
       return e0 * e1 + e1 * e2 + e2 * e0;
+    };
+    auto subArea = [&](int q) {
+      const DSub& x = subs[q];
+      const double e0 = (double)x.hi[0] - x.lo[0], e1 = (double)x.hi[1] - x.lo[1], e2 = (double)x.hi[2] - x.lo[2];
+      return e0 * e1 + e1 * e2 + e2 * e0;
+    };
+    // buildSub(q, deferred): wide subtree over the inner leaf-BVH node q, collapsed four-wide as
+    // below; its boxes are the leaf BVH's (unions of the triangles' padded pre-test boxes, which a
+    // hit triangle's computed interval enters, and so does any box holding them), its leaves list
+    // their triangles in subTri.
+    std::function<int(int, int)> buildSub = [&](int q, int deferred) {
+      int c[4] = {subs[q].first, subs[q].first + 1, -1, -1};
+      int m = 2;
+      while (m < 4) {
+        int best = -1;
+        double ba = -1.0;
+        for (int i = 0; i < m; i++)
+          if (subs[c[i]].count == 0 && subArea(c[i]) > ba) {
+            best = i;
+            ba = subArea(c[i]);
+          }
+        if (best < 0) break;
+        const int x = c[best];
+        for (int j = m; j > best + 1; j--) c[j] = c[j - 1];
+        c[best] = subs[x].first;
+        c[best + 1] = subs[x].first + 1;
+        m++;
+      }
+      const int me = (int)n4.size();
+      n4.emplace_back();
+      wideStack = std::max(wideStack, deferred + m - 1);
+      for (int i = 0; i < 4; i++) {
+        float lo[3], hi[3];
+        int32_t item = -1;
+        if (i < m) {
+          const DSub sq = subs[c[i]];
+          for (int a = 0; a < 3; a++) {
+            lo[a] = sq.lo[a];
+            hi[a] = sq.hi[a];
+          }
+          if (sq.count > 0) {
+            item = kLeafBit | (int32_t)leaves.size();
+            leaves.push_back(DLeaf{sq.first, sq.count, 1, 0});
+          } else {
+            item = buildSub(c[i], deferred + m - 1);
+          }
+        } else {
+          for (int a = 0; a < 3; a++) lo[a] = hi[a] = std::nanf("");
+        }
+        for (int a = 0; a < 3; a++) {
+          n4[me].lo[a][i] = lo[a];
+          n4[me].hi[a][i] = hi[a];
+        }
+        n4[me].item[i] = item;
+      }
+      return me;
     };
     // build(k, deferred): wide node over reference node k (a leaf only at the root); deferred = the
     // siblings the traversal may hold on its stack above this node.
@@ -265,7 +323,10 @@ int Engine::upload(const sbr_scene_desc* d) {
         if (i < m) {
           const DNode& cn = nodesOut[c[i]];
           box_grown(cn.bmin, cn.bmax, 2.0 * delta, lo, hi);
-          item = cn.count > 0 ? (kLeafBit | leafSlot[c[i]]) : build(c[i], deferred + m - 1);
+          if (cn.count == 0)
+            item = build(c[i], deferred + m - 1);
+          else
+            item = cn.sub >= 0 ? buildSub(cn.sub, deferred + m - 1) : (kLeafBit | leafSlot[c[i]]);
         } else {
           for (int a = 0; a < 3; a++) lo[a] = hi[a] = std::nanf("");
         }

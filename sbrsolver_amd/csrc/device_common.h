@@ -205,6 +205,32 @@ __device__ __forceinline__ void leaf_tests(const DTri* __restrict__ tris, const 
   }
 }
 
+// The triangles subTri[first .. first + count - 1] of a leaf-BVH leaf (the ray is covered by the
+// pre-test): pre-tests four at a time, then the exact tests of the survivors.
+template <bool COUNT>
+__device__ __forceinline__ void leaf_tests_listed(const DTri* __restrict__ tris, const DBox32* __restrict__ tbox,
+                                                  const int32_t* __restrict__ subTri, int first, int count,
+                                                  const RayF& f, V3 o, V3 d, Hit& h, float& clip) {
+  for (int j = 0; j < count; j += 4) {
+    const int m = min(4, count - j);
+    const int k0 = subTri[first + j], k1 = subTri[first + j + min(1, m - 1)], k2 = subTri[first + j + min(2, m - 1)],
+              k3 = subTri[first + j + min(3, m - 1)];
+    const DBox32 b0 = tbox[k0];
+    const DBox32 b1 = tbox[k1];
+    const DBox32 b2 = tbox[k2];
+    const DBox32 b3 = tbox[k3];
+    float e;
+    unsigned pass = (box32_may_hit(b0.lo, b0.hi, f, clip, &e) ? 1u : 0u) | (box32_may_hit(b1.lo, b1.hi, f, clip, &e) ? 2u : 0u) |
+                    (box32_may_hit(b2.lo, b2.hi, f, clip, &e) ? 4u : 0u) | (box32_may_hit(b3.lo, b3.hi, f, clip, &e) ? 8u : 0u);
+    pass &= (1u << m) - 1u;   // the clamped repeats of the last triangle
+    while (pass) {
+      const int u = __builtin_ctz(pass);
+      pass &= pass - 1;
+      tri_exact<COUNT>(tris, u == 0 ? k0 : u == 1 ? k1 : u == 2 ? k2 : k3, o, d, h, clip);
+    }
+  }
+}
+
 //
 // SUB: the scene has leaf BVHs (otherwise that code is left out of the kernel).
 // COUNT: h.nodes / h.tests / h.exact64 are counted (diagnostic builds of the kernels only; the
@@ -354,7 +380,10 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
 // ancestors' boxes, pass its FP64 slab test (BVHNode.h:64-86).  If the winner's leaf is reached that
 // way (wide_reaches), it is also the minimum over the triangles the reference tests, i.e. the
 // reference's result; otherwise (a line grazing a box of the winner's path) the caller redoes the
-// ray on the reference tree.  No hit at all means the reference finds none either.
+// ray on the reference tree.  No hit at all means the reference finds none either.  Leaves with a
+// leaf BVH are wide subtrees of the tree itself (the leaf BVH collapsed four-wide), so the
+// traversal has one node kind; SUB: the scene has such leaves (listed-triangle leaves).
+// Stack items: a wide node (>= 0) or a leaf (kLeafBit | DLeaf index).
 // ---------------------------------------------------------------------------------------------
 template <bool SUB, bool COUNT>
 __device__ __forceinline__ void wide_hit(const DNode4* __restrict__ n4, const DLeaf* __restrict__ leaves,
@@ -424,37 +453,11 @@ __device__ __forceinline__ void wide_hit(const DNode4* __restrict__ n4, const DL
     }
     if (!live) break;
     if (COUNT) h.nodes++;
-    if (SUB && cur < 0) {
-      const DSub sn = subs[~cur];
-      if (sn.count > 0) {
-        for (int j = sn.first; j < sn.first + sn.count; j++) {
-          const int k = subTri[j];
-          float e;
-          if (box32_may_hit(tbox[k].lo, tbox[k].hi, f, clip, &e)) tri_exact<COUNT>(tris, k, o, d, h, clip);
-        }
-      } else {
-        float ta, tb;
-        const bool ina = box32_may_hit(subs[sn.first].lo, subs[sn.first].hi, f, clip, &ta);
-        const bool inb = box32_may_hit(subs[sn.first + 1].lo, subs[sn.first + 1].hi, f, clip, &tb);
-        if (ina && inb) {
-          const bool aFirst = ta <= tb;
-          push(~(aFirst ? sn.first + 1 : sn.first), aFirst ? tb : ta);
-          cur = ~(aFirst ? sn.first : sn.first + 1);
-          continue;
-        }
-        if (ina || inb) {
-          cur = ~(ina ? sn.first : sn.first + 1);
-          continue;
-        }
-      }
-    } else {
-      const DLeaf li = leaves[cur & ~kLeafBit];
-      if (SUB && li.sub >= 0) {
-        cur = ~li.sub;   // search the leaf through its leaf BVH
-        continue;
-      }
+    const DLeaf li = leaves[cur & ~kLeafBit];
+    if (SUB && li.listed)
+      leaf_tests_listed<COUNT>(tris, tbox, subTri, li.first, li.count, f, o, d, h, clip);
+    else
       leaf_tests<COUNT>(tris, tbox, li.first, li.count, f, o, d, h, clip);
-    }
     live = pop();
     if (!live) break;
   }

@@ -82,12 +82,14 @@ struct alignas(128) DNode4 {
   int32_t pad[4];
 };
 static_assert(sizeof(DNode4) == 128, "DNode4 layout");
-// Leaf of the reference tree as the wide traversal reads it.
+// Leaf of the wide tree: a reference-tree leaf (its triangles first .. first + count - 1 in DFS
+// order), or a leaf of a leaf BVH (the wide tree holds leaf BVHs as subtrees of its own; its
+// triangles are subTri[first .. first + count - 1]).
 struct alignas(16) DLeaf {
-  int32_t first;   // first triangle (DFS index)
+  int32_t first;
   int32_t count;
-  int32_t sub;     // leaf BVH root (DSub index) or -1
-  int32_t node;    // reference-tree node (DFS index)
+  int32_t listed;  // 1: a leaf-BVH leaf (triangles listed in subTri)
+  int32_t pad;
 };
 // Reference-tree node for the check of a winner's leaf, 32 B: the exact box shrunk by delta and
 // rounded inward (a non-empty computed interval puts the line at depth delta / 2 inside the exact
