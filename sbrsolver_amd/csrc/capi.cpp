@@ -243,9 +243,9 @@ int Engine::upload(const sbr_scene_desc* d) {
     // hit triangle's computed interval enters, and so does any box holding them), its leaves list
     // their triangles in subTri.
     std::function<int(int, int)> buildSub = [&](int q, int deferred) {
-      int c[4] = {subs[q].first, subs[q].first + 1, -1, -1};
+      int c[kWide] = {subs[q].first, subs[q].first + 1};
       int m = 2;
-      while (m < 4) {
+      while (m < kWide) {
         int best = -1;
         double ba = -1.0;
         for (int i = 0; i < m; i++)
@@ -263,7 +263,7 @@ int Engine::upload(const sbr_scene_desc* d) {
       const int me = (int)n4.size();
       n4.emplace_back();
       wideStack = std::max(wideStack, deferred + m - 1);
-      for (int i = 0; i < 4; i++) {
+      for (int i = 0; i < kWide; i++) {
         float lo[3], hi[3];
         int32_t item = -1;
         if (i < m) {
@@ -292,14 +292,14 @@ int Engine::upload(const sbr_scene_desc* d) {
     // build(k, deferred): wide node over reference node k (a leaf only at the root); deferred = the
     // siblings the traversal may hold on its stack above this node.
     std::function<int(int, int)> build = [&](int k, int deferred) {
-      int c[4] = {k, -1, -1, -1};
+      int c[kWide] = {k};
       int m = 1;
       if (nodesOut[k].count == 0) {
         c[0] = nodesOut[k].a;
         c[1] = nodesOut[k].b;
         m = 2;
       }
-      while (m < 4) {
+      while (m < kWide) {
         int best = -1;
         double ba = -1.0;
         for (int i = 0; i < m; i++)
@@ -317,7 +317,7 @@ int Engine::upload(const sbr_scene_desc* d) {
       const int me = (int)n4.size();
       n4.emplace_back();
       wideStack = std::max(wideStack, deferred + m - 1);
-      for (int i = 0; i < 4; i++) {
+      for (int i = 0; i < kWide; i++) {
         float lo[3], hi[3];
         int32_t item = -1;
         if (i < m) {

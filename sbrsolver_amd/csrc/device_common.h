@@ -417,16 +417,16 @@ __device__ __forceinline__ void wide_hit(const DNode4* __restrict__ n4, const DL
     while (live && cur >= 0 && !(cur & kLeafBit)) {
       if (COUNT) h.nodes++;
       const DNode4& n = n4[cur];
-      float t[4];
-      int it[4];
+      float t[kWide];
+      int it[kWide];
 #pragma unroll
-      for (int k = 0; k < 4; k++) {
+      for (int k = 0; k < kWide; k++) {
         float e;
         const bool in = box32_may_hit(n.lo[0][k], n.lo[1][k], n.lo[2][k], n.hi[0][k], n.hi[1][k], n.hi[2][k], f, clip, &e);
         it[k] = n.item[k];
         t[k] = (in && it[k] != -1) ? e : INFINITY;
       }
-      // nearest entry first: a sorting network over the four (entry, item) pairs
+      // nearest entry first: a sorting network over the (entry, item) pairs
       auto cswap = [&](int a, int b) {
         if (t[b] < t[a]) {
           const float tt = t[a];
@@ -437,17 +437,26 @@ __device__ __forceinline__ void wide_hit(const DNode4* __restrict__ n4, const DL
           it[b] = ii;
         }
       };
-      cswap(0, 1);
-      cswap(2, 3);
-      cswap(0, 2);
-      cswap(1, 3);
-      cswap(1, 2);
+      if (kWide == 4) {
+        cswap(0, 1);
+        cswap(2, 3);
+        cswap(0, 2);
+        cswap(1, 3);
+        cswap(1, 2);
+      } else {   // eight: Batcher's odd-even merge network
+        cswap(0, 1); cswap(2, 3); cswap(4, 5); cswap(6, 7);
+        cswap(0, 2); cswap(1, 3); cswap(4, 6); cswap(5, 7);
+        cswap(1, 2); cswap(5, 6);
+        cswap(0, 4); cswap(1, 5); cswap(2, 6); cswap(3, 7);
+        cswap(2, 4); cswap(3, 5);
+        cswap(1, 2); cswap(3, 4); cswap(5, 6);
+      }
       if (t[0] == INFINITY) {
         live = pop();
       } else {
-        if (t[3] != INFINITY) push(it[3], t[3]);
-        if (t[2] != INFINITY) push(it[2], t[2]);
-        if (t[1] != INFINITY) push(it[1], t[1]);
+#pragma unroll
+        for (int k = kWide - 1; k >= 1; k--)
+          if (t[k] != INFINITY) push(it[k], t[k]);
         cur = it[0];
       }
     }

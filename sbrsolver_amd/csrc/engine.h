@@ -75,13 +75,14 @@ constexpr int kTravRef = 2;
 // has NaN bounds, which no interval test passes.  The wide tree only finds the triangles a ray may
 // hit; whether the reference's traversal reaches the winner's leaf is checked on the reference
 // tree itself (DVer).
+constexpr int kWide = 4;   // children per wide node (4: 128 B nodes; 8, 256 B, measured slower)
 struct alignas(128) DNode4 {
-  float lo[3][4];   // lo[axis][child]
-  float hi[3][4];
-  int32_t item[4];  // inner: DNode4 index; leaf: kLeafBit | DLeaf index; empty slot: -1
-  int32_t pad[4];
+  float lo[3][kWide];   // lo[axis][child]
+  float hi[3][kWide];
+  int32_t item[kWide];  // inner: DNode4 index; leaf: kLeafBit | DLeaf index; empty slot: -1
+  int32_t pad[kWide == 4 ? 4 : 8];
 };
-static_assert(sizeof(DNode4) == 128, "DNode4 layout");
+static_assert(sizeof(DNode4) == (kWide == 4 ? 128 : 256), "DNode4 layout");
 // Leaf of the wide tree: a reference-tree leaf (its triangles first .. first + count - 1 in DFS
 // order), or a leaf of a leaf BVH (the wide tree holds leaf BVHs as subtrees of its own; its
 // triangles are subTri[first .. first + count - 1]).
