@@ -376,6 +376,7 @@ struct Engine {
   DBuf<unsigned long long> dNodeKey;   // dedup and order keys of the nodes with records
   DBuf<double> dImg;             // image source of each tree node (reference-sphere traces)
   size_t recCap = 0;
+  size_t heavyCap = 0;   // entries of the heavy-job list (grown when a capture pass fills it)
   DBuf<unsigned long long> dCounters;        // capture: records, corrected, groups, flags, collected, heavy
   DBuf<unsigned long long> dTraceCounters;   // bounce loop: overflow flags, node visits, triangle tests
   DBuf<unsigned long long> dWinKey, dWinKey2;

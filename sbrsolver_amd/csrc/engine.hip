@@ -681,6 +681,7 @@ struct CaptureBufs {
   // image source of each tree node (FP64, the operations of GetImageSources), or null: then the
   // correction's first-step pre-test is off
   const double* img;
+  unsigned long long heavyCap;    // entries of `heavy`
 };
 
 // The wave-per-node kernel removes duplicates of its own nodes' records, against the node's image
@@ -1260,8 +1261,10 @@ __device__ __forceinline__ void capture_pass(const CaptureArgs& A, const Capture
           break;
         }
       }
-      if (heavy) {
-        const unsigned long long h = atomicAdd(&B.counters[5], 1ull);
+      const unsigned long long h = heavy ? atomicAdd(&B.counters[5], 1ull) : 0ull;
+      if (heavy && h >= B.heavyCap) {
+        atomicOr(&B.counters[3], 16ull);   // job list full: the capture is redone with a larger one
+      } else if (heavy) {
         HeavyJob& J = B.heavy[h];
         for (int a = 0; a < 3; a++) {
           J.P0[a] = vget(c.P0, a);
@@ -1487,7 +1490,8 @@ __device__ __forceinline__ void ctx_uniform(NodeCtx& c) {
 // (occlusion tests, deep nodes) the ancestor rule comes first and the records are appended.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_capture_heavy(CaptureArgs A,
                                                                                    CaptureBufs B) {
-  const int64_t nHeavy = (int64_t)B.counters[5];   // appended by k_capture
+  // appended by k_capture (beyond the list's capacity only counted: the capture is then redone)
+  const int64_t nHeavy = (int64_t)std::min<unsigned long long>(B.counters[5], B.heavyCap);
   const int lane = threadIdx.x & 63;
   const unsigned long long lanesBelow = (1ull << lane) - 1ull;
   const int64_t wave = wave_uniform((int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
@@ -2864,7 +2868,11 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
       hipEvent_t kev[4];
       for (auto& e : kev) HIPCHK(hipEventCreate(&e));
       struct KevGuard { hipEvent_t* e; ~KevGuard() { for (int i = 0; i < 4; i++) (void)hipEventDestroy(e[i]); } } kguard{kev};
-      ENSURE(dHeavy, std::max<int64_t>(nodes, 1));
+      // heavy jobs: a list sized from the last trace's (a full list flags the pass, which is redone
+      // with a larger one), at most one per node
+      if (heavyCap == 0) heavyCap = 1 << 16;
+      size_t hcap = std::min<size_t>(heavyCap, (size_t)std::max<int64_t>(nodes, 1));
+      ENSURE(dHeavy, hcap);
       ENSURE(dNodeKey, 3 * (size_t)std::max<int64_t>(nodes, 1));
       CaptureArgs CA{};
       CA.nodes = nodes;   // bound of the listed nodes; their count is read on the device
@@ -2903,6 +2911,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
                        (unsigned long long)(cap - 1), dTabOrder.p, dTabRx.p, dCounters.p, dHeavy.p, dRec.p, part, dRecCount.p,
                        dRecPrefix.p, count ? 1 : 0, dNodeKey.p, dCand.p, dTraceCounters.p + 1};
         CB.img = cone ? dImg.p : nullptr;
+        CB.heavyCap = hcap;
         CB.nodes = dNodes.p;
         CB.nodesF = dNodesF.p;
         CB.tris = dTris.p;
@@ -2941,13 +2950,20 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         nHeavy = ctr[5];
         capNodes = (int64_t)ctr[12];
         nRec = nodes > 0 ? ctr[10] : 0;
-        if (!(ctr[3] & 4ull)) {
+        if (!(ctr[3] & 20ull)) {
           stats.capture_tests += (int64_t)ctr[6];
           break;
         }
         if (recAttempt > 4) return fail(SBR_E_LIMIT, "capture record buffer could not be sized");
-        const unsigned long long maxPart = ctr[11];
-        recCap = std::max<size_t>(recCap * 2, (size_t)(maxPart + maxPart / 4 + 1024) * kRecParts);
+        if (ctr[3] & 4ull) {
+          const unsigned long long maxPart = ctr[11];
+          recCap = std::max<size_t>(recCap * 2, (size_t)(maxPart + maxPart / 4 + 1024) * kRecParts);
+        }
+        if (ctr[3] & 16ull) {
+          heavyCap = std::max<size_t>(heavyCap * 2, (size_t)(ctr[5] + ctr[5] / 4 + 1024));
+          hcap = std::min<size_t>(heavyCap, (size_t)std::max<int64_t>(nodes, 1));
+          ENSURE(dHeavy, hcap);
+        }
       }
       stats.heavy_nodes += (int64_t)nHeavy;
       stats.capture_nodes += capNodes;

@@ -3,10 +3,11 @@ spans more than 12 receiver-grid cells (48 with the reference sphere), a propert
 So the number of such nodes does not change between repeated traces or with the launch-window size
 (which changes how the nodes are numbered and grouped into waves), and neither do the records.
 
-Record-buffer overflow: the dedup pass runs before the host sees that the candidate pass filled a
-partition of the record buffer; the window's capture is then redone with a larger buffer over tables
-that already hold part of the result.  A fresh context's first trace that overflows must equal the
-same trace once the buffer has grown."""
+Record-buffer and heavy-list overflow: the dedup pass runs before the host sees that the candidate
+pass filled a partition of the record buffer or the list of nodes left to the wave-per-node kernel
+(65,536 entries on a fresh context); the window's capture is then redone with larger ones over
+tables that already hold part of the result.  A fresh context's first trace that overflows both must
+equal the same trace once they have grown."""
 import numpy as np
 import pytest
 
@@ -51,6 +52,7 @@ def test_record_overflow_redo_equals_clean_pass():
         b = ctx.trace(p, cfg["rx"])
         print("[case] records", ra, "corrected", a.stats["corrected"], b.stats["corrected"], flush=True)
         assert ra == b.stats["capture_records"] and ra > 2_500_000
+        assert a.stats["heavy_nodes"] == b.stats["heavy_nodes"] > 65536
         # the redo corrected the records of the first pass again: the first trace counts more
         assert a.stats["corrected"] > b.stats["corrected"]
         assert np.array_equal(fa, b.fields, equal_nan=True)
