@@ -168,7 +168,7 @@ typedef struct {
 /* Launch configuration of the traversal kernel (no reference counterpart).  lds_scene: -1 stage
  * the scene in LDS when it fits and does not lower the resident blocks per CU, 1 whenever it
  * fits, 0 never; blocks_per_cu: persistent blocks per compute unit, 0 = the occupancy maximum;
- * window_nodes: interaction-tree nodes held on the device at once (128 B each), 0 = 2^27; smaller
+ * window_nodes: interaction-tree nodes held on the device at once (128 B each), 0 = 2^28; smaller
  * values than 16384 are raised to 16384.  A trace whose trees need more runs in several launch
  * windows.  stack_lds: traversal stack entries per ray held in LDS (1..64), the deeper ones in
  * device memory; 0 = as many as leave LDS room for the blocks per CU the kernel's registers allow.

@@ -22,7 +22,10 @@ constexpr int kBlock = 256;         // threads per block of the traversal kernel
 constexpr size_t kSceneLdsMax = 96 * 1024;   // scenes up to this size are staged in LDS
 constexpr size_t kLdsPerCU = 160 * 1024;     // LDS of a gfx950 compute unit
 constexpr int kTraceBlocksPerCU = 4;         // bounce-kernel blocks per CU its registers allow (127 VGPRs)
-constexpr size_t kWindowNodes = size_t(1) << 27;   // tree nodes per launch window (17 GB of records)
+// Tree nodes per launch window: 2^28 (34 GB of records, about 44 GB with the queues, candidate list
+// and keys; 2^27 measured 12 % slower on the dense indoor trace, 28 windows against 12, and 3 % on
+// the 256-transmitter batch).
+constexpr size_t kWindowNodes = size_t(1) << 28;
 
 // ---- scene, device layout ---------------------------------------------------------------------
 // BVH node in depth-first order (children[0] subtree first), 64 B.
@@ -110,6 +113,9 @@ struct WideTree {
 };
 constexpr int kSubLeaf = 16;      // leaves above this size get a leaf BVH (16 against 64: dense indoor
                                   // bounce kernel -11 %, urban and city unchanged; 8: urban +8 %)
+// Probes of the open-addressing tables (sequence trie, groups) before a table counts as full: at the
+// 70 % load that also restarts a trace the expected run is under ten slots.
+constexpr unsigned long long kMaxProbe = 4096;
 constexpr int kRecParts = 1024;   // partitions of the capture record buffer (one append counter each)
 constexpr int kRecStride = 16;    // counters 128 B apart
 constexpr int kSubLeafSize = 4;   // triangles per leaf-BVH leaf

@@ -830,7 +830,7 @@ __device__ __forceinline__ void node_from_job(const CaptureArgs& A, const Captur
 __device__ int64_t seq_intern(const CaptureBufs& B, int64_t parent, uint32_t elem) {
   const unsigned long long key = seq_key(parent, elem);
   unsigned long long slot = mix64(key) & B.seqMask;
-  for (unsigned long long probe = 0; probe <= B.seqMask; probe++) {
+  for (unsigned long long probe = 0; probe <= B.seqMask && probe < kMaxProbe; probe++) {
     const unsigned long long cur =
         __hip_atomic_load(&B.seqTab[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (cur == key) return (int64_t)slot;
@@ -1091,7 +1091,7 @@ __device__ bool unobstructed(const CaptureBufs& B, const NodeCtx& c, V3 C, int* 
 __device__ __forceinline__ bool group_may_win(const CaptureArgs& A, const CaptureBufs& B, unsigned long long gk,
                                               unsigned long long orderKey) {
   unsigned long long slot = mix64(gk) & A.tabMask;
-  for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
+  for (unsigned long long probe = 0; probe <= A.tabMask && probe < kMaxProbe; probe++) {
     const unsigned long long a = B.tabA[slot];
     if (a == 0ull) return true;
     if (a == gk) return B.tabOrder[slot] >= orderKey;
@@ -1102,12 +1102,13 @@ __device__ __forceinline__ bool group_may_win(const CaptureArgs& A, const Captur
 
 // Inserts a corrected path: the group's slot (claimed by the first insertion, which also records the
 // receiver) keeps the smallest order key.  Returns 1 if this call claimed the slot; flags the table
-// full when no slot is left.
+// full when no slot is left within kMaxProbe probes (the trace is then redone with a larger table:
+// a nearly full table would otherwise make the probes of one pass run over most of it).
 __device__ __forceinline__ unsigned long long group_insert(const CaptureArgs& A, const CaptureBufs& B,
                                                            unsigned long long gk, unsigned long long orderKey,
                                                            int64_t rxOut) {
   unsigned long long slot = mix64(gk) & A.tabMask;
-  for (unsigned long long probe = 0; probe <= A.tabMask; probe++) {
+  for (unsigned long long probe = 0; probe <= A.tabMask && probe < kMaxProbe; probe++) {
     const unsigned long long prev = atomicCAS(&B.tabA[slot], 0ull, gk);
     if (prev == 0ull || prev == gk) {
       if (prev == 0ull) B.tabRx[slot] = (int32_t)rxOut;
@@ -2747,7 +2748,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   // added while w's trees are still resident.  Receiver sums continue in key order across windows.
   const DSurf* surf = P[0].q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
   size_t cap = 1;
-  while (cap < (size_t)std::max<int64_t>(std::max<int64_t>(1 << 16, 4 * nOut), (int64_t)(tabCapHint / 4))) cap <<= 1;
+  while (cap < (size_t)std::max<int64_t>(std::max<int64_t>(1 << 16, 4 * nOut), (int64_t)tabCapHint)) cap <<= 1;
   ENSURE(dLevels, kMaxDepth + 2);
   ENSURE(dCounters, 10);
   ENSURE(dTraceCounters, 8);
