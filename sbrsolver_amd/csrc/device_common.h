@@ -388,8 +388,8 @@ __device__ __forceinline__ bool closest_hit(const DNode* __restrict__ nodes, con
 template <bool SUB, bool COUNT>
 __device__ __forceinline__ void wide_hit(const DNode4* __restrict__ n4, const DLeaf* __restrict__ leaves,
                                          const DTri* __restrict__ tris, const DBox32* __restrict__ tbox,
-                                         const DSub* __restrict__ subs, const int32_t* __restrict__ subTri,
-                                         const RayF& f, V3 o, V3 d, Hit& h, const TravStack& st) {
+                                         const int32_t* __restrict__ subTri, const RayF& f, V3 o, V3 d, Hit& h,
+                                         const TravStack& st) {
   h.idx = INT_MAX;
   h.dist = INFINITY;
   h.t = INFINITY;
@@ -507,7 +507,7 @@ __device__ __forceinline__ bool closest_hit_wide(const WideTree& w, const DNode*
   const RayF f = rayf_pre(o, d, ext);
   bool done = false;
   if (f.on) {
-    wide_hit<SUB, COUNT>(w.n4, w.leaves, tris, tbox, subs, subTri, f, o, d, h, st);
+    wide_hit<SUB, COUNT>(w.n4, w.leaves, tris, tbox, subTri, f, o, d, h, st);
     done = h.idx == INT_MAX || wide_reaches<COUNT>(tris[h.idx].leaf, w.ver, nodes, f, o, d, h);
   }
   if (!done) {

@@ -75,7 +75,9 @@ constexpr int kTravRef = 2;
 // largest box surface is replaced by its two children until four are held.  Each child's box is the
 // exact box grown by 2 delta and rounded outward (a ray whose computed interval of it is empty
 // misses the exact box by more than delta), as single-precision bounds per axis; an empty slot
-// has NaN bounds, which no interval test passes.  The wide tree only finds the triangles a ray may
+// has NaN bounds, which no interval test passes (and item -1, which the traversal checks too).  A
+// reference leaf with a leaf BVH is a subtree of such nodes over the leaf BVH's nodes (their boxes:
+// unions of the triangles' padded pre-test boxes).  The wide tree only finds the triangles a ray may
 // hit; whether the reference's traversal reaches the winner's leaf is checked on the reference
 // tree itself (DVer).
 constexpr int kWide = 4;   // children per wide node (4: 128 B nodes; 8, 256 B, measured slower)
