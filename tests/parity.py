@@ -40,12 +40,14 @@ def compare(orc, gpu_res, label=""):
     assert np.array_equal(gp["ssrc"], op["ssrc"]) and np.array_equal(gp["stgt"], op["stgt"]), \
         f"{label}: corrected vertices differ (max {np.abs(gp['ssrc'] - op['ssrc']).max()})"
     pf_g, pf_o = gpu_res.path_fields(), orc.path_fields
-    worst_path = _worst(pf_g, pf_o, f"{label} (per path)", to)
-    worst = _worst(gpu_res.fields, orc.fields, f"{label} (per receiver)")
+    worst_path = worst_field_error(pf_g, pf_o, f"{label} (per path)", to)
+    worst = worst_field_error(gpu_res.fields, orc.fields, f"{label} (per receiver)")
     return max(worst, worst_path)
 
 
-def _worst(eg, eo, label, table=None):
+def worst_field_error(eg, eo, label, table=None):
+    """Asserts per-receiver (or per-path) fields within FIELD_RTOL of the oracle's, with NaN on the
+    same entries; returns the worst relative error among entries above -180 dB of the strongest."""
     # NaN where the reference produces NaN (the half-wave pattern on its axis is 0/0): the HIP
     # path must produce NaN on exactly the same entries.
     ng, no = ~np.isfinite(eg).all(axis=1), ~np.isfinite(eo).all(axis=1)

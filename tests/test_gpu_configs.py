@@ -15,7 +15,7 @@ import pytest
 
 from oracle import oracle as O
 from sbrsolver_amd import Server, make_params, scenes
-from tests.parity import _worst, compare
+from tests.parity import worst_field_error, compare
 
 pytestmark = pytest.mark.gpu
 
@@ -102,7 +102,7 @@ def test_city_serving_matches_oracle(gpu_ctx, city):
     npaths = []
     for k, ((kw, rx), f) in enumerate(zip(qs, served)):
         o = O.trace(sc, O.make_params(**kw), rx, mats, nthreads=16)
-        _worst(f, o.fields, f"city query {k} (served)")
+        worst_field_error(f, o.fields, f"city query {k} (served)")
         g = gpu_ctx.trace(make_params(**kw), rx)
         compare(o, g, f"city query {k} (traced)")
         say("city query", k, "level", kw["tess"], "receivers", len(rx), "paths", o.np, "bounces", o.bounces)

@@ -1,12 +1,14 @@
 """Serving loop on the GPU: queries submitted while others run (several slots, two submitting
 threads) give exactly the fields of the same query traced alone, and of sbr_trace on a plain
-context."""
+context, and the fields of the oracle's trace of the same transmitter."""
 import threading
 
 import numpy as np
 import pytest
 
+from oracle import oracle as O
 from sbrsolver_amd import Context, Server, make_params, scenes
+from tests.parity import worst_field_error
 
 pytestmark = pytest.mark.gpu
 
@@ -62,3 +64,33 @@ def test_results_identical_alone_and_in_the_mix(urban_ctx):
     for k, (kw, rx) in enumerate(qs):
         ref = ctx.trace(make_params(**kw), rx).fields
         assert np.array_equal(mixed[k], ref, equal_nan=True), f"query {k}"
+
+
+def test_served_fields_match_oracle(urban_ctx):
+    """Served queries (three at once on three slots: a capture-cell query, one with the reference
+    sphere, one with transmission) against the FP64 oracle's trace of the same transmitter."""
+    ctx, cfg = urban_ctx
+    _, boxes, _ = scenes.urban_scene()
+    rng = np.random.default_rng(7)
+    qs = []
+    for kw, _ in scenes.coverage_queries(3, boxes, seed=7, tess=(5, 5), max_rx=1):
+        c = np.asarray(kw["tx"])     # 120 receivers in the 200 m square around the transmitter
+        qs.append((kw, np.column_stack([c[0] + rng.uniform(-100, 100, 120), c[1] + rng.uniform(-100, 100, 120),
+                                        np.full(120, 1.5)])))
+    qs[1] = (dict(qs[1][0], receptionModel=0), qs[1][1])
+    qs[2] = (dict(qs[2][0], maxT=1), qs[2][1])
+    sc = O.Scene.from_soup(*cfg["soup"])
+    print("[case] serving against the oracle: 3 queries, level 5", flush=True)
+    with Server(ctx, slots=3) as srv:
+        ids = [srv.submit(make_params(**kw), rx) for kw, rx in qs]
+        served = []
+        for q, (kw, rx) in zip(ids, qs):
+            rc, _ = srv.poll(q, wait_ms=-1)
+            assert rc == 0, srv.error(q)
+            served.append(srv.fields(q, len(rx)))
+            srv.release(q)
+    for k, ((kw, rx), f) in enumerate(zip(qs, served)):
+        o = O.trace(sc, O.make_params(**kw), rx, cfg["materials"], nthreads=8)
+        assert np.nanmax(np.abs(o.fields)) > 0, f"query {k}: nothing received"
+        worst = worst_field_error(f, o.fields, f"served query {k}")
+        print(f"[case] served query {k}: {len(rx)} receivers, worst field rel err {worst:.2e}", flush=True)

@@ -212,7 +212,7 @@ def test_dedup_survivor_is_first_by_launch():
 def test_planted_mistake_is_caught():
     """A wrong reflection material (planted in a copy of the inputs) changes the field: the field
     comparison used by the parity tests is sensitive to it."""
-    from tests.parity import _worst
+    from tests.parity import worst_field_error
     cfg = scenes.config1()
     sc = O.Scene.from_soup(*cfg["soup"])
     tr = dict(cfg["trace"], tess=2)
@@ -221,7 +221,7 @@ def test_planted_mistake_is_caught():
     bad[0, 3] = 5.2
     b = O.trace(sc, O.make_params(**tr), cfg["rx"], bad)
     with pytest.raises(AssertionError):
-        _worst(b.fields, a.fields, "planted")
+        worst_field_error(b.fields, a.fields, "planted")
 
 
 def test_canonical_traversal_counts_fixture():
