@@ -309,6 +309,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-transmitter line")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-loop line")
     ap.add_argument("--no-indoor", action="store_true", help="skip the dense indoor line")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batched multi-transmitter workload and print its line (profiling aid)")
 
     ap.add_argument("--calib", action="store_true",
                     help="run one plain 256 MiB device-to-device copy (counter calibration for rocprofv3 --pmc)")
@@ -325,6 +327,10 @@ def main():
     ctx.scene_from_soup(*cfg["soup"])
     ctx.materials(cfg["materials"])
     print(f"[bench] scene {cfg['name']} uploaded; warmup {args.warmup}", file=sys.stderr, flush=True)
+    if args.batch_only:
+        print(json.dumps(dict(batched=batched_run(ctx, make_params, scenes))), flush=True)
+        ctx.close()
+        return
 
     # One cold step: the first trace of the context generates the level's launch directions (later
     # steps reuse them) and sizes its buffers.

@@ -163,6 +163,8 @@ typedef struct {
   int64_t capture_nodes;    /* tree nodes whose segment reaches the receivers' capture region */
   int64_t wide_redone;      /* bounces whose wide-tree winner the reference tree's box tests did not
                                confirm, redone on the reference tree (countTraversal) */
+  int64_t table_entries;    /* size of the duplicate-removal tables the trace finished with */
+  int64_t table_restarts;   /* restarts of the trace with larger tables (a table filled up) */
 } sbr_stats;
 
 /* Launch configuration of the traversal kernel (no reference counterpart).  lds_scene: -1 stage
@@ -173,14 +175,19 @@ typedef struct {
  * windows.  stack_lds: traversal stack entries per ray held in LDS (1..64), the deeper ones in
  * device memory; 0 = as many as leave LDS room for the blocks per CU the kernel's registers allow.
  * oversub: blocks of the bounce kernel's grid per resident block (1..64; the chunks of 256 items
- * that the hardware dispatcher hands to freed block slots), 0 = 8.  Results do not depend on any of
- * them. */
+ * that the hardware dispatcher hands to freed block slots), 0 = 8.  table_entries: starting size of
+ * the duplicate-removal hash tables (rounded up to a power of two, at least 1024; a table that
+ * fills up restarts the trace with one four times larger), 0 = sized from the last trace: its final
+ * size, scaled down with the trace when both the (transmitter, receiver) outputs and the launches
+ * are under a sixteenth of the last trace's, and at least 4 entries per output and 65,536.  Results
+ * do not depend on any of them. */
 typedef struct {
   int32_t lds_scene;
   int32_t blocks_per_cu;
   int64_t window_nodes;
   int32_t stack_lds;
   int32_t oversub;
+  int64_t table_entries;
 } sbr_tuning;
 
 /* Autotuner (no reference counterpart): times the traversal kernel on a sample of the given trace

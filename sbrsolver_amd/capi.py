@@ -41,7 +41,8 @@ class Stats(C.Structure):
         ("node_visits", C.c_int64), ("tri_tests", C.c_int64),
         ("trace_lds", C.c_int32), ("trace_blocks_per_cu", C.c_int32), ("heavy_nodes", C.c_int64),
         ("windows", C.c_int64), ("capture_tests", C.c_int64), ("box_tests_fp64", C.c_int64),
-        ("capture_nodes", C.c_int64), ("wide_redone", C.c_int64),
+        ("capture_nodes", C.c_int64), ("wide_redone", C.c_int64), ("table_entries", C.c_int64),
+        ("table_restarts", C.c_int64),
     ]
 
     def as_dict(self):
@@ -50,7 +51,7 @@ class Stats(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("lds_scene", C.c_int32), ("blocks_per_cu", C.c_int32), ("window_nodes", C.c_int64),
-                ("stack_lds", C.c_int32), ("oversub", C.c_int32)]
+                ("stack_lds", C.c_int32), ("oversub", C.c_int32), ("table_entries", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -343,8 +344,9 @@ class Context:
             raise ValueError("gain_info must have shape (n_theta, n_phi, 4)")
         self._chk(lib().sbr_set_tx_pattern(self.h, _a(g), g.shape[0], g.shape[1]))
 
-    def set_tuning(self, lds_scene=-1, blocks_per_cu=0, window_nodes=0, stack_lds=0, oversub=0):
-        t = Tuning(int(lds_scene), int(blocks_per_cu), int(window_nodes), int(stack_lds), int(oversub))
+    def set_tuning(self, lds_scene=-1, blocks_per_cu=0, window_nodes=0, stack_lds=0, oversub=0, table_entries=0):
+        t = Tuning(int(lds_scene), int(blocks_per_cu), int(window_nodes), int(stack_lds), int(oversub),
+                   int(table_entries))
         self._chk(lib().sbr_set_tuning(self.h, C.byref(t)))
 
     def autotune(self, params, rx):

@@ -593,13 +593,14 @@ int sbr_set_tuning(sbr_ctx* ctx, const sbr_tuning* t) {
   return guarded(ctx, [&] {
     if (!t || t->lds_scene < -1 || t->lds_scene > 1 || t->blocks_per_cu < 0 || t->blocks_per_cu > 64 ||
         t->window_nodes < 0 || t->window_nodes > (int64_t(1) << 30) || t->stack_lds < 0 || t->stack_lds > 64 ||
-        t->oversub < 0 || t->oversub > 64)
+        t->oversub < 0 || t->oversub > 64 || t->table_entries < 0 || t->table_entries > (int64_t(1) << 32))
       return ctx->eng.fail(SBR_E_ARG, "tuning out of range");
     ctx->eng.tune.ldsScene = t->lds_scene;
     ctx->eng.tune.blocksPerCU = t->blocks_per_cu;
     ctx->eng.tune.windowNodes = t->window_nodes;
     ctx->eng.tune.stackLds = t->stack_lds;
     ctx->eng.tune.oversub = t->oversub;
+    ctx->eng.tune.tableEntries = t->table_entries;
     return SBR_OK;
   });
 }
@@ -630,9 +631,9 @@ int sbr_autotune(sbr_ctx* ctx, const sbr_trace_params* params, const double* rx,
     };
     // 1. scene staging x blocks per CU; the first trial is the automatic choice, which also
     //    reports the occupancy maximum
-    std::vector<sbr::Tuning> trials{{-1, 0, saved.windowNodes, 0, 0}};
+    std::vector<sbr::Tuning> trials{{-1, 0, saved.windowNodes, 0, 0, saved.tableEntries}};
     for (int lds = 0; lds <= 1; lds++)
-      for (int bpc = 1; bpc <= 8; bpc++) trials.push_back({lds, bpc, saved.windowNodes, 0, 0});
+      for (int bpc = 1; bpc <= 8; bpc++) trials.push_back({lds, bpc, saved.windowNodes, 0, 0, saved.tableEntries});
     int maxBpc = 8;
     for (size_t k = 0; k < trials.size(); k++) {
       if (k > 0 && trials[k].blocksPerCU > maxBpc) continue;
@@ -702,6 +703,7 @@ int sbr_get_tuning(sbr_ctx* ctx, sbr_tuning* t) {
     t->window_nodes = ctx->eng.tune.windowNodes;
     t->stack_lds = ctx->eng.tune.stackLds;
     t->oversub = ctx->eng.tune.oversub;
+    t->table_entries = ctx->eng.tune.tableEntries;
     return SBR_OK;
   });
 }

@@ -284,6 +284,7 @@ struct Tuning {
   int64_t windowNodes = 0;   // tree nodes per launch window; 0: kWindowNodes
   int stackLds = 0;       // traversal stack entries per ray in LDS; 0: what leaves room for the blocks per CU
   int oversub = 0;        // bounce-kernel blocks per resident block; 0: kTraceOversub
+  int64_t tableEntries = 0;   // starting size of the duplicate-removal tables; 0: sized from the last trace
 };
 
 // Kept paths gathered window by window when a trace needs several launch windows.
@@ -402,6 +403,7 @@ struct Engine {
   int64_t winners = 0;
   size_t nodeCap = 0;
   size_t tabCapHint = 0;   // dedup table size of the last trace
+  int64_t tabHintOut = 0, tabHintLaunches = 0;   // its (transmitter, receiver) outputs and launches
   double perRootHint = 0;  // tree nodes per root item of the last trace's densest window (first window size)
   size_t hFieldN = 0;   // doubles of the last trace's receiver fields, held in pinOut
   const double* field_host() const { return reinterpret_cast<const double*>(pinOut.p); }

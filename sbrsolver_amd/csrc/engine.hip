@@ -2749,8 +2749,23 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   // is final when w ends (later windows only hold larger keys): its path is resolved and its field
   // added while w's trees are still resident.  Receiver sums continue in key order across windows.
   const DSurf* surf = P[0].q1MeshIndexing ? dSurfQ1.p : dSurfId.p;
+  // Tables start at the last trace's final size, so a repeated trace does not restart; a trace far
+  // smaller than the last (a single transmitter after a batch, a small query after a large one)
+  // scales it down by the larger of its output and launch ratios, since clearing and scanning the
+  // tables is a cost per entry.  A table that fills up restarts the trace with a larger one.
+  const int64_t nLaunches = nRoot;
+  int64_t want = std::max<int64_t>(1 << 16, 4 * nOut);
+  if (tune.tableEntries > 0) {
+    want = std::max<int64_t>(1024, tune.tableEntries);
+  } else if (tabCapHint > 0) {
+    double scale = 1.0;
+    if (16 * nOut < tabHintOut && 16 * nLaunches < tabHintLaunches)
+      scale = std::max((double)nOut / (double)tabHintOut, (double)nLaunches / (double)tabHintLaunches);
+    want = std::max<int64_t>(want, (int64_t)std::ceil((double)tabCapHint * scale));
+  }
   size_t cap = 1;
-  while (cap < (size_t)std::max<int64_t>(std::max<int64_t>(1 << 16, 4 * nOut), (int64_t)tabCapHint)) cap <<= 1;
+  while (cap < (size_t)want) cap <<= 1;
+  int tabRestarts = 0;
   ENSURE(dLevels, kMaxDepth + 2);
   ENSURE(dCounters, 10);
   ENSURE(dTraceCounters, 8);
@@ -2773,7 +2788,7 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
   }
   double perRootMax = 0;   // densest window of this trace (nodes per root item)
   bool done = false;
-  for (int tabAttempt = 0; tabAttempt < 6 && !done; tabAttempt++) {
+  for (int tabAttempt = 0; tabAttempt < 12 && !done; tabAttempt++) {
     stats.bounces = stats.node_visits = stats.tri_tests = stats.capture_records = stats.corrected = 0;
     stats.box_tests_fp64 = 0;
     stats.ms_k_trace = stats.ms_k_capture = stats.ms_trace = stats.ms_capture = stats.ms_field = 0;
@@ -3044,13 +3059,20 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
     }
     if (tableFull) {
       cap <<= 2;
+      tabRestarts++;
       continue;
     }
     done = true;
   }
   if (!done) return fail(SBR_E_LIMIT, "dedup table could not be sized");
   if (nRoot > 0 && perRootMax > 0) perRootHint = perRootMax;
-  if (m > 0) tabCapHint = cap;
+  if (m > 0) {
+    tabCapHint = cap;
+    tabHintOut = nOut;
+    tabHintLaunches = nLaunches;
+  }
+  stats.table_entries = m > 0 ? (int64_t)cap : 0;
+  stats.table_restarts = tabRestarts;
   if (multiWindow) winners = store.np;
   HIPCHK(hipEventRecord(ev[4], stream));
   HIPCHK(hipStreamSynchronize(stream));

@@ -44,9 +44,11 @@ def test_host_only_context_rejects_device_work():
             c.autotune(make_params(tx=(0, 0, 1), tess=1, maxR=1), np.zeros((1, 3)))
         with pytest.raises(SbrError, match="out of range"):
             c.set_tuning(window_nodes=-5)
-        c.set_tuning(1, 2, 1 << 20, 6, 3)
-        assert c.get_tuning() == dict(lds_scene=1, blocks_per_cu=2, window_nodes=1 << 20, stack_lds=6, oversub=3)
-        for bad in (dict(stack_lds=-1), dict(stack_lds=65), dict(oversub=-2), dict(oversub=65)):
+        c.set_tuning(1, 2, 1 << 20, 6, 3, 5000)
+        assert c.get_tuning() == dict(lds_scene=1, blocks_per_cu=2, window_nodes=1 << 20, stack_lds=6, oversub=3,
+                                      table_entries=5000)
+        for bad in (dict(stack_lds=-1), dict(stack_lds=65), dict(oversub=-2), dict(oversub=65), dict(table_entries=-1),
+                    dict(table_entries=(1 << 32) + 1)):
             with pytest.raises(SbrError):
                 c.set_tuning(**bad)
         with pytest.raises(SbrError):
