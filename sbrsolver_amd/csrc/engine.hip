@@ -1713,8 +1713,12 @@ __global__ void __launch_bounds__(256) k_flag_scatter(const uint8_t* __restrict_
     *nOut = (unsigned long long)tileOffset[nTiles - 1] + (unsigned long long)tileCount[nTiles - 1];
 }
 
-constexpr int kCollectPerThread = 16;   // table slots per thread (a block takes 4096 contiguous slots)
+// Table slots per thread: 16 (a block takes 4096 contiguous slots) on large tables; one on tables
+// up to 4 M slots, where 16 would leave most CUs idle and the loop's dependent loads would set the time.
+constexpr int kCollectPerThread = 16;
+constexpr unsigned long long kCollectSmall = 1ull << 22;
 
+template <int PER>
 __global__ void __launch_bounds__(256) k_collect(const unsigned long long* __restrict__ tabA,
                                                  const unsigned long long* __restrict__ tabOrder,
                                                  const int32_t* __restrict__ tabRx, unsigned long long cap,
@@ -1725,7 +1729,7 @@ __global__ void __launch_bounds__(256) k_collect(const unsigned long long* __res
   // then write (the slots are read again from the cache).  One append per block rather than per wave:
   // appends to a single counter serialise.  The order of the winners does not matter (they are
   // sorted by key next).
-  const unsigned long long tile = (unsigned long long)blockIdx.x * (256ull * kCollectPerThread);
+  const unsigned long long tile = (unsigned long long)blockIdx.x * (256ull * PER);
   auto takes = [&](unsigned long long s, unsigned long long* key) {
     if (s >= cap || tabA[s] == 0ull) return false;
     *key = tabOrder[s];
@@ -1733,7 +1737,7 @@ __global__ void __launch_bounds__(256) k_collect(const unsigned long long* __res
     return gl >= glLo && gl < glHi;
   };
   int cnt = 0;
-  for (int i = 0; i < kCollectPerThread; i++) {
+  for (int i = 0; i < PER; i++) {
     unsigned long long key;
     if (takes(tile + threadIdx.x + 256ull * i, &key)) cnt++;
   }
@@ -1744,7 +1748,7 @@ __global__ void __launch_bounds__(256) k_collect(const unsigned long long* __res
   __syncthreads();
   if (cnt == 0) return;
   unsigned long long w = sBase + (unsigned long long)excl;
-  for (int i = 0; i < kCollectPerThread; i++) {
+  for (int i = 0; i < PER; i++) {
     const unsigned long long sl = tile + threadIdx.x + 256ull * i;
     unsigned long long key;
     if (takes(sl, &key)) {
@@ -3013,10 +3017,11 @@ int Engine::trace(const sbr_trace_params* P, int nt, const double* rx, int m) {
         ENSURE(dWinNode, Wc);
         ENSURE(dWinField, 6 * Wc);
         HIPCHK(hipMemsetAsync(dCounters.p + 4, 0, sizeof(unsigned long long), stream));
-        k_collect<<<blocks_for((int64_t)cap, 256 * kCollectPerThread), 256, 0, stream>>>(dTabA.p, dTabOrder.p, dTabRx.p, cap,
-                                                                      (unsigned long long)ws,
-                                                                      (unsigned long long)(ws + W), dWinKey.p,
-                                                                      dWinRx.p, dCounters.p + 4);
+        const bool smallTab = cap <= kCollectSmall;
+        auto kcol = smallTab ? k_collect<1> : k_collect<kCollectPerThread>;
+        kcol<<<blocks_for((int64_t)cap, 256 * (smallTab ? 1 : kCollectPerThread)), 256, 0, stream>>>(
+            dTabA.p, dTabOrder.p, dTabRx.p, cap, (unsigned long long)ws, (unsigned long long)(ws + W), dWinKey.p,
+            dWinRx.p, dCounters.p + 4);
         HIPCHK(hipGetLastError());
         int txrxBits = 1;
         while ((int64_t(1) << txrxBits) < nOut) txrxBits++;
